@@ -1,0 +1,116 @@
+"""Seeded synthetic column families.
+
+These are the inputs of the decode benchmarks and of the encoding wizard's training set. All generators
+use numpy's PCG64 bit generator with an explicit seed, so every host (including the GPU host) produces the
+same bytes for the same arguments.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+W2U = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+
+
+def rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def config1_values(n: int = 1 << 20, seed: int = 42) -> np.ndarray:
+    """FFOR u32 workload: 1,000,000 + U[0, 2^7), bit width 7."""
+    return (np.uint32(1_000_000) + rng(seed).integers(0, 128, n, dtype=np.uint32)).astype(np.uint32)
+
+
+def sweep_case(W: int, w: int, n_vec: int, seed: int = 0):
+    """Bit-width sweep case: values U[0, 2^w) and per-vector FFOR bases U[0, 2^(W-w)] (inclusive; at w = 0 the
+    bound 2^W is clamped to 2^W - 1). Returns (packed-domain values, bases); FFOR values are value + base."""
+    g = rng(seed * 1000 + W * 100 + w)
+    dt = W2U[W]
+    hi = (1 << w) if w < 64 else None
+    if w == 0:
+        vals = np.zeros(n_vec * 1024, dtype=dt)
+    elif w == W:
+        vals = g.integers(0, np.iinfo(dt).max, n_vec * 1024, dtype=dt, endpoint=True)
+    else:
+        vals = g.integers(0, hi, n_vec * 1024, dtype=np.uint64).astype(dt)
+    top = (1 << (W - w)) if W - w < 64 else (1 << 64) - 1
+    top = min(top, (1 << W) - 1)
+    bases = g.integers(0, top, n_vec, dtype=np.uint64, endpoint=True).astype(dt)
+    return vals, bases
+
+
+def timestamps(n: int, seed: int = 3) -> np.ndarray:
+    """Sorted int64 timestamps starting at 1.7e18 with gaps U[1, 100]."""
+    gaps = rng(seed).integers(1, 101, n, dtype=np.int64)
+    gaps[0] = 0
+    return np.int64(1_700_000_000_000_000_000) + np.cumsum(gaps, dtype=np.int64)
+
+
+def geometric_runs(n: int, mean: float = 16.0, seed: int = 4, value_bits: int = 20,
+                   dtype=np.uint32) -> np.ndarray:
+    """Runs with Geometric(1/mean) lengths and values U[0, 2^value_bits)."""
+    g = rng(seed)
+    lens = g.geometric(1.0 / mean, size=n // max(int(mean), 1) * 2 + 16)
+    while lens.sum() < n:
+        lens = np.concatenate([lens, g.geometric(1.0 / mean, size=lens.size)])
+    vals = g.integers(0, 1 << value_bits, lens.size, dtype=np.uint64).astype(dtype)
+    return np.repeat(vals, lens)[:n].copy()
+
+
+def zipf_codes(n: int, n_keys: int = 4096, s: float = 1.1, seed: int = 5, dtype=np.int32) -> np.ndarray:
+    """Categorical column: n_keys distinct random keys drawn with Zipf(s) over key rank."""
+    g = rng(seed)
+    info = np.iinfo(dtype)
+    keys = g.choice(np.arange(info.min // 2, info.max // 2, dtype=np.int64), size=n_keys, replace=False)
+    p = 1.0 / np.arange(1, n_keys + 1, dtype=np.float64) ** s
+    p /= p.sum()
+    ranks = g.choice(n_keys, size=n, p=p)
+    return keys[ranks].astype(dtype)
+
+
+def vocabulary(n_words: int = 2000, seed: int = 11) -> list[bytes]:
+    g = rng(seed)
+    letters = np.frombuffer(b"abcdefghijklmnopqrstuvwxyz", dtype=np.uint8)
+    words, seen = [], set()
+    while len(words) < n_words:
+        ln = int(g.integers(3, 11))
+        w = letters[g.integers(0, 26, ln)].tobytes()
+        if w not in seen:
+            seen.add(w)
+            words.append(w)
+    return words
+
+
+def vocab_strings(n: int, seed: int = 7, min_len: int = 8, max_len: int = 64, n_words: int = 2000,
+                  words_per_string: tuple[int, int] | None = None):
+    """Strings of a 2000-word lowercase vocabulary. With words_per_string=None each string has a length
+    U[min_len, max_len] (words joined by spaces, cut to length); otherwise it has U[lo, hi] words.
+    Returns (offsets uint32[n+1], bytes uint8)."""
+    g = rng(seed)
+    vocab = vocabulary(n_words)
+    idx = g.integers(0, len(vocab), size=n * 12)
+    out, offs, k = bytearray(), np.zeros(n + 1, dtype=np.uint32), 0
+    for i in range(n):
+        if words_per_string is None:
+            target = int(g.integers(min_len, max_len + 1))
+            s = bytearray()
+            while len(s) < target:
+                if s:
+                    s += b" "
+                s += vocab[idx[k % idx.size]]
+                k += 1
+            s = s[:target]
+        else:
+            cnt = int(g.integers(words_per_string[0], words_per_string[1] + 1))
+            s = b" ".join(vocab[idx[(k + j) % idx.size]] for j in range(cnt))
+            k += cnt
+        out += s
+        offs[i + 1] = len(out)
+    return offs, np.frombuffer(bytes(out), dtype=np.uint8).copy()
+
+
+def for_with_outliers(n: int, width: int = 8, outlier_frac: float = 0.01, seed: int = 6) -> np.ndarray:
+    g = rng(seed)
+    x = np.int64(123_456) + g.integers(0, 1 << width, n, dtype=np.int64)
+    m = g.random(n) < outlier_frac
+    x[m] = g.integers(0, 1 << 40, int(m.sum()), dtype=np.int64)
+    return x

@@ -1,0 +1,1133 @@
+/*
+ * fls_oracle.c -- CPU restatement of FastLanes' decode path (test infrastructure only).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this code, and only as
+ * the checker or the timed CPU baseline. The shipped decoder (fastlanes_amd/csrc) never calls it.
+ *
+ * Citations are relative to the reference tree (cwida/FastLanes). "gen/X" abbreviates
+ * primitives/fls_generated/fallback/scalar_aav_uf1/fallback_scalar_aav_1024_uf1_X.
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   orc_transpose/untranspose  KAT arrays untransposed_arr_owW / transposed_arr_owW (gen/rsum_helper.hpp:222-765)
+ *                              + compiled reference transpose_i/untranspose_i (oracle/_ref) + inverse property.
+ *   orc_pack/unpack/ffor/unffor  packed bytes equal the compiled reference ffor/pack on the 124+124 helper arrays
+ *                              (gen/unffor_helper.hpp, gen/unpack_helper.hpp; fixtures in tests/golden) and on
+ *                              random inputs; bw=0 and bw=W special cases; round trip.
+ *   orc_rsum/unrsum            KAT delta_arr/base_arr/transposed_arr (gen/rsum_helper.hpp:325-896), compiled
+ *                              reference rsum/unrsum, brute-force serial scan per lane.
+ *   orc_ffor_analyze           closed form bw = ceil(log2(max-min+1)) (src/cor/prm/ffor/ffor_analyze.cpp:78-102),
+ *                              checked at every power-of-two boundary and the 64-bit extremes.
+ *   page writer / reader       size closed forms checked against the CSVs in test/local_result/col_result/generated
+ *                              (UNCOMPRESSED 8204 B/1 vec, 131144 B/16 vec; FFOR 2077 B; RLE rows imply the same
+ *                              run count as the DICT rows) and encode->decode round trips.
+ *   HZL decode                 closed form of src/cor/eng/hzl_de_cmp.cpp:28-77 (rec_hdr chain + exceptions + tail),
+ *                              round trip; the encoder's exception search is simplified (no exceptions) -- the
+ *                              decoder handles exceptions and is tested on hand-built exception chunks.
+ *   FSST decode                byte-identical to the compiled reference fsst_decompress on tables and strings
+ *                              produced by the compiled reference fsst_create/fsst_compress (tests/golden fixtures),
+ *                              and round trip with this file's own table builder.
+ */
+#include "fls_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* element access helpers (W-generic through uint64_t)                                                          */
+/* ------------------------------------------------------------------------------------------------------------- */
+static inline uint64_t ld(unsigned W, const void* p, size_t i) {
+    switch (W) {
+    case 8: return ((const uint8_t*)p)[i];
+    case 16: return ((const uint16_t*)p)[i];
+    case 32: return ((const uint32_t*)p)[i];
+    default: return ((const uint64_t*)p)[i];
+    }
+}
+static inline void st(unsigned W, void* p, size_t i, uint64_t v) {
+    switch (W) {
+    case 8: ((uint8_t*)p)[i] = (uint8_t)v; break;
+    case 16: ((uint16_t*)p)[i] = (uint16_t)v; break;
+    case 32: ((uint32_t*)p)[i] = (uint32_t)v; break;
+    default: ((uint64_t*)p)[i] = v; break;
+    }
+}
+static inline uint64_t wmask(unsigned W) { return W == 64 ? ~0ULL : ((1ULL << W) - 1); }
+static inline uint64_t bmask(unsigned bw) { return bw >= 64 ? ~0ULL : ((1ULL << bw) - 1); }
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* Transposed ("unified FastLanes") order.                                                                       */
+/* gen/untranspose_src.cpp:6-8215: transposed index k = 128r + 16b + c maps to original r + 8*ORDER[b] + 64c,  */
+/* the same permutation for every element width; gen/transpose_src.cpp is its inverse.                          */
+/* ------------------------------------------------------------------------------------------------------------- */
+static const unsigned ORDER8[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+
+static inline unsigned orig_of_transposed(unsigned k) {
+    unsigned r = k >> 7, b = (k >> 4) & 7, c = k & 15;
+    return r + 8 * ORDER8[b] + 64 * c;
+}
+
+void orc_untranspose(unsigned W, const void* in, void* out) {
+    for (unsigned k = 0; k < 1024; ++k) st(W, out, orig_of_transposed(k), ld(W, in, k));
+}
+void orc_transpose(unsigned W, const void* in, void* out) {
+    for (unsigned k = 0; k < 1024; ++k) st(W, out, k, ld(W, in, orig_of_transposed(k)));
+}
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* Bit packing (gen/pack_src.cpp, gen/unpack_src.cpp, gen/ffor_src.cpp:29752-29941, gen/unffor_src.cpp:22879-  */
+/* 23077). L = 1024/W lanes; packed word w of lane i is in[w*L + i]; value j of lane i is out[j*L + i]; inside  */
+/* a lane the bw-bit values are concatenated little-endian, a straddling value takes its low bits from word w   */
+/* and its high bits from word w+1 (e.g. gen/unffor_src.cpp:2244-2330 for u32 bw=7).                           */
+/* ------------------------------------------------------------------------------------------------------------- */
+static void pack_impl(unsigned W, const void* in, void* out, unsigned bw, uint64_t base) {
+    const unsigned L = 1024 / W;
+    if (bw == 0) return; /* gen/ffor_src.cpp:6 writes nothing */
+    memset(out, 0, (size_t)bw * 128);
+    for (unsigned i = 0; i < L; ++i) {
+        for (unsigned j = 0; j < W; ++j) {
+            uint64_t v   = ((ld(W, in, (size_t)j * L + i) - base) & wmask(W)) & bmask(bw);
+            unsigned bit = j * bw, w = bit / W, s = bit % W;
+            st(W, out, (size_t)w * L + i, ld(W, out, (size_t)w * L + i) | ((v << s) & wmask(W)));
+            if (s + bw > W) st(W, out, (size_t)(w + 1) * L + i, ld(W, out, (size_t)(w + 1) * L + i) | (v >> (W - s)));
+        }
+    }
+}
+static void unpack_impl(unsigned W, const void* in, void* out, unsigned bw, uint64_t base) {
+    const unsigned L = 1024 / W;
+    for (unsigned i = 0; i < L; ++i) {
+        for (unsigned j = 0; j < W; ++j) {
+            uint64_t v = 0;
+            if (bw) {
+                unsigned bit = j * bw, w = bit / W, s = bit % W;
+                v            = ld(W, in, (size_t)w * L + i) >> s;
+                if (s + bw > W) v |= ld(W, in, (size_t)(w + 1) * L + i) << (W - s);
+                v &= bmask(bw);
+            }
+            st(W, out, (size_t)j * L + i, (v + base) & wmask(W));
+        }
+    }
+}
+void orc_pack(unsigned W, const void* in, void* out, unsigned bw) { pack_impl(W, in, out, bw, 0); }
+void orc_unpack(unsigned W, const void* in, void* out, unsigned bw) { unpack_impl(W, in, out, bw, 0); }
+void orc_ffor(unsigned W, const void* in, void* out, unsigned bw, const void* base) {
+    pack_impl(W, in, out, bw, ld(W, base, 0));
+}
+void orc_unffor(unsigned W, const void* in, void* out, unsigned bw, const void* base) {
+    unpack_impl(W, in, out, bw, ld(W, base, 0));
+}
+
+void orc_ffor_stream(unsigned W, const void* in, void* out, uint64_t n_vec, unsigned bw, const void* bases) {
+    for (uint64_t v = 0; v < n_vec; ++v)
+        orc_ffor(W, (const uint8_t*)in + v * 1024 * (W / 8), (uint8_t*)out + v * 128 * bw, bw,
+                 (const uint8_t*)bases + v * (W / 8));
+}
+void orc_unffor_stream(unsigned W, const void* in, void* out, uint64_t n_vec, unsigned bw, const void* bases) {
+    for (uint64_t v = 0; v < n_vec; ++v)
+        orc_unffor(W, (const uint8_t*)in + v * 128 * bw, (uint8_t*)out + v * 1024 * (W / 8), bw,
+                   (const uint8_t*)bases + v * (W / 8));
+}
+
+/* ffor_analyze (src/cor/prm/ffor/ffor_analyze.cpp:78-102): base = min over the vector (unsigned), bw = bits
+ * needed for max - min. The reference computes ceil(log2(double(delta + 1))), which agrees with the exact bit
+ * length below while delta + 1 is small enough for double rounding not to hit an integer; above ~2^48 (and at
+ * delta = 2^64 - 1, where delta + 1 wraps to 0) it returns one bit too few and the vector decodes wrongly. The
+ * exact bit length is used here and in the shipped encoder. */
+uint8_t orc_ffor_analyze(unsigned W, const void* in, void* base_out) {
+    uint64_t mn = wmask(W), mx = 0;
+    for (unsigned i = 0; i < 1024; ++i) {
+        uint64_t v = ld(W, in, i);
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+    }
+    uint64_t delta = mx - mn;
+    st(W, base_out, 0, mn);
+    return (uint8_t)(delta ? 64 - __builtin_clzll(delta) : 0);
+}
+/* The reference's floating-point formula, kept for tests that show where it and the exact form agree. */
+uint8_t orc_ffor_bw_reference_formula(uint64_t delta) { return (uint8_t)ceil(log2((double)(delta + 1))); }
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* Delta: rsum / unrsum (gen/rsum_src.cpp:6-411, gen/unrsum_src.cpp:6-521, src/cor/prm/rsum/rsum_compress.cpp). */
+/* Lane i (L = 1024/W lanes) visits transposed positions i + 128n + L*ORD[m] for step s = 8m + n, ORD being the  */
+/* bit-reversal order over W/8 items. out = base[i] + running sum; unrsum writes 0 at step 0.                    */
+/* ------------------------------------------------------------------------------------------------------------- */
+static const unsigned ORD1[1] = {0}, ORD2[2] = {0, 1}, ORD4[4] = {0, 2, 1, 3};
+static inline unsigned lane_pos(unsigned W, unsigned i, unsigned s) {
+    const unsigned  L   = 1024 / W;
+    const unsigned* ord = W == 8 ? ORD1 : W == 16 ? ORD2 : W == 32 ? ORD4 : ORDER8;
+    return i + 128 * (s % 8) + L * ord[s / 8];
+}
+void orc_rsum(unsigned W, const void* in, void* out, const void* base) {
+    const unsigned L = 1024 / W;
+    for (unsigned i = 0; i < L; ++i) {
+        uint64_t acc = ld(W, base, i);
+        for (unsigned s = 0; s < W; ++s) {
+            unsigned k = lane_pos(W, i, s);
+            acc        = (acc + ld(W, in, k)) & wmask(W);
+            st(W, out, k, acc);
+        }
+    }
+}
+void orc_unrsum(unsigned W, const void* in, void* out) {
+    const unsigned L = 1024 / W;
+    uint64_t       tmp[1024];
+    for (unsigned k = 0; k < 1024; ++k) tmp[k] = ld(W, in, k);
+    for (unsigned i = 0; i < L; ++i) {
+        st(W, out, lane_pos(W, i, 0), 0);
+        for (unsigned s = 1; s < W; ++s)
+            st(W, out, lane_pos(W, i, s), (tmp[lane_pos(W, i, s)] - tmp[lane_pos(W, i, s - 1)]) & wmask(W));
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* Byte buffer                                                                                                    */
+/* ------------------------------------------------------------------------------------------------------------- */
+typedef struct {
+    uint8_t* p;
+    uint64_t n, cap;
+} buf_t;
+static int buf_put(buf_t* b, const void* src, uint64_t n) {
+    if (b->n + n > b->cap) {
+        uint64_t nc = b->cap ? b->cap * 2 : 4096;
+        while (nc < b->n + n) nc *= 2;
+        uint8_t* q = (uint8_t*)realloc(b->p, nc);
+        if (!q) return -1;
+        b->p = q, b->cap = nc;
+    }
+    if (n) memcpy(b->p + b->n, src, n);
+    b->n += n;
+    return 0;
+}
+static void buf_free(buf_t* b) { free(b->p), b->p = NULL, b->n = b->cap = 0; }
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* Types and expressions                                                                                          */
+/* ------------------------------------------------------------------------------------------------------------- */
+enum { DT_DOUBLE = 1, DT_I8 = 2, DT_I16 = 3, DT_I32 = 4, DT_I64 = 5, DT_U8 = 6, DT_U16 = 7, DT_U32 = 8, DT_U64 = 9,
+       DT_STR = 10 };
+/* ExpT (include/fls/cor/exp/exp_type.hpp:7-28) */
+enum { EXP_UNCOMPRESSED = 2, EXP_FFOR = 4, EXP_DELTA = 5, EXP_RLE = 6, EXP_FSST = 7, EXP_DICT = 8, EXP_DICT_FRQ = 9,
+       EXP_DICT_VAL = 10 };
+
+static unsigned dtype_bits(uint8_t dt) {
+    switch (dt) {
+    case DT_I8:
+    case DT_U8: return 8;
+    case DT_I16:
+    case DT_U16: return 16;
+    case DT_I32:
+    case DT_U32: return 32;
+    case DT_I64:
+    case DT_U64:
+    case DT_DOUBLE: return 64;
+    default: return 0;
+    }
+}
+
+typedef struct {
+    uint8_t exp_t, var, arr_c, dict;
+} expr_t;
+/* Pool ids and (ExpT, var) per src/cor/exp_pool.cpp:23-65 (i64) and :66-100 (str); arr_c from the RPN tables. */
+static int expr_lookup(uint8_t dtype, uint8_t id, expr_t* e) {
+    if (dtype == DT_STR) {
+        if (id == 13) { *e = (expr_t){EXP_FSST, 1, 2, 1}; return 0; }
+        return -1;
+    }
+    switch (id) {
+    case 0: *e = (expr_t){EXP_UNCOMPRESSED, 1, 1, 0}; return 0;
+    case 1: *e = (expr_t){EXP_FFOR, 1, 3, 0}; return 0;
+    case 6: *e = (expr_t){EXP_RLE, 2, 5, 0}; return 0;
+    case 7: *e = (expr_t){EXP_RLE, 1, 2, 0}; return 0;
+    case 11: *e = (expr_t){EXP_DELTA, 1, 4, 0}; return 0;
+    case 16: *e = (expr_t){EXP_DICT, 1, 1, 1}; return 0;
+    case 17: *e = (expr_t){EXP_DICT, 2, 3, 1}; return 0;
+    case 22: *e = (expr_t){EXP_DICT, 7, 4, 1}; return 0;
+    case 28: *e = (expr_t){EXP_DICT_FRQ, 1, 1, 1}; return 0;
+    case 29: *e = (expr_t){EXP_DICT_FRQ, 2, 3, 1}; return 0;
+    default: return -1;
+    }
+}
+static int expr_from_page(uint8_t exp_t, uint8_t var, expr_t* e) {
+    switch (exp_t) {
+    case EXP_UNCOMPRESSED: *e = (expr_t){exp_t, var, 1, 0}; return var == 1 ? 0 : -1;
+    case EXP_FFOR: *e = (expr_t){exp_t, var, 3, 0}; return var == 1 ? 0 : -1;
+    case EXP_DELTA: *e = (expr_t){exp_t, var, 4, 0}; return var == 1 ? 0 : -1;
+    case EXP_RLE:
+        if (var == 1) { *e = (expr_t){exp_t, var, 2, 0}; return 0; }
+        if (var == 2) { *e = (expr_t){exp_t, var, 5, 0}; return 0; }
+        return -1;
+    case EXP_DICT:
+    case EXP_DICT_FRQ:
+    case EXP_DICT_VAL:
+        if (var == 1) { *e = (expr_t){exp_t, var, 1, 1}; return 0; }
+        if (var == 2) { *e = (expr_t){exp_t, var, 3, 1}; return 0; }
+        if (var == 7) { *e = (expr_t){exp_t, var, 4, 1}; return 0; }
+        return -1;
+    case EXP_FSST: *e = (expr_t){exp_t, var, 2, 1}; return var == 1 ? 0 : -1;
+    default: return -1;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* Page writer: TCompressor::FlushPage (src/cor/eng/compressor.cpp:68-86), Page::Absorb/OffsetToSink            */
+/* (src/cor/lyt/page/page.cpp:87-130). Layout: PageHdr{exp_t u8, var u8, 0 u16, vec_c u16, next u16} |          */
+/* arr_c u32 buffer offsets | (vec_c-1)*arr_c u32 per-vector entry points | buffers.                             */
+/* The last u16 of the header (written as 0 by the reference) is 1 when another page entry follows: the         */
+/* reference's u16 vec_c cannot hold 2^26-value columns (include/fls/cor/lyt/page/page_hdr.hpp:17).              */
+/* ------------------------------------------------------------------------------------------------------------- */
+#define MAX_ARR 6
+typedef struct {
+    buf_t    bufs[MAX_ARR];
+    buf_t    eps;
+    uint32_t ep[MAX_ARR];
+    unsigned arr_c, vec_c;
+} page_w;
+
+static void page_begin(page_w* pg, unsigned arr_c) {
+    memset(pg, 0, sizeof(*pg));
+    pg->arr_c = arr_c;
+}
+static int page_absorb(page_w* pg, const uint8_t* const* parts, const uint64_t* sizes) {
+    if (pg->vec_c >= 1 && buf_put(&pg->eps, pg->ep, 4ULL * pg->arr_c)) return -1;
+    for (unsigned i = 0; i < pg->arr_c; ++i) {
+        if (buf_put(&pg->bufs[i], parts[i], sizes[i])) return -1;
+        pg->ep[i] += (uint32_t)sizes[i];
+    }
+    pg->vec_c++;
+    return 0;
+}
+static uint64_t page_size(const page_w* pg) {
+    uint64_t s = 8 + 4ULL * pg->arr_c + pg->eps.n;
+    for (unsigned i = 0; i < pg->arr_c; ++i) s += pg->bufs[i].n;
+    return s;
+}
+static int page_flush(page_w* pg, uint8_t exp_t, uint8_t var, int more, buf_t* out) {
+    uint8_t hdr[8] = {exp_t, var, 0, 0, (uint8_t)(pg->vec_c & 0xff), (uint8_t)(pg->vec_c >> 8), (uint8_t)(more ? 1 : 0),
+                      0};
+    if (buf_put(out, hdr, 8)) return -1;
+    uint32_t off = (uint32_t)(8 + 4 * pg->arr_c + pg->eps.n);
+    for (unsigned i = 0; i < pg->arr_c; ++i) {
+        if (buf_put(out, &off, 4)) return -1;
+        off += (uint32_t)pg->bufs[i].n;
+    }
+    if (buf_put(out, pg->eps.p, pg->eps.n)) return -1;
+    for (unsigned i = 0; i < pg->arr_c; ++i)
+        if (buf_put(out, pg->bufs[i].p, pg->bufs[i].n)) return -1;
+    return 0;
+}
+static void page_free(page_w* pg) {
+    for (unsigned i = 0; i < MAX_ARR; ++i) buf_free(&pg->bufs[i]);
+    buf_free(&pg->eps);
+}
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* HZL dictionary payload: THzlCmp::compress_buf/finalize_buf (src/cor/eng/hzl_cmp.cpp:43-176),                  */
+/* THzlDeCmp::decompress_buf (src/cor/eng/hzl_de_cmp.cpp:28-77). Per full 1024-value chunk: rec_hdr{bw u8,      */
+/* 0 u8, exc_c u16, base[8]} + ffor(chunk) + exc_c u16 positions + exc_c values; the remainder is one rec_hdr    */
+/* with bw = 255 and the raw byte count in base, followed by the raw bytes. A chunk needing the full width ends  */
+/* the FFOR part (the reference returns early). This encoder takes base = min and no exceptions (the reference  */
+/* searches bases with up to 100 exceptions, src/cor/prm/ffor/ffor_analyze.cpp:41-75); the decoder below        */
+/* applies exceptions as the reference does.                                                                      */
+/* ------------------------------------------------------------------------------------------------------------- */
+int64_t orc_hzl_encode(unsigned W, const void* in, uint64_t n, uint8_t* out) {
+    const unsigned sz  = W / 8;
+    uint64_t       pos = 0, o = 0;
+    uint8_t        packed[8 * 1024];
+    while (n - pos >= 1024) {
+        const uint8_t* chunk = (const uint8_t*)in + pos * sz;
+        uint64_t       base  = 0;
+        uint8_t        bw    = orc_ffor_analyze(W, chunk, &base);
+        if (bw >= W) break;
+        uint8_t hdr[12] = {bw, 0, 0, 0};
+        memcpy(hdr + 4, &base, 8); /* little endian; unused high bytes are 0 */
+        if (W == 32) memset(hdr + 8, 0, 4);
+        if (out) memcpy(out + o, hdr, 12);
+        o += 12;
+        orc_ffor(W, chunk, packed, bw, &base);
+        if (out) memcpy(out + o, packed, 128ULL * bw);
+        o += 128ULL * bw;
+        pos += 1024;
+    }
+    uint64_t rest    = (n - pos) * sz;
+    uint8_t  hdr[12] = {255, 0, 0, 0};
+    memcpy(hdr + 4, &rest, 8);
+    if (out) {
+        memcpy(out + o, hdr, 12);
+        memcpy(out + o + 12, (const uint8_t*)in + pos * sz, rest);
+    }
+    o += 12 + rest;
+    return (int64_t)o;
+}
+
+int64_t orc_hzl_decode(unsigned W, const uint8_t* in, void* out, uint64_t* consumed) {
+    const unsigned sz = W / 8;
+    uint64_t       i = 0, n = 0;
+    for (;;) {
+        uint8_t  bw = in[i];
+        uint16_t exc_c;
+        memcpy(&exc_c, in + i + 2, 2);
+        uint64_t base = 0;
+        memcpy(&base, in + i + 4, sz);
+        i += 12;
+        if (bw == 255) {
+            uint64_t bytes = 0;
+            memcpy(&bytes, in + i - 8, 8);
+            memcpy((uint8_t*)out + n * sz, in + i, bytes);
+            i += bytes;
+            n += bytes / sz;
+            break;
+        }
+        if (bw > W) return -1;
+        orc_unffor(W, in + i, (uint8_t*)out + n * sz, bw, &base);
+        i += 128ULL * bw;
+        for (unsigned e = 0; e < exc_c; ++e) {
+            uint16_t p;
+            memcpy(&p, in + i + 2ULL * e, 2);
+            if (p >= 1024) return -1;
+            memcpy((uint8_t*)out + (n + p) * sz, in + i + 2ULL * exc_c + (uint64_t)e * sz, sz);
+        }
+        i += (uint64_t)exc_c * (2 + sz);
+        n += 1024;
+    }
+    if (consumed) *consumed = i;
+    return (int64_t)n;
+}
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* Dictionary: dict_fac::perfect_create (src/cor/lyt/dic/dic_fac.cpp:259-330). DICT assigns codes in first-      */
+/* occurrence order; DICT_FRQ by descending frequency (ties: ascending value, the reference's std::sort leaves    */
+/* ties unspecified). Page: dict_hdr_t{exp_t u8, 0, 0, version u8, entry_c u32} (include/fls/cor/lyt/dic/       */
+/* dic.hpp:32-42) + HZL payload of the values (src/cor/lyt/dic/dic.cpp:60-79). Values narrower than 32 bits are  */
+/* widened to 32 for the payload (the reference's HZL handles 32- and 64-bit types only).                        */
+/* ------------------------------------------------------------------------------------------------------------- */
+typedef struct {
+    uint64_t* keys;
+    uint32_t* vals;
+    uint64_t  cap;
+} hmap_t;
+static uint64_t hmix(uint64_t x) {
+    x ^= x >> 33, x *= 0xff51afd7ed558ccdULL, x ^= x >> 33, x *= 0xc4ceb9fe1a85ec53ULL, x ^= x >> 33;
+    return x;
+}
+static int hmap_init(hmap_t* m, uint64_t n) {
+    m->cap = 16;
+    while (m->cap < 2 * n) m->cap *= 2;
+    m->keys = (uint64_t*)malloc(m->cap * 8);
+    m->vals = (uint32_t*)malloc(m->cap * 4);
+    if (!m->keys || !m->vals) return -1;
+    memset(m->vals, 0xff, m->cap * 4);
+    return 0;
+}
+static uint32_t* hmap_slot(hmap_t* m, uint64_t k, int* found) {
+    uint64_t h = hmix(k) & (m->cap - 1);
+    while (m->vals[h] != 0xffffffffu) {
+        if (m->keys[h] == k) { *found = 1; return &m->vals[h]; }
+        h = (h + 1) & (m->cap - 1);
+    }
+    *found     = 0;
+    m->keys[h] = k;
+    return &m->vals[h];
+}
+static void hmap_free(hmap_t* m) { free(m->keys), free(m->vals); }
+
+typedef struct {
+    uint64_t  value;
+    uint64_t  count;
+    uint32_t  first;
+} dict_ent;
+static int cmp_frq(const void* a, const void* b) {
+    const dict_ent *x = (const dict_ent*)a, *y = (const dict_ent*)b;
+    if (x->count != y->count) return x->count > y->count ? -1 : 1;
+    return x->value < y->value ? -1 : x->value > y->value;
+}
+
+/* Builds codes (n) and the ordered dictionary values; returns entry count or <0. */
+static int64_t dict_build(unsigned W, uint8_t exp_t, const void* in, uint64_t n, uint32_t* codes, uint64_t** dvals) {
+    hmap_t m;
+    if (hmap_init(&m, n < (1u << 20) ? n : (1u << 20))) return -1;
+    dict_ent* ents = NULL;
+    uint64_t  ne = 0, ce = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t  v = ld(W, in, i);
+        int       f;
+        uint32_t* s = hmap_slot(&m, v, &f);
+        if (!f) {
+            if (ne == ce) {
+                ce   = ce ? ce * 2 : 1024;
+                ents = (dict_ent*)realloc(ents, ce * sizeof(dict_ent));
+            }
+            *s       = (uint32_t)ne;
+            ents[ne] = (dict_ent){v, 0, (uint32_t)ne};
+            ne++;
+            if (ne * 2 > m.cap) return -1; /* dictionary larger than the sized map: not a dictionary column */
+        }
+        ents[*s].count++;
+        codes[i] = *s;
+    }
+    uint32_t* remap = (uint32_t*)malloc(ne * 4 + 4);
+    if (exp_t == EXP_DICT_FRQ) {
+        qsort(ents, ne, sizeof(dict_ent), cmp_frq);
+    }
+    *dvals = (uint64_t*)malloc(ne * 8 + 8);
+    for (uint64_t c = 0; c < ne; ++c) remap[ents[c].first] = (uint32_t)c, (*dvals)[c] = ents[c].value;
+    for (uint64_t i = 0; i < n; ++i) codes[i] = remap[codes[i]];
+    free(remap), free(ents), hmap_free(&m);
+    return (int64_t)ne;
+}
+
+static int dict_page_write(unsigned W, uint64_t* dvals, uint64_t ne, buf_t* out) {
+    unsigned dw = W < 32 ? 32 : W;
+    uint8_t  hdr[8] = {EXP_DICT, 0, 0, 0};
+    uint32_t e32    = (uint32_t)ne;
+    memcpy(hdr + 4, &e32, 4);
+    if (buf_put(out, hdr, 8)) return -1;
+    uint8_t* tmp = (uint8_t*)malloc(ne * (dw / 8) + 8);
+    for (uint64_t i = 0; i < ne; ++i) st(dw, tmp, i, dvals[i]);
+    int64_t  sz = orc_hzl_encode(dw, tmp, ne, NULL);
+    uint8_t* enc = (uint8_t*)malloc((size_t)sz);
+    orc_hzl_encode(dw, tmp, ne, enc);
+    int rc = buf_put(out, enc, (uint64_t)sz);
+    free(tmp), free(enc);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* Per-vector encoders, following the compress RPNs: FFOR {TRS, ANALYZE, FFOR} (include/fls/cor/exp/ffor/        */
+/* ffor_exp.hpp:39-46), DELTA {TRS, RSUM, ANALYZE, FFOR} (include/fls/cor/exp/delta/delta_exp.hpp:39-46),         */
+/* RLE {MEM_CPY, TRS16, [RSUM16, ANALYZE16, FFOR16]} (include/fls/cor/exp/rle/rle_exp.hpp:37-74, split in         */
+/* src/cor/lyt/vec.cpp:50-71), DICT on u32 codes (include/fls/cor/exp/dict/dict_exp.hpp:44-60),                  */
+/* UNCOMPRESSED {TRS, MEM_CPY} (include/fls/cor/exp/uncompressed/uncompressed_exp.hpp:32-47).                    */
+/* ------------------------------------------------------------------------------------------------------------- */
+typedef struct {
+    uint8_t  b0[8 * 1024], b1[8 * 1024], b2[8 * 1024], b3[8 * 1024], b4[8 * 1024];
+    uint64_t s[MAX_ARR];
+    const uint8_t* parts[MAX_ARR];
+} vec_parts;
+
+/* FFOR chain on a (already transposed) vector of width W into parts[o..o+2]. */
+static void enc_ffor_parts(unsigned W, const void* t, vec_parts* vp, unsigned o, uint8_t* bwbuf, uint8_t* basebuf,
+                           uint8_t* packed) {
+    uint8_t bw = orc_ffor_analyze(W, t, basebuf);
+    bwbuf[0]   = bw;
+    orc_ffor(W, t, packed, bw, basebuf);
+    vp->parts[o] = bwbuf, vp->s[o] = 1;
+    vp->parts[o + 1] = basebuf, vp->s[o + 1] = W / 8;
+    vp->parts[o + 2] = packed, vp->s[o + 2] = 128ULL * bw;
+}
+
+static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint32_t* codes, vec_parts* vp) {
+    static __thread uint8_t t[8 * 1024], d[8 * 1024], bwb[8], baseb[8];
+    const unsigned          sz = W / 8;
+    switch (e->exp_t) {
+    case EXP_UNCOMPRESSED:
+        orc_transpose(W, vals, vp->b0);
+        vp->parts[0] = vp->b0, vp->s[0] = 1024ULL * sz;
+        break;
+    case EXP_FFOR:
+        orc_transpose(W, vals, t);
+        enc_ffor_parts(W, t, vp, 0, bwb, baseb, vp->b2);
+        break;
+    case EXP_DELTA:
+        orc_transpose(W, vals, t);
+        memcpy(vp->b0, t, 128); /* lane bases (src/cor/prm/rsum/rsum_compress.cpp:15) */
+        vp->parts[0] = vp->b0, vp->s[0] = 128;
+        orc_unrsum(W, t, d);
+        enc_ffor_parts(W, d, vp, 1, bwb, baseb, vp->b3);
+        break;
+    case EXP_RLE: {
+        uint16_t pos[1024], posT[1024], dl[1024];
+        uint64_t r = 0;
+        for (unsigned i = 0; i < 1024; ++i) {
+            uint64_t v = ld(W, vals, i);
+            if (i == 0 || v != ld(W, vals, i - 1)) {
+                if (i) r++;
+                st(W, vp->b0, r, v);
+            }
+            pos[i] = (uint16_t)r;
+        }
+        vp->parts[0] = vp->b0, vp->s[0] = (r + 1) * sz;
+        orc_transpose(16, pos, posT);
+        if (e->var == 1) {
+            memcpy(vp->b1, posT, 2048);
+            vp->parts[1] = vp->b1, vp->s[1] = 2048;
+        } else {
+            memcpy(vp->b1, posT, 128);
+            vp->parts[1] = vp->b1, vp->s[1] = 128;
+            orc_unrsum(16, posT, dl);
+            enc_ffor_parts(16, dl, vp, 2, bwb, baseb, vp->b4);
+        }
+        break;
+    }
+    default: { /* dictionary: codes are u32 */
+        orc_transpose(32, codes, t);
+        if (e->var == 1) {
+            memcpy(vp->b0, t, 4096);
+            vp->parts[0] = vp->b0, vp->s[0] = 4096;
+        } else if (e->var == 2) {
+            enc_ffor_parts(32, t, vp, 0, bwb, baseb, vp->b2);
+        } else {
+            memcpy(vp->b0, t, 128);
+            vp->parts[0] = vp->b0, vp->s[0] = 128;
+            orc_unrsum(32, t, d);
+            enc_ffor_parts(32, d, vp, 1, bwb, baseb, vp->b3);
+        }
+        break;
+    }
+    }
+}
+
+#define DATA_PG_SZ (8ULL * 64 * 256 * 1024) /* include/fls/cfg/cfg.hpp:82 */
+#define MAX_VEC_PER_PAGE 65535u
+
+static int64_t encode_impl(uint8_t dtype, uint8_t expr_id, const void* in, uint64_t n, buf_t* out) {
+    expr_t   e;
+    unsigned W = dtype_bits(dtype);
+    if (!W || expr_lookup(dtype, expr_id, &e)) return -1;
+    if (dtype == DT_DOUBLE && e.exp_t != EXP_UNCOMPRESSED) return -1;
+    if (n % 1024) return -1;
+    uint64_t  n_vec = n / 1024;
+    uint32_t* codes = NULL;
+    uint64_t* dvals = NULL;
+    int64_t   ne    = 0;
+    if (e.dict) {
+        codes = (uint32_t*)malloc(n * 4 + 4);
+        ne    = dict_build(W, e.exp_t, in, n, codes, &dvals);
+        if (ne < 0) { free(codes); return -1; }
+    }
+    uint64_t   page_c = n_vec ? (n_vec + MAX_VEC_PER_PAGE - 1) / MAX_VEC_PER_PAGE : 1;
+    uint64_t   per_pg = n_vec ? (n_vec + page_c - 1) / page_c : 0;
+    vec_parts* vp     = (vec_parts*)malloc(sizeof(vec_parts));
+    uint64_t   v      = 0;
+    int        rc     = 0;
+    while (!rc) {
+        page_w pg;
+        page_begin(&pg, e.arr_c);
+        uint64_t stop = v + per_pg;
+        if (stop > n_vec) stop = n_vec;
+        while (v < stop) {
+            enc_vector(W, &e, (const uint8_t*)in + v * 1024 * (W / 8), codes ? codes + v * 1024 : NULL, vp);
+            uint64_t add = 4ULL * e.arr_c;
+            for (unsigned i = 0; i < e.arr_c; ++i) add += vp->s[i];
+            if (pg.vec_c && page_size(&pg) + add >= DATA_PG_SZ) break; /* TryFlushVec limit, compressor.cpp:47 */
+            if (page_absorb(&pg, vp->parts, vp->s)) { rc = -1; break; }
+            ++v;
+        }
+        int more = v < n_vec;
+        if (!rc && e.dict) rc = dict_page_write(W, dvals, (uint64_t)ne, out);
+        if (!rc) rc = page_flush(&pg, e.exp_t, e.var, more, out);
+        page_free(&pg);
+        if (!more) break;
+    }
+    free(vp), free(codes), free(dvals);
+    return rc ? -1 : (int64_t)out->n;
+}
+
+int64_t orc_encode_column(uint8_t dtype, uint8_t expr_id, const void* in, uint64_t n, uint8_t* out, uint64_t cap) {
+    buf_t   b  = {0};
+    int64_t sz = encode_impl(dtype, expr_id, in, n, &b);
+    if (sz >= 0 && (uint64_t)sz > cap) sz = -2;
+    if (sz >= 0) memcpy(out, b.p, (size_t)sz);
+    buf_free(&b);
+    return sz;
+}
+int64_t orc_encoded_size(uint8_t dtype, uint8_t expr_id, const void* in, uint64_t n) {
+    buf_t   b  = {0};
+    int64_t sz = encode_impl(dtype, expr_id, in, n, &b);
+    buf_free(&b);
+    return sz;
+}
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* Page reader: Page::InitDecompress (src/cor/lyt/page/page.cpp:63-85), off_arr (offset.cpp:11-18),              */
+/* ep_arr::load_and_next (entry_point.cpp:21-41; vector 0's entry points are implicit zeros), PageParam          */
+/* (page_param.cpp:7-12). Decode steps per the decompress RPNs, run in reverse (src/cor/eng/executor.cpp:44-53), */
+/* then Materializer untranspose (src/encoder/materializer.cpp:13-21).                                           */
+/* ------------------------------------------------------------------------------------------------------------- */
+typedef struct {
+    const uint8_t* p;
+    uint8_t        exp_t, var, more;
+    unsigned       vec_c, arr_c;
+    uint32_t       off[MAX_ARR];
+    const uint8_t* eps;
+} page_r;
+
+static int page_open(const uint8_t* p, uint64_t avail, page_r* pg, expr_t* e) {
+    if (avail < 8) return -1;
+    pg->p     = p;
+    pg->exp_t = p[0], pg->var = p[1];
+    pg->vec_c = (unsigned)p[4] | ((unsigned)p[5] << 8);
+    pg->more  = p[6];
+    if (expr_from_page(pg->exp_t, pg->var, e)) return -1;
+    pg->arr_c = e->arr_c;
+    if (avail < 8 + 4ULL * pg->arr_c) return -1;
+    memcpy(pg->off, p + 8, 4ULL * pg->arr_c);
+    pg->eps = p + 8 + 4 * pg->arr_c;
+    for (unsigned i = 0; i < pg->arr_c; ++i)
+        if (pg->off[i] > avail) return -1;
+    return 0;
+}
+static inline const uint8_t* vbuf(const page_r* pg, unsigned v, unsigned i) {
+    uint32_t ep = 0;
+    if (v) memcpy(&ep, pg->eps + 4ULL * ((uint64_t)(v - 1) * pg->arr_c + i), 4);
+    return pg->p + pg->off[i] + ep;
+}
+
+static int unffor_parts(unsigned W, const page_r* pg, unsigned v, unsigned o, void* out) {
+    uint8_t  bw   = *vbuf(pg, v, o);
+    uint64_t base = 0;
+    if (bw > W) return -1;
+    memcpy(&base, vbuf(pg, v, o + 1), W / 8);
+    orc_unffor(W, vbuf(pg, v, o + 2), out, bw, &base);
+    return 0;
+}
+
+/* Byte size of the last vector's slice of the last buffer (needed to find the end of a page). */
+static uint64_t page_end(unsigned W, const page_r* pg, const expr_t* e) {
+    unsigned       v    = pg->vec_c - 1, last = pg->arr_c - 1;
+    const uint8_t* q    = vbuf(pg, v, last);
+    uint64_t       tail = 0;
+    switch (e->exp_t) {
+    case EXP_UNCOMPRESSED: tail = 1024ULL * (W / 8); break;
+    case EXP_FFOR:
+    case EXP_DELTA: tail = 128ULL * *vbuf(pg, v, last - 2); break;
+    case EXP_RLE: tail = e->var == 1 ? 2048 : 128ULL * *vbuf(pg, v, last - 2); break;
+    case EXP_FSST: tail = 4096; break;
+    default: tail = e->var == 1 ? 4096 : 128ULL * *vbuf(pg, v, last - 2); break;
+    }
+    return (uint64_t)(q - pg->p) + tail;
+}
+
+static int decode_vector(unsigned W, const page_r* pg, const expr_t* e, unsigned v, const uint64_t* dict,
+                         uint64_t dict_n, void* out) {
+    uint8_t        t[8 * 1024], d[8 * 1024];
+    const unsigned sz = W / 8;
+    switch (e->exp_t) {
+    case EXP_UNCOMPRESSED: memcpy(t, vbuf(pg, v, 0), 1024ULL * sz); break;
+    case EXP_FFOR:
+        if (unffor_parts(W, pg, v, 0, t)) return -1;
+        break;
+    case EXP_DELTA:
+        if (unffor_parts(W, pg, v, 1, d)) return -1;
+        orc_rsum(W, d, t, vbuf(pg, v, 0));
+        break;
+    case EXP_RLE: {
+        uint16_t posT[1024], dl[1024];
+        if (e->var == 1) memcpy(posT, vbuf(pg, v, 1), 2048);
+        else {
+            if (unffor_parts(16, pg, v, 2, dl)) return -1;
+            orc_rsum(16, dl, posT, vbuf(pg, v, 1));
+        }
+        const uint8_t* runs = vbuf(pg, v, 0);
+        for (unsigned k = 0; k < 1024; ++k) memcpy(t + (size_t)k * sz, runs + (size_t)posT[k] * sz, sz); /* vec.cpp:318-332 */
+        break;
+    }
+    default: { /* dictionary codes */
+        uint32_t c[1024], cd[1024];
+        if (e->var == 1) memcpy(c, vbuf(pg, v, 0), 4096);
+        else if (e->var == 2) {
+            if (unffor_parts(32, pg, v, 0, c)) return -1;
+        } else {
+            if (unffor_parts(32, pg, v, 1, cd)) return -1;
+            orc_rsum(32, cd, c, vbuf(pg, v, 0));
+        }
+        for (unsigned k = 0; k < 1024; ++k) {
+            if (c[k] >= dict_n) return -1;
+            st(W, t, k, dict[c[k]]);
+        }
+        break;
+    }
+    }
+    orc_untranspose(W, t, out);
+    return 0;
+}
+
+/* Reads an optional dictionary page at p; returns bytes consumed (0 if none), <0 on error. */
+static int64_t dict_page_read(unsigned W, const uint8_t* p, uint64_t avail, uint64_t** dvals, uint64_t* dn) {
+    unsigned dw = W < 32 ? 32 : W;
+    if (avail < 8) return -1;
+    uint32_t ne;
+    memcpy(&ne, p + 4, 4);
+    uint8_t* tmp = (uint8_t*)malloc((uint64_t)ne * (dw / 8) + 1024 * 8 + 8);
+    uint64_t consumed;
+    int64_t  n = orc_hzl_decode(dw, p + 8, tmp, &consumed);
+    if (n != (int64_t)ne) { free(tmp); return -1; }
+    *dvals = (uint64_t*)malloc((uint64_t)ne * 8 + 8);
+    for (uint64_t i = 0; i < ne; ++i) (*dvals)[i] = ld(dw, tmp, i);
+    *dn = ne;
+    free(tmp);
+    return (int64_t)(8 + consumed);
+}
+
+int64_t orc_decode_column(uint8_t dtype, const uint8_t* in, uint64_t in_size, void* out, uint64_t cap) {
+    unsigned W = dtype_bits(dtype);
+    if (!W) return -1;
+    uint64_t pos = 0, n = 0;
+    for (;;) {
+        uint64_t* dict = NULL;
+        uint64_t  dn   = 0;
+        if (pos >= in_size) return -1;
+        uint8_t t0 = in[pos];
+        if (t0 == EXP_DICT || t0 == EXP_DICT_FRQ || t0 == EXP_DICT_VAL) {
+            int64_t c = dict_page_read(W, in + pos, in_size - pos, &dict, &dn);
+            if (c < 0) return -1;
+            pos += (uint64_t)c;
+        }
+        page_r pg;
+        expr_t e;
+        if (page_open(in + pos, in_size - pos, &pg, &e)) { free(dict); return -1; }
+        if ((n + 1024ULL * pg.vec_c) * (W / 8) > cap) { free(dict); return -2; }
+        for (unsigned v = 0; v < pg.vec_c; ++v)
+            if (decode_vector(W, &pg, &e, v, dict, dn, (uint8_t*)out + (n + 1024ULL * v) * (W / 8))) {
+                free(dict);
+                return -1;
+            }
+        n += 1024ULL * pg.vec_c;
+        free(dict);
+        if (!pg.more) break;
+        pos += pg.vec_c ? page_end(W, &pg, &e) : 8 + 4ULL * pg.arr_c;
+    }
+    return (int64_t)n;
+}
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* FSST. Serialized table = fsst_export (src/cor/prm/fsst/libfsst.cpp:582-615): u64 version                      */
+/* (20190218 << 32 | suffixLim << 24 | terminator << 16 | nSymbols << 8 | 1), zero_terminated u8, lenHisto[8]    */
+/* (count of symbols of length 1..8), then symbol bytes in code order. fsst_import (:617-652) assigns codes to   */
+/* lengths 2,3,..,8 first and length-1 symbols last; unused codes decode to "corrupt". Decode is the textbook     */
+/* loop of include/fls/cor/prm/fsst/fsst.h:152-258: code < 255 emits symbol[code], 255 escapes the next byte.   */
+/* ------------------------------------------------------------------------------------------------------------- */
+#define FSST_VERSION_NUM 20190218ULL
+#define FSST_CORRUPT_SYM 32774747032022883ULL
+
+unsigned orc_fsst_import(orc_fsst_decoder* d, const uint8_t* buf) {
+    uint64_t version;
+    unsigned code, pos = 17;
+    uint8_t  lh[8];
+    memcpy(&version, buf, 8);
+    if ((version >> 32) != FSST_VERSION_NUM) return 0;
+    d->version         = version;
+    d->zero_terminated = buf[8] & 1;
+    memcpy(lh, buf + 9, 8);
+    d->len[0]    = 1;
+    d->symbol[0] = 0;
+    code         = d->zero_terminated;
+    if (d->zero_terminated) lh[0]--;
+    for (unsigned l = 1; l <= 8; l++)
+        for (unsigned i = 0; i < lh[l & 7]; i++, code++) {
+            if (code >= 255) return 0;
+            d->len[code]    = (uint8_t)((l & 7) + 1);
+            d->symbol[code] = 0;
+            for (unsigned j = 0; j < d->len[code]; j++) ((uint8_t*)&d->symbol[code])[j] = buf[pos++];
+        }
+    while (code < 255) d->symbol[code] = FSST_CORRUPT_SYM, d->len[code++] = 8;
+    return pos;
+}
+
+size_t orc_fsst_decompress(const orc_fsst_decoder* d, size_t len_in, const uint8_t* in, size_t size, uint8_t* out) {
+    size_t pi = 0, po = 0;
+    while (pi < len_in) {
+        unsigned code = in[pi++];
+        if (code < 255) {
+            for (unsigned j = 0; j < d->len[code]; ++j, ++po)
+                if (po < size) out[po] = ((const uint8_t*)&d->symbol[code])[j];
+        } else {
+            if (po < size && pi < len_in) out[po] = in[pi];
+            pi++, po++;
+        }
+    }
+    return po;
+}
+
+/* Symbol-table builder: the iterative candidate-gain construction of Boncz, Neumann, Leis, "FSST: Fast Random  */
+/* Access String Compression", PVLDB 13(11) 2020, Sec. 4 (five rounds; candidates are symbols and concatenations */
+/* of adjacent symbols; gain = count * length), simplified to a hash table of byte strings. Any table in the    */
+/* export format decodes with the reference's decoder; this builder only has to produce a valid, useful table.   */
+typedef struct {
+    uint64_t val;
+    uint8_t  len;
+} fsym;
+typedef struct {
+    fsym     s[256];
+    unsigned n;
+} ftable;
+
+/* Lookup index over a table: codes bucketed by their first two bytes (longest first), plus one-byte codes. */
+typedef struct {
+    int16_t  one[256];
+    uint32_t start[65537];
+    uint8_t  codes[256];
+} fidx;
+static void fidx_build(const ftable* t, fidx* x) {
+    uint32_t cnt[65536] = {0};
+    for (unsigned b = 0; b < 256; ++b) x->one[b] = -1;
+    for (unsigned c = 0; c < t->n; ++c) {
+        if (t->s[c].len == 1) x->one[t->s[c].val & 0xff] = (int16_t)c;
+        else cnt[t->s[c].val & 0xffff]++;
+    }
+    x->start[0] = 0;
+    for (unsigned k = 0; k < 65536; ++k) x->start[k + 1] = x->start[k] + cnt[k];
+    uint32_t fill[65536];
+    memcpy(fill, x->start, sizeof(fill));
+    for (unsigned l = 8; l >= 2; --l)
+        for (unsigned c = 0; c < t->n; ++c)
+            if (t->s[c].len == l) x->codes[fill[t->s[c].val & 0xffff]++] = (uint8_t)c;
+}
+static int fidx_match(const ftable* t, const fidx* x, const uint8_t* p, size_t rem) {
+    if (rem >= 2) {
+        unsigned k = p[0] | ((unsigned)p[1] << 8);
+        for (uint32_t q = x->start[k]; q < x->start[k + 1]; ++q) {
+            unsigned c = x->codes[q], l = t->s[c].len;
+            if (l <= rem && memcmp(&t->s[c].val, p, l) == 0) return (int)c;
+        }
+    }
+    return x->one[p[0]];
+}
+
+typedef struct {
+    uint64_t val;
+    uint8_t  len;
+    uint64_t cnt;
+} cand;
+static int cand_cmp(const void* a, const void* b) {
+    const cand *x = (const cand*)a, *y = (const cand*)b;
+    uint64_t    gx = x->cnt * x->len, gy = y->cnt * y->len;
+    if (gx != gy) return gx > gy ? -1 : 1;
+    if (x->len != y->len) return x->len > y->len ? -1 : 1;
+    return x->val < y->val ? -1 : x->val > y->val;
+}
+static void cand_add(cand* tab, uint64_t cap, uint64_t v, uint8_t l) {
+    uint64_t h = hmix(v * 31 + l) & (cap - 1);
+    for (;;) {
+        if (tab[h].len == 0) { tab[h] = (cand){v, l, 1}; return; }
+        if (tab[h].len == l && tab[h].val == v) { tab[h].cnt++; return; }
+        h = (h + 1) & (cap - 1);
+    }
+}
+
+static void ft_sort_export(ftable* t, uint8_t* buf, unsigned* out_len) {
+    /* order: lengths 2..8 then 1 (fsst_import code assignment) */
+    fsym     tmp[256];
+    unsigned k = 0, lh[8] = {0};
+    for (unsigned L = 2; L <= 9; ++L) {
+        unsigned want = L == 9 ? 1 : L;
+        for (unsigned c = 0; c < t->n; ++c)
+            if (t->s[c].len == want) tmp[k++] = t->s[c];
+    }
+    memcpy(t->s, tmp, k * sizeof(fsym));
+    t->n = k;
+    for (unsigned c = 0; c < k; ++c) lh[t->s[c].len - 1]++;
+    uint64_t version = (FSST_VERSION_NUM << 32) | ((uint64_t)lh[1] << 24) | ((uint64_t)k << 8) | 1;
+    memcpy(buf, &version, 8);
+    buf[8]       = 0;
+    unsigned pos = 17;
+    for (unsigned i = 0; i < 8; ++i) buf[9 + i] = (uint8_t)lh[i];
+    for (unsigned c = 0; c < k; ++c)
+        for (unsigned j = 0; j < t->s[c].len; ++j) buf[pos++] = ((uint8_t*)&t->s[c].val)[j];
+    *out_len = pos;
+}
+
+unsigned orc_fsst_build(const uint32_t* offsets, const uint8_t* bytes, uint64_t n, uint8_t* buf) {
+    ftable   t      = {.n = 0};
+    uint64_t total  = offsets[n] - offsets[0];
+    uint64_t stride = total > (1 << 15) ? total / (1 << 15) + 1 : 1; /* ~32 KB sample */
+    const uint64_t cap = 1 << 16;
+    cand*          tab = (cand*)malloc(cap * sizeof(cand));
+    fidx* x = (fidx*)malloc(sizeof(fidx));
+    for (int round = 0; round < 5; ++round) {
+        fidx_build(&t, x);
+        memset(tab, 0, cap * sizeof(cand));
+        for (uint64_t si = 0; si < n; si += stride) {
+            const uint8_t* p   = bytes + offsets[si];
+            size_t         rem = offsets[si + 1] - offsets[si], i = 0;
+            int            prev = -1;
+            uint64_t       pval = 0;
+            uint8_t        plen = 0;
+            while (i < rem) {
+                int      c   = fidx_match(&t, x, p + i, rem - i);
+                uint8_t  l   = c >= 0 ? t.s[c].len : 1;
+                uint64_t val = 0;
+                memcpy(&val, p + i, l);
+                cand_add(tab, cap, val, l);
+                if (l != 1) { uint64_t b = p[i]; cand_add(tab, cap, b, 1); }
+                if (prev >= 0 && plen + l <= 8) cand_add(tab, cap, pval | (val << (8 * plen)), (uint8_t)(plen + l));
+                prev = 1, pval = val, plen = l;
+                i += l;
+            }
+        }
+        uint64_t nc = 0;
+        for (uint64_t h = 0; h < cap; ++h)
+            if (tab[h].len) tab[nc++] = tab[h];
+        qsort(tab, nc, sizeof(cand), cand_cmp);
+        t.n = 0;
+        for (uint64_t c = 0; c < nc && t.n < 255; ++c)
+            if (tab[c].len > 1 || tab[c].cnt * 2 > 0) t.s[t.n++] = (fsym){tab[c].val, tab[c].len};
+    }
+    free(tab), free(x);
+    unsigned len;
+    ft_sort_export(&t, buf, &len);
+    return len;
+}
+
+typedef struct {
+    ftable t;
+    fidx   x;
+} fenc;
+static int fenc_init(fenc* f, const uint8_t* table) {
+    orc_fsst_decoder d;
+    if (!orc_fsst_import(&d, table)) return -1;
+    unsigned nsym = 0;
+    for (unsigned i = 0; i < 8; ++i) nsym += table[9 + i];
+    f->t.n = 0;
+    for (unsigned c = 0; c < nsym && c < 255; ++c) f->t.s[c] = (fsym){d.symbol[c], d.len[c]}, f->t.n++;
+    fidx_build(&f->t, &f->x);
+    return 0;
+}
+static size_t fenc_one(const fenc* f, const uint8_t* in, size_t len, uint8_t* out) {
+    size_t i = 0, o = 0;
+    while (i < len) {
+        int c = fidx_match(&f->t, &f->x, in + i, len - i);
+        if (c < 0) out[o++] = 255, out[o++] = in[i++];
+        else out[o++] = (uint8_t)c, i += f->t.s[c].len;
+    }
+    return o;
+}
+size_t orc_fsst_compress_one(const uint8_t* table, const uint8_t* in, size_t len, uint8_t* out) {
+    fenc* f = (fenc*)malloc(sizeof(fenc));
+    size_t o = 0;
+    if (!fenc_init(f, table)) o = fenc_one(f, in, len, out);
+    free(f);
+    return o;
+}
+
+/* FSST column: vector split of src/cor/lyt/vec.cpp:160-207 (bytes + cumulative end offsets), page RPN           */
+/* {MEM_CPY, TRS32, MEM_CPY} (include/fls/cor/exp/fsst/fsst_exp.hpp:35-52); the symbol table is the dictionary   */
+/* page dict_hdr_t{FSST,0,0,0,0} + fsst_export (src/encoder/col_encoder.cpp:162-172).                           */
+int64_t orc_encode_str_column(uint8_t expr_id, const uint32_t* offsets, const uint8_t* bytes, uint64_t n,
+                              uint8_t* out, uint64_t cap) {
+    if (expr_id != 13 || n % 1024) return -1;
+    uint8_t  table[2100];
+    unsigned tl    = orc_fsst_build(offsets, bytes, n, table);
+    uint64_t n_vec = n / 1024, page_c = n_vec ? (n_vec + MAX_VEC_PER_PAGE - 1) / MAX_VEC_PER_PAGE : 1;
+    uint64_t per_pg = n_vec ? (n_vec + page_c - 1) / page_c : 0, v = 0;
+    buf_t    ob = {0};
+    uint8_t* cbytes = (uint8_t*)malloc(1 << 22);
+    fenc*    fe     = (fenc*)malloc(sizeof(fenc));
+    int      rc     = fenc_init(fe, table);
+    for (;;) {
+        page_w pg;
+        page_begin(&pg, 2);
+        uint64_t stop = v + per_pg;
+        if (stop > n_vec) stop = n_vec;
+        for (; v < stop && !rc; ++v) {
+            uint32_t ends[1024], endsT[1024], c = 0;
+            for (unsigned i = 0; i < 1024; ++i) {
+                uint64_t s = v * 1024 + i;
+                c += (uint32_t)fenc_one(fe, bytes + offsets[s], offsets[s + 1] - offsets[s], cbytes + c);
+                ends[i] = c;
+            }
+            orc_transpose(32, ends, endsT);
+            const uint8_t* parts[2] = {cbytes, (const uint8_t*)endsT};
+            uint64_t       sizes[2] = {c, 4096};
+            rc                      = page_absorb(&pg, parts, sizes);
+        }
+        int     more   = v < n_vec;
+        uint8_t dh[8]  = {EXP_FSST, 0, 0, 0, 0, 0, 0, 0};
+        if (!rc) rc = buf_put(&ob, dh, 8);
+        if (!rc) rc = buf_put(&ob, table, tl);
+        if (!rc) rc = page_flush(&pg, EXP_FSST, 1, more, &ob);
+        page_free(&pg);
+        if (rc || !more) break;
+    }
+    free(cbytes), free(fe);
+    int64_t sz = rc ? -1 : (int64_t)ob.n;
+    if (sz >= 0 && (uint64_t)sz > cap) sz = -2;
+    if (sz >= 0) memcpy(out, ob.p, (size_t)sz);
+    buf_free(&ob);
+    return sz;
+}
+
+/* Mirrors the per-vector decode of include/fls/encoder/assert_eq.hpp:166-210: untranspose the end offsets, then */
+/* fsst_decompress each string. */
+int64_t orc_decode_str_column(const uint8_t* in, uint64_t in_size, uint32_t* out_off, uint8_t* out_bytes,
+                              uint64_t out_cap, uint64_t* out_n_bytes) {
+    uint64_t pos = 0, n = 0, ob = 0;
+    out_off[0]   = 0;
+    for (;;) {
+        if (pos + 8 > in_size || in[pos] != EXP_FSST) return -1;
+        orc_fsst_decoder d;
+        unsigned         tl = orc_fsst_import(&d, in + pos + 8);
+        if (!tl) return -1;
+        pos += 8 + tl;
+        page_r pg;
+        expr_t e;
+        if (page_open(in + pos, in_size - pos, &pg, &e) || e.exp_t != EXP_FSST) return -1;
+        for (unsigned v = 0; v < pg.vec_c; ++v) {
+            uint32_t       endsT[1024], ends[1024];
+            const uint8_t* cb = vbuf(&pg, v, 0);
+            memcpy(endsT, vbuf(&pg, v, 1), 4096);
+            orc_untranspose(32, endsT, ends);
+            uint32_t start = 0;
+            for (unsigned i = 0; i < 1024; ++i) {
+                size_t got = orc_fsst_decompress(&d, ends[i] - start, cb + start, out_cap - ob, out_bytes + ob);
+                if (got > out_cap - ob) return -2;
+                ob += got;
+                start            = ends[i];
+                out_off[++n] = (uint32_t)ob;
+            }
+        }
+        if (!pg.more) break;
+        pos += page_end(8, &pg, &e);
+    }
+    if (out_n_bytes) *out_n_bytes = ob;
+    return (int64_t)n;
+}
+
+/* ------------------------------------------------------------------------------------------------------------- */
+/* CPU timing helpers (bench.py cpu_baseline leg).                                                               */
+/* ------------------------------------------------------------------------------------------------------------- */
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+typedef struct {
+    unsigned       W, bw;
+    const uint8_t* in;
+    uint8_t*       out;
+    const uint8_t* bases;
+    uint64_t       v0, v1;
+    int            n_iter;
+} ufs_job;
+static void* ufs_run(void* a) {
+    ufs_job* j = (ufs_job*)a;
+    for (int it = 0; it < j->n_iter; ++it)
+        orc_unffor_stream(j->W, j->in + j->v0 * 128 * j->bw, j->out + j->v0 * 1024 * (j->W / 8), j->v1 - j->v0, j->bw,
+                          j->bases + j->v0 * (j->W / 8));
+    return NULL;
+}
+double orc_time_unffor_stream(unsigned W, const void* in, void* out, uint64_t n_vec, unsigned bw, const void* bases,
+                              int n_iter, int n_threads) {
+    if (n_threads < 1) n_threads = 1;
+    pthread_t* th   = (pthread_t*)malloc(sizeof(pthread_t) * n_threads);
+    ufs_job*   jobs = (ufs_job*)malloc(sizeof(ufs_job) * n_threads);
+    double     t0   = now_s();
+    for (int t = 0; t < n_threads; ++t) {
+        jobs[t] = (ufs_job){W, bw, (const uint8_t*)in, (uint8_t*)out, (const uint8_t*)bases, n_vec * t / n_threads,
+                            n_vec * (t + 1) / n_threads, n_iter};
+        pthread_create(&th[t], NULL, ufs_run, &jobs[t]);
+    }
+    for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
+    double dt = now_s() - t0;
+    free(th), free(jobs);
+    return dt / n_iter;
+}
+double orc_time_decode(uint8_t dtype, const uint8_t* in, uint64_t in_size, void* out, uint64_t cap, int n_iter,
+                       int n_threads) {
+    (void)n_threads;
+    double t0 = now_s();
+    for (int it = 0; it < n_iter; ++it) orc_decode_column(dtype, in, in_size, out, cap);
+    return (now_s() - t0) / n_iter;
+}

@@ -1,0 +1,112 @@
+"""Writes the golden fixtures in tests/golden/ from the reference's own test data and compiled code.
+
+Run in a container that has the reference tree (default /root/reference) and `make -C oracle ref` built:
+    python tests/golden/make_golden.py [--ref /root/reference]
+
+Fixtures written (data only: inputs and expected outputs):
+  rsum_kat.npz            known-answer arrays of the reference's rsum test
+                          (primitives/fls_generated/fallback/scalar_aav_uf1/fallback_scalar_aav_1024_uf1_rsum_helper.hpp:
+                          untransposed_arr_owW, transposed_arr_owW, delta_arr_owW, base_arr_owW)
+  unffor_helper.npz       the 124 seeded arrays of the reference's unffor test
+                          (fallback_scalar_aav_1024_uf1_unffor_helper.hpp) with the bytes the compiled reference
+                          ffor (base 10, as in ..._unffor_test.cpp:44-52) and pack produce for them
+  local_result_sizes.json encoded sizes per expression from test/local_result/col_result/generated/i64_pt1vec.csv
+                          and i64_pt16vec.csv
+  fsst_golden.npz         strings of the synthetic vocabulary, the table and compressed bytes the compiled
+                          reference fsst_create/fsst_compress produce, and the reference fsst_decompress output
+  config1_golden.json     config "FFOR u32 bw 7, 2^20 values, seed 42": SHA-256 of the reference-encoded packed
+                          stream and of the decoded values
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import hashlib
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle.py_oracle import Reference  # noqa: E402
+from fastlanes_amd.workloads import config1_values, vocab_strings  # noqa: E402
+
+GEN = "primitives/fls_generated/fallback/scalar_aav_uf1"
+W2DT = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+
+
+def parse_arrays(text: str):
+    """Yields (name, numpy array) for every `uintW_t name[N] = {...};` in a generated helper header."""
+    text = re.sub(r"//[^\n]*", "", text)
+    pat = re.compile(r"uint(8|16|32|64)_t\s+(\w+)\s*\[(\d+)\]\s*=\s*\{(.*?)\};", re.S)
+    for m in pat.finditer(text):
+        w, name, n, body = int(m.group(1)), m.group(2), int(m.group(3)), m.group(4)
+        vals = [int(v.rstrip("UL")) for v in re.findall(r"\d+UL?|\d+", body)]
+        assert len(vals) == n, (name, len(vals), n)
+        yield name, np.array(vals, dtype=np.uint64).astype(W2DT[w])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    a = ap.parse_args()
+    ref = Reference()
+
+    # rsum KATs
+    text = open(os.path.join(a.ref, GEN, "fallback_scalar_aav_1024_uf1_rsum_helper.hpp")).read()
+    kat = {name: arr for name, arr in parse_arrays(text)
+           if re.match(r"(untransposed|transposed|delta|base)_arr_ow(8|16|32|64)$", name)}
+    assert len(kat) == 16, sorted(kat)
+    np.savez_compressed(os.path.join(HERE, "rsum_kat.npz"), **kat)
+
+    # unffor helper arrays + compiled-reference ffor/pack bytes
+    text = open(os.path.join(a.ref, GEN, "fallback_scalar_aav_1024_uf1_unffor_helper.hpp")).read()
+    out = {}
+    for name, arr in parse_arrays(text):
+        m = re.match(r"rand_arr_(\d+)_b(\d+)_w(\d+)_arr$", name)
+        if not m:
+            continue
+        bw, w = int(m.group(2)), int(m.group(3))
+        out[f"in_b{bw}_w{w}"] = arr
+        out[f"ffor10_b{bw}_w{w}"] = ref.ffor(arr, bw, W2DT[w](10))
+        out[f"pack_b{bw}_w{w}"] = ref.pack(arr, bw)
+    assert len([k for k in out if k.startswith("in_")]) == 124
+    np.savez_compressed(os.path.join(HERE, "unffor_helper.npz"), **out)
+
+    # local_result sizes
+    sizes = {}
+    for fname in ("i64_pt1vec.csv", "i64_pt16vec.csv"):
+        rows = list(csv.DictReader(open(os.path.join(a.ref, "test/local_result/col_result/generated", fname))))
+        sizes[fname] = [{"id": int(r["ID"]), "exp": r["EXP_T/ENCODER_T"], "var": r["VAR_T/COMPRESSOR"],
+                         "size": int(r["SIZE"])} for r in rows]
+    json.dump(sizes, open(os.path.join(HERE, "local_result_sizes.json"), "w"), indent=1)
+
+    # FSST golden
+    offs, data = vocab_strings(4096, seed=7)
+    table, comp, coff = ref.fsst_encode(offs, data)
+    doffs, ddata = ref.fsst_decode(table, comp, coff, int(offs[-1]))
+    assert np.array_equal(doffs, offs) and np.array_equal(ddata, data), "reference FSST round trip"
+    np.savez_compressed(os.path.join(HERE, "fsst_golden.npz"), offsets=offs, data=data, table=table, comp=comp,
+                        comp_offsets=coff)
+
+    # config 1: FFOR u32 bw 7, 2^20 values, seed 42
+    x = config1_values()
+    n_vec = x.size // 1024
+    packed = np.concatenate([ref.ffor(x[v * 1024:(v + 1) * 1024], 7, np.uint32(1_000_000)) for v in range(n_vec)])
+    dec = np.concatenate([ref.unffor(packed[v * 224:(v + 1) * 224], 7, np.uint32(1_000_000), np.uint32)
+                          for v in range(n_vec)])
+    assert np.array_equal(dec, x)
+    json.dump({"n": int(x.size), "bw": 7, "base": 1_000_000,
+               "packed_sha256": hashlib.sha256(packed.tobytes()).hexdigest(),
+               "decoded_sha256": hashlib.sha256(dec.tobytes()).hexdigest()},
+              open(os.path.join(HERE, "config1_golden.json"), "w"), indent=1)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

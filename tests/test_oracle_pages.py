@@ -1,0 +1,162 @@
+"""The oracle's page writer/reader: size closed forms against the reference's recorded sizes, round trips,
+HZL dictionary payloads and FSST against the compiled reference."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fastlanes_amd import workloads as wl
+from tests.conftest import GOLDEN
+
+
+def _runs_vector(n_runs=133, seed=0):
+    """One vector of int64 with n_runs runs of distinct values (the shape implied by the recorded sizes)."""
+    g = np.random.default_rng(seed)
+    cuts = np.sort(g.choice(np.arange(1, 1024), n_runs - 1, replace=False))
+    lens = np.diff(np.concatenate([[0], cuts, [1024]]))
+    vals = g.choice(np.arange(10_000, 20_000), n_runs, replace=False).astype(np.int64)
+    return np.repeat(vals, lens)
+
+
+def _ffor16_vector(seed=1):
+    g = np.random.default_rng(seed)
+    x = g.integers(0, 1 << 16, 1024).astype(np.int64)
+    x[0], x[1] = 0, (1 << 16) - 1
+    return x
+
+
+def _delta17_vector(seed=2):
+    g = np.random.default_rng(seed)
+    d = g.integers(0, 1 << 17, 1024).astype(np.int64)
+    d[1] = (1 << 17) - 1
+    return np.cumsum(d).astype(np.int64)
+
+
+@pytest.fixture(scope="module")
+def recorded():
+    return json.load(open(os.path.join(GOLDEN, "local_result_sizes.json")))
+
+
+@pytest.mark.parametrize("n_vec", [1, 16])
+def test_page_sizes_match_recorded(oracle, recorded, n_vec):
+    rows = {r["id"]: r["size"] for r in recorded["i64_pt1vec.csv" if n_vec == 1 else "i64_pt16vec.csv"]}
+    runs = np.tile(_runs_vector(), n_vec)
+    cases = {0: runs, 7: runs, 6: runs, 1: np.tile(_ffor16_vector(), n_vec), 11: np.tile(_delta17_vector(), n_vec)}
+    if n_vec == 1:
+        cases.update({16: runs, 17: runs, 22: runs, 28: runs, 29: runs})
+    for eid, x in cases.items():
+        enc = oracle.encode_column(x, eid)
+        assert enc.size == rows[eid], (eid, enc.size, rows[eid])
+        np.testing.assert_array_equal(oracle.decode_column(enc, np.int64, x.size), x)
+
+
+DTYPES = [np.uint8, np.uint16, np.uint32, np.uint64, np.int8, np.int16, np.int32, np.int64]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("eid", [0, 1, 6, 7, 11, 16, 17, 22, 28, 29])
+def test_round_trip_every_expression(oracle, dtype, eid):
+    g = np.random.default_rng(eid)
+    info = np.iinfo(dtype)
+    n = 8 * 1024
+    if eid in (6, 7):
+        x = wl.geometric_runs(n, 8.0, seed=eid, value_bits=min(20, info.bits - 1)).astype(dtype)
+    elif eid >= 16:
+        keys = g.integers(info.min, info.max, 300, dtype=dtype, endpoint=True)
+        x = keys[g.integers(0, 300, n)]
+    else:
+        x = g.integers(info.min, info.max, n, dtype=dtype, endpoint=True)
+    enc = oracle.encode_column(x, eid)
+    np.testing.assert_array_equal(oracle.decode_column(enc, dtype, n), x)
+
+
+def test_multi_page_column(oracle):
+    # 2^26 values do not fit one page (u16 vector count); a 70000-vector column needs two pages
+    x = wl.timestamps(70_000 * 1024)
+    enc = oracle.encode_column(x, 11)
+    assert enc[6] == 1  # first page flags a following page
+    np.testing.assert_array_equal(oracle.decode_column(enc, np.int64, x.size), x)
+
+
+def test_rejects_ragged_and_unknown(oracle):
+    with pytest.raises(ValueError):
+        oracle.encode_column(np.arange(1000, dtype=np.int64), 1)
+    with pytest.raises(ValueError):
+        oracle.encode_column(np.arange(1024, dtype=np.int64), 3)  # patched variants are not supported
+    empty = oracle.encode_column(np.zeros(0, dtype=np.int64), 1)
+    assert oracle.decode_column(empty, np.int64, 0).size == 0
+
+
+@pytest.mark.parametrize("W,n", [(32, 5000), (64, 3000), (64, 700), (32, 1024)])
+def test_hzl_round_trip(oracle, W, n):
+    dt = {32: np.uint32, 64: np.uint64}[W]
+    x = np.random.default_rng(n).integers(0, 1 << 20, n).astype(dt)
+    enc = oracle.hzl_encode(x)
+    dec, used = oracle.hzl_decode(enc, n, dt)
+    assert used == enc.size
+    np.testing.assert_array_equal(dec, x)
+
+
+def test_hzl_decoder_applies_exceptions(oracle):
+    # hand-built chunk per src/cor/eng/hzl_de_cmp.cpp:40-58: rec_hdr{bw 3, exc_c 2, base 100}, ffor, positions, values
+    vals = (100 + np.arange(1024) % 8).astype(np.uint64)
+    packed = oracle.ffor(vals, 3, np.uint64(100))
+    hdr = np.zeros(12, dtype=np.uint8)
+    hdr[0] = 3
+    hdr[2:4] = np.frombuffer(np.uint16(2).tobytes(), np.uint8)
+    hdr[4:12] = np.frombuffer(np.uint64(100).tobytes(), np.uint8)
+    pos = np.array([7, 1000], dtype=np.uint16)
+    exc = np.array([123456789, 42], dtype=np.uint64)
+    tail = np.zeros(12, dtype=np.uint8)
+    tail[0] = 255
+    blob = np.concatenate([hdr, packed.view(np.uint8), pos.view(np.uint8), exc.view(np.uint8), tail])
+    dec, used = oracle.hzl_decode(blob, 1024, np.uint64)
+    want = vals.copy()
+    want[7], want[1000] = 123456789, 42
+    np.testing.assert_array_equal(dec, want)
+    assert used == blob.size
+
+
+def test_fsst_decoder_matches_reference_golden(oracle):
+    g = np.load(os.path.join(GOLDEN, "fsst_golden.npz"))
+    import ctypes as C
+    from oracle.py_oracle import _ptr
+    lib = oracle.lib
+    lib.orc_fsst_import.argtypes = [C.c_void_p, C.c_void_p]
+    lib.orc_fsst_decompress.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p]
+    lib.orc_fsst_decompress.restype = C.c_size_t
+    dec = (C.c_uint8 * 8192)()
+    assert lib.orc_fsst_import(dec, _ptr(g["table"])) == g["table"].size
+    comp, coff, offs, data = g["comp"], g["comp_offsets"], g["offsets"], g["data"]
+    out = np.zeros(256, dtype=np.uint8)
+    for i in range(offs.size - 1):
+        seg = comp[coff[i]:coff[i + 1]].copy()
+        got = lib.orc_fsst_decompress(dec, seg.size, _ptr(seg), out.size, _ptr(out))
+        np.testing.assert_array_equal(out[:got], data[offs[i]:offs[i + 1]])
+
+
+def test_fsst_column_round_trip(oracle):
+    offs, data = wl.vocab_strings(3 * 1024, seed=3)
+    enc = oracle.encode_str_column(offs, data)
+    assert enc.size < data.size  # the table compresses the vocabulary
+    o2, d2 = oracle.decode_str_column(enc, offs.size - 1, data.size + 64)
+    np.testing.assert_array_equal(o2, offs)
+    np.testing.assert_array_equal(d2, data)
+
+
+def test_fsst_reference_decodes_oracle_tables(oracle, reference):
+    # a table built by the oracle, in the fsst_export format, decoded by the reference's fsst_decompress
+    offs, data = wl.vocab_strings(2048, seed=9)
+    enc = oracle.encode_str_column(offs, data)
+    table_len = 17 + sum(int(b) * (i + 1) for i, b in enumerate(enc[8 + 9:8 + 17]))
+    table = enc[8:8 + table_len].copy()
+    comp = np.zeros(data.size * 2 + 64, dtype=np.uint8)
+    coff = np.zeros(offs.size, dtype=np.uint32)
+    for i in range(offs.size - 1):
+        s = data[offs[i]:offs[i + 1]].copy()
+        n = oracle.lib.orc_fsst_compress_one(table.ctypes.data, s.ctypes.data, s.size, comp[coff[i]:].ctypes.data)
+        coff[i + 1] = coff[i] + n
+    o2, d2 = reference.fsst_decode(table, comp[: coff[-1]].copy(), coff, data.size)
+    np.testing.assert_array_equal(o2, offs)
+    np.testing.assert_array_equal(d2, data)
