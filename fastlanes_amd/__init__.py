@@ -1,0 +1,156 @@
+"""fastlanes_amd -- FastLanes decoding on MI355X (gfx950).
+
+Python access to the C ABI of include/fastlanes.h and include/fls_gpu.h (libfastlanes_amd.so, built in-tree by
+`make -C fastlanes_amd`). Device buffers are passed as raw pointers (torch CUDA tensors' data_ptr() work).
+Importing this module does not touch the GPU; loading the library fails loudly if it has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfastlanes_amd.so")
+
+# data_t (include/fastlanes.h)
+DATA_T = {"float64": 1, "int8": 2, "int16": 3, "int32": 4, "int64": 5, "uint8": 6, "uint16": 7, "uint32": 8,
+          "uint64": 9, "str": 10}
+FLS_OK, FLS_E_DATA = 0, -6
+
+_lib = None
+_vp, _u64, _u32, _int = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+
+
+class FlsError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """The loaded C ABI. Raises if the HIP library is missing (there is no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FlsError(f"{LIB_PATH} is not built: run `make -C fastlanes_amd` (hipcc --offload-arch=gfx950)")
+    L = C.CDLL(LIB_PATH)
+    L.fls_gpu_error_string.restype = C.c_char_p
+    L.fls_gpu_plan_create.argtypes = [_vp, _u64, C.c_uint8, C.POINTER(_vp)]
+    L.fls_gpu_plan_destroy.argtypes = [_vp]
+    L.fls_gpu_plan_destroy.restype = None
+    for n in ("fls_gpu_plan_n_values", "fls_gpu_plan_encoded_size"):
+        getattr(L, n).argtypes = [_vp]
+        getattr(L, n).restype = _u64
+    L.fls_gpu_plan_n_pages.argtypes = [_vp]
+    L.fls_gpu_plan_n_pages.restype = _u32
+    L.fls_gpu_plan_decode.argtypes = [_vp, _vp, _vp, _u64, _vp]
+    L.fls_gpu_plan_check.argtypes = [_vp, _vp]
+    L.fls_gpu_plan_string_offsets.argtypes = [_vp, _vp, _vp, _vp]
+    L.fls_gpu_plan_decode_strings.argtypes = [_vp, _vp, _vp, _vp, _u64, _vp]
+    L.fls_gpu_unffor_stream.argtypes = [C.c_uint, C.c_uint, _vp, _vp, _vp, _u64, _vp]
+    L.fls_gpu_rowgroup_create.argtypes = [_u32, C.POINTER(_vp), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(C.c_uint8),
+                                          C.POINTER(_vp)]
+    L.fls_gpu_rowgroup_destroy.argtypes = [_vp]
+    L.fls_gpu_rowgroup_destroy.restype = None
+    L.fls_gpu_rowgroup_layout.argtypes = [_vp, C.POINTER(_u64), C.POINTER(_u64)]
+    L.fls_gpu_rowgroup_decode.argtypes = [_vp, _vp, _vp, _int, _vp]
+    L.fls_gpu_rowgroup_decode_graph.argtypes = [_vp, _vp, _vp, _vp]
+    L.fls_gpu_rowgroup_check.argtypes = [_vp, _vp]
+    # include/fastlanes.h
+    L.fls_cpi_connection_open.restype = _vp
+    L.fls_cpi_connection_close.argtypes = [_vp]
+    L.fls_cpi_connection_close.restype = None
+    L.encode_from_memory.argtypes = [_vp, _u64, _u64, _vp, C.POINTER(_u64), C.POINTER(_u64), C.c_uint8, C.c_uint8]
+    L.encode_from_memory.restype = C.c_bool
+    L.decode_to_memory.argtypes = [_vp, _vp, C.c_uint8]
+    L.decode_to_memory.restype = None
+    L.init_random_access.argtypes = [_vp, _u64, C.POINTER(_u64), C.POINTER(_u64)]
+    L.init_random_access.restype = None
+    L.random_access.argtypes = [_vp, _u64, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_vp), _vp]
+    L.random_access.restype = None
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != FLS_OK:
+        raise FlsError(f"{what}: {rc} {lib().fls_gpu_error_string().decode()}")
+
+
+def dtype_code(dtype) -> int:
+    return DATA_T[np.dtype(dtype).name] if dtype != "str" else DATA_T["str"]
+
+
+def stream_ptr(stream=None) -> int:
+    """hipStream_t of a torch stream (None -> torch's current stream)."""
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+class Plan:
+    """Host-side parse of one encoded column; decodes device-resident copies of the same bytes."""
+
+    def __init__(self, host_encoded: np.ndarray, dtype):
+        host_encoded = np.ascontiguousarray(host_encoded, dtype=np.uint8)
+        self.dtype = dtype
+        h = _vp()
+        check(lib().fls_gpu_plan_create(host_encoded.ctypes.data, host_encoded.size, dtype_code(dtype), C.byref(h)),
+              "plan_create")
+        self.h = h
+
+    @property
+    def n_values(self) -> int:
+        return lib().fls_gpu_plan_n_values(self.h)
+
+    @property
+    def encoded_size(self) -> int:
+        return lib().fls_gpu_plan_encoded_size(self.h)
+
+    @property
+    def n_pages(self) -> int:
+        return lib().fls_gpu_plan_n_pages(self.h)
+
+    def decode(self, dev_encoded_ptr: int, dev_out_ptr: int, out_capacity: int, stream: int = 0) -> None:
+        check(lib().fls_gpu_plan_decode(self.h, dev_encoded_ptr, dev_out_ptr, out_capacity, stream), "plan_decode")
+
+    def check(self, stream: int = 0) -> None:
+        check(lib().fls_gpu_plan_check(self.h, stream), "plan_check")
+
+    def string_offsets(self, dev_encoded_ptr: int, dev_offsets_ptr: int, stream: int = 0) -> None:
+        check(lib().fls_gpu_plan_string_offsets(self.h, dev_encoded_ptr, dev_offsets_ptr, stream), "string_offsets")
+
+    def decode_strings(self, dev_encoded_ptr, dev_offsets_ptr, dev_bytes_ptr, cap, stream: int = 0) -> None:
+        check(lib().fls_gpu_plan_decode_strings(self.h, dev_encoded_ptr, dev_offsets_ptr, dev_bytes_ptr, cap, stream),
+              "decode_strings")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.fls_gpu_plan_destroy(self.h)
+            self.h = None
+
+
+def unffor_stream(width: int, bw: int, dev_packed: int, dev_bases: int | None, dev_out: int, n_vec: int,
+                  stream: int = 0) -> None:
+    check(lib().fls_gpu_unffor_stream(width, bw, dev_packed, dev_bases or None, dev_out, n_vec, stream),
+          "unffor_stream")
+
+
+def encode_from_memory(values: np.ndarray, expr_id: int, capacity: int | None = None) -> np.ndarray:
+    """include/fastlanes.h encode_from_memory on a host array; returns the encoded bytes."""
+    values = np.ascontiguousarray(values)
+    cap = capacity if capacity is not None else values.nbytes * 2 + (1 << 16) + 64 * (values.size // 1024)
+    out = np.empty(cap, dtype=np.uint8)
+    size, meta = _u64(0), _u64(0)
+    ok = lib().encode_from_memory(values.ctypes.data, values.size, cap, out.ctypes.data, C.byref(size), C.byref(meta),
+                                  dtype_code(values.dtype), expr_id)
+    if not ok:
+        raise FlsError(f"encode_from_memory failed: {lib().fls_gpu_error_string().decode()}")
+    return out[: size.value].copy()
+
+
+def decode_to_memory(encoded_ptr: int, out_ptr: int, dtype) -> None:
+    """include/fastlanes.h decode_to_memory (host or device pointers)."""
+    lib().decode_to_memory(encoded_ptr, out_ptr, dtype_code(dtype))

@@ -1,0 +1,113 @@
+// format.hpp -- host-side view of FastLanes encoded columns: page parsing, validation and the shared constants.
+//
+// A column is a sequence of page entries; an entry is an optional dictionary page followed by a data page.
+//   data page  (src/cor/eng/compressor.cpp:68-86, src/cor/lyt/page/page.cpp:63-130)
+//     PageHdr {exp_t u8, var u8, 0 u16, vec_c u16, next u16}   (include/fls/cor/lyt/page/page_hdr.hpp:8-20)
+//     arr_c x u32 buffer offsets | (vec_c-1) x arr_c x u32 entry points | buffers
+//     `next` (0 in the reference) is 1 when another page entry follows: the reference's u16 vec_c caps a page
+//     at 65535 vectors, so larger columns span several pages.
+//   dictionary page: dict_hdr_t {exp_t u8, 0, 0, version u8, entry_c u32} (include/fls/cor/lyt/dic/dic.hpp:32-42)
+//     DICT: HZL payload of the values (src/cor/eng/hzl_cmp.cpp:43-176); FSST: fsst_export symbol table
+//     (src/cor/prm/fsst/libfsst.cpp:582-615).
+#pragma once
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace fls {
+
+// ExpT (include/fls/cor/exp/exp_type.hpp:7-28)
+enum : uint8_t {
+    EXP_UNCOMPRESSED = 2,
+    EXP_FFOR         = 4,
+    EXP_DELTA        = 5,
+    EXP_RLE          = 6,
+    EXP_FSST         = 7,
+    EXP_DICT         = 8,
+    EXP_DICT_FRQ     = 9,
+    EXP_DICT_VAL     = 10,
+};
+// data_t (include/fastlanes.h)
+enum : uint8_t {
+    DT_DOUBLE = 1,
+    DT_I8     = 2,
+    DT_I16    = 3,
+    DT_I32    = 4,
+    DT_I64    = 5,
+    DT_U8     = 6,
+    DT_U16    = 7,
+    DT_U32    = 8,
+    DT_U64    = 9,
+    DT_STR    = 10,
+};
+
+constexpr uint64_t kDataPageMax   = 8ull * 64 * 256 * 1024;  // CFG::PQ::DATA_PG_SZ, include/fls/cfg/cfg.hpp:82
+constexpr unsigned kMaxVecPerPage = 65535;                    // u16 PageHdr::vec_c
+
+unsigned dtype_bits(uint8_t dt);
+inline bool is_dict_exp(uint8_t e) { return e == EXP_DICT || e == EXP_DICT_FRQ || e == EXP_DICT_VAL; }
+
+struct Expr {
+    uint8_t exp_t, var, arr_c;
+    bool    dict;
+};
+// Pool ids (src/cor/exp_pool.cpp:23-100) -> expression; false if not supported.
+bool expr_from_id(uint8_t dtype, uint8_t id, Expr& e);
+bool expr_from_page(uint8_t exp_t, uint8_t var, Expr& e);
+
+struct HzlChunkH {
+    uint64_t src, dst, base;
+    uint32_t bw, exc_c, n;
+};
+
+struct FsstTableH {
+    uint8_t  len[255];
+    uint64_t symbol[255];
+};
+
+struct PageEntry {
+    uint64_t dict_off  = ~0ull;  // dictionary page offset, if any
+    uint64_t page_off  = 0;      // data page offset
+    uint64_t page_size = 0;      // bytes of the data page
+    uint64_t out_index = 0;      // first decoded value
+    uint32_t vec_c = 0, arr_c = 0;
+    uint8_t  exp_t = 0, var = 0;
+    // dictionary (integer columns)
+    uint32_t               dict_n = 0;
+    std::vector<HzlChunkH> hzl;
+    // FSST (string columns)
+    FsstTableH fsst;
+};
+
+struct Column {
+    uint8_t                dtype = 0;
+    uint64_t               n_values = 0, size = 0;
+    std::vector<PageEntry> pages;
+};
+
+// Parses and validates an encoded column (every offset, entry point, bit width and extent is checked so that
+// device kernels only touch bytes inside [0, size)). Returns an empty string on success, else the reason.
+std::string parse_column(const uint8_t* p, uint64_t size, uint8_t dtype, Column& out);
+// Size of the page-entry chain starting at p when the total size is not known (bounded by `limit`).
+std::string measure_column(const uint8_t* p, uint64_t limit, uint8_t dtype, uint64_t& size);
+
+// Vector v's slice of buffer b (host view).
+inline uint64_t ep_of(const uint8_t* page, unsigned arr_c, unsigned v, unsigned b) {
+    uint32_t ep = 0;
+    if (v) std::memcpy(&ep, page + 8 + 4ull * arr_c + 4ull * ((uint64_t)(v - 1) * arr_c + b), 4);
+    return ep;
+}
+inline uint64_t off_of(const uint8_t* page, unsigned b) {
+    uint32_t o;
+    std::memcpy(&o, page + 8 + 4ull * b, 4);
+    return o;
+}
+
+// Encoder (product side). Returns the encoded size, or -1 unsupported / -2 capacity.
+int64_t encode_column(uint8_t dtype, uint8_t expr_id, const void* in, uint64_t n, uint8_t* out, uint64_t cap,
+                      uint64_t* metadata_size);
+int64_t encode_str_column(uint8_t expr_id, const uint32_t* offsets, const uint8_t* bytes, uint64_t n, uint8_t* out,
+                          uint64_t cap);
+
+}  // namespace fls

@@ -1,0 +1,165 @@
+// gpu_api.hip -- the C-ABI of include/fls_gpu.h: column plans, launches, error reporting.
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fls_gpu.h"
+#include "format.hpp"
+#include "launch.hpp"
+#include "page_kernels.hpp"
+#include "gpu_internal.hpp"
+
+namespace fls {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+    return fail(FLS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace fls
+
+using namespace fls;
+
+struct DevDict {
+    uint8_t*  values = nullptr;  // decoded dictionary (DW-bit values)
+    HzlChunk* chunks = nullptr;
+    uint32_t  n_chunks = 0;
+};
+
+struct fls_gpu_plan {
+    Column               col;
+    unsigned             W  = 0;
+    unsigned             DW = 0;
+    uint32_t*            d_err = nullptr;
+    std::vector<DevDict> dicts;  // one per page entry (empty for non-dictionary pages)
+    StringPlan           strings;
+};
+
+extern "C" {
+
+const char* fls_gpu_error_string(void) { return g_err.c_str(); }
+
+int fls_gpu_plan_create(const uint8_t* host_encoded, uint64_t encoded_size, enum data_t type,
+                        fls_gpu_plan** out_plan) {
+    if (!host_encoded || !out_plan) return fail(FLS_E_ARG, "null argument");
+    *out_plan = nullptr;
+    auto plan = std::make_unique<fls_gpu_plan>();
+    auto err  = parse_column(host_encoded, encoded_size, (uint8_t)type, plan->col);
+    if (!err.empty()) return fail(FLS_E_FORMAT, err);
+    plan->W  = type == STR ? 8 : dtype_bits((uint8_t)type);
+    plan->DW = plan->W < 32 ? 32 : plan->W;
+    hipError_t e;
+    if ((e = hipMalloc(&plan->d_err, 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
+    if ((e = hipMemset(plan->d_err, 0, 4)) != hipSuccess) return hip_fail(e, "hipMemset");
+    plan->dicts.resize(plan->col.pages.size());
+    for (size_t i = 0; i < plan->col.pages.size(); ++i) {
+        const PageEntry& pe = plan->col.pages[i];
+        if (pe.hzl.empty()) continue;
+        DevDict&              d = plan->dicts[i];
+        std::vector<HzlChunk> ch(pe.hzl.size());
+        for (size_t k = 0; k < ch.size(); ++k)
+            ch[k] = {pe.hzl[k].src, pe.hzl[k].dst, pe.hzl[k].base, pe.hzl[k].bw, pe.hzl[k].exc_c, pe.hzl[k].n, 0};
+        d.n_chunks          = (uint32_t)ch.size();
+        const uint64_t vals = (uint64_t)pe.dict_n + 1024;  // chunk writes are whole 1024-value vectors
+        if ((e = hipMalloc(&d.values, vals * (plan->DW / 8))) != hipSuccess) return hip_fail(e, "hipMalloc");
+        if ((e = hipMalloc(&d.chunks, ch.size() * sizeof(HzlChunk))) != hipSuccess) return hip_fail(e, "hipMalloc");
+        if ((e = hipMemcpy(d.chunks, ch.data(), ch.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, "hipMemcpy");
+    }
+    if (type == STR) {
+        int rc = strings_prepare(plan->col, host_encoded, plan->strings);
+        if (rc) return rc;
+    }
+    *out_plan = plan.release();
+    return FLS_OK;
+}
+
+void fls_gpu_plan_destroy(fls_gpu_plan* plan) {
+    if (!plan) return;
+    for (auto& d : plan->dicts) {
+        if (d.values) (void)hipFree(d.values);
+        if (d.chunks) (void)hipFree(d.chunks);
+    }
+    strings_release(plan->strings);
+    if (plan->d_err) (void)hipFree(plan->d_err);
+    delete plan;
+}
+
+uint64_t fls_gpu_plan_n_values(const fls_gpu_plan* plan) { return plan ? plan->col.n_values : 0; }
+uint64_t fls_gpu_plan_encoded_size(const fls_gpu_plan* plan) { return plan ? plan->col.size : 0; }
+uint32_t fls_gpu_plan_n_pages(const fls_gpu_plan* plan) { return plan ? (uint32_t)plan->col.pages.size() : 0; }
+
+int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* dev_out, uint64_t out_capacity,
+                        void* stream) {
+    if (!plan || !dev_encoded || (!dev_out && plan->col.n_values)) return fail(FLS_E_ARG, "null argument");
+    if (plan->col.dtype == STR) return fail(FLS_E_ARG, "string column: use fls_gpu_plan_decode_strings");
+    if (((uintptr_t)dev_out & 15) != 0) return fail(FLS_E_ARG, "output must be 16-byte aligned");
+    const unsigned W = plan->W;
+    if (out_capacity < plan->col.n_values * (W / 8)) return fail(FLS_E_CAPACITY, "output capacity too small");
+    hipStream_t s = (hipStream_t)stream;
+    for (size_t i = 0; i < plan->col.pages.size(); ++i) {
+        const PageEntry& pe   = plan->col.pages[i];
+        const uint8_t*   page = dev_encoded + pe.page_off;
+        uint8_t*         out  = (uint8_t*)dev_out + pe.out_index * (W / 8);
+        hipError_t       e    = hipSuccess;
+        switch (pe.exp_t) {
+        case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pe.arr_c, pe.vec_c, out, s); break;
+        case EXP_FFOR: e = launch_page_int(W, 1, page, pe.arr_c, pe.vec_c, out, s); break;
+        case EXP_DELTA: e = launch_page_int(W, 2, page, pe.arr_c, pe.vec_c, out, s); break;
+        case EXP_RLE: e = launch_page_rle(W, pe.var, page, pe.arr_c, pe.vec_c, out, plan->d_err, s); break;
+        default: {
+            DevDict& d = plan->dicts[i];
+            e          = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, s);
+            if (e == hipSuccess)
+                e = launch_page_dict(W, pe.var, page, pe.arr_c, pe.vec_c, d.values, pe.dict_n, out, plan->d_err, s);
+        }
+        }
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    }
+    return FLS_OK;
+}
+
+int fls_gpu_plan_check(fls_gpu_plan* plan, void* stream) {
+    if (!plan) return fail(FLS_E_ARG, "null plan");
+    uint32_t   h = 0;
+    hipError_t e = hipMemcpyAsync(&h, plan->d_err, 4, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "check");
+    if (h) return fail(FLS_E_DATA, "decoded data referenced an out-of-range run or dictionary entry");
+    return FLS_OK;
+}
+
+int fls_gpu_plan_string_offsets(fls_gpu_plan* plan, const uint8_t* dev_encoded, uint64_t* dev_offsets,
+                                void* stream) {
+    if (!plan || !dev_encoded || !dev_offsets) return fail(FLS_E_ARG, "null argument");
+    if (plan->col.dtype != STR) return fail(FLS_E_ARG, "not a string column");
+    return strings_offsets(plan->col, plan->strings, dev_encoded, dev_offsets, (hipStream_t)stream);
+}
+
+int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, const uint64_t* dev_offsets,
+                                uint8_t* dev_bytes, uint64_t bytes_capacity, void* stream) {
+    if (!plan || !dev_encoded || !dev_offsets || !dev_bytes) return fail(FLS_E_ARG, "null argument");
+    if (plan->col.dtype != STR) return fail(FLS_E_ARG, "not a string column");
+    return strings_decode(plan->col, plan->strings, dev_encoded, dev_offsets, dev_bytes, bytes_capacity,
+                          (hipStream_t)stream);
+}
+
+int fls_gpu_unffor_stream(unsigned width, unsigned bw, const void* dev_packed, const void* dev_bases, void* dev_out,
+                          uint64_t n_vec, void* stream) {
+    if (width != 8 && width != 16 && width != 32 && width != 64) return fail(FLS_E_ARG, "width must be 8/16/32/64");
+    if (bw > width) return fail(FLS_E_ARG, "bit width larger than the element width");
+    if (n_vec && (!dev_out || (bw && !dev_packed))) return fail(FLS_E_ARG, "null argument");
+    if (((uintptr_t)dev_out & 15) != 0) return fail(FLS_E_ARG, "output must be 16-byte aligned");
+    hipError_t e = launch_unffor_stream(width, bw, (const uint8_t*)dev_packed, dev_bases, (uint8_t*)dev_out, n_vec,
+                                        (hipStream_t)stream);
+    return e == hipSuccess ? FLS_OK : hip_fail(e, "kernel launch");
+}
+
+}  // extern "C"

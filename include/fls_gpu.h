@@ -1,0 +1,94 @@
+/*
+ * fls_gpu.h -- device-resident entry points of the FastLanes decoder for MI355X (gfx950).
+ *
+ * Plain C ABI: pointers, sizes and an opaque plan handle. `stream` is a hipStream_t passed as void*
+ * (NULL = the default stream). Device pointers are HIP device memory (e.g. hipMalloc or a torch CUDA tensor).
+ * Decoded output buffers must be 16-byte aligned. All functions return 0 on success, a negative FLS_E_* code
+ * otherwise; fls_gpu_error_string() describes the last failure of the calling thread.
+ *
+ * The reference has no device boundary; each function states which reference interface it stands in for.
+ */
+#ifndef FLS_GPU_H
+#define FLS_GPU_H
+
+#include <stdint.h>
+
+#include "fastlanes.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FLS_OK            = 0,
+    FLS_E_ARG         = -1, /* bad argument (null pointer, misaligned output, unknown type) */
+    FLS_E_FORMAT      = -2, /* encoded bytes fail validation */
+    FLS_E_UNSUPPORTED = -3, /* expression or type not supported */
+    FLS_E_CAPACITY    = -4, /* output or encode capacity too small */
+    FLS_E_HIP         = -5, /* HIP runtime error */
+    FLS_E_DATA        = -6  /* decoded data referenced an out-of-range run or dictionary entry */
+};
+
+const char* fls_gpu_error_string(void);
+
+/* ---- column plans ------------------------------------------------------------------------------------------------
+ * A plan is the host-side parse of one encoded column (all page entries): it stands in for the per-call parsing of
+ * Decompressor::Init / InitDict (reference src/cor/eng/decompressor.cpp:21-94) and ExpColDecoder::full_decode
+ * (src/encoder/exp_col_decoder.cpp:13-26), done once. host_encoded must hold the whole column; it is only read
+ * during fls_gpu_plan_create. Decoding then reads the same bytes from device memory. */
+typedef struct fls_gpu_plan fls_gpu_plan;
+
+int      fls_gpu_plan_create(const uint8_t* host_encoded, uint64_t encoded_size, enum data_t type,
+                             fls_gpu_plan** out_plan);
+void     fls_gpu_plan_destroy(fls_gpu_plan* plan);
+uint64_t fls_gpu_plan_n_values(const fls_gpu_plan* plan);  /* values (or strings) in the column */
+uint64_t fls_gpu_plan_encoded_size(const fls_gpu_plan* plan);
+uint32_t fls_gpu_plan_n_pages(const fls_gpu_plan* plan);
+
+/* Decode the whole integer column to dev_out (n_values * sizeof(T) bytes, original value order).
+ * Replaces decode_to_memory (reference include/fastlanes.h:84-86) for device-resident data. Asynchronous. */
+int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* dev_out, uint64_t out_capacity,
+                        void* stream);
+/* Synchronizes the stream and reports FLS_E_DATA if a decode on this plan met an out-of-range position/code. */
+int fls_gpu_plan_check(fls_gpu_plan* plan, void* stream);
+
+/* Strings (FSST pages, reference include/fls/cor/exp/fsst/fsst_exp.hpp:35-52 decoded as
+ * include/fls/encoder/assert_eq.hpp:166-210 does). Step 1 writes n+1 uint64 output offsets (Arrow layout);
+ * dev_offsets[n] is the total decoded size. Step 2 writes the bytes. Both asynchronous. */
+int fls_gpu_plan_string_offsets(fls_gpu_plan* plan, const uint8_t* dev_encoded, uint64_t* dev_offsets,
+                                void* stream);
+int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, const uint64_t* dev_offsets,
+                                uint8_t* dev_bytes, uint64_t bytes_capacity, void* stream);
+
+/* ---- raw primitive streams -----------------------------------------------------------------------------------------
+ * n_vec packed vectors back to back (128*bw bytes each), decoded into n_vec*1024 values of `width` bits in the
+ * lane-major order of the reference's generated unffor/unpack (primitives/fls_generated/fallback/scalar_aav_uf1/
+ * fallback_scalar_aav_1024_uf1_unffor_src.cpp:22879-23077, ..._unpack_src.cpp:17191-17380). dev_bases holds one
+ * base per vector (FFOR) or is NULL (plain unpack). */
+int fls_gpu_unffor_stream(unsigned width, unsigned bw, const void* dev_packed, const void* dev_bases, void* dev_out,
+                          uint64_t n_vec, void* stream);
+
+/* ---- row groups --------------------------------------------------------------------------------------------------
+ * Many columns (each an encoded column resident in device memory at its own offset of one buffer) decoded by one
+ * persistent launch over a device-side work list of (column, vector range) tiles. Stands in for the per-column
+ * decode loop behind Reader::get_chunk / Interpreter::Decoding (reference src/reader/reader.cpp:19-61,
+ * src/expression/interpreter.cpp:105-137). */
+typedef struct fls_gpu_rowgroup fls_gpu_rowgroup;
+
+int  fls_gpu_rowgroup_create(uint32_t n_columns, const uint8_t* const* host_columns, const uint64_t* column_sizes,
+                             const uint64_t* device_offsets, const enum data_t* types, fls_gpu_rowgroup** out);
+void fls_gpu_rowgroup_destroy(fls_gpu_rowgroup* rg);
+/* Output offsets (bytes, 16-aligned) of each column's values in the row-group's output buffer, and its size. */
+int  fls_gpu_rowgroup_layout(const fls_gpu_rowgroup* rg, uint64_t* out_offsets, uint64_t* out_total_bytes);
+/* mode 0: one persistent launch over the work list; mode 1: one launch per column page (baseline form). */
+int  fls_gpu_rowgroup_decode(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, int mode,
+                             void* stream);
+/* Captures the persistent launch in a HIP graph once and replays it. */
+int  fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, void* stream);
+int  fls_gpu_rowgroup_check(fls_gpu_rowgroup* rg, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLS_GPU_H */

@@ -1,0 +1,239 @@
+"""GPU decode parity: every kernel through the C ABI against the CPU oracle, bit-exact."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import fastlanes_amd as fa
+from fastlanes_amd import workloads as wl
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+W2DT = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def dev(torch, arr: np.ndarray):
+    """Device copy of a host byte/array buffer (16-byte padded)."""
+    b = np.ascontiguousarray(arr).view(np.uint8).ravel()
+    t = torch.zeros(b.size + 16, dtype=torch.uint8, device="cuda")
+    if b.size:
+        t[: b.size] = torch.from_numpy(b).cuda()
+    return t
+
+
+def empty(torch, nbytes):
+    return torch.empty(max(nbytes, 16), dtype=torch.uint8, device="cuda")
+
+
+def to_np(t, dtype, n):
+    return t.cpu().numpy().view(np.uint8)[: n * np.dtype(dtype).itemsize].view(dtype)
+
+
+@pytest.mark.parametrize("W", [8, 16, 32, 64])
+def test_unffor_stream_every_bit_width(torch_cuda, oracle, W):
+    torch = torch_cuda
+    dt = W2DT[W]
+    n_vec = 24
+    for bw in range(W + 1):
+        print(f"stream W={W} bw={bw}", flush=True)
+        vals, bases = wl.sweep_case(W, bw, n_vec, seed=1)
+        x = (vals + np.repeat(bases, 1024)).astype(dt)
+        packed = oracle.ffor_stream(x, bw, bases)
+        want = oracle.unffor_stream(packed, n_vec, bw, bases, dt)
+        np.testing.assert_array_equal(want, x)
+        d_in, d_b, d_out = dev(torch, packed), dev(torch, bases), empty(torch, x.nbytes)
+        fa.unffor_stream(W, bw, d_in.data_ptr(), d_b.data_ptr(), d_out.data_ptr(), n_vec)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(to_np(d_out, dt, x.size), want, err_msg=f"W={W} bw={bw}")
+        # plain unpack (no bases)
+        pk = oracle.ffor_stream(vals.astype(dt), bw, np.zeros(n_vec, dtype=dt))
+        d_in = dev(torch, pk)
+        fa.unffor_stream(W, bw, d_in.data_ptr(), None, d_out.data_ptr(), n_vec)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(to_np(d_out, dt, x.size), vals.astype(dt), err_msg=f"unpack W={W} bw={bw}")
+
+
+def test_config1_golden(torch_cuda, oracle):
+    torch = torch_cuda
+    g = json.load(open(os.path.join(GOLDEN, "config1_golden.json")))
+    x = wl.config1_values()
+    n_vec = x.size // 1024
+    bases = np.full(n_vec, g["base"], dtype=np.uint32)
+    packed = oracle.ffor_stream(x, 7, bases)
+    assert hashlib.sha256(packed.tobytes()).hexdigest() == g["packed_sha256"]  # same bytes as the reference encoder
+    d_in, d_b, d_out = dev(torch, packed), dev(torch, bases), empty(torch, x.nbytes)
+    fa.unffor_stream(32, 7, d_in.data_ptr(), d_b.data_ptr(), d_out.data_ptr(), n_vec)
+    torch.cuda.synchronize()
+    got = to_np(d_out, np.uint32, x.size)
+    assert hashlib.sha256(got.tobytes()).hexdigest() == g["decoded_sha256"]
+
+
+def _column(dtype, eid, n, seed):
+    g = np.random.default_rng(seed)
+    info = np.iinfo(dtype)
+    if eid in (6, 7):
+        return wl.geometric_runs(n, 16.0, seed=seed, value_bits=min(20, info.bits - 1)).astype(dtype)
+    if eid >= 16:
+        keys = g.integers(info.min, info.max, 4096 if info.bits > 8 else 200, dtype=dtype, endpoint=True)
+        return keys[g.integers(0, keys.size, n)]
+    if eid == 11:
+        return np.cumsum(g.integers(0, 100, n)).astype(dtype)
+    return g.integers(info.min, info.max, n, dtype=dtype, endpoint=True)
+
+
+DTYPES = [np.uint8, np.int16, np.uint32, np.int64, np.int8, np.uint16, np.int32, np.uint64]
+
+
+@pytest.mark.parametrize("eid", [0, 1, 6, 7, 11, 16, 17, 22, 28, 29])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_page_decode_matches_oracle(torch_cuda, oracle, dtype, eid):
+    torch = torch_cuda
+    print(f"page dtype={np.dtype(dtype).name} expr={eid}", flush=True)
+    n = 40 * 1024
+    x = _column(dtype, eid, n, seed=eid * 7 + np.dtype(dtype).itemsize)
+    enc = oracle.encode_column(x, eid)
+    want = oracle.decode_column(enc, dtype, n)
+    np.testing.assert_array_equal(want, x)
+    plan = fa.Plan(enc, dtype)
+    assert plan.n_values == n
+    d_in, d_out = dev(torch, enc), empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    plan.check()
+    np.testing.assert_array_equal(to_np(d_out, dtype, n), want)
+
+
+def test_multi_page_delta(torch_cuda, oracle):
+    torch = torch_cuda
+    x = wl.timestamps(65_600 * 1024)
+    enc = oracle.encode_column(x, 11)
+    plan = fa.Plan(enc, np.int64)
+    assert plan.n_pages == 2
+    d_in, d_out = dev(torch, enc), empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    plan.check()
+    got = to_np(d_out, np.int64, x.size)
+    assert np.array_equal(got, x)
+
+
+def test_out_of_range_codes_are_clamped_and_reported(torch_cuda, oracle):
+    torch = torch_cuda
+    x = np.tile(np.arange(4, dtype=np.int64), 1024)
+    enc = oracle.encode_column(x, 16)  # DICT, uncompressed codes
+    # corrupt one code of vector 0 (codes buffer follows the dictionary page and the page metadata)
+    page = enc.size - 4096
+    bad = enc.copy()
+    bad[page + 100: page + 104] = np.frombuffer(np.uint32(999).tobytes(), np.uint8)
+    plan = fa.Plan(bad, np.int64)
+    d_in, d_out = dev(torch, bad), empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    with pytest.raises(fa.FlsError):
+        plan.check()
+
+
+def test_fsst_strings_match_oracle(torch_cuda, oracle):
+    torch = torch_cuda
+    offs, data = wl.vocab_strings(6 * 1024, seed=5)
+    enc = oracle.encode_str_column(offs, data)
+    plan = fa.Plan(enc, "str")
+    n = plan.n_values
+    assert n == offs.size - 1
+    d_in = dev(torch, enc)
+    d_off = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
+    plan.string_offsets(d_in.data_ptr(), d_off.data_ptr())
+    got_off = d_off.cpu().numpy()
+    np.testing.assert_array_equal(got_off, offs.astype(np.int64))
+    total = int(got_off[-1])
+    d_bytes = empty(torch, total)
+    plan.decode_strings(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), total)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d_bytes.cpu().numpy()[:total], data)
+
+
+def test_fsst_reference_table_golden(torch_cuda, oracle):
+    """Strings compressed by the reference's own fsst_create/fsst_compress (fixture), packed into pages."""
+    torch = torch_cuda
+    g = np.load(os.path.join(GOLDEN, "fsst_golden.npz"))
+    table, comp, coff, offs, data = g["table"], g["comp"], g["comp_offsets"], g["offsets"], g["data"]
+    n = offs.size - 1
+    # build one FSST page per the reference layout: [dict hdr][table] then page {bytes, transposed end offsets}
+    pages = []
+    vec_c = n // 1024
+    ends_all, bytes_all, eps0, eps1 = [], [], [], []
+    b0 = 0
+    for v in range(vec_c):
+        s0, s1 = coff[v * 1024], coff[(v + 1) * 1024]
+        ends = (coff[v * 1024 + 1:(v + 1) * 1024 + 1] - s0).astype(np.uint32)
+        ends_all.append(oracle.transpose(ends))
+        bytes_all.append(comp[s0:s1])
+        if v:
+            eps0.append(b0)
+            eps1.append(4096 * v)
+        b0 += s1 - s0
+    hdr = np.array([7, 1, 0, 0, vec_c & 255, vec_c >> 8, 0, 0], dtype=np.uint8)
+    meta = 8 + 8 + 8 * (vec_c - 1)
+    offs_arr = np.array([meta, meta + b0], dtype=np.uint32)
+    eps = np.array([x for pair in zip(eps0, eps1) for x in pair], dtype=np.uint32)
+    page = np.concatenate([hdr, offs_arr.view(np.uint8), eps.view(np.uint8), np.concatenate(bytes_all),
+                           np.concatenate(ends_all).view(np.uint8)])
+    dh = np.array([7, 0, 0, 0, 0, 0, 0, 0], dtype=np.uint8)
+    enc = np.concatenate([dh, table, page])
+    o2, d2 = oracle.decode_str_column(enc, n, data.size + 64)
+    np.testing.assert_array_equal(d2, data)
+    plan = fa.Plan(enc, "str")
+    d_in = dev(torch, enc)
+    d_off = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
+    plan.string_offsets(d_in.data_ptr(), d_off.data_ptr())
+    d_bytes = empty(torch, data.size)
+    plan.decode_strings(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d_off.cpu().numpy(), offs.astype(np.int64))
+    np.testing.assert_array_equal(d_bytes.cpu().numpy()[: data.size], data)
+
+
+@pytest.mark.parametrize("eid", [0, 1, 11, 6, 16])
+def test_c_api_round_trip_host_pointers(torch_cuda, eid):
+    x = _column(np.int64, eid, 8 * 1024, seed=3)
+    enc = fa.encode_from_memory(x, eid)
+    out = np.zeros_like(x)
+    fa.decode_to_memory(enc.ctypes.data, out.ctypes.data, np.int64)
+    np.testing.assert_array_equal(out, x)
+
+
+def test_c_api_device_pointers(torch_cuda):
+    torch = torch_cuda
+    x = _column(np.int64, 11, 8 * 1024, seed=4)
+    enc = fa.encode_from_memory(x, 11)
+    d_in, d_out = dev(torch, enc), empty(torch, x.nbytes)
+    fa.decode_to_memory(d_in.data_ptr(), d_out.data_ptr(), np.int64)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(to_np(d_out, np.int64, x.size), x)
+
+
+@pytest.mark.parametrize("eid", [1, 11, 6, 7, 16, 17])
+def test_random_access(torch_cuda, eid):
+    import ctypes as C
+    x = _column(np.int64, eid, 5 * 1024, seed=9)
+    enc = fa.encode_from_memory(x, eid)
+    lib = fa.lib()
+    for v in (0, 3, 4):
+        nb = C.c_uint64(0)
+        pairs = (C.c_uint64 * 16)()
+        lib.init_random_access(enc.ctypes.data, v, C.byref(nb), pairs)
+        assert nb.value > 0
+        bufs = [enc[pairs[2 * b]: pairs[2 * b] + pairs[2 * b + 1]].copy() for b in range(nb.value)]
+        ptrs = (C.c_void_p * nb.value)(*[b.ctypes.data for b in bufs])
+        out = np.zeros(1024, dtype=np.int64)
+        lib.random_access(enc.ctypes.data, v, C.byref(nb), pairs, ptrs, out.ctypes.data)
+        np.testing.assert_array_equal(out, x[v * 1024:(v + 1) * 1024], err_msg=f"expr {eid} vector {v}")
