@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Decode benchmark: FastLanes bit-unpack / FFOR bit-width sweep on one MI355X.
+
+Workload (one step): for every element type uint8/16/32/64 and every bit width w in [0, W], decode 2^28 values
+FFOR-packed (per-vector base) and 2^28 values plain-packed, all inputs resident in HBM. Values are U[0, 2^w), so
+their packed bits are uniform random bytes; bases are U[0, 2^(W-w)] (bound clamped at w = 0). Output in the
+reference's unffor/unpack order. A bounded sample of every case is checked bit-exact against the CPU oracle.
+
+Prints one JSON line (metric, value in Gvalues/s, roofline of the dominant kernel, CPU baseline of the
+reference's own compiled unffor on the host cores). --detail adds the cascade / row-group / wizard sub-benchmarks.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gvalues/s decoded + % HBM peak; learned vs exhaustive wizard time @<=2% size"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log2n", type=int, default=28, help="values per (type, width) case")
+    ap.add_argument("--widths", default="8,16,32,64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--detail", action="store_true", help="also run cascade, row-group and wizard sub-benchmarks")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/traffic.txt: `<kernel> <bytes>` lines)."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel:
+            return float(parts[1])
+    return None
+
+
+def cpu_baseline(widths, threads):
+    """The reference's compiled generated::unffor (oracle/_ref) on the host cores, over a bounded sample: every
+    (type, width) FFOR case at 2^20 values (the same per-case shape as the GPU sweep)."""
+    from oracle.py_oracle import Reference, reference_available
+
+    if not reference_available():
+        return None
+    ref = Reference()
+    rng = np.random.default_rng(0)
+    total_vals, total_t = 0, 0.0
+    n_vec = 1024
+    for W in widths:
+        dt = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[W]
+        out = np.empty(n_vec * 1024, dtype=dt)
+        for w in range(W + 1):
+            packed = rng.integers(0, 256, n_vec * 128 * w + 64, dtype=np.uint8)
+            bases = rng.integers(0, 2 ** min(W - w, 62) + 1, n_vec).astype(dt)
+            t = ref.lib.ref_time_unffor(W, packed.ctypes.data, out.ctypes.data, n_vec, w, bases.ctypes.data, 0, 3,
+                                        threads)
+            total_vals += n_vec * 1024
+            total_t += t
+    return {"value": round(total_vals / total_t / 1e9, 3), "unit": "Gvalues/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"reference generated::unffor, every (type,width) FFOR case of {widths} at 2^20 values, "
+                      f"median of 3 passes, {threads} threads"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+
+    import fastlanes_amd as fa
+
+    fa.lib()
+    dev = torch.device("cuda:0")
+    widths = [int(x) for x in a.widths.split(",")]
+    n = 1 << a.log2n
+    n_vec = n // 1024
+    g = torch.Generator(device=dev)
+    g.manual_seed(1234)
+    dt_of = {8: torch.uint8, 16: torch.int16, 32: torch.int32, 64: torch.int64}
+
+    # ---- inputs resident in HBM --------------------------------------------------------------------------------
+    cases = []  # (W, w, packed, bases)
+    for W in widths:
+        for w in range(W + 1):
+            packed = torch.randint(0, 256, (max(n_vec * 128 * w, 16),), dtype=torch.uint8, device=dev, generator=g)
+            hi = min(W - w, 62)
+            bases = torch.randint(0, 2 ** hi + (1 if hi < 62 else 0), (n_vec,), dtype=torch.int64, device=dev,
+                                  generator=g)
+            bases = bases.to(dt_of[W]) if W < 64 else bases
+            cases.append((W, w, packed, bases))
+    outs = {W: torch.empty(n * W // 8 + 16, dtype=torch.uint8, device=dev) for W in widths}
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    sp = int(stream.cuda_stream)
+
+    def launch(W, w, packed, bases, plain):
+        fa.unffor_stream(W, w, packed.data_ptr(), 0 if plain else bases.data_ptr(), outs[W].data_ptr(), n_vec, sp)
+
+    def step(events=None):
+        for plain in (False, True):
+            for (W, w, packed, bases) in cases:
+                if events is not None:
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    launch(W, w, packed, bases, plain)
+                    e1.record(stream)
+                    events.append((W, w, plain, e0, e1))
+                else:
+                    launch(W, w, packed, bases, plain)
+
+    # ---- parity on a bounded sample of every case (first and last 4 vectors) ----------------------------------
+    from oracle.py_oracle import Oracle
+
+    orc = Oracle()
+    bad = []
+    for plain in (False, True):
+        for (W, w, packed, bases) in cases:
+            launch(W, w, packed, bases, plain)
+            torch.cuda.synchronize()
+            npdt = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[W]
+            for v0 in (0, n_vec - 4):
+                pk = packed[v0 * 128 * w:(v0 + 4) * 128 * w].cpu().numpy() if w else np.zeros(16, np.uint8)
+                bs = np.zeros(4, npdt) if plain else bases[v0:v0 + 4].cpu().numpy().view(npdt)
+                want = orc.unffor_stream(pk, 4, w, np.ascontiguousarray(bs), npdt)
+                got = outs[W][v0 * 1024 * W // 8:(v0 + 4) * 1024 * W // 8].cpu().numpy().view(npdt)
+                if not np.array_equal(got, want):
+                    bad.append((W, w, plain, v0))
+    if bad:
+        print(json.dumps({"error": "decode mismatch against the oracle", "cases": bad[:10]}))
+        sys.exit(1)
+
+    # ---- timed region --------------------------------------------------------------------------------------------
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step(events)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    per_case = {}
+    for (W, w, plain, e0, e1) in events:
+        per_case.setdefault((W, w, plain), []).append(e0.elapsed_time(e1) * 1e-3)
+    values_per_step = 2 * len(cases) * n
+    kernel_time = sum(sum(v) for v in per_case.values()) / a.steps
+
+    def alg_bytes(W, w, plain):  # packed bits + stored values + (FFOR) one base per vector
+        return n * w / 8 + n * W / 8 + (0 if plain else n_vec * W / 8)
+
+    # dominant kernel: largest total time
+    (dW, dw, dplain), dts = max(per_case.items(), key=lambda kv: sum(kv[1]))
+    d_avg = sum(dts) / len(dts)
+    d_bytes = alg_bytes(dW, dw, dplain)
+    kname = f"k_unffor_stream<{dW},{dw}>"
+    achieved = d_bytes / d_avg / 1e9
+    traffic = traffic_from_profiles(kname)
+
+    sweep = {}
+    for W in widths:
+        fr = []
+        for w in range(W + 1):
+            for plain in (False, True):
+                t = sum(per_case[(W, w, plain)]) / len(per_case[(W, w, plain)])
+                fr.append(alg_bytes(W, w, plain) / t / 1e9 / HBM_PEAK_GBS)
+        sweep[f"u{W}"] = {"min_frac": round(min(fr), 3), "mean_frac": round(float(np.mean(fr)), 3),
+                          "max_frac": round(max(fr), 3)}
+    agg_bytes = sum(alg_bytes(W, w, p) for (W, w, p) in per_case)
+    u32w7 = per_case.get((32, 7, False))
+    line = {
+        "metric": METRIC,
+        "value": round(values_per_step / (wall / a.steps) / 1e9, 2),
+        "unit": "Gvalues/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/u16/u32/u64",
+        "data": "synthetic (seeded on-device random packed bytes and bases)",
+        "config": {"workload": f"bit-width sweep: FFOR and plain unpack, uint8/16/32/64, every w in [0,W], "
+                               f"2^{a.log2n} values per case", "cases": 2 * len(cases),
+                   "values_per_step": values_per_step},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": kname + (" (plain unpack)" if dplain else " (FFOR)"),
+                     "bytes_per_launch": d_bytes, "avg_launch_s": d_avg},
+        "sweep_hbm_frac": sweep,
+        "sweep_aggregate_hbm_frac": round(agg_bytes / kernel_time / 1e9 / HBM_PEAK_GBS, 4),
+        "u32_w7_ffor_hbm_frac": round(alg_bytes(32, 7, False) / (sum(u32w7) / len(u32w7)) / 1e9 / HBM_PEAK_GBS, 4)
+        if u32w7 else None,
+        "parity": f"bit-exact vs oracle on 8 sampled vectors of each of the {2 * len(cases)} cases",
+    }
+    del cases, outs
+    torch.cuda.empty_cache()
+    if a.detail:
+        from fastlanes_amd import bench_detail
+
+        line["detail"] = bench_detail.run_all(dev)
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(widths, threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -151,6 +151,22 @@ int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, 
                           (hipStream_t)stream);
 }
 
+int fls_gpu_tune_set(int family, unsigned width, unsigned bw, unsigned block, unsigned nontemporal,
+                     unsigned max_blocks) {
+    if (family < 0 || family >= FAM_COUNT) return fail(FLS_E_ARG, "unknown kernel family");
+    if (block == 0 || block % 64 || block > kMaxBlock) return fail(FLS_E_ARG, "block must be a multiple of 64, <= 512");
+    (void)tune_for(family, width, bw);  // make sure the default table is loaded first
+    tune_set(family, width, bw, Tune {(uint16_t)block, (uint16_t)(nontemporal != 0), max_blocks});
+    return FLS_OK;
+}
+
+int fls_gpu_tune_load(const char* path) {
+    if (!path) return fail(FLS_E_ARG, "null path");
+    (void)tune_for(0, 8, 0);
+    int n = tune_load(path);
+    return n < 0 ? fail(FLS_E_ARG, "cannot read tuning file") : n;
+}
+
 int fls_gpu_unffor_stream(unsigned width, unsigned bw, const void* dev_packed, const void* dev_bases, void* dev_out,
                           uint64_t n_vec, void* stream) {
     if (width != 8 && width != 16 && width != 32 && width != 64) return fail(FLS_E_ARG, "width must be 8/16/32/64");

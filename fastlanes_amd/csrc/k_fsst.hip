@@ -15,6 +15,7 @@ namespace fls {
 
 namespace {
 
+constexpr unsigned kBlock        = 256;
 constexpr unsigned kStrPerThread = 1024 / kBlock;
 
 // transposed index of original position i (inverse of untranspose_i): i = r + 8*ORDER[b] + 64c -> 128r + 16b + c

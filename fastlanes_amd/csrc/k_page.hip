@@ -1,17 +1,69 @@
-// k_page.hip -- page decode kernels (UNCOMPRESSED / FFOR / DELTA / RLE / DICT) and dictionary payload kernels.
+// k_page.hip -- width dispatch for page kernels, dictionary payload kernels and the tuning table.
+#include <dlfcn.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "launch.hpp"
 #include "page_kernels.hpp"
 
 namespace fls {
 
-template <int W>
-static hipError_t page_int_w(int mode, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out,
-                             hipStream_t s) {
-    const unsigned blocks = (vec_c + Geo<W>::VPB - 1) / Geo<W>::VPB;
-    if (mode == 0) hipLaunchKernelGGL((k_page_int<W, 0>), dim3(blocks), dim3(kBlock), 0, s, page, arr_c, vec_c, out);
-    else if (mode == 1) hipLaunchKernelGGL((k_page_int<W, 1>), dim3(blocks), dim3(kBlock), 0, s, page, arr_c, vec_c, out);
-    else hipLaunchKernelGGL((k_page_int<W, 2>), dim3(blocks), dim3(kBlock), 0, s, page, arr_c, vec_c, out);
-    return hipGetLastError();
+namespace {
+std::mutex                                          g_tune_mu;
+std::map<std::tuple<int, unsigned, unsigned>, Tune> g_tune;
+std::once_flag                                      g_tune_once;
+
+// The tuning table shipped next to the library (or named by FLS_TUNING) is loaded on first use.
+void load_default_tuning() {
+    const char* env = std::getenv("FLS_TUNING");
+    if (env && *env) {
+        tune_load(env);
+        return;
+    }
+    Dl_info info;
+    if (dladdr((void*)&load_default_tuning, &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        p = p.substr(0, p.find_last_of('/') + 1) + "tuning.txt";
+        tune_load(p.c_str());
+    }
+}
+}  // namespace
+
+Tune tune_for(int family, unsigned W, unsigned bw) {
+    std::call_once(g_tune_once, load_default_tuning);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto                        it = g_tune.find({family, W, bw});
+    if (it != g_tune.end()) return it->second;
+    it = g_tune.find({family, W, 255u});
+    if (it != g_tune.end()) return it->second;
+    return Tune {};
+}
+void tune_set(int family, unsigned W, unsigned bw, Tune t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[{family, W, bw}] = t;
+}
+int tune_load(const char* path) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return -1;
+    char line[256];
+    int  n = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        int      fam;
+        unsigned W, bw, block, nt, maxb;
+        if (std::sscanf(line, "%d %u %u %u %u %u", &fam, &W, &bw, &block, &nt, &maxb) != 6) continue;
+        if (fam < 0 || fam >= FAM_COUNT || block == 0 || block % 64 || block > kMaxBlock) continue;
+        tune_set(fam, W, bw, Tune {(uint16_t)block, (uint16_t)(nt != 0), maxb});
+        ++n;
+    }
+    std::fclose(f);
+    return n;
 }
 
 hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out,
@@ -26,51 +78,26 @@ hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, uint32_t a
     }
 }
 
-template <typename T>
-static hipError_t page_rle_t(int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out,
-                             uint32_t* err, hipStream_t s) {
-    const unsigned blocks = (vec_c + Geo<16>::VPB - 1) / Geo<16>::VPB;
-    T*             o      = reinterpret_cast<T*>(out);
-    if (var == 1) hipLaunchKernelGGL((k_page_rle<T, 1>), dim3(blocks), dim3(kBlock), 0, s, page, arr_c, vec_c, o, err);
-    else hipLaunchKernelGGL((k_page_rle<T, 2>), dim3(blocks), dim3(kBlock), 0, s, page, arr_c, vec_c, o, err);
-    return hipGetLastError();
-}
-
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out,
                            uint32_t* err, hipStream_t s) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_rle_t<uint8_t>(var, page, arr_c, vec_c, out, err, s);
-    case 16: return page_rle_t<uint16_t>(var, page, arr_c, vec_c, out, err, s);
-    case 32: return page_rle_t<uint32_t>(var, page, arr_c, vec_c, out, err, s);
-    case 64: return page_rle_t<uint64_t>(var, page, arr_c, vec_c, out, err, s);
+    case 8: return page_rle_w<8>(var, page, arr_c, vec_c, out, err, s);
+    case 16: return page_rle_w<16>(var, page, arr_c, vec_c, out, err, s);
+    case 32: return page_rle_w<32>(var, page, arr_c, vec_c, out, err, s);
+    case 64: return page_rle_w<64>(var, page, arr_c, vec_c, out, err, s);
     default: return hipErrorInvalidValue;
     }
-}
-
-template <typename T, typename D>
-static hipError_t page_dict_t(int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, const void* dict,
-                              uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s) {
-    const unsigned blocks = (vec_c + Geo<32>::VPB - 1) / Geo<32>::VPB;
-    T*             o      = reinterpret_cast<T*>(out);
-    const D*       d      = reinterpret_cast<const D*>(dict);
-    if (var == 1)
-        hipLaunchKernelGGL((k_page_dict<T, D, 1>), dim3(blocks), dim3(kBlock), 0, s, page, arr_c, vec_c, d, dict_n, o, err);
-    else if (var == 2)
-        hipLaunchKernelGGL((k_page_dict<T, D, 2>), dim3(blocks), dim3(kBlock), 0, s, page, arr_c, vec_c, d, dict_n, o, err);
-    else
-        hipLaunchKernelGGL((k_page_dict<T, D, 7>), dim3(blocks), dim3(kBlock), 0, s, page, arr_c, vec_c, d, dict_n, o, err);
-    return hipGetLastError();
 }
 
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c,
                             const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_dict_t<uint8_t, uint32_t>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
-    case 16: return page_dict_t<uint16_t, uint32_t>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
-    case 32: return page_dict_t<uint32_t, uint32_t>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
-    case 64: return page_dict_t<uint64_t, uint64_t>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
+    case 8: return page_dict_w<8>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
+    case 16: return page_dict_w<16>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
+    case 32: return page_dict_w<32>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
+    case 64: return page_dict_w<64>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -78,16 +105,14 @@ hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, uint32_t a
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
                       hipStream_t s) {
     if (!n_chunks) return hipSuccess;
-    const unsigned vpb    = DW == 64 ? Geo<64>::VPB : Geo<32>::VPB;
-    const unsigned blocks = (n_chunks + vpb - 1) / vpb;
+    const Tune t {256, 0, 0};
+    const dim3 g(grid_for(n_chunks, t, kTPV)), b(256), ge(n_chunks < 1024 ? n_chunks : 1024);
     if (DW == 32) {
-        hipLaunchKernelGGL((k_hzl_chunks<32>), dim3(blocks), dim3(kBlock), 0, s, enc, chunks, n_chunks, dict);
-        hipLaunchKernelGGL((k_hzl_exceptions<32>), dim3(n_chunks < 1024 ? n_chunks : 1024), dim3(kBlock), 0, s, enc,
-                           chunks, n_chunks, dict);
+        hipLaunchKernelGGL((k_hzl_chunks<32>), g, b, 0, s, enc, chunks, n_chunks, dict);
+        hipLaunchKernelGGL((k_hzl_exceptions<32>), ge, b, 0, s, enc, chunks, n_chunks, dict);
     } else {
-        hipLaunchKernelGGL((k_hzl_chunks<64>), dim3(blocks), dim3(kBlock), 0, s, enc, chunks, n_chunks, dict);
-        hipLaunchKernelGGL((k_hzl_exceptions<64>), dim3(n_chunks < 1024 ? n_chunks : 1024), dim3(kBlock), 0, s, enc,
-                           chunks, n_chunks, dict);
+        hipLaunchKernelGGL((k_hzl_chunks<64>), g, b, 0, s, enc, chunks, n_chunks, dict);
+        hipLaunchKernelGGL((k_hzl_exceptions<64>), ge, b, 0, s, enc, chunks, n_chunks, dict);
     }
     return hipGetLastError();
 }

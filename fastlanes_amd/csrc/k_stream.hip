@@ -1,4 +1,6 @@
-// k_stream.hip -- raw FFOR / unpack stream kernels, one instantiation per (width, bit width).
+// k_stream.hip -- raw FFOR / unpack stream kernels, one instantiation per (width, bit width, store mode).
+#include <array>
+
 #include "launch.hpp"
 #include "page_kernels.hpp"
 
@@ -6,25 +8,28 @@ namespace fls {
 namespace {
 using StreamFn = void (*)(const uint8_t*, const void*, uint8_t*, uint64_t);
 
-template <int W, int... B>
+template <int W, bool NT, int... B>
 constexpr auto stream_table(std::integer_sequence<int, B...>) {
-    return std::array<StreamFn, sizeof...(B)> {&k_unffor_stream<W, B>...};
+    return std::array<StreamFn, sizeof...(B)> {&k_stream<W, B, NT>...};
 }
-const auto kS8  = stream_table<8>(std::make_integer_sequence<int, 9> {});
-const auto kS16 = stream_table<16>(std::make_integer_sequence<int, 17> {});
-const auto kS32 = stream_table<32>(std::make_integer_sequence<int, 33> {});
-const auto kS64 = stream_table<64>(std::make_integer_sequence<int, 65> {});
+template <int W, bool NT>
+const auto kTab = stream_table<W, NT>(std::make_integer_sequence<int, W + 1> {});
 }  // namespace
 
 hipError_t launch_unffor_stream(unsigned W, unsigned bw, const uint8_t* packed, const void* bases, uint8_t* out,
                                 uint64_t n_vec, hipStream_t s) {
     if (bw > W) return hipErrorInvalidValue;
-    StreamFn f = W == 8 ? kS8[bw] : W == 16 ? kS16[bw] : W == 32 ? kS32[bw] : kS64[bw];
-    const unsigned vpb = W == 64 ? Geo<64>::VPB : Geo<32>::VPB;
     if (n_vec == 0) return hipSuccess;
-    const uint64_t blocks = (n_vec + vpb - 1) / vpb;
-    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(f, dim3((unsigned)blocks), dim3(kBlock), 0, s, packed, bases, out, n_vec);
+    const Tune t = tune_for(FAM_STREAM, W, bw);
+    StreamFn   f = nullptr;
+    switch (W) {
+    case 8: f = t.nt ? kTab<8, true>[bw] : kTab<8, false>[bw]; break;
+    case 16: f = t.nt ? kTab<16, true>[bw] : kTab<16, false>[bw]; break;
+    case 32: f = t.nt ? kTab<32, true>[bw] : kTab<32, false>[bw]; break;
+    case 64: f = t.nt ? kTab<64, true>[bw] : kTab<64, false>[bw]; break;
+    default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(f, dim3(grid_for(n_vec, t, kTPV)), dim3(t.block), 0, s, packed, bases, out, n_vec);
     return hipGetLastError();
 }
 

@@ -1,4 +1,4 @@
-// launch.hpp -- host-side launchers for the decode kernels (one translation unit per kernel family).
+// launch.hpp -- host-side launchers for the decode kernels and the launch-shape tuning table.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -6,6 +6,36 @@
 namespace fls {
 
 struct HzlChunk;
+
+// Kernel families with separately tuned launch shapes.
+enum Family : int {
+    FAM_STREAM = 0,   // raw FFOR / unpack stream (tuned per width and bit width)
+    FAM_PAGE_UNC,     // UNCOMPRESSED pages
+    FAM_PAGE_FFOR,    // FFOR pages
+    FAM_PAGE_DELTA,   // DELTA pages
+    FAM_PAGE_RLE,     // RLE pages (both variants)
+    FAM_PAGE_DICT,    // dictionary pages (all variants)
+    FAM_COUNT
+};
+
+// Launch shape: threads per block (multiple of 64, <= 512), non-temporal stores, and a cap on the grid
+// (0 = one block per block-sized group of vectors; otherwise blocks stride over the vectors).
+struct Tune {
+    uint16_t block      = 256;
+    uint16_t nt         = 1;
+    uint32_t max_blocks = 0;
+};
+Tune tune_for(int family, unsigned W, unsigned bw);
+void tune_set(int family, unsigned W, unsigned bw, Tune t);  // bw 255 = every bit width
+int  tune_load(const char* path);                             // returns entries read, <0 on error
+
+inline unsigned grid_for(uint64_t n_vec, const Tune& t, unsigned tpv) {
+    const uint64_t vpb    = t.block / tpv;
+    uint64_t       blocks = (n_vec + vpb - 1) / vpb;
+    if (t.max_blocks && blocks > t.max_blocks) blocks = t.max_blocks;
+    if (blocks > 0x7fffffffull) blocks = 0x7fffffffull;
+    return (unsigned)(blocks ? blocks : 1);
+}
 
 hipError_t launch_unffor_stream(unsigned W, unsigned bw, const uint8_t* packed, const void* bases, uint8_t* out,
                                 uint64_t n_vec, hipStream_t s);
@@ -17,5 +47,15 @@ hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, uint32_t a
                             const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s);
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
                       hipStream_t s);
+
+// per-width instantiation units (k_page_w*.hip)
+template <int W>
+hipError_t page_int_w(int mode, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out, hipStream_t s);
+template <int W>
+hipError_t page_rle_w(int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out, uint32_t* err,
+                      hipStream_t s);
+template <int W>
+hipError_t page_dict_w(int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, const void* dict, uint32_t dict_n,
+                       uint8_t* out, uint32_t* err, hipStream_t s);
 
 }  // namespace fls

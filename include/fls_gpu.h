@@ -68,6 +68,14 @@ int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, 
 int fls_gpu_unffor_stream(unsigned width, unsigned bw, const void* dev_packed, const void* dev_bases, void* dev_out,
                           uint64_t n_vec, void* stream);
 
+/* ---- launch-shape tuning ------------------------------------------------------------------------------------------
+ * The decoder reads a table of launch shapes (threads per block, non-temporal stores, grid cap) from tuning.txt
+ * next to the library (or the file named by FLS_TUNING) on first use; fastlanes_amd/autotune.py measures and writes
+ * it. family: 0 raw streams, 1 UNCOMPRESSED, 2 FFOR, 3 DELTA, 4 RLE, 5 dictionary pages; bw 255 = all bit widths. */
+int fls_gpu_tune_set(int family, unsigned width, unsigned bw, unsigned block, unsigned nontemporal,
+                     unsigned max_blocks);
+int fls_gpu_tune_load(const char* path);
+
 /* ---- row groups --------------------------------------------------------------------------------------------------
  * Many columns (each an encoded column resident in device memory at its own offset of one buffer) decoded by one
  * persistent launch over a device-side work list of (column, vector range) tiles. Stands in for the per-column
