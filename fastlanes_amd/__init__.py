@@ -154,3 +154,18 @@ def encode_from_memory(values: np.ndarray, expr_id: int, capacity: int | None = 
 def decode_to_memory(encoded_ptr: int, out_ptr: int, dtype) -> None:
     """include/fastlanes.h decode_to_memory (host or device pointers)."""
     lib().decode_to_memory(encoded_ptr, out_ptr, dtype_code(dtype))
+
+
+def encode_strings(offsets: np.ndarray, data: np.ndarray) -> np.ndarray:
+    """FSST-encode a string column given Arrow-style uint32 offsets (n+1) and bytes (fls_encode_strings)."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    n = offsets.size - 1
+    cap = int(offsets[-1]) * 2 + 8 * n + (1 << 16)
+    out = np.empty(cap, dtype=np.uint8)
+    size = _u64(0)
+    L = lib()
+    L.fls_encode_strings.argtypes = [_vp, _vp, _u64, _vp, _u64, C.POINTER(_u64)]
+    check(L.fls_encode_strings(offsets.ctypes.data, data.ctypes.data if data.size else None, n, out.ctypes.data, cap,
+                               C.byref(size)), "encode_strings")
+    return out[: size.value].copy()

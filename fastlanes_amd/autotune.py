@@ -104,7 +104,7 @@ def main():
         rng = np.random.default_rng(3)
         for W in widths:
             dt = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[W]
-            m = min(n, 1 << 24)
+            m = min(n, 1 << 25)
             cases = {
                 FAM_UNC: (0, rng.integers(0, np.iinfo(dt).max, m, dtype=dt, endpoint=True)),
                 FAM_FFOR: (1, rng.integers(0, 1 << min(W - 1, 7), m, dtype=dt)),

@@ -1,0 +1,133 @@
+"""Sub-benchmarks behind `bench.py --detail`: page decode per expression family, the cascade configurations,
+the row-group decode and the encoding wizard. Every figure is a median of timed launches with inputs resident in
+HBM; bytes are the algorithmic bytes (encoded bytes read + decoded bytes written)."""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+import fastlanes_amd as fa
+from fastlanes_amd import workloads as wl
+
+HBM = 8000.0
+
+
+def _events_time(fn, stream, reps=5, warm=2):
+    import torch
+
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b) * 1e-3)
+    return float(np.median(ts))
+
+
+def decode_column_timed(x: np.ndarray, eid: int, dev, stream, check=True):
+    import torch
+
+    enc = fa.encode_from_memory(x, eid)
+    plan = fa.Plan(enc, x.dtype)
+    d_in = torch.from_numpy(enc).to(dev)
+    d_out = torch.empty(x.nbytes + 16, dtype=torch.uint8, device=dev)
+    sp = int(stream.cuda_stream)
+
+    def fn():
+        plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes, sp)
+
+    t = _events_time(fn, stream)
+    plan.check(sp)
+    ok = None
+    if check:
+        got = d_out[: x.nbytes].cpu().numpy().view(x.dtype)
+        ok = bool(np.array_equal(got, x))
+    nbytes = enc.size + x.nbytes
+    return {"values": int(x.size), "encoded_bytes": int(enc.size), "ms": round(t * 1e3, 4),
+            "gvalues_s": round(x.size / t / 1e9, 2), "gb_s": round(nbytes / t / 1e9, 1),
+            "hbm_frac": round(nbytes / t / 1e9 / HBM, 4), "bit_exact": ok, "pages": plan.n_pages}
+
+
+def pages(dev, stream, log2n=26):
+    """Every page family at each element width (bit-exact check against the input)."""
+    n = 1 << log2n
+    rng = np.random.default_rng(1)
+    res = {}
+    for W, dt in ((8, np.uint8), (16, np.uint16), (32, np.uint32), (64, np.uint64)):
+        m = n if W < 64 else n // 2
+        cases = {
+            "uncompressed": (0, rng.integers(0, np.iinfo(dt).max, m, dtype=dt, endpoint=True)),
+            "ffor_w7": (1, rng.integers(0, 1 << min(W - 1, 7), m, dtype=dt)),
+            "delta": (11, np.cumsum(rng.integers(0, 100, m)).astype(dt)),
+            "rle": (6, wl.geometric_runs(m, 16.0, seed=1, value_bits=min(W - 1, 20)).astype(dt)),
+            "dict_ffor": (17, rng.integers(0, 1 << min(W - 1, 12), m).astype(dt)),
+        }
+        for name, (eid, x) in cases.items():
+            res[f"u{W}_{name}"] = decode_column_timed(x, eid, dev, stream)
+            print(f"page u{W} {name}: {res[f'u{W}_{name}']}", flush=True)
+    return res
+
+
+def cascade(dev, stream, log2n=26, log2_strings=24):
+    """Configs: (a) int64 timestamps DELTA+FFOR, (b) uint32 runs RLE, (c) int32 Zipf dictionary codes, (d) FSST."""
+    import torch
+
+    n = 1 << log2n
+    out = {}
+    out["a_timestamps_delta"] = decode_column_timed(wl.timestamps(n), 11, dev, stream)
+    out["b_runs_rle"] = decode_column_timed(wl.geometric_runs(n, 16.0, seed=4), 6, dev, stream)
+    out["c_zipf_dict"] = decode_column_timed(wl.zipf_codes(n, 4096, 1.1, seed=5), 17, dev, stream)
+    # (d) strings
+    ns = 1 << log2_strings
+    offs, data = wl.vocab_strings_fast(ns, seed=7)
+    enc = fa.encode_strings(offs, data)
+    plan = fa.Plan(enc, "str")
+    d_in = torch.from_numpy(enc).to(dev)
+    d_off = torch.empty(ns + 1, dtype=torch.int64, device=dev)
+    d_bytes = torch.empty(data.size + 64, dtype=torch.uint8, device=dev)
+    sp = int(stream.cuda_stream)
+
+    def fn():
+        plan.string_offsets(d_in.data_ptr(), d_off.data_ptr(), sp)
+        plan.decode_strings(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size, sp)
+
+    t = _events_time(fn, stream, reps=3)
+    ok = bool(np.array_equal(d_bytes[: data.size].cpu().numpy(), data)) and bool(
+        np.array_equal(d_off.cpu().numpy(), offs.astype(np.int64)))
+    nbytes = enc.size + data.size + 8 * (ns + 1)
+    out["d_strings_fsst"] = {"strings": ns, "raw_bytes": int(data.size), "encoded_bytes": int(enc.size),
+                             "ms": round(t * 1e3, 3), "gstrings_s": round(ns / t / 1e9, 3),
+                             "decoded_gb_s": round(data.size / t / 1e9, 1), "gb_s": round(nbytes / t / 1e9, 1),
+                             "hbm_frac": round(nbytes / t / 1e9 / HBM, 4), "bit_exact": ok}
+    for k, v in out.items():
+        print(f"cascade {k}: {v}", flush=True)
+    return out
+
+
+def run_all(dev):
+    import torch
+
+    stream = torch.cuda.current_stream()
+    res = {}
+    t0 = time.time()
+    res["pages"] = pages(dev, stream)
+    res["cascade"] = cascade(dev, stream)
+    try:
+        from fastlanes_amd import rowgroup_bench
+
+        res["rowgroup"] = rowgroup_bench.run(dev, stream)
+    except ImportError:
+        pass
+    try:
+        from fastlanes_amd import wizard
+
+        res["wizard"] = wizard.bench(dev)
+    except ImportError:
+        pass
+    res["seconds"] = round(time.time() - t0, 1)
+    return res

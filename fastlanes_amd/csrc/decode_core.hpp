@@ -65,6 +65,26 @@ __device__ __forceinline__ void st8(uint8_t* p, v2u x) {
     else *reinterpret_cast<v2u*>(p) = x;
 }
 
+// Output sinks for the lane-column (page) bodies: byte offsets are relative to the vector's first value.
+// LdsSink stages a vector in LDS with 16 bytes of padding per 256 (conflict-free 16-byte lane writes); the kernel
+// then writes whole lines to HBM. GlobalSink writes in place.
+__device__ __forceinline__ uint32_t lds_pad(uint32_t off) { return off + (off >> 8) * 16; }
+struct LdsSink {
+    uint8_t* base;
+    __device__ __forceinline__ void put16(uint32_t off, v4u x) const { *reinterpret_cast<v4u*>(base + lds_pad(off)) = x; }
+    __device__ __forceinline__ void put8(uint32_t off, v2u x) const { *reinterpret_cast<v2u*>(base + lds_pad(off)) = x; }
+    template <typename T>
+    __device__ __forceinline__ void put(uint32_t off, T x) const { *reinterpret_cast<T*>(base + lds_pad(off)) = x; }
+};
+template <bool NT>
+struct GlobalSink {
+    uint8_t* base;
+    __device__ __forceinline__ void put16(uint32_t off, v4u x) const { st16<NT>(base + off, x); }
+    __device__ __forceinline__ void put8(uint32_t off, v2u x) const { st8<NT>(base + off, x); }
+    template <typename T>
+    __device__ __forceinline__ void put(uint32_t off, T x) const { *reinterpret_cast<T*>(base + off) = x; }
+};
+
 template <int BW>
 __device__ __forceinline__ void load_rows(const uint8_t* packed, unsigned q, v4u (&R)[BW > 0 ? BW : 1]) {
     static_for<BW>([&](auto w) { R[w] = ld16(packed + 128 * w + 16 * q); });
@@ -159,30 +179,30 @@ __device__ __forceinline__ void lanes_p32(const uint8_t* packed, uint32_t base, 
 }
 
 // Stores the untransposed segments of a 32-bit column's lanes. out points at the vector's first element.
-template <int W, bool NT>
-__device__ __forceinline__ void store_segments32(const uint32_t (&V)[W], unsigned t, uint8_t* out) {
+template <int W, typename S>
+__device__ __forceinline__ void store_segments32(const uint32_t (&V)[W], unsigned t, const S& out) {
     if constexpr (W == 32) {
-        uint8_t* o = out + 4 * seg_start<32>(t);
-        static_for<8>([&](auto q) { st16<NT>(o + 16 * q, v4u {V[4 * q], V[4 * q + 1], V[4 * q + 2], V[4 * q + 3]}); });
+        const uint32_t o = 4 * seg_start<32>(t);
+        static_for<8>([&](auto q) { out.put16(o + 16 * q, v4u {V[4 * q], V[4 * q + 1], V[4 * q + 2], V[4 * q + 3]}); });
     } else if constexpr (W == 16) {
         static_for<2>([&](auto l) {
-            uint8_t* o = out + 2 * seg_start<16>(2 * t + l);
+            const uint32_t o = 2 * seg_start<16>(2 * t + l);
             static_for<2>([&](auto q) {
-                st16<NT>(o + 16 * q, v4u {gather_half<l>(V[8 * q + 0], V[8 * q + 1]), gather_half<l>(V[8 * q + 2], V[8 * q + 3]),
-                                          gather_half<l>(V[8 * q + 4], V[8 * q + 5]), gather_half<l>(V[8 * q + 6], V[8 * q + 7])});
+                out.put16(o + 16 * q, v4u {gather_half<l>(V[8 * q + 0], V[8 * q + 1]), gather_half<l>(V[8 * q + 2], V[8 * q + 3]),
+                                           gather_half<l>(V[8 * q + 4], V[8 * q + 5]), gather_half<l>(V[8 * q + 6], V[8 * q + 7])});
             });
         });
     } else {
         static_for<4>([&](auto l) {
-            st8<NT>(out + seg_start<8>(4 * t + l), v2u {gather_byte<l>(V[0], V[1], V[2], V[3]), gather_byte<l>(V[4], V[5], V[6], V[7])});
+            out.put8(seg_start<8>(4 * t + l), v2u {gather_byte<l>(V[0], V[1], V[2], V[3]), gather_byte<l>(V[4], V[5], V[6], V[7])});
         });
     }
 }
 
 // W = 64: one lane per thread, values produced and stored in p order.
-template <int BW, bool DELTA, bool NT>
+template <int BW, bool DELTA, typename S>
 __device__ __forceinline__ void decode64(const uint8_t* packed, uint64_t base, uint64_t lane_base, unsigned t,
-                                         uint8_t* out) {
+                                         const S& out) {
     uint64_t R[BW > 0 ? BW : 1];
     lane_rows64<BW>(packed, t, R);
     uint64_t acc = lane_base;
@@ -194,7 +214,7 @@ __device__ __forceinline__ void decode64(const uint8_t* packed, uint64_t base, u
             v = acc;
         }
         if constexpr (p % 2 == 0) prev = v;
-        else st16<NT>(out + 8 * (64 * t + p - 1), v4u {(uint32_t)prev, (uint32_t)(prev >> 32), (uint32_t)v, (uint32_t)(v >> 32)});
+        else out.put16(8 * (64 * t + p - 1), v4u {(uint32_t)prev, (uint32_t)(prev >> 32), (uint32_t)v, (uint32_t)(v >> 32)});
     });
 }
 
@@ -209,9 +229,9 @@ __device__ __forceinline__ T load_t(const uint8_t* p) {
     else return *reinterpret_cast<const u64_ua*>(p);
 }
 
-// Writes N consecutive T values to out; ALIGN is the byte alignment guaranteed for out.
-template <typename T, int N, int ALIGN, bool NT>
-__device__ __forceinline__ void store_run(const T (&vals)[N], uint8_t* out) {
+// Writes N consecutive T values at byte offset off of the sink; ALIGN is the alignment guaranteed for off.
+template <typename T, int N, int ALIGN, typename S>
+__device__ __forceinline__ void store_run(const T (&vals)[N], uint32_t off, const S& out) {
     constexpr int bytes = ALIGN < 16 ? ALIGN : 16;
     constexpr int per   = bytes / (int)sizeof(T);
     if constexpr (bytes == 16 && (N * sizeof(T)) % 16 == 0) {
@@ -221,7 +241,7 @@ __device__ __forceinline__ void store_run(const T (&vals)[N], uint8_t* out) {
                 v4u u;
             } b;
             static_for<per>([&](auto e) { b.e[e] = vals[k * per + e]; });
-            st16<NT>(out + 16 * k, b.u);
+            out.put16(off + 16 * k, b.u);
         });
     } else if constexpr (bytes == 8 && (N * sizeof(T)) % 8 == 0) {
         static_for<N / per>([&](auto k) {
@@ -230,10 +250,10 @@ __device__ __forceinline__ void store_run(const T (&vals)[N], uint8_t* out) {
                 v2u u;
             } b;
             static_for<per>([&](auto e) { b.e[e] = vals[k * per + e]; });
-            st8<NT>(out + 8 * k, b.u);
+            out.put8(off + 8 * k, b.u);
         });
     } else {
-        static_for<N>([&](auto k) { reinterpret_cast<T*>(out)[k] = vals[k]; });
+        static_for<N>([&](auto k) { out.template put<T>(off + sizeof(T) * k, vals[k]); });
     }
 }
 

@@ -2,6 +2,7 @@
 #include "format.hpp"
 
 #include <algorithm>
+#include <thread>
 #include <unordered_map>
 
 namespace fls {
@@ -441,29 +442,27 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
         for (uint32_t c = 0; c < order.size(); ++c) remap[order[c]] = c, dvals[c] = (uint64_t)ents[order[c]].first;
         for (auto& c : codes) c = remap[c];
     }
-    const uint64_t page_c = n_vec ? (n_vec + kMaxVecPerPage - 1) / kMaxVecPerPage : 1;
-    const uint64_t per_pg = n_vec ? (n_vec + page_c - 1) / page_c : 0;
-    uint64_t       v      = 0;
-    constexpr unsigned W  = bits<T>();
-    std::vector<uint8_t> b0(8192), b1(8192), b2(8192), b3(8192), b4(8192);
-    T                    t[1024], d[1024];
-    uint32_t             ct[1024], cd[1024];
-    uint8_t              bwb;
-    uint64_t             baseb;
-    bool                 first_page = true;
-    for (;;) {
-        PageW pg;
-        pg.arr_c = e.arr_c;
-        const uint64_t stop = std::min(n_vec, v + per_pg);
-        for (; v < stop; ++v) {
+    // 1) per-vector encodes, independent of each other: run on all host threads
+    struct VecOut {
+        std::vector<uint8_t>  bytes;
+        std::vector<uint64_t> sizes;
+    };
+    std::vector<VecOut> vo(n_vec);
+    auto encode_range = [&](uint64_t v0, uint64_t v1) {
+        std::vector<uint8_t> b0(8192), b1(8192), b2(8192), b3(8192), b4(8192);
+        T                    t[1024], d[1024];
+        uint32_t             ct[1024], cd[1024];
+        uint8_t              bwb;
+        uint64_t             baseb;
+        for (uint64_t v = v0; v < v1; ++v) {
             const T* x = in + v * 1024;
             std::vector<std::pair<const uint8_t*, uint64_t>> parts(e.arr_c);
             auto ffor_parts = [&](auto* src, unsigned o, uint8_t* packed) {
                 using S = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
-                S              base;
-                const uint8_t  bw = analyze<S>(src, base);
-                bwb               = bw;
-                baseb             = 0;
+                S             base;
+                const uint8_t bw = analyze<S>(src, base);
+                bwb              = bw;
+                baseb            = 0;
                 std::memcpy(&baseb, &base, sizeof(S));
                 ffor<S>(src, packed, bw, base);
                 parts[o]     = {&bwb, 1};
@@ -524,10 +523,34 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
                 }
             }
             }
-            uint64_t add = 4ull * e.arr_c;
-            for (auto& pp : parts) add += pp.second;
+            VecOut& o = vo[v];
+            for (auto& pp : parts) o.bytes.insert(o.bytes.end(), pp.first, pp.first + pp.second), o.sizes.push_back(pp.second);
+        }
+    };
+    {
+        const unsigned hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+        const unsigned nt = n_vec < 64 ? 1 : hw;
+        std::vector<std::thread> th;
+        for (unsigned k = 0; k < nt; ++k) th.emplace_back(encode_range, n_vec * k / nt, n_vec * (k + 1) / nt);
+        for (auto& x : th) x.join();
+    }
+    // 2) sequential page assembly (Page::Absorb order)
+    const uint64_t page_c = n_vec ? (n_vec + kMaxVecPerPage - 1) / kMaxVecPerPage : 1;
+    const uint64_t per_pg = n_vec ? (n_vec + page_c - 1) / page_c : 0;
+    uint64_t       v      = 0;
+    constexpr unsigned W  = bits<T>();
+    bool           first_page = true;
+    for (;;) {
+        PageW pg;
+        pg.arr_c = e.arr_c;
+        const uint64_t stop = std::min(n_vec, v + per_pg);
+        for (; v < stop; ++v) {
+            std::vector<std::pair<const uint8_t*, uint64_t>> parts(e.arr_c);
+            uint64_t off = 0, add = 4ull * e.arr_c;
+            for (unsigned i = 0; i < e.arr_c; ++i) parts[i] = {vo[v].bytes.data() + off, vo[v].sizes[i]}, off += vo[v].sizes[i], add += vo[v].sizes[i];
             if (pg.vec_c && pg.size() + add >= kDataPageMax) break;  // TryFlushVec limit (compressor.cpp:47)
             pg.absorb(parts);
+            std::vector<uint8_t>().swap(vo[v].bytes);
         }
         const bool more = v < n_vec;
         if (e.dict) {

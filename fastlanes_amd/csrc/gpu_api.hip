@@ -128,6 +128,7 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
 
 int fls_gpu_plan_check(fls_gpu_plan* plan, void* stream) {
     if (!plan) return fail(FLS_E_ARG, "null plan");
+    if (plan->col.dtype == STR) return strings_check(plan->strings, (hipStream_t)stream);
     uint32_t   h = 0;
     hipError_t e = hipMemcpyAsync(&h, plan->d_err, 4, hipMemcpyDeviceToHost, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
@@ -149,6 +150,16 @@ int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, 
     if (plan->col.dtype != STR) return fail(FLS_E_ARG, "not a string column");
     return strings_decode(plan->col, plan->strings, dev_encoded, dev_offsets, dev_bytes, bytes_capacity,
                           (hipStream_t)stream);
+}
+
+int fls_encode_strings(const uint32_t* offsets, const uint8_t* bytes, uint64_t n, uint8_t* out, uint64_t capacity,
+                       uint64_t* out_size) {
+    if (!offsets || (!bytes && n) || !out || !out_size) return fail(FLS_E_ARG, "null argument");
+    const int64_t r = encode_str_column(13, offsets, bytes, n, out, capacity);
+    if (r == -2) return fail(FLS_E_CAPACITY, "encoded column exceeds capacity");
+    if (r < 0) return fail(FLS_E_UNSUPPORTED, "string count must be a multiple of 1024");
+    *out_size = (uint64_t)r;
+    return FLS_OK;
 }
 
 int fls_gpu_tune_set(int family, unsigned width, unsigned bw, unsigned block, unsigned nontemporal,

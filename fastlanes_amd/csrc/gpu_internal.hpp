@@ -20,13 +20,15 @@ struct FsstTableDev {
 
 struct StringPlan {
     FsstTableDev* d_tables  = nullptr;  // one per page entry
-    uint64_t*     d_vec_tot = nullptr;  // decoded bytes per vector, then exclusive prefix
+    uint64_t*     d_vec_tot = nullptr;  // decoded bytes per half-vector segment, then their exclusive prefix
+    uint32_t*     d_err     = nullptr;  // set when string offsets exceed the output capacity
     uint64_t      n_vec     = 0;
 };
 
 int  strings_prepare(const Column& col, const uint8_t* host, StringPlan& sp);
 void strings_release(StringPlan& sp);
 int  strings_offsets(const Column& col, StringPlan& sp, const uint8_t* dev_enc, uint64_t* dev_offsets, hipStream_t s);
+int  strings_check(StringPlan& sp, hipStream_t s);
 int  strings_decode(const Column& col, StringPlan& sp, const uint8_t* dev_enc, const uint64_t* dev_offsets,
                     uint8_t* dev_bytes, uint64_t cap, hipStream_t s);
 

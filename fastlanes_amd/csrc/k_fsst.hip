@@ -6,7 +6,10 @@
 // with fsst_decompress (include/fls/cor/prm/fsst/fsst.h:152-258): code < 255 emits symbol[code] (len[code] bytes),
 // code 255 emits the next byte literally.
 //
-// One 256-thread block per vector, 4 strings per thread; symbol table in LDS.
+// A block of 512 threads owns one half-vector segment (512 strings, one per thread). The segment's compressed
+// bytes are staged in LDS with a coalesced copy, strings are decoded from LDS into an LDS output buffer, and the
+// decoded segment is written out with consecutive lanes storing consecutive bytes. Segments larger than the LDS
+// budget decode straight from / to global memory.
 #include "../../include/fls_gpu.h"
 #include "gpu_internal.hpp"
 #include "page_kernels.hpp"
@@ -15,8 +18,10 @@ namespace fls {
 
 namespace {
 
-constexpr unsigned kBlock        = 256;
-constexpr unsigned kStrPerThread = 1024 / kBlock;
+constexpr unsigned kSeg      = 512;        // strings per segment (= threads per block)
+constexpr unsigned kSegPerV  = 1024 / kSeg;
+constexpr uint32_t kCompLds  = 24 * 1024;  // staged compressed bytes per segment
+constexpr uint32_t kOutLds   = 32 * 1024;  // staged decoded bytes per segment
 
 // transposed index of original position i (inverse of untranspose_i): i = r + 8*ORDER[b] + 64c -> 128r + 16b + c
 __device__ __forceinline__ unsigned transposed_of(unsigned i) {
@@ -24,55 +29,76 @@ __device__ __forceinline__ unsigned transposed_of(unsigned i) {
     return 128 * r + 16 * b + c;
 }
 
-__device__ __forceinline__ void string_range(const uint8_t* ends, unsigned i, uint32_t& s, uint32_t& e) {
-    const u32_ua* E = reinterpret_cast<const u32_ua*>(ends);
-    e               = E[transposed_of(i)];
-    s               = i ? E[transposed_of(i - 1)] : 0u;
-    if (s > e) s = e;  // malformed offsets decode to an empty string
+__device__ __forceinline__ uint32_t end_of(const uint8_t* ends, int i) {  // end offset of string i, 0 for i = -1
+    return i < 0 ? 0u : reinterpret_cast<const u32_ua*>(ends)[transposed_of((unsigned)i)];
 }
 
-__global__ __launch_bounds__(kBlock) void k_fsst_len(const uint8_t* __restrict__ page, uint32_t vec_c,
-                                                     const FsstTableDev* __restrict__ table, uint64_t first_string,
-                                                     uint64_t* __restrict__ offsets, uint64_t* __restrict__ vec_tot,
-                                                     uint64_t first_vec) {
-    __shared__ uint8_t  len[256];
-    __shared__ uint64_t part[kBlock];
-    const unsigned      v = blockIdx.x, t = threadIdx.x;
-    if (t < 256) len[t] = table->len[t];
+// Exclusive scan of one u64 per thread over a block of kSeg threads; returns the block total in *total.
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t x, uint64_t* total) {
+    __shared__ uint64_t wsum[kSeg / 64];
+    const unsigned      lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    uint64_t            inc  = x;
+    for (unsigned d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+    }
+    if (lane == 63) wsum[w] = inc;
     __syncthreads();
+    uint64_t before = 0, all = 0;
+    for (unsigned k = 0; k < kSeg / 64; ++k) {
+        const uint64_t s = wsum[k];
+        if (k < w) before += s;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - x;
+}
+
+// Stages compressed bytes [s0, s1) of a segment into LDS; returns whether they fit.
+__device__ __forceinline__ bool stage_bytes(const uint8_t* cb, uint32_t s0, uint32_t s1, uint8_t* lds) {
+    const uint32_t n = s1 - s0;
+    if (n > kCompLds) return false;
+    for (uint32_t k = threadIdx.x; k < n; k += kSeg) lds[k] = cb[s0 + k];
+    return true;
+}
+
+__global__ __launch_bounds__(kSeg) void k_fsst_len(const uint8_t* __restrict__ page, const FsstTableDev* __restrict__ table,
+                                                   uint64_t first_string, uint64_t* __restrict__ offsets,
+                                                   uint64_t* __restrict__ seg_tot, uint64_t first_seg) {
+    __shared__ uint8_t len[256];
+    __shared__ uint8_t cbuf[kCompLds];
+    const unsigned     v = blockIdx.x / kSegPerV, h = blockIdx.x % kSegPerV, i = h * kSeg + threadIdx.x;
+    if (threadIdx.x < 256) len[threadIdx.x] = table->len[threadIdx.x];
     const uint8_t* cb   = vbuf(page, 2, v, 0);
     const uint8_t* ends = vbuf(page, 2, v, 1);
-    uint64_t       l[kStrPerThread], run = 0;
-    for (unsigned k = 0; k < kStrPerThread; ++k) {
-        const unsigned i = t * kStrPerThread + k;
-        uint32_t       s, e;
-        string_range(ends, i, s, e);
-        uint64_t n = 0;
+    const uint32_t s0 = end_of(ends, (int)(h * kSeg) - 1), s1 = end_of(ends, (int)(h * kSeg + kSeg) - 1);
+    const bool     in_lds = stage_bytes(cb, s0, s1 > s0 ? s1 : s0, cbuf);
+    __syncthreads();
+    uint32_t s = end_of(ends, (int)i - 1), e = end_of(ends, (int)i);
+    if (s > e) s = e;  // malformed offsets decode to an empty string
+    uint64_t n = 0;
+    if (in_lds && s >= s0 && e <= s1) {
+        for (uint32_t q = s - s0; q < e - s0;) {
+            const unsigned c = cbuf[q];
+            if (c == 255) n += 1, q += 2;
+            else n += len[c], q += 1;
+        }
+    } else {
         for (uint32_t q = s; q < e;) {
             const unsigned c = cb[q];
             if (c == 255) n += 1, q += 2;
             else n += len[c], q += 1;
         }
-        run += n;
-        l[k] = run;
     }
-    // block exclusive scan of per-thread totals
-    part[t] = run;
-    __syncthreads();
-    for (unsigned d = 1; d < kBlock; d <<= 1) {
-        const uint64_t x = t >= d ? part[t - d] : 0;
-        __syncthreads();
-        part[t] += x;
-        __syncthreads();
-    }
-    const uint64_t before = t ? part[t - 1] : 0;
-    uint64_t*      o      = offsets + first_string + (uint64_t)v * 1024 + 1;
-    for (unsigned k = 0; k < kStrPerThread; ++k) o[t * kStrPerThread + k] = before + l[k];
-    if (t == kBlock - 1) vec_tot[first_vec + v] = part[kBlock - 1];
+    uint64_t total;
+    const uint64_t before = block_exclusive_scan(n, &total);
+    offsets[first_string + (uint64_t)v * 1024 + i + 1] = before + n;  // relative to the segment; fixed below
+    if (threadIdx.x == 0) seg_tot[first_seg + blockIdx.x] = total;
 }
 
-// Single-block exclusive scan over the per-vector totals (n <= a few million).
-__global__ __launch_bounds__(1024) void k_scan_vec(uint64_t* __restrict__ tot, uint64_t n, uint64_t* __restrict__ offsets) {
+// Single-block exclusive scan over the per-segment totals.
+__global__ __launch_bounds__(1024) void k_scan_seg(uint64_t* __restrict__ tot, uint64_t n, uint64_t* __restrict__ offsets) {
     __shared__ uint64_t part[1024];
     const unsigned      t     = threadIdx.x;
     const uint64_t      chunk = (n + 1023) / 1024, a = t * chunk, b = a + chunk < n ? a + chunk : n;
@@ -95,37 +121,51 @@ __global__ __launch_bounds__(1024) void k_scan_vec(uint64_t* __restrict__ tot, u
     if (t == 0) offsets[0] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_fsst_fix(uint64_t* __restrict__ offsets, const uint64_t* __restrict__ vbase,
-                                                     uint64_t n_vec) {
-    const uint64_t v = blockIdx.x;
-    if (v >= n_vec) return;
-    const uint64_t b = vbase[v];
-    for (unsigned k = threadIdx.x; k < 1024; k += kBlock) offsets[v * 1024 + 1 + k] += b;
+__global__ __launch_bounds__(kSeg) void k_fsst_fix(uint64_t* __restrict__ offsets, const uint64_t* __restrict__ sbase,
+                                                   uint64_t n_seg) {
+    const uint64_t g = blockIdx.x;
+    if (g < n_seg) offsets[g * kSeg + 1 + threadIdx.x] += sbase[g];
 }
 
-__global__ __launch_bounds__(kBlock) void k_fsst_decode(const uint8_t* __restrict__ page, uint32_t vec_c,
-                                                        const FsstTableDev* __restrict__ table, uint64_t first_string,
-                                                        const uint64_t* __restrict__ offsets, uint8_t* __restrict__ out,
-                                                        uint64_t cap, uint32_t* err) {
+__global__ __launch_bounds__(kSeg) void k_fsst_decode(const uint8_t* __restrict__ page, const FsstTableDev* __restrict__ table,
+                                                      uint64_t first_string, const uint64_t* __restrict__ offsets,
+                                                      uint8_t* __restrict__ out, uint64_t cap, uint32_t* err) {
     __shared__ uint64_t sym[256];
     __shared__ uint8_t  len[256];
-    const unsigned      v = blockIdx.x, t = threadIdx.x;
-    sym[t] = table->symbol[t];
-    len[t] = table->len[t];
-    __syncthreads();
+    __shared__ uint8_t  cbuf[kCompLds];
+    __shared__ uint8_t  obuf[kOutLds];
+    const unsigned      v = blockIdx.x / kSegPerV, h = blockIdx.x % kSegPerV, i = h * kSeg + threadIdx.x;
+    if (threadIdx.x < 256) sym[threadIdx.x] = table->symbol[threadIdx.x], len[threadIdx.x] = table->len[threadIdx.x];
     const uint8_t*  cb   = vbuf(page, 2, v, 0);
     const uint8_t*  ends = vbuf(page, 2, v, 1);
     const uint64_t* o    = offsets + first_string + (uint64_t)v * 1024;
-    for (unsigned k = 0; k < kStrPerThread; ++k) {
-        const unsigned i = t * kStrPerThread + k;
-        uint32_t       s, e;
-        string_range(ends, i, s, e);
-        uint64_t       w = o[i];
-        const uint64_t stop = o[i + 1];
-        if (stop > cap) {
-            atomicOr(err, 4u);
-            continue;
+    const uint32_t  s0 = end_of(ends, (int)(h * kSeg) - 1), s1 = end_of(ends, (int)(h * kSeg + kSeg) - 1);
+    const uint64_t  o0 = o[h * kSeg], o1 = o[h * kSeg + kSeg];
+    const bool      in_lds  = stage_bytes(cb, s0, s1 > s0 ? s1 : s0, cbuf);
+    const bool      out_lds = in_lds && o1 >= o0 && o1 - o0 <= kOutLds && o1 <= cap;
+    __syncthreads();
+    uint32_t s = end_of(ends, (int)i - 1), e = end_of(ends, (int)i);
+    if (s > e) s = e;
+    const uint64_t w0 = o[i], stop = o[i + 1];
+    if (stop > cap || stop < w0) {
+        atomicOr(err, 4u);
+    } else if (out_lds && s >= s0 && e <= s1) {
+        uint32_t w = (uint32_t)(w0 - o0);
+        const uint32_t wend = (uint32_t)(stop - o0);
+        for (uint32_t q = s - s0; q < e - s0 && w < wend;) {
+            const unsigned c = cbuf[q];
+            if (c == 255) {
+                obuf[w++] = q + 1 < e - s0 ? cbuf[q + 1] : 0;
+                q += 2;
+            } else {
+                const uint64_t x = sym[c];
+                const unsigned n = len[c];
+                for (unsigned b = 0; b < n && w < wend; ++b) obuf[w++] = (uint8_t)(x >> (8 * b));
+                q += 1;
+            }
         }
+    } else {
+        uint64_t w = w0;
         for (uint32_t q = s; q < e && w < stop;) {
             const unsigned c = cb[q];
             if (c == 255) {
@@ -138,6 +178,11 @@ __global__ __launch_bounds__(kBlock) void k_fsst_decode(const uint8_t* __restric
                 q += 1;
             }
         }
+    }
+    if (out_lds) {
+        __syncthreads();
+        const uint32_t n = (uint32_t)(o1 - o0);
+        for (uint32_t k = threadIdx.x; k < n; k += kSeg) out[o0 + k] = obuf[k];
     }
 }
 
@@ -156,13 +201,17 @@ int strings_prepare(const Column& col, const uint8_t* host, StringPlan& sp) {
     if ((e = hipMalloc(&sp.d_tables, tabs.size() * sizeof(FsstTableDev) + 1)) != hipSuccess) return hip_fail(e, "hipMalloc");
     if ((e = hipMemcpy(sp.d_tables, tabs.data(), tabs.size() * sizeof(FsstTableDev), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "hipMemcpy");
-    if ((e = hipMalloc(&sp.d_vec_tot, (sp.n_vec + 1) * sizeof(uint64_t))) != hipSuccess) return hip_fail(e, "hipMalloc");
+    if ((e = hipMalloc(&sp.d_vec_tot, (sp.n_vec * kSegPerV + 1) * sizeof(uint64_t))) != hipSuccess)
+        return hip_fail(e, "hipMalloc");
+    if ((e = hipMalloc(&sp.d_err, 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
+    if ((e = hipMemset(sp.d_err, 0, 4)) != hipSuccess) return hip_fail(e, "hipMemset");
     return FLS_OK;
 }
 
 void strings_release(StringPlan& sp) {
     if (sp.d_tables) (void)hipFree(sp.d_tables);
     if (sp.d_vec_tot) (void)hipFree(sp.d_vec_tot);
+    if (sp.d_err) (void)hipFree(sp.d_err);
     sp = StringPlan {};
 }
 
@@ -170,12 +219,13 @@ int strings_offsets(const Column& col, StringPlan& sp, const uint8_t* dev_enc, u
     for (size_t i = 0; i < col.pages.size(); ++i) {
         const PageEntry& pe = col.pages[i];
         if (!pe.vec_c) continue;
-        hipLaunchKernelGGL(k_fsst_len, dim3(pe.vec_c), dim3(kBlock), 0, s, dev_enc + pe.page_off, pe.vec_c,
-                           sp.d_tables + i, pe.out_index, dev_offsets, sp.d_vec_tot, pe.out_index / 1024);
+        hipLaunchKernelGGL(k_fsst_len, dim3(pe.vec_c * kSegPerV), dim3(kSeg), 0, s, dev_enc + pe.page_off, sp.d_tables + i,
+                           pe.out_index, dev_offsets, sp.d_vec_tot, (pe.out_index / 1024) * kSegPerV);
     }
-    if (sp.n_vec) {
-        hipLaunchKernelGGL(k_scan_vec, dim3(1), dim3(1024), 0, s, sp.d_vec_tot, sp.n_vec, dev_offsets);
-        hipLaunchKernelGGL(k_fsst_fix, dim3((unsigned)sp.n_vec), dim3(kBlock), 0, s, dev_offsets, sp.d_vec_tot, sp.n_vec);
+    const uint64_t n_seg = sp.n_vec * kSegPerV;
+    if (n_seg) {
+        hipLaunchKernelGGL(k_scan_seg, dim3(1), dim3(1024), 0, s, sp.d_vec_tot, n_seg, dev_offsets);
+        hipLaunchKernelGGL(k_fsst_fix, dim3((unsigned)n_seg), dim3(kSeg), 0, s, dev_offsets, sp.d_vec_tot, n_seg);
     } else {
         hipError_t e = hipMemsetAsync(dev_offsets, 0, 8, s);
         if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
@@ -186,21 +236,22 @@ int strings_offsets(const Column& col, StringPlan& sp, const uint8_t* dev_enc, u
 
 int strings_decode(const Column& col, StringPlan& sp, const uint8_t* dev_enc, const uint64_t* dev_offsets,
                    uint8_t* dev_bytes, uint64_t cap, hipStream_t s) {
-    static uint32_t* d_err = nullptr;  // error word for string pages (never read back on the hot path)
-    if (!d_err) {
-        hipError_t e = hipMalloc(&d_err, 4);
-        if (e != hipSuccess) return hip_fail(e, "hipMalloc");
-        (void)hipMemset(d_err, 0, 4);
-    }
     for (size_t i = 0; i < col.pages.size(); ++i) {
         const PageEntry& pe = col.pages[i];
         if (!pe.vec_c) continue;
-        hipLaunchKernelGGL(k_fsst_decode, dim3(pe.vec_c), dim3(kBlock), 0, s, dev_enc + pe.page_off, pe.vec_c,
-                           sp.d_tables + i, pe.out_index, dev_offsets, dev_bytes, cap, d_err);
+        hipLaunchKernelGGL(k_fsst_decode, dim3(pe.vec_c * kSegPerV), dim3(kSeg), 0, s, dev_enc + pe.page_off,
+                           sp.d_tables + i, pe.out_index, dev_offsets, dev_bytes, cap, sp.d_err);
     }
     hipError_t e = hipGetLastError();
-    (void)sp;
     return e == hipSuccess ? FLS_OK : hip_fail(e, "kernel launch");
+}
+
+int strings_check(StringPlan& sp, hipStream_t s) {
+    uint32_t   h = 0;
+    hipError_t e = hipMemcpyAsync(&h, sp.d_err, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "check");
+    return h ? fail(FLS_E_DATA, "string offsets exceed the output capacity") : FLS_OK;
 }
 
 }  // namespace fls

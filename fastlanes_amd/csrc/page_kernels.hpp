@@ -54,15 +54,59 @@ __global__ __launch_bounds__(kMaxBlock) void k_stream(const uint8_t* __restrict_
 }
 
 // One vector of an integer page, untransposed: FFOR / UNCOMPRESSED (BW = W, base 0) / DELTA.
-template <int W, int BW, bool DELTA, bool NT>
+template <int W, int BW, bool DELTA, typename S>
 __device__ __forceinline__ void ffor_untransposed(const uint8_t* pk, uint64_t base, uint64_t lane_base, unsigned t,
-                                                  uint8_t* dst) {
+                                                  const S& dst) {
     if constexpr (W == 64) {
-        decode64<BW, DELTA, NT>(pk, base, lane_base, t, dst);
+        decode64<BW, DELTA>(pk, base, lane_base, t, dst);
     } else {
         uint32_t V[W];
         lanes_p32<W, BW, DELTA>(pk, (uint32_t)base, (uint32_t)lane_base, t, V);
-        store_segments32<W, NT>(V, t, dst);
+        store_segments32<W>(V, t, dst);
+    }
+}
+
+// Per-wave LDS staging of decoded vectors. A wave decodes VPW = 64 / TPV vectors (one per TPV-thread group) into
+// LDS, then all 64 lanes write them to HBM as consecutive 16-byte chunks: every store instruction writes whole
+// 128-byte lines, which the lane-segment layout alone cannot do.
+template <unsigned TPV, unsigned VB>
+struct WaveStage {
+    static constexpr unsigned VPW   = 64 / TPV;
+    static constexpr unsigned PVB   = VB + VB / 16;  // padded bytes per vector
+    static constexpr unsigned BYTES = VPW * PVB;     // LDS bytes per wave
+};
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <unsigned TPV, unsigned VB, bool NT>
+__device__ __forceinline__ void wave_copy_out(const uint8_t* wl, uint8_t* dst, uint32_t n_valid, unsigned lane) {
+    using WS              = WaveStage<TPV, VB>;
+    constexpr unsigned CV = VB / 16;  // 16-byte chunks per vector
+    for (uint32_t c = lane; c < n_valid * CV; c += 64) {
+        const uint32_t slot = c / CV, off = (c % CV) * 16;
+        st16<NT>(dst + slot * VB + off, *reinterpret_cast<const v4u*>(wl + slot * WS::PVB + lds_pad(off)));
+    }
+}
+
+// Loop over the kernel's vectors wave by wave; body(v, t, sink) decodes vector v (< N) into the sink.
+template <unsigned TPV, unsigned VB, bool NT, typename F>
+__device__ __forceinline__ void staged_vector_loop(uint64_t N, uint8_t* out, F&& body) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
+    using WS              = WaveStage<TPV, VB>;
+    const unsigned lane   = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
+    const unsigned t      = lane % TPV, slot = lane / TPV;
+    uint8_t*       wl     = fls_lds + wave * WS::BYTES;
+    const LdsSink  sink {wl + slot * WS::PVB};
+    const uint64_t stride = (uint64_t)gridDim.x * waves * WS::VPW;
+    for (uint64_t v0 = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW; v0 < N; v0 += stride) {
+        const uint64_t v = v0 + slot;
+        if (v < N) body(v, t, sink);
+        wave_sync();
+        const uint64_t left = N - v0;
+        wave_copy_out<TPV, VB, NT>(wl, out + v0 * VB, left < WS::VPW ? (uint32_t)left : WS::VPW, lane);
+        wave_sync();
     }
 }
 
@@ -80,19 +124,18 @@ template <int W, int MODE, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
                                                         uint8_t* __restrict__ out) {
     using T = typename uint_of<W>::type;
-    FLS_VECTOR_LOOP(vec_c, lane_tpv<W>()) {
-        uint8_t* dst = out + v * 128 * W;
+    staged_vector_loop<lane_tpv<W>(), 128 * W, NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
         if constexpr (MODE == 0) {
-            ffor_untransposed<W, W, false, NT>(vbuf(page, arr_c, v, 0), 0, 0, t, dst);
+            ffor_untransposed<W, W, false>(vbuf(page, arr_c, v, 0), 0, 0, t, dst);
         } else {
             constexpr unsigned o    = MODE == 2 ? 1 : 0;
             const unsigned     bw   = *vbuf(page, arr_c, v, o);
             const uint64_t     base = load_t<T>(vbuf(page, arr_c, v, o + 1));
             const uint64_t     lb   = MODE == 2 ? lane_base_of<W>(vbuf(page, arr_c, v, 0), t) : 0ull;
             const uint8_t*     pk   = vbuf(page, arr_c, v, o + 2);
-            dispatch_bw<W>(bw, [&](auto BW) { ffor_untransposed<W, BW, MODE == 2, NT>(pk, base, lb, t, dst); });
+            dispatch_bw<W>(bw, [&](auto BW) { ffor_untransposed<W, BW, MODE == 2>(pk, base, lb, t, dst); });
         }
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -101,9 +144,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restric
 // (src/cor/lyt/vec.cpp:318-332). Positions are clamped to the vector's run count; a clamp sets err bit 1.
 // A thread owns one 32-bit column of the u16 position rows: two position lanes.
 // ---------------------------------------------------------------------------------------------------------------
-template <typename T, int BW, bool DELTA, bool NT>
+template <typename T, int BW, bool DELTA, typename S>
 __device__ __forceinline__ void rle_vector(const uint8_t* packed, uint32_t base, uint32_t lane_base, unsigned t,
-                                           const uint8_t* runs, uint32_t n_runs, uint8_t* out, uint32_t* err) {
+                                           const uint8_t* runs, uint32_t n_runs, const S& out, uint32_t* err) {
     uint32_t V[16];
     lanes_p32<16, BW, DELTA>(packed, base, lane_base, t, V);
     const uint32_t last = n_runs - 1;
@@ -116,7 +159,7 @@ __device__ __forceinline__ void rle_vector(const uint8_t* packed, uint32_t base,
             idx     = idx > last ? last : idx;
             vals[p] = load_t<T>(runs + (uint64_t)idx * sizeof(T));
         });
-        store_run<T, 16, (8 * sizeof(T) < 16 ? 8 * sizeof(T) : 16), NT>(vals, out + sizeof(T) * seg_start<16>(2 * t + l));
+        store_run<T, 16, (8 * sizeof(T) < 16 ? 8 * sizeof(T) : 16)>(vals, sizeof(T) * seg_start<16>(2 * t + l), out);
     });
     if (bad) atomicOr(err, 1u);
 }
@@ -124,26 +167,25 @@ __device__ __forceinline__ void rle_vector(const uint8_t* packed, uint32_t base,
 template <typename T, int VAR, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
                                                         uint8_t* __restrict__ out, uint32_t* err) {
-    FLS_VECTOR_LOOP(vec_c, 32) {
+    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
         const uint8_t* runs = vbuf(page, arr_c, v, 0);
         // run count from the run buffer's extent: next vector's entry point, or the next buffer's offset
         const uint8_t* runs_end = v + 1 < vec_c ? vbuf(page, arr_c, v + 1, 0) : vbuf(page, arr_c, 0, 1);
         const uint32_t n_runs   = (uint32_t)((runs_end - runs) / sizeof(T));
-        uint8_t*       dst      = out + v * 1024 * sizeof(T);
         if (n_runs == 0) {  // the host rejects such pages; never read an empty run buffer
             atomicOr(err, 1u);
-            continue;
+            return;
         }
         if constexpr (VAR == 1) {
-            rle_vector<T, 16, false, NT>(vbuf(page, arr_c, v, 1), 0, 0, t, runs, n_runs, dst, err);
+            rle_vector<T, 16, false>(vbuf(page, arr_c, v, 1), 0, 0, t, runs, n_runs, dst, err);
         } else {
             const uint32_t lb   = reinterpret_cast<const u32_ua*>(vbuf(page, arr_c, v, 1))[t];
             const unsigned bw   = *vbuf(page, arr_c, v, 2);
             const uint32_t base = *reinterpret_cast<const u16_ua*>(vbuf(page, arr_c, v, 3));
             const uint8_t* pk   = vbuf(page, arr_c, v, 4);
-            dispatch_bw<16>(bw, [&](auto BW) { rle_vector<T, BW, true, NT>(pk, base, lb, t, runs, n_runs, dst, err); });
+            dispatch_bw<16>(bw, [&](auto BW) { rle_vector<T, BW, true>(pk, base, lb, t, runs, n_runs, dst, err); });
         }
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -151,9 +193,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
 // VAR 7 [lane bases][bw][base][packed]; out[i] = dict[code[i]]. Codes >= dict_n are clamped (err bit 2).
 // The decoded dictionary holds D-typed values (D = max(32, bits of T)). One u32 code lane per thread.
 // ---------------------------------------------------------------------------------------------------------------
-template <typename T, typename D, int BW, bool DELTA, bool NT>
+template <typename T, typename D, int BW, bool DELTA, typename S>
 __device__ __forceinline__ void dict_vector(const uint8_t* packed, uint32_t base, uint32_t lane_base, unsigned t,
-                                            const D* __restrict__ dict, uint32_t dict_n, uint8_t* out, uint32_t* err) {
+                                            const D* __restrict__ dict, uint32_t dict_n, const S& out, uint32_t* err) {
     uint32_t V[32];
     lanes_p32<32, BW, DELTA>(packed, base, lane_base, t, V);
     const uint32_t last = dict_n - 1;
@@ -165,7 +207,7 @@ __device__ __forceinline__ void dict_vector(const uint8_t* packed, uint32_t base
         c       = c > last ? last : c;
         vals[p] = (T)dict[c];
     });
-    store_run<T, 32, 16, NT>(vals, out + sizeof(T) * seg_start<32>(t));
+    store_run<T, 32, 16>(vals, sizeof(T) * seg_start<32>(t), out);
     if (bad) atomicOr(err, 2u);
 }
 
@@ -174,10 +216,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restri
                                                          const D* __restrict__ dict, uint32_t dict_n,
                                                          uint8_t* __restrict__ out, uint32_t* err) {
     if (dict_n == 0) return;  // the host rejects such pages
-    FLS_VECTOR_LOOP(vec_c, 32) {
-        uint8_t* dst = out + v * 1024 * sizeof(T);
+    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
         if constexpr (VAR == 1) {
-            dict_vector<T, D, 32, false, NT>(vbuf(page, arr_c, v, 0), 0, 0, t, dict, dict_n, dst, err);
+            dict_vector<T, D, 32, false>(vbuf(page, arr_c, v, 0), 0, 0, t, dict, dict_n, dst, err);
         } else {
             constexpr unsigned o    = VAR == 7 ? 1 : 0;
             const unsigned     bw   = *vbuf(page, arr_c, v, o);
@@ -185,10 +226,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restri
             const uint32_t     lb   = VAR == 7 ? reinterpret_cast<const u32_ua*>(vbuf(page, arr_c, v, 0))[t] : 0u;
             const uint8_t*     pk   = vbuf(page, arr_c, v, o + 2);
             dispatch_bw<32>(bw, [&](auto BW) {
-                dict_vector<T, D, BW, VAR == 7, NT>(pk, base, lb, t, dict, dict_n, dst, err);
+                dict_vector<T, D, BW, VAR == 7>(pk, base, lb, t, dict, dict_n, dst, err);
             });
         }
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------------------------------------------

@@ -114,3 +114,33 @@ def for_with_outliers(n: int, width: int = 8, outlier_frac: float = 0.01, seed: 
     m = g.random(n) < outlier_frac
     x[m] = g.integers(0, 1 << 40, int(m.sum()), dtype=np.int64)
     return x
+
+
+def vocab_strings_fast(n: int, seed: int = 7, min_len: int = 8, max_len: int = 64, n_words: int = 2000):
+    """Vectorized variant for large n: a text of space-separated vocabulary words is cut into consecutive strings
+    of length U[min_len, max_len]. Returns (offsets uint32[n+1], bytes uint8)."""
+    g = rng(seed)
+    vocab = vocabulary(n_words)
+    lens = g.integers(min_len, max_len + 1, n).astype(np.int64)
+    total = int(lens.sum())
+    wl = np.array([len(w) + 1 for w in vocab], dtype=np.int64)  # word + space
+    wbytes = np.frombuffer(b"".join(w + b" " for w in vocab), dtype=np.uint8)
+    wstart = np.concatenate([[0], np.cumsum(wl)[:-1]])
+    n_words_needed = int(total / wl.mean() * 1.1) + 16
+    idx = g.integers(0, len(vocab), n_words_needed)
+    pieces_len = wl[idx]
+    while pieces_len.sum() < total:
+        more = g.integers(0, len(vocab), n_words_needed // 10 + 16)
+        idx = np.concatenate([idx, more])
+        pieces_len = wl[idx]
+    # the text is the concatenation of the chosen word pieces (+ space), cut to `total` bytes
+    ends = np.cumsum(pieces_len)
+    cut = int(np.searchsorted(ends, total)) + 1
+    idx, pieces_len, ends = idx[:cut], pieces_len[:cut], ends[:cut]
+    rel = np.repeat(wstart[idx] - (ends - pieces_len), pieces_len)
+    text = wbytes[rel[:total] + np.arange(total, dtype=np.int64)]
+    offs = np.zeros(n + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(lens)
+    if offs[-1] >= 1 << 32:
+        raise ValueError("string column too large for 32-bit offsets")
+    return offs.astype(np.uint32), text

@@ -60,6 +60,13 @@ int fls_gpu_plan_string_offsets(fls_gpu_plan* plan, const uint8_t* dev_encoded, 
 int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, const uint64_t* dev_offsets,
                                 uint8_t* dev_bytes, uint64_t bytes_capacity, void* stream);
 
+/* Encode a string column with FSST (1024-string vectors; n must be a multiple of 1024): strings i are
+ * bytes[offsets[i], offsets[i+1]). Writes *out_size bytes. No counterpart in the reference's C API, which accepts
+ * INT64 and DOUBLE only (reference src/encoder/single_col_encoder.cpp:13-33); the layout is the reference's FSST
+ * expression (include/fls/cor/exp/fsst/fsst_exp.hpp:35-52). Host-side, no GPU involved. */
+int fls_encode_strings(const uint32_t* offsets, const uint8_t* bytes, uint64_t n, uint8_t* out, uint64_t capacity,
+                       uint64_t* out_size);
+
 /* ---- raw primitive streams -----------------------------------------------------------------------------------------
  * n_vec packed vectors back to back (128*bw bytes each), decoded into n_vec*1024 values of `width` bits in the
  * lane-major order of the reference's generated unffor/unpack (primitives/fls_generated/fallback/scalar_aav_uf1/
