@@ -4,7 +4,8 @@
 Workload (one step): for every element type uint8/16/32/64 and every bit width w in [0, W], decode 2^28 values
 FFOR-packed (per-vector base) and 2^28 values plain-packed, all inputs resident in HBM. Values are U[0, 2^w), so
 their packed bits are uniform random bytes; bases are U[0, 2^(W-w)] (bound clamped at w = 0). Output in the
-reference's unffor/unpack order. A bounded sample of every case is checked bit-exact against the CPU oracle.
+reference's unffor/unpack order. A bounded sample of every case is checked bit-exact against the CPU oracle in the
+CPU-baseline leg.
 
 Prints one JSON line (metric, value in Gvalues/s, roofline of the dominant kernel, CPU baseline of the
 reference's own compiled unffor on the host cores). --detail adds the cascade / row-group / wizard sub-benchmarks.
@@ -51,13 +52,20 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def cpu_baseline(widths, threads):
-    """The reference's compiled generated::unffor (oracle/_ref) on the host cores, over a bounded sample: every
-    (type, width) FFOR case at 2^20 values (the same per-case shape as the GPU sweep)."""
-    from oracle.py_oracle import Reference, reference_available
+def cpu_baseline(widths, threads, samples):
+    """The CPU leg: (1) checks the sampled GPU outputs against the oracle's unffor; (2) times the reference's
+    compiled generated::unffor (oracle/_ref) on the host cores over a bounded sample: every (type, width) FFOR case
+    at 2^20 values (the same per-case shape as the GPU sweep). Returns (baseline, mismatching cases)."""
+    from oracle.py_oracle import Oracle, Reference, reference_available
 
+    orc = Oracle()
+    bad = []
+    for (W, w, plain, v0, pk, bs, got) in samples:
+        want = orc.unffor_stream(pk, 4, w, bs, got.dtype)
+        if not np.array_equal(got, want):
+            bad.append((W, w, plain, v0))
     if not reference_available():
-        return None
+        return None, bad
     ref = Reference()
     rng = np.random.default_rng(0)
     total_vals, total_t = 0, 0.0
@@ -75,7 +83,7 @@ def cpu_baseline(widths, threads):
     return {"value": round(total_vals / total_t / 1e9, 3), "unit": "Gvalues/s", "cores": threads,
             "kind": "reference",
             "sample": f"reference generated::unffor, every (type,width) FFOR case of {widths} at 2^20 values, "
-                      f"median of 3 passes, {threads} threads"}
+                      f"median of 3 passes, {threads} threads"}, bad
 
 
 def main():
@@ -128,11 +136,8 @@ def main():
                 else:
                     launch(W, w, packed, bases, plain)
 
-    # ---- parity on a bounded sample of every case (first and last 4 vectors) ----------------------------------
-    from oracle.py_oracle import Oracle
-
-    orc = Oracle()
-    bad = []
+    # ---- bounded sample of every case's output (first and last 4 vectors), checked in the cpu_baseline leg ------
+    samples = []
     for plain in (False, True):
         for (W, w, packed, bases) in cases:
             launch(W, w, packed, bases, plain)
@@ -141,13 +146,8 @@ def main():
             for v0 in (0, n_vec - 4):
                 pk = packed[v0 * 128 * w:(v0 + 4) * 128 * w].cpu().numpy() if w else np.zeros(16, np.uint8)
                 bs = np.zeros(4, npdt) if plain else bases[v0:v0 + 4].cpu().numpy().view(npdt)
-                want = orc.unffor_stream(pk, 4, w, np.ascontiguousarray(bs), npdt)
                 got = outs[W][v0 * 1024 * W // 8:(v0 + 4) * 1024 * W // 8].cpu().numpy().view(npdt)
-                if not np.array_equal(got, want):
-                    bad.append((W, w, plain, v0))
-    if bad:
-        print(json.dumps({"error": "decode mismatch against the oracle", "cases": bad[:10]}))
-        sys.exit(1)
+                samples.append((W, w, plain, v0, pk, np.ascontiguousarray(bs), got))
 
     # ---- timed region --------------------------------------------------------------------------------------------
     for _ in range(a.warmup):
@@ -212,7 +212,6 @@ def main():
         "sweep_aggregate_hbm_frac": round(agg_bytes / kernel_time / 1e9 / HBM_PEAK_GBS, 4),
         "u32_w7_ffor_hbm_frac": round(alg_bytes(32, 7, False) / (sum(u32w7) / len(u32w7)) / 1e9 / HBM_PEAK_GBS, 4)
         if u32w7 else None,
-        "parity": f"bit-exact vs oracle on 8 sampled vectors of each of the {2 * len(cases)} cases",
     }
     del cases, outs
     torch.cuda.empty_cache()
@@ -222,7 +221,13 @@ def main():
         line["detail"] = bench_detail.run_all(dev)
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(widths, threads)
+        line["cpu_baseline"], bad = cpu_baseline(widths, threads, samples)
+        if bad:
+            print(json.dumps({"error": "decode mismatch against the oracle", "cases": bad[:10]}))
+            sys.exit(1)
+        line["parity"] = f"bit-exact vs oracle on 8 sampled vectors of each of the {len(samples) // 2} cases"
+    else:
+        line["parity"] = "not checked (--no-cpu)"
     print(json.dumps(line), flush=True)
 
 

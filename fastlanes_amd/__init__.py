@@ -132,6 +132,44 @@ class Plan:
             self.h = None
 
 
+class RowGroup:
+    """Many encoded columns decoded together (fls_gpu_rowgroup_*). The encoded columns live in one device buffer at
+    `device_offsets`; decoded columns land in one output buffer at `offsets` (256-byte aligned)."""
+
+    def __init__(self, host_columns, dtypes, device_offsets):
+        n = len(host_columns)
+        self._cols = [np.ascontiguousarray(c, dtype=np.uint8) for c in host_columns]
+        ptrs = (_vp * max(n, 1))(*[c.ctypes.data for c in self._cols])
+        sizes = (_u64 * max(n, 1))(*[c.size for c in self._cols])
+        offs = (_u64 * max(n, 1))(*[int(o) for o in device_offsets])
+        types = (C.c_uint8 * max(n, 1))(*[dtype_code(d) for d in dtypes])
+        h = _vp()
+        check(lib().fls_gpu_rowgroup_create(n, ptrs, sizes, offs, types, C.byref(h)), "rowgroup_create")
+        self.h = h
+        self.n = n
+        out_offs = (_u64 * max(n, 1))()
+        total = _u64(0)
+        check(lib().fls_gpu_rowgroup_layout(h, out_offs, C.byref(total)), "rowgroup_layout")
+        self.offsets = [int(out_offs[i]) for i in range(n)]
+        self.total_bytes = int(total.value)
+
+    def decode(self, dev_encoded: int, dev_out: int, mode: int = 0, stream: int = 0) -> None:
+        """mode 0: one persistent launch over the row-group's tile list; mode 1: one launch per page."""
+        check(lib().fls_gpu_rowgroup_decode(self.h, dev_encoded, dev_out, mode, stream), "rowgroup_decode")
+
+    def decode_graph(self, dev_encoded: int, dev_out: int, stream: int = 0) -> None:
+        """The persistent decode captured once in a HIP graph, then replayed."""
+        check(lib().fls_gpu_rowgroup_decode_graph(self.h, dev_encoded, dev_out, stream), "rowgroup_decode_graph")
+
+    def check(self, stream: int = 0) -> None:
+        check(lib().fls_gpu_rowgroup_check(self.h, stream), "rowgroup_check")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.fls_gpu_rowgroup_destroy(self.h)
+            self.h = None
+
+
 def unffor_stream(width: int, bw: int, dev_packed: int, dev_bases: int | None, dev_out: int, n_vec: int,
                   stream: int = 0) -> None:
     check(lib().fls_gpu_unffor_stream(width, bw, dev_packed, dev_bases or None, dev_out, n_vec, stream),

@@ -42,6 +42,22 @@ struct fls_gpu_plan {
     StringPlan           strings;
 };
 
+namespace fls {
+const Column& plan_column(const fls_gpu_plan* p) { return p->col; }
+const void*   plan_dictionary(const fls_gpu_plan* p, size_t page) {
+    return page < p->dicts.size() ? p->dicts[page].values : nullptr;
+}
+int plan_decode_dictionaries(fls_gpu_plan* p, const uint8_t* dev_encoded, hipStream_t s) {
+    for (size_t i = 0; i < p->col.pages.size(); ++i) {
+        const DevDict& d = p->dicts[i];
+        if (!d.n_chunks) continue;
+        hipError_t e = launch_hzl(p->DW, dev_encoded, d.chunks, d.n_chunks, d.values, s);
+        if (e != hipSuccess) return hip_fail(e, "dictionary launch");
+    }
+    return FLS_OK;
+}
+}  // namespace fls
+
 extern "C" {
 
 const char* fls_gpu_error_string(void) { return g_err.c_str(); }
@@ -165,7 +181,9 @@ int fls_encode_strings(const uint32_t* offsets, const uint8_t* bytes, uint64_t n
 int fls_gpu_tune_set(int family, unsigned width, unsigned bw, unsigned block, unsigned nontemporal,
                      unsigned max_blocks) {
     if (family < 0 || family >= FAM_COUNT) return fail(FLS_E_ARG, "unknown kernel family");
-    if (block == 0 || block % 64 || block > kMaxBlock) return fail(FLS_E_ARG, "block must be a multiple of 64, <= 512");
+    if (family != FAM_ROWGROUP && (block == 0 || block % 64 || block > kMaxBlock))
+        return fail(FLS_E_ARG, "block must be a multiple of 64, <= 512");
+    if (family == FAM_ROWGROUP && block == 0) return fail(FLS_E_ARG, "tile size must be positive");
     (void)tune_for(family, width, bw);  // make sure the default table is loaded first
     tune_set(family, width, bw, Tune {(uint16_t)block, (uint16_t)(nontemporal != 0), max_blocks});
     return FLS_OK;

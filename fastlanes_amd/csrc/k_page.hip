@@ -58,7 +58,8 @@ int tune_load(const char* path) {
         int      fam;
         unsigned W, bw, block, nt, maxb;
         if (std::sscanf(line, "%d %u %u %u %u %u", &fam, &W, &bw, &block, &nt, &maxb) != 6) continue;
-        if (fam < 0 || fam >= FAM_COUNT || block == 0 || block % 64 || block > kMaxBlock) continue;
+        if (fam < 0 || fam >= FAM_COUNT || block == 0) continue;
+        if (fam != FAM_ROWGROUP && (block % 64 || block > kMaxBlock)) continue;
         tune_set(fam, W, bw, Tune {(uint16_t)block, (uint16_t)(nt != 0), maxb});
         ++n;
     }

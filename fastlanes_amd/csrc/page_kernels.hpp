@@ -120,21 +120,26 @@ __device__ __forceinline__ uint64_t lane_base_of(const uint8_t* p, unsigned t) {
 // Integer pages. MODE 0: UNCOMPRESSED [values], 1: FFOR [bw][base][packed], 2: DELTA [lane bases][bw][base][packed]
 // (include/fls/cor/exp/{uncompressed,ffor,delta}/*_exp.hpp). Output untransposed.
 // ---------------------------------------------------------------------------------------------------------------
+template <int W, int MODE, typename S>
+__device__ __forceinline__ void int_page_vector(const uint8_t* page, uint32_t arr_c, uint64_t v, unsigned t, const S& dst) {
+    using T = typename uint_of<W>::type;
+    if constexpr (MODE == 0) {
+        ffor_untransposed<W, W, false>(vbuf(page, arr_c, v, 0), 0, 0, t, dst);
+    } else {
+        constexpr unsigned o    = MODE == 2 ? 1 : 0;
+        const unsigned     bw   = *vbuf(page, arr_c, v, o);
+        const uint64_t     base = load_t<T>(vbuf(page, arr_c, v, o + 1));
+        const uint64_t     lb   = MODE == 2 ? lane_base_of<W>(vbuf(page, arr_c, v, 0), t) : 0ull;
+        const uint8_t*     pk   = vbuf(page, arr_c, v, o + 2);
+        dispatch_bw<W>(bw, [&](auto BW) { ffor_untransposed<W, BW, MODE == 2>(pk, base, lb, t, dst); });
+    }
+}
+
 template <int W, int MODE, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
                                                         uint8_t* __restrict__ out) {
-    using T = typename uint_of<W>::type;
     staged_vector_loop<lane_tpv<W>(), 128 * W, NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
-        if constexpr (MODE == 0) {
-            ffor_untransposed<W, W, false>(vbuf(page, arr_c, v, 0), 0, 0, t, dst);
-        } else {
-            constexpr unsigned o    = MODE == 2 ? 1 : 0;
-            const unsigned     bw   = *vbuf(page, arr_c, v, o);
-            const uint64_t     base = load_t<T>(vbuf(page, arr_c, v, o + 1));
-            const uint64_t     lb   = MODE == 2 ? lane_base_of<W>(vbuf(page, arr_c, v, 0), t) : 0ull;
-            const uint8_t*     pk   = vbuf(page, arr_c, v, o + 2);
-            dispatch_bw<W>(bw, [&](auto BW) { ffor_untransposed<W, BW, MODE == 2>(pk, base, lb, t, dst); });
-        }
+        int_page_vector<W, MODE>(page, arr_c, v, t, dst);
     });
 }
 
@@ -164,10 +169,10 @@ __device__ __forceinline__ void rle_vector(const uint8_t* packed, uint32_t base,
     if (bad) atomicOr(err, 1u);
 }
 
-template <typename T, int VAR, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
-                                                        uint8_t* __restrict__ out, uint32_t* err) {
-    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
+template <typename T, int VAR, typename S>
+__device__ __forceinline__ void rle_page_vector(const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint64_t v,
+                                                unsigned t, const S& dst, uint32_t* err) {
+    {
         const uint8_t* runs = vbuf(page, arr_c, v, 0);
         // run count from the run buffer's extent: next vector's entry point, or the next buffer's offset
         const uint8_t* runs_end = v + 1 < vec_c ? vbuf(page, arr_c, v + 1, 0) : vbuf(page, arr_c, 0, 1);
@@ -185,6 +190,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
             const uint8_t* pk   = vbuf(page, arr_c, v, 4);
             dispatch_bw<16>(bw, [&](auto BW) { rle_vector<T, BW, true>(pk, base, lb, t, runs, n_runs, dst, err); });
         }
+    }
+}
+
+template <typename T, int VAR, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
+                                                        uint8_t* __restrict__ out, uint32_t* err) {
+    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
+        rle_page_vector<T, VAR>(page, arr_c, vec_c, v, t, dst, err);
     });
 }
 
@@ -211,12 +224,11 @@ __device__ __forceinline__ void dict_vector(const uint8_t* packed, uint32_t base
     if (bad) atomicOr(err, 2u);
 }
 
-template <typename T, typename D, int VAR, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
-                                                         const D* __restrict__ dict, uint32_t dict_n,
-                                                         uint8_t* __restrict__ out, uint32_t* err) {
-    if (dict_n == 0) return;  // the host rejects such pages
-    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
+template <typename T, typename D, int VAR, typename S>
+__device__ __forceinline__ void dict_page_vector(const uint8_t* page, uint32_t arr_c, uint64_t v, unsigned t,
+                                                 const D* __restrict__ dict, uint32_t dict_n, const S& dst,
+                                                 uint32_t* err) {
+    {
         if constexpr (VAR == 1) {
             dict_vector<T, D, 32, false>(vbuf(page, arr_c, v, 0), 0, 0, t, dict, dict_n, dst, err);
         } else {
@@ -229,6 +241,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restri
                 dict_vector<T, D, BW, VAR == 7>(pk, base, lb, t, dict, dict_n, dst, err);
             });
         }
+    }
+}
+
+template <typename T, typename D, int VAR, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
+                                                         const D* __restrict__ dict, uint32_t dict_n,
+                                                         uint8_t* __restrict__ out, uint32_t* err) {
+    if (dict_n == 0) return;  // the host rejects such pages
+    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
+        dict_page_vector<T, D, VAR>(page, arr_c, v, t, dict, dict_n, dst, err);
     });
 }
 

@@ -1,54 +1,162 @@
-// rowgroup.hip -- row-group decode: many columns, one launch.
+// rowgroup.hip -- row-group decode: many columns, one persistent launch over a device-side work list.
+//
+// Replaces the per-column decode loop that sits behind Reader::get_chunk / Interpreter::Decoding in the reference
+// (src/reader/reader.cpp:19-61, src/expression/interpreter.cpp:105-137) with a work list of (page, vector range)
+// tiles decoded by one persistent kernel (k_rowgroup.hip); mode 1 keeps the one-launch-per-page form for
+// comparison. The persistent launch can be captured in a HIP graph and replayed.
+#include <algorithm>
 #include <memory>
 #include <vector>
 
 #include "../../include/fls_gpu.h"
 #include "format.hpp"
 #include "gpu_internal.hpp"
+#include "launch.hpp"
+#include "page_kernels.hpp"
+#include "rowgroup.hpp"
 
 using namespace fls;
+
+// fls_gpu_plan internals used here (defined in gpu_api.hip)
+namespace fls {
+const Column& plan_column(const fls_gpu_plan* p);
+int           plan_decode_dictionaries(fls_gpu_plan* p, const uint8_t* dev_encoded, hipStream_t s);
+const void*   plan_dictionary(const fls_gpu_plan* p, size_t page);
+}  // namespace fls
 
 struct fls_gpu_rowgroup {
     std::vector<fls_gpu_plan*> plans;
     std::vector<uint64_t>      dev_off, out_off;
     std::vector<uint8_t>       types;
+    std::vector<bool>          fused;  // column decoded by the persistent kernel
     uint64_t                   out_total = 0;
+    std::vector<RgTile>        tiles;
+    RgTile*                    d_tiles   = nullptr;
+    uint32_t*                  d_counter = nullptr;
+    uint32_t*                  d_err     = nullptr;
+    unsigned                   lds = 0, blocks = 0;
+    // captured graph
+    hipGraph_t     graph = nullptr;
+    hipGraphExec_t exec  = nullptr;
+    const uint8_t* g_enc = nullptr;
+    uint8_t*       g_out = nullptr;
 };
+
+namespace {
+
+int kind_of(const PageEntry& pe, unsigned W) {
+    const int wc = W == 8 ? 0 : W == 16 ? 1 : W == 32 ? 2 : 3;
+    switch (pe.exp_t) {
+    case EXP_UNCOMPRESSED: return RG_UNC8 + wc;
+    case EXP_FFOR: return RG_FFOR8 + wc;
+    case EXP_DELTA: return RG_DELTA8 + wc;
+    case EXP_RLE: return pe.var == 2 ? RG_RLE8 + wc : -1;
+    default: return is_dict_exp(pe.exp_t) && pe.var == 2 ? RG_DICT8 + wc : -1;
+    }
+}
+
+void free_rg(fls_gpu_rowgroup* rg) {
+    for (auto* p : rg->plans) fls_gpu_plan_destroy(p);
+    if (rg->d_tiles) (void)hipFree(rg->d_tiles);
+    if (rg->d_counter) (void)hipFree(rg->d_counter);
+    if (rg->d_err) (void)hipFree(rg->d_err);
+    if (rg->exec) (void)hipGraphExecDestroy(rg->exec);
+    if (rg->graph) (void)hipGraphDestroy(rg->graph);
+    delete rg;
+}
+
+int decode_fused(fls_gpu_rowgroup* rg, const uint8_t* enc, uint8_t* out, hipStream_t s) {
+    for (size_t c = 0; c < rg->plans.size(); ++c) {
+        if (!rg->fused[c]) {
+            const uint64_t bytes = fls_gpu_plan_n_values(rg->plans[c]) * (dtype_bits(rg->types[c]) / 8);
+            int rc = fls_gpu_plan_decode(rg->plans[c], enc + rg->dev_off[c], out + rg->out_off[c], bytes, s);
+            if (rc) return rc;
+        } else {
+            int rc = plan_decode_dictionaries(rg->plans[c], enc + rg->dev_off[c], s);
+            if (rc) return rc;
+        }
+    }
+    if (rg->tiles.empty()) return FLS_OK;
+    hipError_t e = launch_rowgroup(enc, out, rg->d_tiles, (uint32_t)rg->tiles.size(), rg->d_counter, rg->d_err, rg->lds,
+                                   rg->blocks, s);
+    return e == hipSuccess ? FLS_OK : hip_fail(e, "row-group launch");
+}
+
+}  // namespace
 
 extern "C" {
 
 int fls_gpu_rowgroup_create(uint32_t n_columns, const uint8_t* const* host_columns, const uint64_t* column_sizes,
-                            const uint64_t* device_offsets, const enum data_t* types, fls_gpu_rowgroup** out) {
+                            const uint64_t* device_offsets, const uint8_t* types, fls_gpu_rowgroup** out) {
     if (!out || (n_columns && (!host_columns || !column_sizes || !device_offsets || !types)))
         return fail(FLS_E_ARG, "null argument");
-    auto rg = std::make_unique<fls_gpu_rowgroup>();
+    *out    = nullptr;
+    auto* rg = new fls_gpu_rowgroup();
+    const Tune tn = tune_for(FAM_ROWGROUP, 0, 255);
+    const unsigned tile_vecs = tn.block ? tn.block : 64;
     for (uint32_t c = 0; c < n_columns; ++c) {
         fls_gpu_plan* p  = nullptr;
-        int           rc = fls_gpu_plan_create(host_columns[c], column_sizes[c], types[c], &p);
-        if (rc) {
-            for (auto* q : rg->plans) fls_gpu_plan_destroy(q);
-            return rc;
-        }
-        if (types[c] == STR) {
-            fls_gpu_plan_destroy(p);
-            for (auto* q : rg->plans) fls_gpu_plan_destroy(q);
-            return fail(FLS_E_UNSUPPORTED, "string columns are decoded with fls_gpu_plan_decode_strings");
-        }
+        int           rc = fls_gpu_plan_create(host_columns[c], column_sizes[c], (enum data_t)types[c], &p);
+        if (rc) return free_rg(rg), rc;
         rg->plans.push_back(p);
+        if (types[c] == STR) return free_rg(rg), fail(FLS_E_UNSUPPORTED, "string columns use fls_gpu_plan_decode_strings");
         rg->dev_off.push_back(device_offsets[c]);
         rg->types.push_back((uint8_t)types[c]);
         rg->out_off.push_back(rg->out_total);
-        const uint64_t bytes = fls_gpu_plan_n_values(p) * (dtype_bits((uint8_t)types[c]) / 8);
+        const unsigned W     = dtype_bits((uint8_t)types[c]);
+        const uint64_t bytes = fls_gpu_plan_n_values(p) * (W / 8);
+        const Column&  col   = plan_column(p);
+        bool           fused = true;
+        for (const auto& pe : col.pages) fused &= kind_of(pe, W) >= 0;
+        rg->fused.push_back(fused);
+        if (fused) {
+            for (size_t i = 0; i < col.pages.size(); ++i) {
+                const PageEntry& pe   = col.pages[i];
+                const int        kind = kind_of(pe, W);
+                rg->lds               = std::max(rg->lds, rg_stage_bytes(kind));
+                for (uint32_t v0 = 0; v0 < pe.vec_c; v0 += tile_vecs) {
+                    RgTile t {};
+                    t.page   = device_offsets[c] + pe.page_off;
+                    t.out    = rg->out_total + pe.out_index * (W / 8);
+                    t.dict   = (uint64_t)(uintptr_t)plan_dictionary(p, i);
+                    t.dict_n = pe.dict_n;
+                    t.v0     = v0;
+                    t.nv     = std::min(tile_vecs, pe.vec_c - v0);
+                    t.vec_c  = pe.vec_c;
+                    t.kind   = (uint16_t)kind;
+                    t.arr_c  = (uint16_t)pe.arr_c;
+                    rg->tiles.push_back(t);
+                }
+            }
+        }
         rg->out_total += (bytes + 255) & ~255ull;
     }
-    *out = rg.release();
+    // group tiles by kind so a compute unit's consecutive tiles share one code path
+    std::stable_sort(rg->tiles.begin(), rg->tiles.end(), [](const RgTile& a, const RgTile& b) { return a.kind < b.kind; });
+    hipError_t e;
+    if ((e = hipMalloc(&rg->d_counter, 4)) != hipSuccess) return free_rg(rg), hip_fail(e, "hipMalloc");
+    if ((e = hipMalloc(&rg->d_err, 4)) != hipSuccess) return free_rg(rg), hip_fail(e, "hipMalloc");
+    if ((e = hipMemset(rg->d_err, 0, 4)) != hipSuccess) return free_rg(rg), hip_fail(e, "hipMemset");
+    if (!rg->tiles.empty()) {
+        const size_t b = rg->tiles.size() * sizeof(RgTile);
+        if ((e = hipMalloc(&rg->d_tiles, b)) != hipSuccess) return free_rg(rg), hip_fail(e, "hipMalloc");
+        if ((e = hipMemcpy(rg->d_tiles, rg->tiles.data(), b, hipMemcpyHostToDevice)) != hipSuccess)
+            return free_rg(rg), hip_fail(e, "hipMemcpy");
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return free_rg(rg), hip_fail(e, "device properties");
+    const unsigned lds_cu  = prop.maxSharedMemoryPerMultiProcessor ? (unsigned)prop.maxSharedMemoryPerMultiProcessor : 65536u;
+    const unsigned per_cu  = tn.max_blocks ? tn.max_blocks : 8;
+    const unsigned fit     = rg->lds ? std::max(1u, lds_cu / rg->lds) : per_cu;
+    rg->blocks             = (unsigned)prop.multiProcessorCount * std::min(per_cu, fit);
+    *out                   = rg;
     return FLS_OK;
 }
 
 void fls_gpu_rowgroup_destroy(fls_gpu_rowgroup* rg) {
-    if (!rg) return;
-    for (auto* p : rg->plans) fls_gpu_plan_destroy(p);
-    delete rg;
+    if (rg) free_rg(rg);
 }
 
 int fls_gpu_rowgroup_layout(const fls_gpu_rowgroup* rg, uint64_t* out_offsets, uint64_t* out_total_bytes) {
@@ -61,22 +169,52 @@ int fls_gpu_rowgroup_layout(const fls_gpu_rowgroup* rg, uint64_t* out_offsets, u
 
 int fls_gpu_rowgroup_decode(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, int mode, void* stream) {
     if (!rg || !dev_encoded || !dev_out) return fail(FLS_E_ARG, "null argument");
-    if (mode != 1) return fail(FLS_E_UNSUPPORTED, "persistent row-group launch not built");
+    if (((uintptr_t)dev_out & 255) != 0) return fail(FLS_E_ARG, "row-group output must be 256-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    if (mode == 0) return decode_fused(rg, dev_encoded, dev_out, s);
+    if (mode != 1) return fail(FLS_E_ARG, "mode must be 0 (persistent) or 1 (per page)");
     for (size_t c = 0; c < rg->plans.size(); ++c) {
         const uint64_t bytes = fls_gpu_plan_n_values(rg->plans[c]) * (dtype_bits(rg->types[c]) / 8);
-        int rc = fls_gpu_plan_decode(rg->plans[c], dev_encoded + rg->dev_off[c], dev_out + rg->out_off[c], bytes, stream);
+        int rc = fls_gpu_plan_decode(rg->plans[c], dev_encoded + rg->dev_off[c], dev_out + rg->out_off[c], bytes, s);
         if (rc) return rc;
     }
     return FLS_OK;
 }
 
 int fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, void* stream) {
-    (void)rg, (void)dev_encoded, (void)dev_out, (void)stream;
-    return fail(FLS_E_UNSUPPORTED, "graph capture not built");
+    if (!rg || !dev_encoded || !dev_out) return fail(FLS_E_ARG, "null argument");
+    if (((uintptr_t)dev_out & 255) != 0) return fail(FLS_E_ARG, "row-group output must be 256-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t  e;
+    if (!rg->exec || rg->g_enc != dev_encoded || rg->g_out != dev_out) {
+        if (rg->exec) (void)hipGraphExecDestroy(rg->exec), rg->exec = nullptr;
+        if (rg->graph) (void)hipGraphDestroy(rg->graph), rg->graph = nullptr;
+        hipStream_t cs;
+        if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
+        if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+            (void)hipStreamDestroy(cs);
+            return hip_fail(e, "begin capture");
+        }
+        int rc = decode_fused(rg, dev_encoded, dev_out, cs);
+        e      = hipStreamEndCapture(cs, &rg->graph);
+        (void)hipStreamDestroy(cs);
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail(e, "end capture");
+        if ((e = hipGraphInstantiate(&rg->exec, rg->graph, nullptr, nullptr, 0)) != hipSuccess)
+            return hip_fail(e, "graph instantiate");
+        rg->g_enc = dev_encoded, rg->g_out = dev_out;
+    }
+    e = hipGraphLaunch(rg->exec, s);
+    return e == hipSuccess ? FLS_OK : hip_fail(e, "graph launch");
 }
 
 int fls_gpu_rowgroup_check(fls_gpu_rowgroup* rg, void* stream) {
     if (!rg) return fail(FLS_E_ARG, "null argument");
+    uint32_t   h = 0;
+    hipError_t e = hipMemcpyAsync(&h, rg->d_err, 4, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "check");
+    if (h) return fail(FLS_E_DATA, "row-group decode met an out-of-range run or dictionary entry");
     for (auto* p : rg->plans) {
         int rc = fls_gpu_plan_check(p, stream);
         if (rc) return rc;

@@ -1,0 +1,53 @@
+"""Launch one decode kernel (and a copy of known size) a few times, for hardware-counter runs.
+
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -- python -m fastlanes_amd.profile_kernel --width 64 --bw 63
+
+Prints the algorithmic bytes of one launch of each, so counter readings (KiB per dispatch) can be compared with
+them: `copy_f32` reads and writes `copy_bytes` each, which calibrates FETCH_SIZE / WRITE_SIZE against bytes that
+are known to cross HBM.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+
+import fastlanes_amd as fa
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--width", type=int, default=64)
+    ap.add_argument("--bw", type=int, default=63)
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--plain", action="store_true", help="plain unpack (no FFOR bases)")
+    ap.add_argument("--copy-log2-bytes", type=int, default=30)
+    a = ap.parse_args()
+    import torch
+
+    dev = torch.device("cuda:0")
+    n = 1 << a.log2n
+    n_vec = n // 1024
+    W, bw = a.width, a.bw
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    packed = torch.randint(0, 256, (max(n_vec * 128 * bw, 16),), dtype=torch.uint8, device=dev, generator=g)
+    bases = torch.zeros(n_vec * W // 8 + 16, dtype=torch.uint8, device=dev)
+    out = torch.empty(n * W // 8 + 16, dtype=torch.uint8, device=dev)
+    sp = fa.stream_ptr()
+    for _ in range(a.reps):
+        fa.unffor_stream(W, bw, packed.data_ptr(), None if a.plain else bases.data_ptr(), out.data_ptr(), n_vec, sp)
+    src = torch.ones(1 << (a.copy_log2_bytes - 2), dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(a.reps):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    kbytes = n * bw / 8 + n * W / 8 + (0 if a.plain else n_vec * W / 8)
+    print(json.dumps({"kernel": f"k_unffor_stream<{W},{bw}>", "algorithmic_bytes": kbytes,
+                      "read_bytes": n * bw / 8 + (0 if a.plain else n_vec * W / 8), "write_bytes": n * W / 8,
+                      "copy_kernel": "torch mul (float32, elementwise)", "copy_bytes": src.numel() * 4,
+                      "reps": a.reps}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

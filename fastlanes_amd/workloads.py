@@ -144,3 +144,85 @@ def vocab_strings_fast(n: int, seed: int = 7, min_len: int = 8, max_len: int = 6
     if offs[-1] >= 1 << 32:
         raise ValueError("string column too large for 32-bit offsets")
     return offs.astype(np.uint32), text
+
+
+# expression ids of the mixed row-group (include/fls/expression/expression_pool, i64 pool numbering)
+ROWGROUP_EXPRS = {"bitpack": 1, "ffor": 1, "delta": 11, "rle": 6, "dict": 17}
+
+
+def rowgroup_column(kind: str, W: int, n: int, seed: int, w: int | None = None) -> np.ndarray:
+    """One column of the mixed row-group: `kind` in bitpack / ffor / delta / rle / dict, element width W bits."""
+    g = rng(seed)
+    dt = W2U[W]
+    if kind == "bitpack":  # uniform in [0, 2^w), w in 1..min(32, W)
+        w = w if w is not None else int(g.integers(1, min(32, W) + 1))
+        return g.integers(0, 1 << w, n, dtype=np.uint64).astype(dt)
+    if kind == "ffor":  # a large base plus uniform in [0, 2^w)
+        w = w if w is not None else int(g.integers(1, min(32, W - 1) + 1))
+        base = int(g.integers(0, (1 << W) - (1 << w), dtype=np.uint64, endpoint=True))
+        return (np.uint64(base) + g.integers(0, 1 << w, n, dtype=np.uint64)).astype(dt)
+    if kind == "delta":  # sorted (wrapping in narrow types) with gaps uniform in [0, 2^gw)
+        gw = int(g.integers(1, min(12, W - 1) + 1))
+        x = np.cumsum(g.integers(0, 1 << gw, n, dtype=np.uint64), dtype=np.uint64) + np.uint64(g.integers(0, 1 << 30))
+        return x.astype(dt)
+    if kind == "rle":  # geometric runs of mean 16
+        return geometric_runs(n, 16.0, seed=seed, value_bits=min(W, 20), dtype=dt)
+    if kind == "dict":  # few distinct wide values
+        keys = g.integers(0, np.iinfo(dt).max, int(g.integers(16, 4097)), dtype=dt, endpoint=True)
+        return keys[g.integers(0, keys.size, n)]
+    raise ValueError(kind)
+
+
+def rowgroup_spec(n_columns: int = 64, seed: int = 17, min_log2: int = 20, max_log2: int = 26):
+    """Mixed row-group: per column (W, kind, n) with W in {8, 16, 32, 64}, kind in bitpack / ffor / delta / rle /
+    dict and n uniform in [2^min_log2, 2^max_log2] rounded to a multiple of 1024."""
+    g = rng(seed)
+    kinds = list(ROWGROUP_EXPRS)
+    out = []
+    for c in range(n_columns):
+        W = int(g.choice([8, 16, 32, 64]))
+        kind = kinds[int(g.integers(0, len(kinds)))]
+        n = int(g.integers(1 << min_log2, (1 << max_log2) + 1))
+        n = max(1024, n // 1024 * 1024)
+        out.append((W, kind, n, seed * 1000 + c))
+    return out
+
+
+WIZARD_FAMILIES = ("uniform", "sorted", "runs", "zipf", "outliers")
+
+
+def wizard_column(family: str, n: int = 1 << 20, seed: int = 0) -> np.ndarray:
+    """int64 column of one of the encoding-wizard families: uniform ints of width 1..40; sorted sequences; geometric
+    runs; Zipf categoricals with 16..65536 keys; FOR-able data with 1% uniform outliers."""
+    g = rng(seed)
+    if family == "uniform":
+        w = int(g.integers(1, 41))
+        return g.integers(0, 1 << w, n, dtype=np.int64)
+    if family == "sorted":
+        gap = int(1 << int(g.integers(0, 17)))
+        start = int(g.integers(0, 1 << 40))
+        return np.int64(start) + np.cumsum(g.integers(0, gap + 1, n, dtype=np.int64))
+    if family == "runs":
+        mean = float(2 ** g.uniform(1, 7))
+        return geometric_runs(n, mean, seed=seed, value_bits=int(g.integers(4, 33)), dtype=np.int64)
+    if family == "zipf":
+        n_keys = int(2 ** g.uniform(4, 16))
+        kw = int(g.integers(16, 48))
+        keys = g.choice(1 << kw, size=n_keys, replace=False).astype(np.int64)
+        s = float(g.uniform(0.8, 1.6))
+        p = 1.0 / np.arange(1, n_keys + 1, dtype=np.float64) ** s
+        p /= p.sum()
+        return keys[g.choice(n_keys, size=n, p=p)]
+    if family == "outliers":
+        w = int(g.integers(1, 25))
+        x = np.int64(g.integers(0, 1 << 40)) + g.integers(0, 1 << w, n, dtype=np.int64)
+        m = g.random(n) < 0.01
+        x[m] = g.integers(0, 1 << 40, int(m.sum()), dtype=np.int64)
+        return x
+    raise ValueError(family)
+
+
+def wizard_columns(n_columns: int = 512, n: int = 1 << 20, seed: int = 99):
+    """The wizard data set: column c belongs to family c mod 5, with its own seed."""
+    return [(WIZARD_FAMILIES[c % len(WIZARD_FAMILIES)], wizard_column(WIZARD_FAMILIES[c % len(WIZARD_FAMILIES)], n,
+                                                                       seed * 100003 + c)) for c in range(n_columns)]

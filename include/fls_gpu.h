@@ -90,12 +90,16 @@ int fls_gpu_tune_load(const char* path);
  * src/expression/interpreter.cpp:105-137). */
 typedef struct fls_gpu_rowgroup fls_gpu_rowgroup;
 
+/* types[c] is the column's enum data_t value stored in one byte (the enum's size differs between C and C++). */
 int  fls_gpu_rowgroup_create(uint32_t n_columns, const uint8_t* const* host_columns, const uint64_t* column_sizes,
-                             const uint64_t* device_offsets, const enum data_t* types, fls_gpu_rowgroup** out);
+                             const uint64_t* device_offsets, const uint8_t* types, fls_gpu_rowgroup** out);
 void fls_gpu_rowgroup_destroy(fls_gpu_rowgroup* rg);
-/* Output offsets (bytes, 16-aligned) of each column's values in the row-group's output buffer, and its size. */
+/* Output offsets (bytes, 256-aligned) of each column's values in the row-group's output buffer, and its size.
+ * Each column occupies n_values rounded up to whole 1024-value vectors. */
 int  fls_gpu_rowgroup_layout(const fls_gpu_rowgroup* rg, uint64_t* out_offsets, uint64_t* out_total_bytes);
-/* mode 0: one persistent launch over the work list; mode 1: one launch per column page (baseline form). */
+/* mode 0: one persistent launch over the work list (dictionary pages are expanded first; expressions outside the
+ * persistent kernel's set decode with their per-page kernels); mode 1: one launch per column page (baseline form).
+ * dev_out must be 256-byte aligned and hold out_total_bytes. */
 int  fls_gpu_rowgroup_decode(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, int mode,
                              void* stream);
 /* Captures the persistent launch in a HIP graph once and replays it. */

@@ -1,0 +1,264 @@
+"""CPU oracle of the encoding wizard -- TEST INFRASTRUCTURE ONLY (imported by tests/ as the checker; never by the
+product path).
+
+What it restates, and how each part is pinned:
+  * candidate_sizes(x): exact encoded size of one 65536-value int64 row-group under every candidate expression of the
+    wizard's pool, in closed form. The reference finds the best expression by encoding the column with every
+    expression and comparing sizes (ExpColEncoder<PT>::encode_all, AllExpEncodedCol::GetBest,
+    src/encoder/exp_col_encoder.cpp:19-28, 117-129; pool src/cor/exp_pool.cpp:23-144). Page layout
+    (src/cor/lyt/page/page.cpp:120-130: header 8 B, arr_c u32 offsets, (vec_c-1)*arr_c u32 entry points, buffers),
+    per-vector buffers (include/fls/cor/exp/*/..._exp.hpp RPN tables), bit widths from ffor_analyze
+    (src/cor/prm/ffor/ffor_analyze.cpp:78-102, exact bit length), RLE split (src/cor/lyt/vec.cpp:50-71), dictionary
+    insertion order (src/cor/lyt/dic/dic_fac.cpp:259-330) and the HZL dictionary payload
+    (src/cor/eng/hzl_cmp.cpp:43-176). PINNED by tests/test_wizard_oracle.py against the byte length of the C oracle's
+    own encoder (oracle/fls_oracle.c, itself pinned to the reference's local_result size CSVs) for every candidate on
+    every data family; brute-force restatements of the delta/segment rules pin the vectorized forms.
+  * features(x): the 64 sample statistics the learned selector reads (a definition of this build, not of the
+    reference). PINNED by brute-force Python loops on small samples (tests/test_wizard_oracle.py).
+  * mlp_forward_backward(...): softmax cross-entropy MLP 64-256-256-K with the GPU kernels' rounding points
+    (BF16 operands, FP32 accumulation). PINNED by float64 finite differences of the loss (tests/test_wizard_oracle.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+CANDIDATES = (0, 1, 11, 6, 7, 16, 17, 22)  # expression ids (i64 pool)
+K = len(CANDIDATES)
+RG = 65536
+V = RG // 1024
+SAMPLE_VECTORS = (0, 21, 42, 63)
+N_FEATURES = 64
+H = 256
+KPAD = 32
+
+_U64 = np.uint64
+
+
+def bitlen(d: np.ndarray) -> np.ndarray:
+    """Exact bit length of unsigned 64-bit values (0 -> 0)."""
+    d = np.asarray(d, dtype=np.uint64)
+    out = np.zeros(d.shape, dtype=np.int64)
+    for b in range(64):
+        out = np.where((d >> _U64(b)) != 0, b + 1, out)
+    return out
+
+
+def _seg_deltas(x: np.ndarray, seg: int, mask=None) -> np.ndarray:
+    """Differences of consecutive values (mod 2^bits) inside aligned segments of `seg` values, 0 at each segment
+    start -- the transposed lane delta (unrsum) order expressed in the original order."""
+    d = np.zeros_like(x)
+    d[1:] = x[1:] - x[:-1]
+    d[::seg] = 0
+    if mask is not None:
+        d &= mask
+    return d
+
+
+def _vec_bw(x2: np.ndarray) -> np.ndarray:
+    """Per-row FFOR bit width of a (n_vec, 1024) unsigned array: bitlen(max - min)."""
+    mn = x2.min(axis=1)
+    mx = x2.max(axis=1)
+    return bitlen((mx - mn).astype(np.uint64))
+
+
+def hzl_size(dvals: np.ndarray) -> int:
+    """Bytes of the HZL-compressed dictionary payload (no exceptions; a full-width chunk ends the FFOR part)."""
+    n = dvals.size
+    pos, size = 0, 0
+    while n - pos >= 1024:
+        c = dvals[pos: pos + 1024].astype(np.uint64)
+        bw = int(bitlen(np.array([c.max() - c.min()], dtype=np.uint64))[0])
+        if bw >= 64:
+            break
+        size += 12 + 128 * bw
+        pos += 1024
+    return size + 12 + 8 * (n - pos)
+
+
+def dictionary(x: np.ndarray):
+    """Insertion-order dictionary: (values in code order, code of every element)."""
+    uniq, first, inv = np.unique(x, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")  # code c -> unique index
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    return uniq[order], rank[inv].astype(np.uint32)
+
+
+def candidate_sizes(x: np.ndarray) -> np.ndarray:
+    """Exact encoded bytes of a 65536-value int64 row-group under each candidate (order of CANDIDATES)."""
+    assert x.size == RG
+    u = np.ascontiguousarray(x).view(np.uint64)
+    u2 = u.reshape(V, 1024)
+    bwf = _vec_bw(u2)
+    bwd = _vec_bw(_seg_deltas(u, 64).reshape(V, 1024))
+    chg = np.ones(RG, dtype=bool)
+    chg[1:] = u[1:] != u[:-1]
+    chg[::1024] = True  # runs restart in every vector
+    runs = chg.reshape(V, 1024).sum(axis=1)
+    inner = chg.copy()
+    inner[::16] = False  # position deltas are 0 at every 16-value segment start
+    bwp = inner.reshape(V, 1024).any(axis=1).astype(np.int64)
+    dvals, codes = dictionary(u)
+    c2 = codes.reshape(V, 1024).astype(np.uint32)
+    bwc = bitlen((c2.max(axis=1) - c2.min(axis=1)).astype(np.uint64))
+    cd = _seg_deltas(codes.astype(np.uint32), 32).reshape(V, 1024)
+    bwcd = bitlen((cd.max(axis=1) - cd.min(axis=1)).astype(np.uint64))
+    dict_page = 8 + hzl_size(dvals)
+
+    def page(arr_c, bufs):
+        return 8 + 4 * arr_c * V + int(np.sum(bufs))
+
+    sizes = [
+        page(1, np.full(V, 8192)),                              # 0  UNCOMPRESSED
+        page(3, 9 + 128 * bwf),                                  # 1  FFOR
+        page(4, 137 + 128 * bwd),                                # 11 DELTA
+        page(5, 8 * runs + 131 + 128 * bwp),                     # 6  RLE, FFOR'd delta positions
+        page(2, 8 * runs + 2048),                                # 7  RLE, plain positions
+        dict_page + page(1, np.full(V, 4096)),                   # 16 DICT, plain codes
+        dict_page + page(3, 5 + 128 * bwc),                      # 17 DICT, FFOR codes
+        dict_page + page(4, 133 + 128 * bwcd),                   # 22 DICT, delta codes
+    ]
+    return np.array(sizes, dtype=np.uint64)
+
+
+def best(sizes: np.ndarray) -> np.ndarray:
+    return np.argmin(sizes, axis=-1).astype(np.int32)
+
+
+# ---- features ---------------------------------------------------------------------------------------------------
+def _log2f(v):
+    return np.log2(np.float32(v)).astype(np.float32)
+
+
+def features(x: np.ndarray) -> np.ndarray:
+    """64 float32 statistics of a 65536-value row-group, read from SAMPLE_VECTORS only."""
+    u = np.ascontiguousarray(x).view(np.uint64).reshape(V, 1024)
+    s = np.ascontiguousarray(u[list(SAMPLE_VECTORS)])  # (4, 1024)
+    si = s.view(np.int64)
+    S = s.shape[0]
+    flat = s.ravel()
+    bwf = _vec_bw(s)
+    bwd = np.array([_vec_bw(_seg_deltas(s[i], 64)[None])[0] for i in range(S)])
+    chg = np.ones_like(s, dtype=bool)
+    chg[:, 1:] = s[:, 1:] != s[:, :-1]
+    runs = chg.sum(axis=1)
+    inner = chg.copy()
+    inner[:, ::16] = False
+    bwp = inner.any(axis=1).astype(np.int64)
+    eq = int((s[:, 1:] == s[:, :-1]).sum())
+    nondec = int((si[:, 1:] >= si[:, :-1]).sum())
+    inc = int((si[:, 1:] > si[:, :-1]).sum())
+    uniq, cnt = np.unique(flat, return_counts=True)
+    D = int(uniq.size)
+    top = int(cnt.max())
+    _, codes = dictionary(flat)
+    c2 = codes.reshape(S, 1024).astype(np.uint32)
+    bwc = bitlen((c2.max(axis=1) - c2.min(axis=1)).astype(np.uint64))
+    cdl = np.stack([_seg_deltas(c2[i], 32) for i in range(S)])
+    bwcd = bitlen((cdl.max(axis=1) - cdl.min(axis=1)).astype(np.uint64))
+    umax, umin = int(flat.max()), int(flat.min())
+    smax, smin = int(si.max()), int(si.min())
+    bws = int(bitlen(np.array([(smax - smin) % (1 << 64)], dtype=np.uint64))[0])
+
+    f32 = np.float32
+    mbwf, mbwd, mruns, mbwp = (f32(np.sum(a)) / f32(S) for a in (bwf, bwd, runs, bwp))
+    mbwc, mbwcd = f32(np.sum(bwc)) / f32(S), f32(np.sum(bwcd)) / f32(S)
+    vv = f32(V)
+    est = np.array([
+        f32(8) + f32(4) * vv + f32(8192) * vv,
+        f32(8) + f32(12) * vv + vv * (f32(9) + f32(128) * mbwf),
+        f32(8) + f32(16) * vv + vv * (f32(137) + f32(128) * mbwd),
+        f32(8) + f32(20) * vv + vv * (f32(8) * mruns + f32(131) + f32(128) * mbwp),
+        f32(8) + f32(8) * vv + vv * (f32(8) * mruns + f32(2048)),
+        f32(28) + f32(8) * f32(D) + f32(4) * vv + f32(4096) * vv,
+        f32(28) + f32(8) * f32(D) + f32(12) * vv + vv * (f32(5) + f32(128) * mbwc),
+        f32(28) + f32(8) * f32(D) + f32(16) * vv + vv * (f32(133) + f32(128) * mbwcd),
+    ], dtype=np.float32)
+    F = np.zeros(N_FEATURES, dtype=np.float32)
+    F[0:8] = est / f32(RG * 8)
+    F[8:16] = _log2f(est) / f32(24)
+    F[16], F[17], F[18] = mbwf / f32(64), f32(bwf.max()) / f32(64), f32(bwf.min()) / f32(64)
+    F[19], F[20], F[21] = mbwd / f32(64), f32(bwd.max()) / f32(64), f32(bwd.min()) / f32(64)
+    F[22], F[23], F[24] = mruns / f32(1024), f32(runs.max()) / f32(1024), f32(runs.min()) / f32(1024)
+    F[25] = _log2f(mruns) / f32(10)
+    F[26] = mbwp
+    F[27] = f32(D) / f32(S * 1024)
+    F[28] = _log2f(D) / f32(12)
+    F[29] = f32(int(bitlen(np.array([D - 1], dtype=np.uint64))[0])) / f32(16)
+    F[30] = f32(top) / f32(S * 1024)
+    F[31] = f32(eq) / f32(S * 1023)
+    F[32] = f32(nondec) / f32(S * 1023)
+    F[33] = f32(inc) / f32(S * 1023)
+    F[34] = f32(int(bitlen(np.array([umax], dtype=np.uint64))[0])) / f32(64)
+    F[35] = f32(int(bitlen(np.array([umin], dtype=np.uint64))[0])) / f32(64)
+    F[36] = f32(bws) / f32(64)
+    F[37] = mbwc / f32(32)
+    F[38] = mbwcd / f32(32)
+    for i in range(S):
+        F[39 + 6 * i: 45 + 6 * i] = [f32(bwf[i]) / f32(64), f32(bwd[i]) / f32(64), f32(runs[i]) / f32(1024),
+                                      f32(bwp[i]), f32(bwc[i]) / f32(32), f32(bwcd[i]) / f32(32)]
+    F[63] = f32(1)
+    return F
+
+
+# ---- MLP ---------------------------------------------------------------------------------------------------------
+def bf16(a: np.ndarray) -> np.ndarray:
+    """Round float32 to bfloat16 (round to nearest even), returned as float32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def unpack_params(p: np.ndarray):
+    """Parameter block -> (W1t[256,64], b1, W2t[256,256], b2, W3t[32,256], b3)."""
+    o = 0
+    out = []
+    for shape in ((H, N_FEATURES), (H,), (H, H), (H,), (KPAD, H), (KPAD,)):
+        n = int(np.prod(shape))
+        out.append(p[o: o + n].reshape(shape))
+        o += n
+    return out
+
+
+N_PARAMS = H * N_FEATURES + H + H * H + H + KPAD * H + KPAD
+
+
+def mlp_forward_backward(p: np.ndarray, X: np.ndarray, y: np.ndarray, n_classes: int, rounding: bool = True):
+    """Loss (mean softmax cross-entropy over the first n_classes logits) and its gradient block. With rounding,
+    operands are rounded to bf16 where the GPU kernels store bf16 (inputs, weights, activations, upstream
+    gradients); sums are float32 (float64 without rounding)."""
+    r = bf16 if rounding else (lambda a: a)
+    dt = np.float32 if rounding else np.float64
+    W1t, b1, W2t, b2, W3t, b3 = (a.astype(dt) for a in unpack_params(p.astype(dt)))
+    B = X.shape[0]
+    Xb = r(X.astype(dt))
+    A1 = Xb @ r(W1t).T + b1
+    H1 = r(np.maximum(A1, 0))
+    A2 = H1 @ r(W2t).T + b2
+    H2 = r(np.maximum(A2, 0))
+    Z = H2 @ r(W3t).T + b3
+    Zk = Z[:, :n_classes]
+    m = Zk.max(axis=1, keepdims=True)
+    e = np.exp(Zk - m)
+    P = e / e.sum(axis=1, keepdims=True)
+    loss = float(np.mean(-np.log(P[np.arange(B), y])))
+    dZ = np.zeros((B, KPAD), dtype=dt)
+    dZ[:, :n_classes] = P
+    dZ[np.arange(B), y] -= 1
+    dZ /= B
+    dZb = r(dZ)
+    gW3t = dZb.T @ H2
+    gb3 = dZ.sum(axis=0)
+    dH2 = r((dZb @ r(W3t)) * (H2 > 0))
+    gW2t = dH2.T @ H1
+    gb2 = dH2.sum(axis=0)
+    dH1 = r((dH2 @ r(W2t)) * (H1 > 0))
+    gW1t = dH1.T @ Xb
+    gb1 = dH1.sum(axis=0)
+    g = np.concatenate([gW1t.ravel(), gb1, gW2t.ravel(), gb2, gW3t.ravel(), gb3]).astype(dt)
+    return loss, g, Z
+
+
+def mlp_loss(p, X, y, n_classes):
+    return mlp_forward_backward(p, X, y, n_classes, rounding=False)[0]

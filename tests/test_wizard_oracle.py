@@ -1,0 +1,123 @@
+"""CPU checks of the wizard oracle: closed-form candidate sizes against the encoders' byte counts, sample statistics
+against brute-force loops, and the MLP's analytic gradients against finite differences."""
+import numpy as np
+import pytest
+
+import fastlanes_amd as fa
+from fastlanes_amd import workloads as wl
+from oracle import wizard_oracle as wo
+
+
+def edge_rowgroups():
+    g = np.random.default_rng(5)
+    yield "constant", np.full(wo.RG, 7, dtype=np.int64)
+    yield "minus_one", np.full(wo.RG, -1, dtype=np.int64)  # all bits set
+    x = g.integers(-3, 3, wo.RG).astype(np.int64)
+    yield "small_signed", x
+    yield "alternating", np.tile(np.array([0, np.iinfo(np.int64).max], dtype=np.int64), wo.RG // 2)
+    yield "extremes", g.choice(np.array([np.iinfo(np.int64).min, -1, 0, np.iinfo(np.int64).max]), wo.RG)
+    yield "unique_desc", np.arange(wo.RG, 0, -1, dtype=np.int64) * 1_000_003
+    y = np.repeat(g.integers(0, 1 << 50, 5000), 14)[: wo.RG].astype(np.int64)
+    yield "runs_many_distinct", np.concatenate([y, np.zeros(wo.RG - y.size, dtype=np.int64)])
+
+
+@pytest.mark.parametrize("family", wl.WIZARD_FAMILIES)
+def test_candidate_sizes_equal_encoded_sizes(oracle, family):
+    for seed in range(2):
+        x = wl.wizard_column(family, wo.RG, seed=seed + 10)
+        s = wo.candidate_sizes(x)
+        for k, eid in enumerate(wo.CANDIDATES):
+            assert int(s[k]) == fa.encode_from_memory(x, eid).size == oracle.encode_column(x, eid).size, (family, eid)
+
+
+def test_candidate_sizes_edge_cases(oracle):
+    for name, x in edge_rowgroups():
+        s = wo.candidate_sizes(x)
+        for k, eid in enumerate(wo.CANDIDATES):
+            assert int(s[k]) == oracle.encode_column(x, eid).size, (name, eid)
+
+
+def test_hzl_size_full_width_chunk_ends_ffor_part(oracle):
+    """A 1024-entry dictionary chunk spanning all 64 bits switches the rest of the dictionary to raw values."""
+    g = np.random.default_rng(1)
+    keys = np.concatenate([g.integers(0, 100, 1024), [np.iinfo(np.int64).min, np.iinfo(np.int64).max],
+                           g.integers(0, 1 << 62, 3000)]).astype(np.int64)
+    keys = np.unique(keys)[: 2048 + 100]
+    g.shuffle(keys)
+    x = np.concatenate([keys, np.repeat(keys[:1], wo.RG - keys.size)])
+    s = wo.candidate_sizes(x)
+    for k in (5, 6, 7):
+        assert int(s[k]) == oracle.encode_column(x, wo.CANDIDATES[k]).size
+
+
+def _brute_features(x):
+    """Loop restatement of the integer statistics behind the features."""
+    s = [list(map(int, x[v * 1024:(v + 1) * 1024].view(np.uint64))) for v in wo.SAMPLE_VECTORS]
+    M = (1 << 64) - 1
+    out = {"bwf": [], "bwd": [], "runs": [], "bwp": [], "bwc": [], "bwcd": []}
+    seen, codes = {}, []
+    for vec in s:
+        for v in vec:
+            if v not in seen:
+                seen[v] = len(seen)
+            codes.append(seen[v])
+    for i, vec in enumerate(s):
+        out["bwf"].append((max(vec) - min(vec)).bit_length())
+        d = [0 if o % 64 == 0 else (vec[o] - vec[o - 1]) & M for o in range(1024)]
+        out["bwd"].append((max(d) - min(d)).bit_length())
+        out["runs"].append(1 + sum(vec[o] != vec[o - 1] for o in range(1, 1024)))
+        out["bwp"].append(int(any(vec[o] != vec[o - 1] for o in range(1, 1024) if o % 16)))
+        c = codes[i * 1024:(i + 1) * 1024]
+        out["bwc"].append((max(c) - min(c)).bit_length())
+        cd = [0 if o % 32 == 0 else (c[o] - c[o - 1]) & 0xFFFFFFFF for o in range(1024)]
+        out["bwcd"].append((max(cd) - min(cd)).bit_length())
+    flat = [v for vec in s for v in vec]
+    out["D"] = len(seen)
+    out["top"] = max(flat.count(v) for v in set(flat)) if len(seen) < 64 else max(
+        np.unique(np.array(flat, dtype=np.uint64), return_counts=True)[1])
+    return out
+
+
+@pytest.mark.parametrize("family", wl.WIZARD_FAMILIES)
+def test_features_against_loops(family):
+    x = wl.wizard_column(family, wo.RG, seed=3)
+    F = wo.features(x)
+    b = _brute_features(x)
+    assert F.dtype == np.float32 and F.shape == (64,)
+    for i in range(4):
+        assert F[39 + 6 * i] * 64 == b["bwf"][i]
+        assert F[40 + 6 * i] * 64 == b["bwd"][i]
+        assert F[41 + 6 * i] * 1024 == b["runs"][i]
+        assert F[42 + 6 * i] == b["bwp"][i]
+        assert F[43 + 6 * i] * 32 == b["bwc"][i]
+        assert F[44 + 6 * i] * 32 == b["bwcd"][i]
+    assert F[27] * 4096 == b["D"]
+    assert F[30] * 4096 == b["top"]
+    assert F[63] == 1.0
+    # the sample-size estimates of the non-dictionary candidates use the closed forms of candidate_sizes
+    assert abs(F[1] * wo.RG * 8 - (8 + 12 * 64 + 64 * (9 + 128 * np.mean(b["bwf"])))) < 1e-2 * wo.RG
+
+
+def test_mlp_gradients_match_finite_differences():
+    g = np.random.default_rng(0)
+    p = (g.standard_normal(wo.N_PARAMS) * 0.1).astype(np.float64)
+    X = g.standard_normal((6, 64))
+    y = g.integers(0, 8, 6)
+    loss, grad, _ = wo.mlp_forward_backward(p, X, y, 8, rounding=False)
+    idx = np.concatenate([g.choice(wo.N_PARAMS, 60, replace=False),
+                          [wo.N_PARAMS - 32, wo.N_PARAMS - 25, 256 * 64 + 3]])  # includes b3 and b1 entries
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        fd = (wo.mlp_loss(p + e, X, y, 8) - wo.mlp_loss(p - e, X, y, 8)) / 2e-6
+        assert abs(fd - grad[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, grad[i])
+    # padded classes get no gradient
+    W3t_g = grad[256 * 64 + 256 + 256 * 256 + 256:][: 32 * 256].reshape(32, 256)
+    assert np.all(W3t_g[8:] == 0)
+
+
+def test_bf16_rounding_is_nearest_even():
+    x = np.array([1.0, 1.00390625, 1.01171875, -2.5e-3, 3.0e38], dtype=np.float32)
+    r = wo.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.015625  # ties to even, above-half rounds up
+    assert abs(r[3] - -2.5e-3) < 2e-5
