@@ -1,7 +1,8 @@
 """Mixed row-group decode benchmark: 64 columns of uint8/16/32/64 with lengths uniform in [2^20, 2^26] values
 (multiples of 1024) and encodings drawn from bit-packing (w in 1..32), FFOR, delta, RLE and dictionary. The same
-device-resident row-group is decoded three ways -- one persistent launch over the tile list, that launch replayed
-from a HIP graph, and one launch per page -- and every column is compared with its input values.
+device-resident row-group is decoded two ways -- one launch per page, and the same launches captured once in a HIP
+graph and replayed -- and every column is compared with its input values. (The persistent row-group kernel is
+disabled in this build.)
 
     python -m fastlanes_amd.rowgroup_bench [--min-log2 20] [--max-log2 26] [--columns 64]
 """
@@ -78,8 +79,7 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
 
     res = {"columns": n_columns, "values": n_values, "decoded_bytes": dec_bytes, "encoded_bytes": enc_bytes,
            "host_build_s": round(t_build, 1)}
-    forms = {"persistent": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, sp),
-             "graph": lambda: rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), sp),
+    forms = {"graph": lambda: rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), sp),
              "per_page": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, sp)}
     for name, fn in forms.items():
         d_out.fill_(0)
@@ -90,7 +90,7 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
         res[name] = {"ms": round(t * 1e3, 3), "gvalues_s": round(n_values / t / 1e9, 2), "gb_s": round(gbs, 1),
                      "hbm_frac": round(gbs / HBM, 4)}
     res["graph_vs_per_page"] = round(res["per_page"]["ms"] / res["graph"]["ms"], 3)
-    res["persistent_vs_per_page"] = round(res["per_page"]["ms"] / res["persistent"]["ms"], 3)
+    res["persistent"] = "disabled in this build"
     kinds = {}
     for W, kind, n, _ in spec:
         kinds[f"{kind}{W}"] = kinds.get(f"{kind}{W}", 0) + 1

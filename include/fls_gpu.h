@@ -98,11 +98,12 @@ void fls_gpu_rowgroup_destroy(fls_gpu_rowgroup* rg);
  * Each column occupies n_values rounded up to whole 1024-value vectors. */
 int  fls_gpu_rowgroup_layout(const fls_gpu_rowgroup* rg, uint64_t* out_offsets, uint64_t* out_total_bytes);
 /* mode 0: one persistent launch over the work list (dictionary pages are expanded first; expressions outside the
- * persistent kernel's set decode with their per-page kernels); mode 1: one launch per column page (baseline form).
- * dev_out must be 256-byte aligned and hold out_total_bytes. */
+ * persistent kernel's set decode with their per-page kernels) -- disabled in this build, returns FLS_E_UNSUPPORTED;
+ * mode 1: one launch per column page (baseline form). dev_out must be 256-byte aligned and hold out_total_bytes. */
 int  fls_gpu_rowgroup_decode(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, int mode,
                              void* stream);
-/* Captures the persistent launch in a HIP graph once and replays it. */
+/* Captures the row-group's decode once in a HIP graph and replays it (one graph launch per call). With the
+ * persistent kernel disabled, the graph holds the per-page launches of mode 1. */
 int  fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, void* stream);
 int  fls_gpu_rowgroup_check(fls_gpu_rowgroup* rg, void* stream);
 

@@ -1,5 +1,8 @@
-"""Row-group decode parity: many columns of mixed widths and encodings through fls_gpu_rowgroup_* (persistent
-launch, per-page launches, HIP graph replay) against the CPU oracle, bit-exact."""
+"""Row-group decode parity: many columns of mixed widths and encodings through fls_gpu_rowgroup_* (per-page
+launches, and the same launches captured once in a HIP graph and replayed) against the CPU oracle, bit-exact.
+
+The persistent row-group kernel (mode 0) is disabled in this build: its first GPU run did not finish and the cause
+has not been found, so mode 0 must report FLS_E_UNSUPPORTED and its parity test is skipped."""
 import numpy as np
 import pytest
 
@@ -67,11 +70,10 @@ def test_rowgroup_mixed_all_modes(torch_cuda, oracle):
     rg, d_enc, want = build(torch, oracle, mixed_columns())
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
     assert d_out.data_ptr() % 256 == 0
-    for mode in (0, 1):
-        print(f"rowgroup mixed: mode {mode}", flush=True)
-        d_out.fill_(0xA5)
-        rg.decode(d_enc.data_ptr(), d_out.data_ptr(), mode, fa.stream_ptr())
-        verify(torch, rg, d_out, want, f"mode {mode}")
+    print("rowgroup mixed: per-page launches", flush=True)
+    d_out.fill_(0xA5)
+    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, fa.stream_ptr())
+    verify(torch, rg, d_out, want, "mode 1")
     # graph: captured once, replayed with the program's own work in between, every replay checked
     for rep in range(3):
         print(f"rowgroup mixed: graph replay {rep}", flush=True)
@@ -95,11 +97,13 @@ def test_rowgroup_fallback_expressions(torch_cuda, oracle):
         cols.append((wl.rowgroup_column("dict", 8 * np.dtype(dt).itemsize, 9216, 5).view(dt), 17))
     rg, d_enc, want = build(torch, oracle, cols)
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
-    for mode in (0, 1):
-        print(f"rowgroup fallback: mode {mode}", flush=True)
-        d_out.fill_(0x5A)
-        rg.decode(d_enc.data_ptr(), d_out.data_ptr(), mode, fa.stream_ptr())
-        verify(torch, rg, d_out, want, f"fallback mode {mode}")
+    print("rowgroup fallback: per-page launches and graph", flush=True)
+    d_out.fill_(0x5A)
+    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, fa.stream_ptr())
+    verify(torch, rg, d_out, want, "fallback mode 1")
+    d_out.fill_(0x5A)
+    rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), fa.stream_ptr())
+    verify(torch, rg, d_out, want, "fallback graph")
 
 
 def test_rowgroup_multi_page_columns(torch_cuda, oracle):
@@ -109,8 +113,8 @@ def test_rowgroup_multi_page_columns(torch_cuda, oracle):
     cols = [(wl.rowgroup_column("ffor", 8, n, 21), 1), (wl.rowgroup_column("delta", 16, n, 22), 11)]
     rg, d_enc, want = build(torch, oracle, cols)
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
-    print("rowgroup multi-page: persistent", flush=True)
-    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, fa.stream_ptr())
+    print("rowgroup multi-page: graph", flush=True)
+    rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), fa.stream_ptr())
     verify(torch, rg, d_out, want, "multi-page")
 
 
@@ -122,9 +126,9 @@ def test_rowgroup_spec_sample(torch_cuda, oracle):
         cols.append((wl.rowgroup_column(kind, W, n, seed), wl.ROWGROUP_EXPRS[kind]))
     rg, d_enc, want = build(torch, oracle, cols)
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
-    print("rowgroup spec sample: persistent + graph", flush=True)
-    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, fa.stream_ptr())
-    verify(torch, rg, d_out, want, "spec persistent")
+    print("rowgroup spec sample: per-page + graph", flush=True)
+    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, fa.stream_ptr())
+    verify(torch, rg, d_out, want, "spec per-page")
     d_out.zero_()
     rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), fa.stream_ptr())
     verify(torch, rg, d_out, want, "spec graph")
@@ -135,7 +139,7 @@ def test_rowgroup_empty_and_errors(torch_cuda):
     rg = fa.RowGroup([], [], [])
     assert rg.total_bytes == 0
     buf = torch.empty(256, dtype=torch.uint8, device="cuda")
-    rg.decode(buf.data_ptr(), buf.data_ptr(), 0)
+    rg.decode(buf.data_ptr(), buf.data_ptr(), 1)
     rg.check()
     x = np.arange(4096, dtype=np.uint32)
     enc = fa.encode_from_memory(x, 1)
@@ -144,3 +148,15 @@ def test_rowgroup_empty_and_errors(torch_cuda):
         rg.decode(buf.data_ptr(), buf.data_ptr() + 4, 0)  # output not 256-byte aligned
     with pytest.raises(fa.FlsError):
         rg.decode(buf.data_ptr(), buf.data_ptr(), 2)  # unknown mode
+    with pytest.raises(fa.FlsError, match="disabled"):
+        rg.decode(buf.data_ptr(), buf.data_ptr(), 0)  # persistent kernel switched off: error, no launch
+
+
+@pytest.mark.skip(reason="persistent row-group kernel disabled: its first GPU run did not finish and the cause has "
+                         "not been found; it stays off the GPU until one is shown")
+def test_rowgroup_persistent_kernel(torch_cuda, oracle):
+    torch = torch_cuda
+    rg, d_enc, want = build(torch, oracle, mixed_columns())
+    d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
+    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, fa.stream_ptr())
+    verify(torch, rg, d_out, want, "mode 0")
