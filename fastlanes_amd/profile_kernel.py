@@ -17,7 +17,7 @@ import fastlanes_amd as fa
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--width", type=int, default=64)
-    ap.add_argument("--bw", type=int, default=63)
+    ap.add_argument("--bw", default="63", help="bit width, or a comma-separated list")
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--plain", action="store_true", help="plain unpack (no FFOR bases)")
@@ -28,25 +28,30 @@ def main():
     dev = torch.device("cuda:0")
     n = 1 << a.log2n
     n_vec = n // 1024
-    W, bw = a.width, a.bw
+    W = a.width
     g = torch.Generator(device=dev)
     g.manual_seed(5)
-    packed = torch.randint(0, 256, (max(n_vec * 128 * bw, 16),), dtype=torch.uint8, device=dev, generator=g)
+    bws = [int(b) for b in a.bw.split(",")]
     bases = torch.zeros(n_vec * W // 8 + 16, dtype=torch.uint8, device=dev)
     out = torch.empty(n * W // 8 + 16, dtype=torch.uint8, device=dev)
     sp = fa.stream_ptr()
-    for _ in range(a.reps):
-        fa.unffor_stream(W, bw, packed.data_ptr(), None if a.plain else bases.data_ptr(), out.data_ptr(), n_vec, sp)
+    for bw in bws:
+        packed = torch.randint(0, 256, (max(n_vec * 128 * bw, 16),), dtype=torch.uint8, device=dev, generator=g)
+        for _ in range(a.reps):
+            fa.unffor_stream(W, bw, packed.data_ptr(), None if a.plain else bases.data_ptr(), out.data_ptr(), n_vec,
+                             sp)
+        del packed
     src = torch.ones(1 << (a.copy_log2_bytes - 2), dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(a.reps):
         torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
-    kbytes = n * bw / 8 + n * W / 8 + (0 if a.plain else n_vec * W / 8)
-    print(json.dumps({"kernel": f"k_unffor_stream<{W},{bw}>", "algorithmic_bytes": kbytes,
-                      "read_bytes": n * bw / 8 + (0 if a.plain else n_vec * W / 8), "write_bytes": n * W / 8,
-                      "copy_kernel": "torch mul (float32, elementwise)", "copy_bytes": src.numel() * 4,
-                      "reps": a.reps}), flush=True)
+    for bw in bws:
+        print(json.dumps({"kernel": f"k_unffor_stream<{W},{bw}>", "launch_order": bws.index(bw),
+                          "algorithmic_bytes": n * bw / 8 + n * W / 8 + (0 if a.plain else n_vec * W / 8),
+                          "read_bytes": n * bw / 8 + (0 if a.plain else n_vec * W / 8), "write_bytes": n * W / 8,
+                          "reps": a.reps}), flush=True)
+    print(json.dumps({"copy_kernel": "torch mul (float32, elementwise)", "copy_bytes": src.numel() * 4}), flush=True)
 
 
 if __name__ == "__main__":
