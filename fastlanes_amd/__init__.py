@@ -154,7 +154,8 @@ class RowGroup:
         self.total_bytes = int(total.value)
 
     def decode(self, dev_encoded: int, dev_out: int, mode: int = 0, stream: int = 0) -> None:
-        """mode 0: one persistent launch over the row-group's tile list; mode 1: one launch per page."""
+        """mode 1: one launch per page in column order; mode 2: the same launches over 4 parallel stream branches;
+        mode 0 (one persistent launch over the tile list) is disabled in this build."""
         check(lib().fls_gpu_rowgroup_decode(self.h, dev_encoded, dev_out, mode, stream), "rowgroup_decode")
 
     def decode_graph(self, dev_encoded: int, dev_out: int, stream: int = 0) -> None:

@@ -40,6 +40,11 @@ struct fls_gpu_rowgroup {
     hipGraphExec_t exec  = nullptr;
     const uint8_t* g_enc = nullptr;
     uint8_t*       g_out = nullptr;
+    // parallel branches of the per-page form
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t>  join;
+    hipEvent_t               fork = nullptr;
+    std::vector<size_t>      order;  // columns, largest first
 };
 
 namespace {
@@ -62,6 +67,9 @@ void free_rg(fls_gpu_rowgroup* rg) {
     if (rg->d_err) (void)hipFree(rg->d_err);
     if (rg->exec) (void)hipGraphExecDestroy(rg->exec);
     if (rg->graph) (void)hipGraphDestroy(rg->graph);
+    for (auto s : rg->branch) (void)hipStreamDestroy(s);
+    for (auto ev : rg->join) (void)hipEventDestroy(ev);
+    if (rg->fork) (void)hipEventDestroy(rg->fork);
     delete rg;
 }
 
@@ -76,6 +84,29 @@ int decode_pages(fls_gpu_rowgroup* rg, const uint8_t* enc, uint8_t* out, hipStre
         if (rc) return rc;
     }
     return FLS_OK;
+}
+
+// Per-page launches spread over parallel branches: `cs` forks into rg->branches streams (columns assigned longest
+// first to the least-loaded branch), which join back into `cs`. Inside a graph capture the branches become parallel
+// graph paths; outside, they overlap the ramp-up and drain of small kernels.
+int decode_pages_branched(fls_gpu_rowgroup* rg, const uint8_t* enc, uint8_t* out, hipStream_t cs) {
+    const unsigned nb = (unsigned)rg->branch.size();
+    hipError_t     e  = hipEventRecord(rg->fork, cs);
+    for (unsigned b = 0; b < nb && e == hipSuccess; ++b) e = hipStreamWaitEvent(rg->branch[b], rg->fork, 0);
+    if (e != hipSuccess) return hip_fail(e, "fork");
+    std::vector<uint64_t> load(nb, 0);
+    for (size_t c : rg->order) {
+        const unsigned b = (unsigned)(std::min_element(load.begin(), load.end()) - load.begin());
+        const uint64_t bytes = fls_gpu_plan_n_values(rg->plans[c]) * (dtype_bits(rg->types[c]) / 8);
+        int rc = fls_gpu_plan_decode(rg->plans[c], enc + rg->dev_off[c], out + rg->out_off[c], bytes, rg->branch[b]);
+        if (rc) return rc;
+        load[b] += bytes + fls_gpu_plan_encoded_size(rg->plans[c]);
+    }
+    for (unsigned b = 0; b < nb && e == hipSuccess; ++b) {
+        e = hipEventRecord(rg->join[b], rg->branch[b]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(cs, rg->join[b], 0);
+    }
+    return e == hipSuccess ? FLS_OK : hip_fail(e, "join");
 }
 
 int decode_fused(fls_gpu_rowgroup* rg, const uint8_t* enc, uint8_t* out, hipStream_t s) {
@@ -157,6 +188,24 @@ int fls_gpu_rowgroup_create(uint32_t n_columns, const uint8_t* const* host_colum
         if ((e = hipMemcpy(rg->d_tiles, rg->tiles.data(), b, hipMemcpyHostToDevice)) != hipSuccess)
             return free_rg(rg), hip_fail(e, "hipMemcpy");
     }
+    // parallel branches (default-priority, non-blocking streams) and the column order for them
+    const unsigned nb = (unsigned)std::min<size_t>(4, std::max<size_t>(1, rg->plans.size()));
+    if ((e = hipEventCreateWithFlags(&rg->fork, hipEventDisableTiming)) != hipSuccess) return free_rg(rg), hip_fail(e, "event");
+    for (unsigned b = 0; b < nb; ++b) {
+        hipStream_t st;
+        hipEvent_t  ev;
+        if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return free_rg(rg), hip_fail(e, "stream");
+        rg->branch.push_back(st);
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return free_rg(rg), hip_fail(e, "event");
+        rg->join.push_back(ev);
+    }
+    for (size_t c = 0; c < rg->plans.size(); ++c) rg->order.push_back(c);
+    std::stable_sort(rg->order.begin(), rg->order.end(), [&](size_t a, size_t b) {
+        auto work = [&](size_t c) {
+            return fls_gpu_plan_n_values(rg->plans[c]) * (dtype_bits(rg->types[c]) / 8) + fls_gpu_plan_encoded_size(rg->plans[c]);
+        };
+        return work(a) > work(b);
+    });
     int dev = 0;
     (void)hipGetDevice(&dev);
     hipDeviceProp_t prop;
@@ -186,8 +235,9 @@ int fls_gpu_rowgroup_decode(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, ui
     if (((uintptr_t)dev_out & 255) != 0) return fail(FLS_E_ARG, "row-group output must be 256-byte aligned");
     hipStream_t s = (hipStream_t)stream;
     if (mode == 0) return decode_fused(rg, dev_encoded, dev_out, s);
-    if (mode != 1) return fail(FLS_E_ARG, "mode must be 0 (persistent) or 1 (per page)");
-    return decode_pages(rg, dev_encoded, dev_out, s);
+    if (mode == 1) return decode_pages(rg, dev_encoded, dev_out, s);
+    if (mode == 2) return decode_pages_branched(rg, dev_encoded, dev_out, s);
+    return fail(FLS_E_ARG, "mode must be 0 (persistent), 1 (per page) or 2 (per page, parallel branches)");
 }
 
 int fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, void* stream) {
@@ -204,7 +254,8 @@ int fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encod
             (void)hipStreamDestroy(cs);
             return hip_fail(e, "begin capture");
         }
-        int rc = kPersistentEnabled ? decode_fused(rg, dev_encoded, dev_out, cs) : decode_pages(rg, dev_encoded, dev_out, cs);
+        int rc = kPersistentEnabled ? decode_fused(rg, dev_encoded, dev_out, cs)
+                                    : decode_pages_branched(rg, dev_encoded, dev_out, cs);
         e      = hipStreamEndCapture(cs, &rg->graph);
         (void)hipStreamDestroy(cs);
         if (rc) return rc;

@@ -1,8 +1,8 @@
 """Mixed row-group decode benchmark: 64 columns of uint8/16/32/64 with lengths uniform in [2^20, 2^26] values
 (multiples of 1024) and encodings drawn from bit-packing (w in 1..32), FFOR, delta, RLE and dictionary. The same
-device-resident row-group is decoded two ways -- one launch per page, and the same launches captured once in a HIP
-graph and replayed -- and every column is compared with its input values. (The persistent row-group kernel is
-disabled in this build.)
+device-resident row-group is decoded three ways -- one launch per page in column order (the per-column form), the
+same launches spread over 4 parallel stream branches, and those branches captured once in a HIP graph and replayed
+-- and every column is compared with its input values. (The persistent row-group kernel is disabled in this build.)
 
     python -m fastlanes_amd.rowgroup_bench [--min-log2 20] [--max-log2 26] [--columns 64]
 """
@@ -80,6 +80,7 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
     res = {"columns": n_columns, "values": n_values, "decoded_bytes": dec_bytes, "encoded_bytes": enc_bytes,
            "host_build_s": round(t_build, 1)}
     forms = {"graph": lambda: rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), sp),
+             "branched": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 2, sp),
              "per_page": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, sp)}
     for name, fn in forms.items():
         d_out.fill_(0)
@@ -90,11 +91,23 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
         res[name] = {"ms": round(t * 1e3, 3), "gvalues_s": round(n_values / t / 1e9, 2), "gb_s": round(gbs, 1),
                      "hbm_frac": round(gbs / HBM, 4)}
     res["graph_vs_per_page"] = round(res["per_page"]["ms"] / res["graph"]["ms"], 3)
+    res["branched_vs_per_page"] = round(res["per_page"]["ms"] / res["branched"]["ms"], 3)
     res["persistent"] = "disabled in this build"
-    kinds = {}
-    for W, kind, n, _ in spec:
-        kinds[f"{kind}{W}"] = kinds.get(f"{kind}{W}", 0) + 1
-    res["mix"] = kinds
+    # where the per-column time goes: each column decoded alone, grouped by (encoding, width)
+    per_kind = {}
+    for c, (W, kind, n, _) in enumerate(spec):
+        plan = fa.Plan(encs[c], vals[c].dtype)
+        base = d_enc.data_ptr() + offs[c]
+        dst = d_out.data_ptr() + rg.offsets[c]
+        t = _time(lambda: plan.decode(base, dst, vals[c].nbytes, sp), stream, 3, warm=1)
+        k = per_kind.setdefault(f"{kind}{W}", {"columns": 0, "ms": 0.0, "bytes": 0})
+        k["columns"] += 1
+        k["ms"] += t * 1e3
+        k["bytes"] += vals[c].nbytes + encs[c].size
+    for k in per_kind.values():
+        k["hbm_frac"] = round(k["bytes"] / (k["ms"] * 1e-3) / 1e9 / HBM, 3)
+        k["ms"] = round(k["ms"], 3)
+    res["per_kind_alone"] = dict(sorted(per_kind.items(), key=lambda kv: -kv[1]["ms"]))
     return res
 
 

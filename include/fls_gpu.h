@@ -99,11 +99,13 @@ void fls_gpu_rowgroup_destroy(fls_gpu_rowgroup* rg);
 int  fls_gpu_rowgroup_layout(const fls_gpu_rowgroup* rg, uint64_t* out_offsets, uint64_t* out_total_bytes);
 /* mode 0: one persistent launch over the work list (dictionary pages are expanded first; expressions outside the
  * persistent kernel's set decode with their per-page kernels) -- disabled in this build, returns FLS_E_UNSUPPORTED;
- * mode 1: one launch per column page (baseline form). dev_out must be 256-byte aligned and hold out_total_bytes. */
+ * mode 1: one launch per column page, in column order on `stream` (baseline form); mode 2: the same launches spread
+ * over parallel branches (4 streams forked from and joined back into `stream`). dev_out must be 256-byte aligned and
+ * hold out_total_bytes. */
 int  fls_gpu_rowgroup_decode(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, int mode,
                              void* stream);
 /* Captures the row-group's decode once in a HIP graph and replays it (one graph launch per call). With the
- * persistent kernel disabled, the graph holds the per-page launches of mode 1. */
+ * persistent kernel disabled, the graph holds the per-page launches of mode 2 as parallel graph branches. */
 int  fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, void* stream);
 int  fls_gpu_rowgroup_check(fls_gpu_rowgroup* rg, void* stream);
 

@@ -70,10 +70,11 @@ def test_rowgroup_mixed_all_modes(torch_cuda, oracle):
     rg, d_enc, want = build(torch, oracle, mixed_columns())
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
     assert d_out.data_ptr() % 256 == 0
-    print("rowgroup mixed: per-page launches", flush=True)
-    d_out.fill_(0xA5)
-    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, fa.stream_ptr())
-    verify(torch, rg, d_out, want, "mode 1")
+    for mode in (1, 2):
+        print(f"rowgroup mixed: per-page launches, mode {mode}", flush=True)
+        d_out.fill_(0xA5)
+        rg.decode(d_enc.data_ptr(), d_out.data_ptr(), mode, fa.stream_ptr())
+        verify(torch, rg, d_out, want, f"mode {mode}")
     # graph: captured once, replayed with the program's own work in between, every replay checked
     for rep in range(3):
         print(f"rowgroup mixed: graph replay {rep}", flush=True)
@@ -147,7 +148,7 @@ def test_rowgroup_empty_and_errors(torch_cuda):
     with pytest.raises(fa.FlsError):
         rg.decode(buf.data_ptr(), buf.data_ptr() + 4, 0)  # output not 256-byte aligned
     with pytest.raises(fa.FlsError):
-        rg.decode(buf.data_ptr(), buf.data_ptr(), 2)  # unknown mode
+        rg.decode(buf.data_ptr(), buf.data_ptr(), 3)  # unknown mode
     with pytest.raises(fa.FlsError, match="disabled"):
         rg.decode(buf.data_ptr(), buf.data_ptr(), 0)  # persistent kernel switched off: error, no launch
 
