@@ -109,6 +109,35 @@ def cascade(dev, stream, log2n=26, log2_strings=24):
     return out
 
 
+def main():
+    import argparse
+    import json
+
+    import torch
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--sections", default="pages,cascade", help="pages, cascade, rowgroup, wizard")
+    ap.add_argument("--log2n", type=int, default=26)
+    a = ap.parse_args()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    res = {}
+    for sec in a.sections.split(","):
+        if sec == "pages":
+            res["pages"] = pages(dev, stream, a.log2n)
+        elif sec == "cascade":
+            res["cascade"] = cascade(dev, stream, a.log2n)
+        elif sec == "rowgroup":
+            from fastlanes_amd import rowgroup_bench
+
+            res["rowgroup"] = rowgroup_bench.run(dev, stream)
+        elif sec == "wizard":
+            from fastlanes_amd import wizard
+
+            res["wizard"] = wizard.bench(dev)
+    print(json.dumps(res), flush=True)
+
+
 def run_all(dev):
     import torch
 
@@ -131,3 +160,7 @@ def run_all(dev):
         pass
     res["seconds"] = round(time.time() - t0, 1)
     return res
+
+
+if __name__ == "__main__":
+    main()

@@ -106,8 +106,7 @@ hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, uint32_t a
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
                       hipStream_t s) {
     if (!n_chunks) return hipSuccess;
-    const Tune t {256, 0, 0};
-    const dim3 g(grid_for(n_chunks, t, kTPV)), b(256), ge(n_chunks < 1024 ? n_chunks : 1024);
+    const dim3 g(n_chunks < 65535 ? n_chunks : 65535), b(256), ge(n_chunks < 1024 ? n_chunks : 1024);
     if (DW == 32) {
         hipLaunchKernelGGL((k_hzl_chunks<32>), g, b, 0, s, enc, chunks, n_chunks, dict);
         hipLaunchKernelGGL((k_hzl_exceptions<32>), ge, b, 0, s, enc, chunks, n_chunks, dict);

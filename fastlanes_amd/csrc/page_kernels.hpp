@@ -254,6 +254,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restri
     });
 }
 
+// Same, with the decoded dictionary copied into LDS once per block (after the waves' staging areas) so the code
+// gathers read LDS instead of L2. The host picks this form when dict_n * sizeof(D) fits beside the staging.
+template <typename T, typename D, int VAR, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_page_dict_lds(const uint8_t* __restrict__ page, uint32_t arr_c,
+                                                             uint32_t vec_c, const D* __restrict__ dict, uint32_t dict_n,
+                                                             uint8_t* __restrict__ out, uint32_t* err) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
+    if (dict_n == 0) return;  // the host rejects such pages
+    using WS  = WaveStage<32, 1024 * sizeof(T)>;
+    D* ldict  = reinterpret_cast<D*>(fls_lds + (blockDim.x / 64) * WS::BYTES);
+    for (uint32_t i = threadIdx.x; i < dict_n; i += blockDim.x) ldict[i] = dict[i];
+    __syncthreads();
+    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
+        dict_page_vector<T, D, VAR>(page, arr_c, v, t, ldict, dict_n, dst, err);
+    });
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // HZL dictionary payload chunks (src/cor/eng/hzl_de_cmp.cpp:28-77): an FFOR chunk decodes to 1024 values in array
 // order (the lane-major order of unffor); the raw tail is copied; exceptions are patched by a second launch.
@@ -268,19 +285,36 @@ struct HzlChunk {
     uint32_t pad;
 };
 
+// One block per chunk, one thread per (value row j, lane): every thread extracts its value straight from the two
+// packed words that hold it (word floor(j*bw/DW) and the next of its lane), so a chunk decodes with one round of
+// loads instead of a serial walk over its rows. Output in array order: out[j * L + lane] = value + base.
 template <int DW>
-__global__ __launch_bounds__(kMaxBlock) void k_hzl_chunks(const uint8_t* __restrict__ enc, const HzlChunk* chunks,
-                                                          uint32_t n_chunks, uint8_t* __restrict__ dict) {
-    using D = typename uint_of<DW>::type;
-    FLS_VECTOR_LOOP(n_chunks, kTPV) {
-        const HzlChunk ch  = chunks[v];
-        uint8_t*       out = dict + ch.dst * sizeof(D);
+__global__ __launch_bounds__(1024) void k_hzl_chunks(const uint8_t* __restrict__ enc, const HzlChunk* chunks,
+                                                     uint32_t n_chunks, uint8_t* __restrict__ dict) {
+    using D                = typename uint_of<DW>::type;
+    constexpr unsigned L   = 1024 / DW;
+    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        const HzlChunk ch  = chunks[c];
+        D*             out = reinterpret_cast<D*>(dict + ch.dst * sizeof(D));
+        const uint8_t* in  = enc + ch.src;
         if (ch.bw == 255) {
-            for (uint32_t k = t; k < ch.n; k += kTPV)
-                reinterpret_cast<D*>(out)[k] = load_t<D>(enc + ch.src + (uint64_t)k * sizeof(D));
+            for (uint32_t k = threadIdx.x; k < ch.n; k += blockDim.x) out[k] = load_t<D>(in + (uint64_t)k * sizeof(D));
             continue;
         }
-        dispatch_bw<DW>(ch.bw, [&](auto BW) { ffor_lane_major<DW, BW, false>(enc + ch.src, ch.base, t, out); });
+        const unsigned bw   = ch.bw;
+        const D        base = (D)ch.base;
+        const D        mask = bw >= DW ? (D)~(D)0 : (D)(((D)1 << bw) - 1);
+        for (uint32_t k = threadIdx.x; k < 1024; k += blockDim.x) {
+            const unsigned j = k / L, lane = k % L;
+            D              v = 0;
+            if (bw) {
+                const unsigned bit = j * bw, w = bit / DW, sh = bit % DW;
+                v                  = load_t<D>(in + ((uint64_t)w * L + lane) * sizeof(D)) >> sh;
+                if (sh + bw > DW) v |= load_t<D>(in + ((uint64_t)(w + 1) * L + lane) * sizeof(D)) << (DW - sh);
+                v &= mask;
+            }
+            out[k] = (D)(v + base);
+        }
     }
 }
 template <int DW>
