@@ -21,10 +21,16 @@ __device__ __forceinline__ const uint8_t* vbuf(const uint8_t* page, unsigned arr
 }
 
 // Calls f(integral_constant<BW>) for the runtime bw (bw > W calls nothing; the host validated every bw).
+// Each case ends with an empty asm statement naming its bit width: the cases then never share a tail, so the
+// compiler cannot sink their identical final stores into one block fed by per-case values (which keeps every
+// case's registers live across the whole switch and spills).
 template <int W, typename F>
 __device__ __forceinline__ void dispatch_bw(unsigned bw, F&& f) {
     static_for<W + 1>([&](auto b) {
-        if (bw == b) f(b);
+        if (bw == b) {
+            f(b);
+            asm volatile("; bw %0" ::"n"(decltype(b)::value));
+        }
     });
 }
 
