@@ -155,9 +155,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restric
 // (src/cor/lyt/vec.cpp:318-332). Positions are clamped to the vector's run count; a clamp sets err bit 1.
 // A thread owns one 32-bit column of the u16 position rows: two position lanes.
 // ---------------------------------------------------------------------------------------------------------------
-template <typename T, int BW, bool DELTA, typename S>
+// Run value idx: from the page's (possibly unaligned) run buffer (R = uint8_t), or from an aligned copy (R = T).
+template <typename T, typename R>
+__device__ __forceinline__ T run_value(const R* runs, uint32_t idx) {
+    if constexpr (sizeof(R) == 1) return load_t<T>(reinterpret_cast<const uint8_t*>(runs) + (uint64_t)idx * sizeof(T));
+    else return runs[idx];
+}
+
+template <typename T, int BW, bool DELTA, typename S, typename R>
 __device__ __forceinline__ void rle_vector(const uint8_t* packed, uint32_t base, uint32_t lane_base, unsigned t,
-                                           const uint8_t* runs, uint32_t n_runs, const S& out, uint32_t* err) {
+                                           const R* runs, uint32_t n_runs, const S& out, uint32_t* err) {
     uint32_t V[16];
     lanes_p32<16, BW, DELTA>(packed, base, lane_base, t, V);
     const uint32_t last = n_runs - 1;
@@ -168,43 +175,82 @@ __device__ __forceinline__ void rle_vector(const uint8_t* packed, uint32_t base,
             uint32_t idx = (V[p] >> (16 * l)) & 0xffffu;
             bad |= idx > last;
             idx     = idx > last ? last : idx;
-            vals[p] = load_t<T>(runs + (uint64_t)idx * sizeof(T));
+            vals[p] = run_value<T>(runs, idx);
         });
         store_run<T, 16, (8 * sizeof(T) < 16 ? 8 * sizeof(T) : 16)>(vals, sizeof(T) * seg_start<16>(2 * t + l), out);
     });
     if (bad) atomicOr(err, 1u);
 }
 
-template <typename T, int VAR, typename S>
-__device__ __forceinline__ void rle_page_vector(const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint64_t v,
-                                                unsigned t, const S& dst, uint32_t* err) {
-    {
-        const uint8_t* runs = vbuf(page, arr_c, v, 0);
-        // run count from the run buffer's extent: next vector's entry point, or the next buffer's offset
-        const uint8_t* runs_end = v + 1 < vec_c ? vbuf(page, arr_c, v + 1, 0) : vbuf(page, arr_c, 0, 1);
-        const uint32_t n_runs   = (uint32_t)((runs_end - runs) / sizeof(T));
-        if (n_runs == 0) {  // the host rejects such pages; never read an empty run buffer
-            atomicOr(err, 1u);
-            return;
-        }
-        if constexpr (VAR == 1) {
-            rle_vector<T, 16, false>(vbuf(page, arr_c, v, 1), 0, 0, t, runs, n_runs, dst, err);
-        } else {
-            const uint32_t lb   = reinterpret_cast<const u32_ua*>(vbuf(page, arr_c, v, 1))[t];
-            const unsigned bw   = *vbuf(page, arr_c, v, 2);
-            const uint32_t base = *reinterpret_cast<const u16_ua*>(vbuf(page, arr_c, v, 3));
-            const uint8_t* pk   = vbuf(page, arr_c, v, 4);
-            dispatch_bw<16>(bw, [&](auto BW) { rle_vector<T, BW, true>(pk, base, lb, t, runs, n_runs, dst, err); });
-        }
+// Run values of a vector with at most kRleLds runs are copied into LDS first and gathered from there.
+constexpr unsigned kRleLds = 256;
+
+template <typename T, int VAR, typename S, typename R>
+__device__ __forceinline__ void rle_positions(const uint8_t* page, uint32_t arr_c, uint64_t v, unsigned t, const R* runs,
+                                              uint32_t n_runs, const S& dst, uint32_t* err) {
+    if constexpr (VAR == 1) {
+        rle_vector<T, 16, false>(vbuf(page, arr_c, v, 1), 0, 0, t, runs, n_runs, dst, err);
+    } else {
+        const uint32_t lb   = reinterpret_cast<const u32_ua*>(vbuf(page, arr_c, v, 1))[t];
+        const unsigned bw   = *vbuf(page, arr_c, v, 2);
+        const uint32_t base = *reinterpret_cast<const u16_ua*>(vbuf(page, arr_c, v, 3));
+        const uint8_t* pk   = vbuf(page, arr_c, v, 4);
+        dispatch_bw<16>(bw, [&](auto BW) { rle_vector<T, BW, true>(pk, base, lb, t, runs, n_runs, dst, err); });
     }
 }
 
+template <typename T, int VAR, typename S>
+__device__ __forceinline__ void rle_page_vector(const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint64_t v,
+                                                unsigned t, const S& dst, uint32_t* err) {
+    const uint8_t* runs = vbuf(page, arr_c, v, 0);
+    // run count from the run buffer's extent: next vector's entry point, or the next buffer's offset
+    const uint8_t* runs_end = v + 1 < vec_c ? vbuf(page, arr_c, v + 1, 0) : vbuf(page, arr_c, 0, 1);
+    const uint32_t n_runs   = (uint32_t)((runs_end - runs) / sizeof(T));
+    if (n_runs == 0) {  // the host rejects such pages; never read an empty run buffer
+        atomicOr(err, 1u);
+        return;
+    }
+    rle_positions<T, VAR>(page, arr_c, v, t, runs, n_runs, dst, err);
+}
+
+// A wave decodes two vectors per step (32 threads each). Per step: each vector's runs go to LDS when they fit,
+// positions are decoded and the run values gathered into the wave's output staging, which is then written out.
 template <typename T, int VAR, bool NT>
 __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
                                                         uint8_t* __restrict__ out, uint32_t* err) {
-    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
-        rle_page_vector<T, VAR>(page, arr_c, vec_c, v, t, dst, err);
-    });
+    extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
+    using WS              = WaveStage<32, 1024 * sizeof(T)>;
+    constexpr unsigned RB = kRleLds * sizeof(T);
+    const unsigned     lane = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
+    const unsigned     t = lane % 32, slot = lane / 32;
+    uint8_t*           wl    = fls_lds + wave * (WS::BYTES + WS::VPW * RB);
+    T*                 lruns = reinterpret_cast<T*>(wl + WS::BYTES + slot * RB);
+    const LdsSink      sink {wl + slot * WS::PVB};
+    const uint64_t     N      = vec_c;
+    const uint64_t     stride = (uint64_t)gridDim.x * waves * WS::VPW;
+    for (uint64_t v0 = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW; v0 < N; v0 += stride) {
+        const uint64_t v      = v0 + slot;
+        const uint8_t* runs   = nullptr;
+        uint32_t       n_runs = 0;
+        if (v < N) {
+            runs                    = vbuf(page, arr_c, v, 0);
+            const uint8_t* runs_end = v + 1 < N ? vbuf(page, arr_c, v + 1, 0) : vbuf(page, arr_c, 0, 1);
+            n_runs                  = (uint32_t)((runs_end - runs) / sizeof(T));
+            if (n_runs <= kRleLds)
+                for (uint32_t i = t; i < n_runs; i += 32) lruns[i] = load_t<T>(runs + (uint64_t)i * sizeof(T));
+        }
+        wave_sync();
+        if (v < N) {
+            if (n_runs == 0) atomicOr(err, 1u);  // the host rejects such pages; never read an empty run buffer
+            else if (n_runs <= kRleLds) rle_positions<T, VAR>(page, arr_c, v, t, lruns, n_runs, sink, err);
+            else rle_positions<T, VAR>(page, arr_c, v, t, runs, n_runs, sink, err);
+        }
+        wave_sync();
+        const uint64_t left = N - v0;
+        wave_copy_out<32, 1024 * sizeof(T), NT>(wl, out + v0 * 1024 * sizeof(T), left < WS::VPW ? (uint32_t)left : WS::VPW,
+                                                lane);
+        wave_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
