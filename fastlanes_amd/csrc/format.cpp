@@ -210,6 +210,29 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
         }
         if (!ok) return "vector " + std::to_string(v) + " exceeds its page buffers";
     }
+    for (unsigned b = 0; b < arr_c; ++b) pe.boff[b] = (uint32_t)boff[b];
+    {  // fixed per-vector sizes of the metadata buffers (0 = variable size)
+        unsigned fsz[8] = {};
+        switch (e.exp_t) {
+        case EXP_UNCOMPRESSED: fsz[0] = 1024 * sz; break;
+        case EXP_FFOR: fsz[0] = 1, fsz[1] = sz; break;
+        case EXP_DELTA: fsz[0] = 128, fsz[1] = 1, fsz[2] = sz; break;
+        case EXP_RLE:
+            if (e.var == 1) fsz[1] = 2048;
+            else fsz[1] = 128, fsz[2] = 1, fsz[3] = 2;
+            break;
+        case EXP_FSST: fsz[1] = 4096; break;
+        default:
+            if (e.var == 1) fsz[0] = 4096;
+            else if (e.var == 2) fsz[0] = 1, fsz[1] = 4;
+            else fsz[0] = 128, fsz[1] = 1, fsz[2] = 4;
+        }
+        bool fixed = true;
+        for (unsigned v = 0; v < pe.vec_c && fixed; ++v)
+            for (unsigned b = 0; b < arr_c; ++b)
+                if (fsz[b] && ep(v, b) != (uint64_t)v * fsz[b]) fixed = false;
+        pe.fixed_meta = fixed;
+    }
     // page size: last buffer's start + last vector's slice end
     {
         const unsigned v = pe.vec_c - 1, b = arr_c - 1;

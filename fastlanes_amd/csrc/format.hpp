@@ -73,6 +73,8 @@ struct PageEntry {
     uint64_t out_index = 0;      // first decoded value
     uint32_t vec_c = 0, arr_c = 0;
     uint8_t  exp_t = 0, var = 0;
+    uint32_t boff[8]    = {};     // buffer offsets from the page start
+    bool     fixed_meta = false;  // every fixed-size buffer (bw, base, lane bases, plain values) sits at v * size
     // dictionary (integer columns)
     uint32_t               dict_n = 0;
     std::vector<HzlChunkH> hzl;

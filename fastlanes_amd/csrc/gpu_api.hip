@@ -125,10 +125,13 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
         const uint8_t*   page = dev_encoded + pe.page_off;
         uint8_t*         out  = (uint8_t*)dev_out + pe.out_index * (W / 8);
         hipError_t       e    = hipSuccess;
+        PageMeta pm {};
+        for (unsigned b = 0; b < pe.arr_c && b < 6; ++b) pm.off[b] = pe.boff[b];
+        pm.arr_c = pe.arr_c, pm.fixed = pe.fixed_meta;
         switch (pe.exp_t) {
-        case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pe.arr_c, pe.vec_c, out, s); break;
-        case EXP_FFOR: e = launch_page_int(W, 1, page, pe.arr_c, pe.vec_c, out, s); break;
-        case EXP_DELTA: e = launch_page_int(W, 2, page, pe.arr_c, pe.vec_c, out, s); break;
+        case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pm, pe.vec_c, out, s); break;
+        case EXP_FFOR: e = launch_page_int(W, 1, page, pm, pe.vec_c, out, s); break;
+        case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, out, s); break;
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pe.arr_c, pe.vec_c, out, plan->d_err, s); break;
         default: {
             DevDict& d = plan->dicts[i];

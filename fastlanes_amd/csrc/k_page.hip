@@ -67,14 +67,14 @@ int tune_load(const char* path) {
     return n;
 }
 
-hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out,
+hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                            hipStream_t s) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_int_w<8>(mode, page, arr_c, vec_c, out, s);
-    case 16: return page_int_w<16>(mode, page, arr_c, vec_c, out, s);
-    case 32: return page_int_w<32>(mode, page, arr_c, vec_c, out, s);
-    case 64: return page_int_w<64>(mode, page, arr_c, vec_c, out, s);
+    case 8: return page_int_w<8>(mode, page, pm, vec_c, out, s);
+    case 16: return page_int_w<16>(mode, page, pm, vec_c, out, s);
+    case 32: return page_int_w<32>(mode, page, pm, vec_c, out, s);
+    case 64: return page_int_w<64>(mode, page, pm, vec_c, out, s);
     default: return hipErrorInvalidValue;
     }
 }

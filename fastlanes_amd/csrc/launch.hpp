@@ -6,6 +6,7 @@
 namespace fls {
 
 struct HzlChunk;
+struct PageMeta;
 
 // Kernel families with separately tuned launch shapes.
 enum Family : int {
@@ -40,7 +41,7 @@ inline unsigned grid_for(uint64_t n_vec, const Tune& t, unsigned tpv) {
 
 hipError_t launch_unffor_stream(unsigned W, unsigned bw, const uint8_t* packed, const void* bases, uint8_t* out,
                                 uint64_t n_vec, hipStream_t s);
-hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out,
+hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                            hipStream_t s);
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out,
                            uint32_t* err, hipStream_t s);
@@ -51,7 +52,7 @@ hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, u
 
 // per-width instantiation units (k_page_w*.hip)
 template <int W>
-hipError_t page_int_w(int mode, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out, hipStream_t s);
+hipError_t page_int_w(int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, hipStream_t s);
 template <int W>
 hipError_t page_rle_w(int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out, uint32_t* err,
                       hipStream_t s);

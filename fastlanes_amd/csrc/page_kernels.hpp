@@ -141,12 +141,81 @@ __device__ __forceinline__ void int_page_vector(const uint8_t* page, uint32_t ar
     }
 }
 
+// Page layout handed to the kernels by value: buffer offsets from the page start, buffer count, and whether the
+// fixed-size metadata buffers sit at v * size (then their addresses need no entry-point load).
+struct PageMeta {
+    uint32_t off[6];
+    uint32_t arr_c;
+    uint32_t fixed;
+};
+
+// Per-vector metadata of an integer page: bit width, FFOR base, the thread's lane base (DELTA) and the entry point of
+// the packed (or plain) value buffer. With fixed-stride metadata buffers the loads are independent of each other.
+struct IntMeta {
+    uint64_t base, lb;
+    uint32_t bw, ep;
+};
+
+template <int W, int MODE>
+__device__ __forceinline__ IntMeta int_meta(const uint8_t* page, const PageMeta& pm, uint64_t v, unsigned t) {
+    using T              = typename uint_of<W>::type;
+    constexpr unsigned o = MODE == 2 ? 1 : 0;            // bw buffer
+    constexpr unsigned P = MODE == 0 ? 0 : o + 2;        // value buffer
+    const u32_ua*      eps = reinterpret_cast<const u32_ua*>(page + 8 + 4 * pm.arr_c);
+    IntMeta            m {};
+    if constexpr (MODE == 0) {
+        m.ep = pm.fixed ? (uint32_t)(v * (128 * W)) : (v ? eps[(v - 1) * pm.arr_c] : 0u);
+        return m;
+    } else {
+        if (pm.fixed) {
+            m.bw   = page[pm.off[o] + v];
+            m.base = load_t<T>(page + pm.off[o + 1] + v * (W / 8));
+            if constexpr (MODE == 2) m.lb = lane_base_of<W>(page + pm.off[0] + v * 128, t);
+        } else {
+            m.bw   = *vbuf(page, pm.arr_c, v, o);
+            m.base = load_t<T>(vbuf(page, pm.arr_c, v, o + 1));
+            if constexpr (MODE == 2) m.lb = lane_base_of<W>(vbuf(page, pm.arr_c, v, 0), t);
+        }
+        m.ep = v ? eps[(v - 1) * pm.arr_c + P] : 0u;
+        return m;
+    }
+}
+
+// Staged loop as staged_vector_loop, software-pipelined one step deep: the metadata of the vector a thread decodes
+// next is loaded right after the current vector's values are decoded, so it arrives while the wave writes its
+// output, and each step waits on one round of loads (the packed values) instead of three dependent ones.
 template <int W, int MODE, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
+__global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
                                                         uint8_t* __restrict__ out) {
-    staged_vector_loop<lane_tpv<W>(), 128 * W, NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
-        int_page_vector<W, MODE>(page, arr_c, v, t, dst);
-    });
+    extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
+    constexpr unsigned TPV = lane_tpv<W>(), VB = 128 * W;
+    constexpr unsigned P   = MODE == 0 ? 0 : (MODE == 2 ? 3 : 2);
+    using WS               = WaveStage<TPV, VB>;
+    const unsigned lane = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
+    const unsigned t = lane % TPV, slot = lane / TPV;
+    uint8_t*       wl = fls_lds + wave * WS::BYTES;
+    const LdsSink  sink {wl + slot * WS::PVB};
+    const uint64_t N      = vec_c;
+    const uint64_t stride = (uint64_t)gridDim.x * waves * WS::VPW;
+    uint64_t       v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
+    IntMeta        m {};
+    if (v0 + slot < N) m = int_meta<W, MODE>(page, pm, v0 + slot, t);
+    for (; v0 < N; v0 += stride) {
+        const uint64_t v = v0 + slot;
+        if (v < N) {
+            const uint8_t* pk = page + pm.off[P] + m.ep;
+            if constexpr (MODE == 0) {
+                ffor_untransposed<W, W, false>(pk, 0, 0, t, sink);
+            } else {
+                dispatch_bw<W>(m.bw, [&](auto BW) { ffor_untransposed<W, BW, MODE == 2>(pk, m.base, m.lb, t, sink); });
+            }
+        }
+        if (v + stride < N) m = int_meta<W, MODE>(page, pm, v + stride, t);
+        wave_sync();
+        const uint64_t left = N - v0;
+        wave_copy_out<TPV, VB, NT>(wl, out + v0 * VB, left < WS::VPW ? (uint32_t)left : WS::VPW, lane);
+        wave_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------

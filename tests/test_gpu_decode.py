@@ -274,3 +274,46 @@ def test_dictionary_page_with_hzl_exceptions(torch_cuda, oracle, dtype):
     torch.cuda.synchronize()
     plan.check()
     np.testing.assert_array_equal(to_np(d_out, dtype, x.size), expect)
+
+
+def _respace_ffor_page(enc: np.ndarray, W: int) -> np.ndarray:
+    """Rewrite a single-page FFOR column so that its bw buffer holds 2 bytes per vector (bw, padding): a valid page
+    whose metadata does not sit at fixed strides, which sends the kernels down the entry-point path."""
+    hdr = enc[:8].copy()
+    vec_c = int(hdr[4]) | (int(hdr[5]) << 8)
+    arr_c = 3
+    off = enc[8:8 + 4 * arr_c].view(np.uint32).astype(np.int64)
+    ep = np.zeros((vec_c, arr_c), dtype=np.int64)
+    if vec_c > 1:
+        ep[1:] = enc[8 + 4 * arr_c: 8 + 4 * arr_c + 4 * arr_c * (vec_c - 1)].view(np.uint32).reshape(vec_c - 1, arr_c)
+    sz = W // 8
+    bws = np.array([enc[off[0] + ep[v, 0]] for v in range(vec_c)], dtype=np.uint8)
+    bw_buf = np.zeros(2 * vec_c, dtype=np.uint8)
+    bw_buf[0::2] = bws
+    bw_buf[1::2] = 0xEE
+    base_buf = enc[off[1]: off[1] + sz * vec_c]
+    packed_buf = enc[off[2]: off[2] + int(ep[-1, 2]) + 128 * int(bws[-1])]
+    new_ep = ep.copy()
+    new_ep[:, 0] = 2 * np.arange(vec_c)
+    meta = 8 + 4 * arr_c + 4 * arr_c * (vec_c - 1)
+    o0 = meta
+    o1 = o0 + bw_buf.size
+    o2 = o1 + base_buf.size
+    parts = [hdr, np.array([o0, o1, o2], dtype=np.uint32).view(np.uint8),
+             new_ep[1:].astype(np.uint32).ravel().view(np.uint8), bw_buf, base_buf, packed_buf]
+    return np.concatenate(parts)
+
+
+@pytest.mark.parametrize("W", [8, 32, 64])
+def test_ffor_page_with_strided_metadata(torch_cuda, oracle, W):
+    torch = torch_cuda
+    dt = W2DT[W]
+    x = np.random.default_rng(W).integers(0, 1 << min(W - 1, 11), 40 * 1024).astype(dt)
+    enc = fa.encode_from_memory(x, 1)
+    page = _respace_ffor_page(enc, W)
+    np.testing.assert_array_equal(oracle.decode_column(page, dt, x.size), x)
+    plan = fa.Plan(page, dt)
+    d_in, d_out = dev(torch, page), empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(to_np(d_out, dt, x.size), x)
