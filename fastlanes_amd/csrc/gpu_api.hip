@@ -132,12 +132,12 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
         case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pm, pe.vec_c, out, s); break;
         case EXP_FFOR: e = launch_page_int(W, 1, page, pm, pe.vec_c, out, s); break;
         case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, out, s); break;
-        case EXP_RLE: e = launch_page_rle(W, pe.var, page, pe.arr_c, pe.vec_c, out, plan->d_err, s); break;
+        case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s); break;
         default: {
             DevDict& d = plan->dicts[i];
             e          = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, s);
             if (e == hipSuccess)
-                e = launch_page_dict(W, pe.var, page, pe.arr_c, pe.vec_c, d.values, pe.dict_n, out, plan->d_err, s);
+                e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, out, plan->d_err, s);
         }
         }
         if (e != hipSuccess) return hip_fail(e, "kernel launch");

@@ -79,26 +79,26 @@ hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const Page
     }
 }
 
-hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out,
+hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                            uint32_t* err, hipStream_t s) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_rle_w<8>(var, page, arr_c, vec_c, out, err, s);
-    case 16: return page_rle_w<16>(var, page, arr_c, vec_c, out, err, s);
-    case 32: return page_rle_w<32>(var, page, arr_c, vec_c, out, err, s);
-    case 64: return page_rle_w<64>(var, page, arr_c, vec_c, out, err, s);
+    case 8: return page_rle_w<8>(var, page, pm, vec_c, out, err, s);
+    case 16: return page_rle_w<16>(var, page, pm, vec_c, out, err, s);
+    case 32: return page_rle_w<32>(var, page, pm, vec_c, out, err, s);
+    case 64: return page_rle_w<64>(var, page, pm, vec_c, out, err, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c,
+hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,
                             const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_dict_w<8>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
-    case 16: return page_dict_w<16>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
-    case 32: return page_dict_w<32>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
-    case 64: return page_dict_w<64>(var, page, arr_c, vec_c, dict, dict_n, out, err, s);
+    case 8: return page_dict_w<8>(var, page, pm, vec_c, dict, dict_n, out, err, s);
+    case 16: return page_dict_w<16>(var, page, pm, vec_c, dict, dict_n, out, err, s);
+    case 32: return page_dict_w<32>(var, page, pm, vec_c, dict, dict_n, out, err, s);
+    case 64: return page_dict_w<64>(var, page, pm, vec_c, dict, dict_n, out, err, s);
     default: return hipErrorInvalidValue;
     }
 }

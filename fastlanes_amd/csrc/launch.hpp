@@ -43,9 +43,9 @@ hipError_t launch_unffor_stream(unsigned W, unsigned bw, const uint8_t* packed, 
                                 uint64_t n_vec, hipStream_t s);
 hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                            hipStream_t s);
-hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out,
+hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                            uint32_t* err, hipStream_t s);
-hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c,
+hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,
                             const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s);
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
                       hipStream_t s);
@@ -54,10 +54,10 @@ hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, u
 template <int W>
 hipError_t page_int_w(int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, hipStream_t s);
 template <int W>
-hipError_t page_rle_w(int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, uint8_t* out, uint32_t* err,
+hipError_t page_rle_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err,
                       hipStream_t s);
 template <int W>
-hipError_t page_dict_w(int var, const uint8_t* page, uint32_t arr_c, uint32_t vec_c, const void* dict, uint32_t dict_n,
-                       uint8_t* out, uint32_t* err, hipStream_t s);
+hipError_t page_dict_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, const void* dict,
+                       uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s);
 
 }  // namespace fls

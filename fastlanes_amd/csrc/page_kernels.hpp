@@ -282,14 +282,45 @@ __device__ __forceinline__ void rle_page_vector(const uint8_t* page, uint32_t ar
     rle_positions<T, VAR>(page, arr_c, v, t, runs, n_runs, dst, err);
 }
 
-// A wave decodes two vectors per step (32 threads each). Per step: each vector's runs go to LDS when they fit,
-// positions are decoded and the run values gathered into the wave's output staging, which is then written out.
+// Per-vector metadata of an RLE page: run-value bytes [r0, r1) of buffer 0, and for the positions: lane base
+// (VAR 2), bit width, base, entry point of the position buffer.
+struct RleMeta {
+    uint32_t r0, r1, lb, bw, base, ep;
+};
+
+template <int VAR>
+__device__ __forceinline__ RleMeta rle_meta(const uint8_t* page, const PageMeta& pm, uint64_t v, uint64_t N, unsigned t) {
+    const u32_ua* eps = reinterpret_cast<const u32_ua*>(page + 8 + 4 * pm.arr_c);
+    RleMeta       m {};
+    m.r0 = v ? eps[(v - 1) * pm.arr_c] : 0u;
+    m.r1 = v + 1 < N ? eps[v * pm.arr_c] : pm.off[1] - pm.off[0];
+    if constexpr (VAR == 1) {
+        m.ep = pm.fixed ? (uint32_t)(v * 2048) : (v ? eps[(v - 1) * pm.arr_c + 1] : 0u);
+    } else {
+        if (pm.fixed) {
+            m.lb   = reinterpret_cast<const u32_ua*>(page + pm.off[1] + 128 * v)[t];
+            m.bw   = page[pm.off[2] + v];
+            m.base = *reinterpret_cast<const u16_ua*>(page + pm.off[3] + 2 * v);
+        } else {
+            m.lb   = reinterpret_cast<const u32_ua*>(vbuf(page, pm.arr_c, v, 1))[t];
+            m.bw   = *vbuf(page, pm.arr_c, v, 2);
+            m.base = *reinterpret_cast<const u16_ua*>(vbuf(page, pm.arr_c, v, 3));
+        }
+        m.ep = v ? eps[(v - 1) * pm.arr_c + 4] : 0u;
+    }
+    return m;
+}
+
+// A wave decodes two vectors per step (32 threads each), pipelined like k_page_int. Per step: each vector's runs go
+// to LDS when they fit, positions are decoded and the run values gathered into the wave's output staging, which is
+// then written out.
 template <typename T, int VAR, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
+__global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
                                                         uint8_t* __restrict__ out, uint32_t* err) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     using WS              = WaveStage<32, 1024 * sizeof(T)>;
     constexpr unsigned RB = kRleLds * sizeof(T);
+    constexpr unsigned P  = VAR == 1 ? 1 : 4;
     const unsigned     lane = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
     const unsigned     t = lane % 32, slot = lane / 32;
     uint8_t*           wl    = fls_lds + wave * (WS::BYTES + WS::VPW * RB);
@@ -297,23 +328,32 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
     const LdsSink      sink {wl + slot * WS::PVB};
     const uint64_t     N      = vec_c;
     const uint64_t     stride = (uint64_t)gridDim.x * waves * WS::VPW;
-    for (uint64_t v0 = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW; v0 < N; v0 += stride) {
+    uint64_t           v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
+    RleMeta            m {};
+    if (v0 + slot < N) m = rle_meta<VAR>(page, pm, v0 + slot, N, t);
+    for (; v0 < N; v0 += stride) {
         const uint64_t v      = v0 + slot;
-        const uint8_t* runs   = nullptr;
-        uint32_t       n_runs = 0;
-        if (v < N) {
-            runs                    = vbuf(page, arr_c, v, 0);
-            const uint8_t* runs_end = v + 1 < N ? vbuf(page, arr_c, v + 1, 0) : vbuf(page, arr_c, 0, 1);
-            n_runs                  = (uint32_t)((runs_end - runs) / sizeof(T));
-            if (n_runs <= kRleLds)
-                for (uint32_t i = t; i < n_runs; i += 32) lruns[i] = load_t<T>(runs + (uint64_t)i * sizeof(T));
-        }
+        const uint8_t* runs   = page + pm.off[0] + m.r0;
+        const uint32_t n_runs = (uint32_t)((m.r1 - m.r0) / sizeof(T));
+        if (v < N && n_runs <= kRleLds)
+            for (uint32_t i = t; i < n_runs; i += 32) lruns[i] = load_t<T>(runs + (uint64_t)i * sizeof(T));
         wave_sync();
         if (v < N) {
+            const uint8_t* pk = page + pm.off[P] + m.ep;
+            auto positions    = [&](auto runs_src) {
+                if constexpr (VAR == 1) {
+                    rle_vector<T, 16, false>(pk, 0, 0, t, runs_src, n_runs, sink, err);
+                } else {
+                    dispatch_bw<16>(m.bw, [&](auto BW) {
+                        rle_vector<T, BW, true>(pk, m.base, m.lb, t, runs_src, n_runs, sink, err);
+                    });
+                }
+            };
             if (n_runs == 0) atomicOr(err, 1u);  // the host rejects such pages; never read an empty run buffer
-            else if (n_runs <= kRleLds) rle_positions<T, VAR>(page, arr_c, v, t, lruns, n_runs, sink, err);
-            else rle_positions<T, VAR>(page, arr_c, v, t, runs, n_runs, sink, err);
+            else if (n_runs <= kRleLds) positions(static_cast<const T*>(lruns));
+            else positions(runs);
         }
+        if (v + stride < N) m = rle_meta<VAR>(page, pm, v + stride, N, t);
         wave_sync();
         const uint64_t left = N - v0;
         wave_copy_out<32, 1024 * sizeof(T), NT>(wl, out + v0 * 1024 * sizeof(T), left < WS::VPW ? (uint32_t)left : WS::VPW,
@@ -365,31 +405,79 @@ __device__ __forceinline__ void dict_page_vector(const uint8_t* page, uint32_t a
     }
 }
 
-template <typename T, typename D, int VAR, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restrict__ page, uint32_t arr_c, uint32_t vec_c,
-                                                         const D* __restrict__ dict, uint32_t dict_n,
-                                                         uint8_t* __restrict__ out, uint32_t* err) {
-    if (dict_n == 0) return;  // the host rejects such pages
-    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
-        dict_page_vector<T, D, VAR>(page, arr_c, v, t, dict, dict_n, dst, err);
-    });
+// Per-vector metadata of a dictionary page (u32 codes): bit width, FFOR base, lane base (VAR 7), entry point of
+// the code buffer.
+struct CodeMeta {
+    uint32_t bw, base, lb, ep;
+};
+
+template <int VAR>
+__device__ __forceinline__ CodeMeta code_meta(const uint8_t* page, const PageMeta& pm, uint64_t v, unsigned t) {
+    const u32_ua* eps = reinterpret_cast<const u32_ua*>(page + 8 + 4 * pm.arr_c);
+    CodeMeta      m {};
+    if constexpr (VAR == 1) {
+        m.ep = pm.fixed ? (uint32_t)(v * 4096) : (v ? eps[(v - 1) * pm.arr_c] : 0u);
+    } else {
+        constexpr unsigned o = VAR == 7 ? 1 : 0;
+        if (pm.fixed) {
+            m.bw   = page[pm.off[o] + v];
+            m.base = *reinterpret_cast<const u32_ua*>(page + pm.off[o + 1] + 4 * v);
+            if constexpr (VAR == 7) m.lb = reinterpret_cast<const u32_ua*>(page + pm.off[0] + 128 * v)[t];
+        } else {
+            m.bw   = *vbuf(page, pm.arr_c, v, o);
+            m.base = *reinterpret_cast<const u32_ua*>(vbuf(page, pm.arr_c, v, o + 1));
+            if constexpr (VAR == 7) m.lb = reinterpret_cast<const u32_ua*>(vbuf(page, pm.arr_c, v, 0))[t];
+        }
+        m.ep = v ? eps[(v - 1) * pm.arr_c + o + 2] : 0u;
+    }
+    return m;
 }
 
-// Same, with the decoded dictionary copied into LDS once per block (after the waves' staging areas) so the code
-// gathers read LDS instead of L2. The host picks this form when dict_n * sizeof(D) fits beside the staging.
-template <typename T, typename D, int VAR, bool NT>
-__global__ __launch_bounds__(kMaxBlock) void k_page_dict_lds(const uint8_t* __restrict__ page, uint32_t arr_c,
-                                                             uint32_t vec_c, const D* __restrict__ dict, uint32_t dict_n,
-                                                             uint8_t* __restrict__ out, uint32_t* err) {
+// Dictionary pages, pipelined like k_page_int. IN_LDS: the decoded dictionary is first copied into LDS (after the
+// waves' staging areas) and the code gathers read LDS instead of L2; the host picks it when the dictionary fits.
+template <typename T, typename D, int VAR, bool NT, bool IN_LDS>
+__global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
+                                                         const D* __restrict__ dict, uint32_t dict_n,
+                                                         uint8_t* __restrict__ out, uint32_t* err) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     if (dict_n == 0) return;  // the host rejects such pages
-    using WS  = WaveStage<32, 1024 * sizeof(T)>;
-    D* ldict  = reinterpret_cast<D*>(fls_lds + (blockDim.x / 64) * WS::BYTES);
-    for (uint32_t i = threadIdx.x; i < dict_n; i += blockDim.x) ldict[i] = dict[i];
-    __syncthreads();
-    staged_vector_loop<32, 1024 * sizeof(T), NT>(vec_c, out, [&](uint64_t v, unsigned t, const LdsSink& dst) {
-        dict_page_vector<T, D, VAR>(page, arr_c, v, t, ldict, dict_n, dst, err);
-    });
+    constexpr unsigned VB = 1024 * sizeof(T);
+    constexpr unsigned P  = VAR == 1 ? 0 : (VAR == 7 ? 3 : 2);
+    using WS              = WaveStage<32, VB>;
+    const D* dp           = dict;
+    if constexpr (IN_LDS) {
+        D* ldict = reinterpret_cast<D*>(fls_lds + (blockDim.x / 64) * WS::BYTES);
+        for (uint32_t i = threadIdx.x; i < dict_n; i += blockDim.x) ldict[i] = dict[i];
+        __syncthreads();
+        dp = ldict;
+    }
+    const unsigned lane = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
+    const unsigned t = lane % 32, slot = lane / 32;
+    uint8_t*       wl = fls_lds + wave * WS::BYTES;
+    const LdsSink  sink {wl + slot * WS::PVB};
+    const uint64_t N      = vec_c;
+    const uint64_t stride = (uint64_t)gridDim.x * waves * WS::VPW;
+    uint64_t       v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
+    CodeMeta       m {};
+    if (v0 + slot < N) m = code_meta<VAR>(page, pm, v0 + slot, t);
+    for (; v0 < N; v0 += stride) {
+        const uint64_t v = v0 + slot;
+        if (v < N) {
+            const uint8_t* pk = page + pm.off[P] + m.ep;
+            if constexpr (VAR == 1) {
+                dict_vector<T, D, 32, false>(pk, 0, 0, t, dp, dict_n, sink, err);
+            } else {
+                dispatch_bw<32>(m.bw, [&](auto BW) {
+                    dict_vector<T, D, BW, VAR == 7>(pk, m.base, m.lb, t, dp, dict_n, sink, err);
+                });
+            }
+        }
+        if (v + stride < N) m = code_meta<VAR>(page, pm, v + stride, t);
+        wave_sync();
+        const uint64_t left = N - v0;
+        wave_copy_out<32, VB, NT>(wl, out + v0 * VB, left < WS::VPW ? (uint32_t)left : WS::VPW, lane);
+        wave_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
