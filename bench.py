@@ -8,7 +8,8 @@ reference's unffor/unpack order. A bounded sample of every case is checked bit-e
 CPU-baseline leg.
 
 Prints one JSON line (metric, value in Gvalues/s, roofline of the dominant kernel, CPU baseline of the
-reference's own compiled unffor on the host cores). --detail adds the cascade / row-group / wizard sub-benchmarks.
+reference's own compiled unffor on the host cores), with summaries of the cascade, row-group and wizard
+configurations under "configs" (--no-configs skips them); --detail adds the per-family page sub-benchmarks.
 """
 from __future__ import annotations
 
@@ -35,7 +36,9 @@ def parse():
     ap.add_argument("--log2n", type=int, default=28, help="values per (type, width) case")
     ap.add_argument("--widths", default="8,16,32,64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--detail", action="store_true", help="also run cascade, row-group and wizard sub-benchmarks")
+    ap.add_argument("--detail", action="store_true", help="also run the per-family page sub-benchmarks (full output)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the cascade (2^26 values per column), row-group and wizard summaries")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
@@ -84,6 +87,41 @@ def cpu_baseline(widths, threads, samples):
             "kind": "reference",
             "sample": f"reference generated::unffor, every (type,width) FFOR case of {widths} at 2^20 values, "
                       f"median of 3 passes, {threads} threads"}, bad
+
+
+def other_configs(dev):
+    """Summaries of the cascade decoders (2^26 values per column, 2^24 strings), the mixed 64-column row-group and the
+    encoding wizard (512 columns of 2^20 values); each bit-exact-checked by its own module. A failure is reported
+    in place of its summary."""
+    import torch
+
+    from fastlanes_amd import bench_detail, rowgroup_bench, wizard
+
+    stream = torch.cuda.current_stream()
+    out = {}
+    keep = ("ms", "gvalues_s", "gstrings_s", "decoded_gb_s", "gb_s", "hbm_frac", "bit_exact")
+    try:
+        c = bench_detail.cascade(dev, stream, verbose=False)
+        out["cascade"] = {k: {kk: v[kk] for kk in keep if kk in v} for k, v in c.items()}
+    except Exception as e:  # noqa: BLE001 -- reported, not raised: the headline line must still print
+        out["cascade"] = {"error": repr(e)}
+    torch.cuda.empty_cache()
+    try:
+        r = rowgroup_bench.run(dev, stream)
+        out["rowgroup"] = {k: r[k] for k in ("columns", "values", "decoded_bytes", "encoded_bytes", "graph", "branched",
+                                             "per_page", "graph_vs_per_page", "persistent")}
+    except Exception as e:  # noqa: BLE001
+        out["rowgroup"] = {"error": repr(e)}
+    torch.cuda.empty_cache()
+    try:
+        w = wizard.run(dev, stream)
+        out["wizard"] = {k: w[k] for k in ("columns", "rowgroups", "exhaustive_ms", "learned_ms", "speedup",
+                                           "test_regret", "test_accuracy", "no_network_test_regret",
+                                           "no_network_test_accuracy", "train_s", "meets_targets")}
+    except Exception as e:  # noqa: BLE001
+        out["wizard"] = {"error": repr(e)}
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -215,10 +253,12 @@ def main():
     }
     del cases, outs
     torch.cuda.empty_cache()
+    if not a.no_configs:
+        line["configs"] = other_configs(dev)
     if a.detail:
         from fastlanes_amd import bench_detail
 
-        line["detail"] = bench_detail.run_all(dev)
+        line["detail"] = {"pages": bench_detail.pages(dev, torch.cuda.current_stream())}
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"], bad = cpu_baseline(widths, threads, samples)

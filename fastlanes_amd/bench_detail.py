@@ -73,7 +73,7 @@ def pages(dev, stream, log2n=26):
     return res
 
 
-def cascade(dev, stream, log2n=26, log2_strings=24):
+def cascade(dev, stream, log2n=26, log2_strings=24, verbose=True):
     """Configs: (a) int64 timestamps DELTA+FFOR, (b) uint32 runs RLE, (c) int32 Zipf dictionary codes, (d) FSST."""
     import torch
 
@@ -104,8 +104,9 @@ def cascade(dev, stream, log2n=26, log2_strings=24):
                              "ms": round(t * 1e3, 3), "gstrings_s": round(ns / t / 1e9, 3),
                              "decoded_gb_s": round(data.size / t / 1e9, 1), "gb_s": round(nbytes / t / 1e9, 1),
                              "hbm_frac": round(nbytes / t / 1e9 / HBM, 4), "bit_exact": ok}
-    for k, v in out.items():
-        print(f"cascade {k}: {v}", flush=True)
+    if verbose:
+        for k, v in out.items():
+            print(f"cascade {k}: {v}", flush=True)
     return out
 
 
