@@ -224,6 +224,14 @@ def main():
                 fr.append(alg_bytes(W, w, plain) / t / 1e9 / HBM_PEAK_GBS)
         sweep[f"u{W}"] = {"min_frac": round(min(fr), 3), "mean_frac": round(float(np.mean(fr)), 3),
                           "max_frac": round(max(fr), 3)}
+    by_width = {}
+    for W in widths:
+        row = {"ffor": [], "unpack": []}
+        for w in range(W + 1):
+            for plain in (False, True):
+                t = sum(per_case[(W, w, plain)]) / len(per_case[(W, w, plain)])
+                row["unpack" if plain else "ffor"].append(round(alg_bytes(W, w, plain) / t / 1e9 / HBM_PEAK_GBS, 3))
+        by_width[f"u{W}"] = row
     agg_bytes = sum(alg_bytes(W, w, p) for (W, w, p) in per_case)
     u32w7 = per_case.get((32, 7, False))
     line = {
@@ -248,6 +256,7 @@ def main():
                      "bytes_per_launch": d_bytes, "avg_launch_s": d_avg},
         "sweep_hbm_frac": sweep,
         "sweep_aggregate_hbm_frac": round(agg_bytes / kernel_time / 1e9 / HBM_PEAK_GBS, 4),
+        "sweep_hbm_frac_by_bit_width": by_width,
         "u32_w7_ffor_hbm_frac": round(alg_bytes(32, 7, False) / (sum(u32w7) / len(u32w7)) / 1e9 / HBM_PEAK_GBS, 4)
         if u32w7 else None,
     }
@@ -258,7 +267,7 @@ def main():
     if a.detail:
         from fastlanes_amd import bench_detail
 
-        line["detail"] = {"pages": bench_detail.pages(dev, torch.cuda.current_stream())}
+        line["detail"] = {"pages": bench_detail.pages(dev, torch.cuda.current_stream(), verbose=False)}
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"], bad = cpu_baseline(widths, threads, samples)

@@ -53,7 +53,7 @@ def decode_column_timed(x: np.ndarray, eid: int, dev, stream, check=True):
             "hbm_frac": round(nbytes / t / 1e9 / HBM, 4), "bit_exact": ok, "pages": plan.n_pages}
 
 
-def pages(dev, stream, log2n=26):
+def pages(dev, stream, log2n=26, verbose=True):
     """Every page family at each element width (bit-exact check against the input)."""
     n = 1 << log2n
     rng = np.random.default_rng(1)
@@ -69,7 +69,8 @@ def pages(dev, stream, log2n=26):
         }
         for name, (eid, x) in cases.items():
             res[f"u{W}_{name}"] = decode_column_timed(x, eid, dev, stream)
-            print(f"page u{W} {name}: {res[f'u{W}_{name}']}", flush=True)
+            if verbose:
+                print(f"page u{W} {name}: {res[f'u{W}_{name}']}", flush=True)
     return res
 
 
