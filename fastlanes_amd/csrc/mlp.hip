@@ -126,13 +126,13 @@ __global__ void k_softmax_xent(const float* __restrict__ Z, const int32_t* __res
     }
 }
 
-// bias gradients: g[n] = sum_b dZf[b][n] (fp32 rows of kKP) ...
+// bias gradients: g[n] = sum_b dZf[b][n] (fp32 rows of ld), one wave per column ...
 __global__ void k_colsum_f32(const float* __restrict__ src, uint32_t B, uint32_t ld, float* __restrict__ g) {
-    const uint32_t n = threadIdx.x;
-    if (n >= ld) return;
-    float s = 0.f;
-    for (uint32_t b = 0; b < B; ++b) s += src[(uint64_t)b * ld + n];
-    g[n] = s;
+    const uint32_t n = blockIdx.x, lane = threadIdx.x;
+    float          s = 0.f;
+    for (uint32_t b = lane; b < B; b += 64) s += src[(uint64_t)b * ld + n];
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) g[n] = s;
 }
 // ... or g[n] = sum_b srcT[n][b] (bf16 transposed rows), one wave per row
 __global__ void k_rowsum_bf16(const uint16_t* __restrict__ srcT, uint32_t B, float* __restrict__ g) {
@@ -341,7 +341,7 @@ int fls_mlp_forward_backward(fls_mlp* m, const float* dev_features, const int32_
     GemmArgs g = gemm(m->dZbT, B, m->H2T, B, kKP, kH, B);
     g.Cf = m->G + kOffW3, g.ldcf = kH;
     if ((rc = launch_gemm(g, s))) return rc;
-    hipLaunchKernelGGL(k_colsum_f32, dim3(1), dim3(kKP), 0, s, m->dZf, batch, kKP, m->G + kOffB3);
+    hipLaunchKernelGGL(k_colsum_f32, dim3(kKP), dim3(64), 0, s, m->dZf, batch, kKP, m->G + kOffB3);
     // hidden layer 2
     g      = gemm(m->dZb, kKP, m->w3, kKP, B, kH, kKP);
     g.mask = m->H2, g.ldm = kH, g.Cb = m->dH2, g.ldcb = kH, g.CbT = m->dH2T, g.ldct = B;
