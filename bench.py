@@ -100,6 +100,24 @@ def other_configs(dev):
     stream = torch.cuda.current_stream()
     out = {}
     keep = ("ms", "gvalues_s", "gstrings_s", "decoded_gb_s", "gb_s", "hbm_frac", "bit_exact")
+    try:  # FFOR u32 bw 7, 2^20 values (1,000,000 + U[0, 2^7), seed 42), encoded by the library's own encoder
+        import fastlanes_amd as fa
+        from fastlanes_amd import workloads as wl
+
+        x = wl.config1_values()
+        enc = fa.encode_from_memory(x, 1)
+        plan = fa.Plan(enc, x.dtype)
+        d_in = torch.from_numpy(enc).to(dev)
+        d_out = torch.empty(x.nbytes, dtype=torch.uint8, device=dev)
+        sp = int(stream.cuda_stream)
+        t = bench_detail._events_time(lambda: plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes, sp), stream,
+                                      reps=21, warm=5)
+        ok = bool(np.array_equal(d_out.cpu().numpy().view(np.uint32), x))
+        out["config1_ffor_u32_bw7"] = {"values": int(x.size), "us": round(t * 1e6, 2),
+                                       "gvalues_s": round(x.size / t / 1e9, 2), "bit_exact": ok,
+                                       "note": "5.1 MB of traffic: resident in the last-level cache, latency-bound"}
+    except Exception as e:  # noqa: BLE001
+        out["config1_ffor_u32_bw7"] = {"error": repr(e)}
     try:
         c = bench_detail.cascade(dev, stream, verbose=False)
         out["cascade"] = {k: {kk: v[kk] for kk in keep if kk in v} for k, v in c.items()}
