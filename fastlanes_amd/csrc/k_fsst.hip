@@ -55,11 +55,15 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t x, uint64_t* t
     return before + inc - x;
 }
 
-// Stages compressed bytes [s0, s1) of a segment into LDS; returns whether they fit.
+// Stages compressed bytes [s0, s1) of a segment into LDS (16-byte loads from the unaligned source, aligned LDS
+// stores, the last n % 16 bytes one by one so nothing past s1 is read); returns whether they fit.
 __device__ __forceinline__ bool stage_bytes(const uint8_t* cb, uint32_t s0, uint32_t s1, uint8_t* lds) {
     const uint32_t n = s1 - s0;
     if (n > kCompLds) return false;
-    for (uint32_t k = threadIdx.x; k < n; k += kSeg) lds[k] = cb[s0 + k];
+    const uint32_t n16 = n & ~15u;
+    for (uint32_t k = 16 * threadIdx.x; k < n16; k += 16 * kSeg)
+        *reinterpret_cast<v4u*>(lds + k) = *reinterpret_cast<const v4u_ua*>(cb + s0 + k);
+    for (uint32_t k = n16 + threadIdx.x; k < n; k += kSeg) lds[k] = cb[s0 + k];
     return true;
 }
 
@@ -67,7 +71,7 @@ __global__ __launch_bounds__(kSeg) void k_fsst_len(const uint8_t* __restrict__ p
                                                    uint64_t first_string, uint64_t* __restrict__ offsets,
                                                    uint64_t* __restrict__ seg_tot, uint64_t first_seg) {
     __shared__ uint8_t len[256];
-    __shared__ uint8_t cbuf[kCompLds];
+    __shared__ __attribute__((aligned(16))) uint8_t cbuf[kCompLds];
     const unsigned     v = blockIdx.x / kSegPerV, h = blockIdx.x % kSegPerV, i = h * kSeg + threadIdx.x;
     if (threadIdx.x < 256) len[threadIdx.x] = table->len[threadIdx.x];
     const uint8_t* cb   = vbuf(page, 2, v, 0);
@@ -80,9 +84,10 @@ __global__ __launch_bounds__(kSeg) void k_fsst_len(const uint8_t* __restrict__ p
     uint64_t n = 0;
     if (in_lds && s >= s0 && e <= s1) {
         for (uint32_t q = s - s0; q < e - s0;) {
-            const unsigned c = cbuf[q];
-            if (c == 255) n += 1, q += 2;
-            else n += len[c], q += 1;
+            const unsigned c   = cbuf[q];
+            const bool     esc = c == 255;
+            n += esc ? 1u : len[c];
+            q += esc ? 2u : 1u;
         }
     } else {
         for (uint32_t q = s; q < e;) {
@@ -127,13 +132,44 @@ __global__ __launch_bounds__(kSeg) void k_fsst_fix(uint64_t* __restrict__ offset
     if (g < n_seg) offsets[g * kSeg + 1 + threadIdx.x] += sbase[g];
 }
 
+// Writes a string's decoded bytes to [w, wend) of an LDS buffer a word at a time: bytes gather in a 128-bit
+// pending register; single bytes are written until w is 4-byte aligned, then whole 32-bit words. Strings own
+// disjoint byte ranges, so no write leaves its own string.
+struct WordWriter {
+    uint8_t*          buf;
+    uint32_t          w, wend;
+    unsigned __int128 pend = 0;
+    unsigned          pb   = 0;  // pending bytes
+    __device__ __forceinline__ void flush() {
+        while (pb && (w & 3) && w < wend) {
+            buf[w++] = (uint8_t)pend;
+            pend >>= 8, --pb;
+        }
+        while (pb >= 4 && w + 4 <= wend) {
+            *reinterpret_cast<uint32_t*>(buf + w) = (uint32_t)pend;
+            w += 4, pend >>= 32, pb -= 4;
+        }
+    }
+    __device__ __forceinline__ void put(uint64_t x, unsigned n) {  // n <= 8 bytes of x
+        pend |= (unsigned __int128)x << (8 * pb);
+        pb += n;
+        flush();
+    }
+    __device__ __forceinline__ void finish() {
+        while (pb && w < wend) {
+            buf[w++] = (uint8_t)pend;
+            pend >>= 8, --pb;
+        }
+    }
+};
+
 __global__ __launch_bounds__(kSeg) void k_fsst_decode(const uint8_t* __restrict__ page, const FsstTableDev* __restrict__ table,
                                                       uint64_t first_string, const uint64_t* __restrict__ offsets,
                                                       uint8_t* __restrict__ out, uint64_t cap, uint32_t* err) {
     __shared__ uint64_t sym[256];
     __shared__ uint8_t  len[256];
-    __shared__ uint8_t  cbuf[kCompLds];
-    __shared__ uint8_t  obuf[kOutLds];
+    __shared__ __attribute__((aligned(16))) uint8_t cbuf[kCompLds];
+    __shared__ __attribute__((aligned(16))) uint8_t obuf[kOutLds];
     const unsigned      v = blockIdx.x / kSegPerV, h = blockIdx.x % kSegPerV, i = h * kSeg + threadIdx.x;
     if (threadIdx.x < 256) sym[threadIdx.x] = table->symbol[threadIdx.x], len[threadIdx.x] = table->len[threadIdx.x];
     const uint8_t*  cb   = vbuf(page, 2, v, 0);
@@ -150,20 +186,18 @@ __global__ __launch_bounds__(kSeg) void k_fsst_decode(const uint8_t* __restrict_
     if (stop > cap || stop < w0) {
         atomicOr(err, 4u);
     } else if (out_lds && s >= s0 && e <= s1) {
-        uint32_t w = (uint32_t)(w0 - o0);
-        const uint32_t wend = (uint32_t)(stop - o0);
-        for (uint32_t q = s - s0; q < e - s0 && w < wend;) {
+        WordWriter ww {obuf, (uint32_t)(w0 - o0), (uint32_t)(stop - o0)};
+        for (uint32_t q = s - s0; q < e - s0 && ww.w < ww.wend;) {
             const unsigned c = cbuf[q];
             if (c == 255) {
-                obuf[w++] = q + 1 < e - s0 ? cbuf[q + 1] : 0;
+                ww.put(q + 1 < e - s0 ? cbuf[q + 1] : 0, 1);
                 q += 2;
             } else {
-                const uint64_t x = sym[c];
-                const unsigned n = len[c];
-                for (unsigned b = 0; b < n && w < wend; ++b) obuf[w++] = (uint8_t)(x >> (8 * b));
+                ww.put(sym[c], len[c]);
                 q += 1;
             }
         }
+        ww.finish();
     } else {
         uint64_t w = w0;
         for (uint32_t q = s; q < e && w < stop;) {
@@ -181,8 +215,12 @@ __global__ __launch_bounds__(kSeg) void k_fsst_decode(const uint8_t* __restrict_
     }
     if (out_lds) {
         __syncthreads();
-        const uint32_t n = (uint32_t)(o1 - o0);
-        for (uint32_t k = threadIdx.x; k < n; k += kSeg) out[o0 + k] = obuf[k];
+        // copy-out: 16 bytes per thread per step (unaligned global stores), then the tail byte by byte
+        const uint32_t n = (uint32_t)(o1 - o0), n16 = n & ~15u;
+        uint8_t*       dst = out + o0;
+        for (uint32_t k = 16 * threadIdx.x; k < n16; k += 16 * kSeg)
+            *reinterpret_cast<v4u_ua*>(dst + k) = *reinterpret_cast<const v4u*>(obuf + k);
+        for (uint32_t k = n16 + threadIdx.x; k < n; k += kSeg) dst[k] = obuf[k];
     }
 }
 

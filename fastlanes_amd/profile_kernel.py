@@ -22,8 +22,28 @@ def main():
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--plain", action="store_true", help="plain unpack (no FFOR bases)")
     ap.add_argument("--copy-log2-bytes", type=int, default=30)
+    ap.add_argument("--fsst-log2-strings", type=int, default=0,
+                    help="instead: decode an FSST string column of 2^k vocabulary strings (offsets + bytes)")
     a = ap.parse_args()
     import torch
+
+    if a.fsst_log2_strings:
+        from fastlanes_amd import workloads as wl
+
+        ns = 1 << a.fsst_log2_strings
+        offs, data = wl.vocab_strings_fast(ns, seed=7)
+        enc = fa.encode_strings(offs, data)
+        plan = fa.Plan(enc, "str")
+        d_in = torch.from_numpy(enc).cuda()
+        d_off = torch.empty(ns + 1, dtype=torch.int64, device="cuda")
+        d_bytes = torch.empty(data.size + 64, dtype=torch.uint8, device="cuda")
+        for _ in range(a.reps):
+            plan.string_offsets(d_in.data_ptr(), d_off.data_ptr())
+            plan.decode_strings(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size)
+        torch.cuda.synchronize()
+        print(json.dumps({"strings": ns, "raw_bytes": int(data.size), "encoded_bytes": int(enc.size),
+                          "pages": plan.n_pages, "reps": a.reps}), flush=True)
+        return
 
     dev = torch.device("cuda:0")
     n = 1 << a.log2n
