@@ -30,6 +30,9 @@ bool expr_from_id(uint8_t dtype, uint8_t id, Expr& e) {
     switch (id) {
     case 0: e = {EXP_UNCOMPRESSED, 1, 1, false}; return true;
     case 1: e = {EXP_FFOR, 1, 3, false}; return true;
+    case 2: e = {EXP_FFOR, kFforS, 5, false}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE;   // S_PATCH
+    case 4: e = {EXP_FFOR, kFforB, 5, false}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE;   // B_PATCH
+    case 5: e = {EXP_FFOR, kFforLL, 5, false}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE;  // LL_PATCH
     case 6: e = {EXP_RLE, 2, 5, false}; return true;
     case 7: e = {EXP_RLE, 1, 2, false}; return true;
     case 11: e = {EXP_DELTA, 1, 4, false}; return true;
@@ -45,7 +48,10 @@ bool expr_from_id(uint8_t dtype, uint8_t id, Expr& e) {
 bool expr_from_page(uint8_t exp_t, uint8_t var, Expr& e) {
     switch (exp_t) {
     case EXP_UNCOMPRESSED: e = {exp_t, var, 1, false}; return var == 1;
-    case EXP_FFOR: e = {exp_t, var, 3, false}; return var == 1;
+    case EXP_FFOR:
+        if (var == 1) return e = {exp_t, var, 3, false}, true;
+        if (var == kFforLL || var == kFforS || var == kFforB) return e = {exp_t, var, 5, false}, true;
+        return false;
     case EXP_DELTA: e = {exp_t, var, 4, false}; return var == 1;
     case EXP_RLE:
         if (var == 1) return e = {exp_t, var, 2, false}, true;
@@ -175,6 +181,35 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
         switch (e.exp_t) {
         case EXP_UNCOMPRESSED: ok = need(v, 0, 1024ull * sz); break;
         case EXP_FFOR:
+            if (e.var != 1) {  // patched: [exc values][exc positions | bitmap | count+first][bw][base][packed]
+                if (W < 32) return "patched FFOR needs 32- or 64-bit values";
+                ok = need(v, 2, 1) && need(v, 3, sz);
+                if (!ok) break;
+                const unsigned bw = bwc(v, 2);
+                ok = bw <= W && need(v, 4, 128ull * bw);
+                if (!ok) break;
+                const uint8_t* b1  = pg + boff[1] + ep(v, 1);
+                uint32_t       exc = 0;
+                if (e.var == kFforB) {
+                    ok = need(v, 1, 128);
+                    if (!ok) break;
+                    for (unsigned k = 0; k < 128; ++k) exc += (uint32_t)__builtin_popcount(b1[k]);
+                } else {
+                    ok = need(v, 1, e.var == kFforS ? 2 : 4);
+                    if (!ok) break;
+                    exc = (uint32_t)b1[0] | ((uint32_t)b1[1] << 8);
+                    if (e.var == kFforS) {
+                        ok = exc <= 1024 && need(v, 1, 2 + 2ull * exc);
+                        for (uint32_t k = 0; ok && k < exc; ++k) ok = ((uint32_t)b1[2 + 2 * k] | ((uint32_t)b1[3 + 2 * k] << 8)) < 1024;
+                    } else {
+                        ok = exc <= 1024 && (exc == 0 || ((uint32_t)b1[2] | ((uint32_t)b1[3] << 8)) < 1024);
+                    }
+                    if (!ok) break;
+                }
+                ok = need(v, 0, (uint64_t)exc * sz);
+                break;
+            }
+            [[fallthrough]];
         case EXP_DELTA: {
             const unsigned o = e.exp_t == EXP_DELTA;
             if (o) ok &= need(v, 0, 128);
@@ -215,7 +250,10 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
         unsigned fsz[8] = {};
         switch (e.exp_t) {
         case EXP_UNCOMPRESSED: fsz[0] = 1024 * sz; break;
-        case EXP_FFOR: fsz[0] = 1, fsz[1] = sz; break;
+        case EXP_FFOR:
+            if (e.var == 1) fsz[0] = 1, fsz[1] = sz;
+            else fsz[2] = 1, fsz[3] = sz, fsz[1] = e.var == kFforB ? 128 : e.var == kFforLL ? 4 : 0;
+            break;
         case EXP_DELTA: fsz[0] = 128, fsz[1] = 1, fsz[2] = sz; break;
         case EXP_RLE:
             if (e.var == 1) fsz[1] = 2048;
@@ -240,7 +278,7 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
         switch (e.exp_t) {
         case EXP_UNCOMPRESSED: tail = 1024ull * sz; break;
         case EXP_FFOR:
-        case EXP_DELTA: tail = 128ull * bwc(v, b - 2); break;
+        case EXP_DELTA: tail = 128ull * bwc(v, b - 2); break;  // patched FFOR: b - 2 is its bw buffer too
         case EXP_RLE: tail = e.var == 1 ? 2048 : 128ull * bwc(v, b - 2); break;
         case EXP_FSST: tail = 4096; break;
         default: tail = e.var == 1 ? 4096 : 128ull * bwc(v, b - 2); break;
@@ -365,6 +403,74 @@ uint8_t analyze(const T* in, T& base) {  // base = min, bw = exact bit length of
     const uint64_t d = (uint64_t)(T)(mx - mn);
     return (uint8_t)(d ? 64 - __builtin_clzll(d) : 0);
 }
+// Patched FFOR parameters of one transposed vector, as ffor_patch_analyze / find_best_params choose them
+// (src/cor/prm/ffor/ffor_analyze.cpp:41-75, :105-140): every distinct value is tried as the base with every bit width
+// below W; the values outside [base, base + 2^bw - 1] are exceptions (bw 0 admits only the value 0:
+// SafeUpperBound(base, 0) is 0, include/fls/utl/util.hpp:87-101); the size 128*bw + exceptions*sizeof(T) is
+// minimised over settings with at most kExcLimit exceptions, ties going to the smaller base, then the larger bw.
+// Without a better setting the vector is packed at full width with base 0 and no exceptions. The counts come from
+// the sorted vector with one moving pointer per bit width. LL_PATCH then adds its compulsory links so that
+// consecutive exceptions lie at most 2^bw apart (src/cor/prm/patch/ll_patch/ll_patch_compress.cpp:13-43).
+template <typename T>
+struct PatchPlan {
+    unsigned              bw = 0;
+    T                     base {};
+    std::vector<uint16_t> pos;  // exception positions, ascending (LL: with its compulsory links)
+};
+template <typename T>
+PatchPlan<T> plan_patch(const T* t, uint8_t var) {
+    constexpr unsigned W = bits<T>();
+    T                  s[1024];
+    std::copy(t, t + 1024, s);
+    std::sort(s, s + 1024);
+    uint64_t best_size = 128ull * W;
+    unsigned best_i = 0, best_bw = W;
+    bool     found  = false;
+    for (unsigned bw = 0; bw < W; ++bw) {
+        const T  range = (T)((1ull << bw) - 1);
+        unsigned j     = 0;  // first index above the window of base s[i]
+        for (unsigned i = 0; i < 1024; ++i) {
+            if (i && s[i] == s[i - 1]) continue;
+            unsigned in_window;
+            if (bw == 0) {
+                in_window = 0;
+                if (s[i] == 0)
+                    while (in_window < 1024 && s[in_window] == 0) ++in_window;
+            } else {
+                const T upper = s[i] > (T)(~(T)0 - range) ? (T)~(T)0 : (T)(s[i] + range);
+                if (j < i) j = i;
+                while (j < 1024 && s[j] <= upper) ++j;
+                in_window = j - i;
+            }
+            const uint64_t exc  = 1024 - in_window;
+            const uint64_t size = 128ull * bw + exc * sizeof(T);
+            if (exc > kExcLimit) continue;
+            const bool better = size < best_size || (size == best_size && found && (i < best_i || (i == best_i && bw > best_bw)));
+            if (better) best_size = size, best_i = i, best_bw = bw, found = true;
+        }
+    }
+    PatchPlan<T> pp;
+    if (!found) return pp.bw = W, pp.base = 0, pp;
+    pp.bw   = best_bw;
+    pp.base = s[best_i];
+    const T lo = pp.base, hi = best_bw == 0 ? (T)0 : (lo > (T)(~(T)0 - (T)((1ull << best_bw) - 1)) ? (T)~(T)0 : (T)(lo + (T)((1ull << best_bw) - 1)));
+    for (unsigned k = 0; k < 1024; ++k)
+        if (t[k] < lo || t[k] > hi) pp.pos.push_back((uint16_t)k);
+    if (var == kFforLL && !pp.pos.empty()) {
+        const uint64_t        hop = 1ull << (best_bw < 11 ? best_bw : 11);
+        std::vector<uint16_t> chain;
+        uint64_t              cur = pp.pos[0];
+        chain.push_back((uint16_t)cur);
+        for (size_t i = 1; i < pp.pos.size(); ++i) {
+            while (pp.pos[i] - cur > hop) cur += hop, chain.push_back((uint16_t)cur);
+            cur = pp.pos[i];
+            chain.push_back((uint16_t)cur);
+        }
+        pp.pos = std::move(chain);
+    }
+    return pp;
+}
+
 template <typename T>
 void ffor(const T* in, uint8_t* out, unsigned bw, T base) {
     constexpr unsigned W = bits<T>(), L = 1024 / W;
@@ -499,7 +605,42 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
                 break;
             case EXP_FFOR:
                 transpose<T>(x, t);
-                ffor_parts(t, 0, b2.data());
+                if (e.var == 1) {
+                    ffor_parts(t, 0, b2.data());
+                } else if constexpr (sizeof(T) >= 4) {
+                    const PatchPlan<T> pp = plan_patch<T>(t, e.var);
+                    const uint16_t     n  = (uint16_t)pp.pos.size();
+                    for (uint16_t k = 0; k < n; ++k) std::memcpy(b0.data() + k * sizeof(T), &t[pp.pos[k]], sizeof(T));
+                    parts[0] = {b0.data(), (uint64_t)n * sizeof(T)};
+                    if (e.var == kFforB) {
+                        std::memset(b1.data(), 0, 128);
+                        for (uint16_t p : pp.pos) b1[p >> 3] |= (uint8_t)(1u << (p & 7));
+                        parts[1] = {b1.data(), 128};
+                    } else if (e.var == kFforS) {
+                        std::memcpy(b1.data(), &n, 2);
+                        for (uint16_t k = 0; k < n; ++k) std::memcpy(b1.data() + 2 + 2 * k, &pp.pos[k], 2);
+                        parts[1] = {b1.data(), 2ull + 2ull * n};
+                    } else {
+                        const uint16_t first = n ? pp.pos[0] : 0;
+                        std::memcpy(b1.data(), &n, 2), std::memcpy(b1.data() + 2, &first, 2);
+                        parts[1] = {b1.data(), 4};
+                    }
+                    // S and B leave the exceptions in place (FFOR keeps their low bw bits of value - base); LL
+                    // stores in each exception's slot the distance to the next one, minus one, plus base (the last
+                    // slot's link is never followed: it gets distance 1)
+                    if (e.var == kFforLL)
+                        for (uint16_t k = 0; k < n; ++k) {
+                            const uint64_t gap = k + 1 < n ? (uint64_t)(pp.pos[k + 1] - pp.pos[k]) : 1;
+                            t[pp.pos[k]]       = (T)(pp.base + (T)(gap - 1));
+                        }
+                    bwb   = (uint8_t)pp.bw;
+                    baseb = 0;
+                    std::memcpy(&baseb, &pp.base, sizeof(T));
+                    ffor<T>(t, b2.data(), pp.bw, pp.base);
+                    parts[2] = {&bwb, 1};
+                    parts[3] = {reinterpret_cast<const uint8_t*>(&baseb), sizeof(T)};
+                    parts[4] = {b2.data(), 128ull * pp.bw};
+                }
                 break;
             case EXP_DELTA:
                 transpose<T>(x, t);

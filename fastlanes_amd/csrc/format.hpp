@@ -44,6 +44,9 @@ enum : uint8_t {
 
 constexpr uint64_t kDataPageMax   = 8ull * 64 * 256 * 1024;  // CFG::PQ::DATA_PG_SZ, include/fls/cfg/cfg.hpp:82
 constexpr unsigned kMaxVecPerPage = 65535;                    // u16 PageHdr::vec_c
+constexpr unsigned kExcLimit      = 100;                      // CFG::CMPR::EXC_LIMIT_C, include/fls/cfg/cfg.hpp:67
+// FFOR variants (include/fls/cor/exp/ffor/ffor_exp.hpp var_t): 1 NO_PATCH, 2 LL_PATCH, 3 S_PATCH, 4 B_PATCH
+constexpr uint8_t kFforLL = 2, kFforS = 3, kFforB = 4;
 
 unsigned dtype_bits(uint8_t dt);
 inline bool is_dict_exp(uint8_t e) { return e == EXP_DICT || e == EXP_DICT_FRQ || e == EXP_DICT_VAL; }

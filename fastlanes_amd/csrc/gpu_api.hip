@@ -130,7 +130,10 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
         pm.arr_c = pe.arr_c, pm.fixed = pe.fixed_meta;
         switch (pe.exp_t) {
         case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pm, pe.vec_c, out, s); break;
-        case EXP_FFOR: e = launch_page_int(W, 1, page, pm, pe.vec_c, out, s); break;
+        case EXP_FFOR:
+            e = pe.var == 1 ? launch_page_int(W, 1, page, pm, pe.vec_c, out, s)
+                            : launch_page_patch(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s);
+            break;
         case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, out, s); break;
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s); break;
         default: {
@@ -152,7 +155,7 @@ int fls_gpu_plan_check(fls_gpu_plan* plan, void* stream) {
     hipError_t e = hipMemcpyAsync(&h, plan->d_err, 4, hipMemcpyDeviceToHost, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "check");
-    if (h) return fail(FLS_E_DATA, "decoded data referenced an out-of-range run or dictionary entry");
+    if (h) return fail(FLS_E_DATA, "decoded data referenced an out-of-range run, dictionary entry or exception position");
     return FLS_OK;
 }
 

@@ -79,6 +79,16 @@ hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const Page
     }
 }
 
+hipError_t launch_page_patch(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
+                             uint32_t* err, hipStream_t s) {
+    if (!vec_c) return hipSuccess;
+    switch (W) {
+    case 32: return page_patch_w<32>(var, page, pm, vec_c, out, err, s);
+    case 64: return page_patch_w<64>(var, page, pm, vec_c, out, err, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                            uint32_t* err, hipStream_t s) {
     if (!vec_c) return hipSuccess;

@@ -47,12 +47,17 @@ hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageM
                            uint32_t* err, hipStream_t s);
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,
                             const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s);
+hipError_t launch_page_patch(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
+                             uint32_t* err, hipStream_t s);
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
                       hipStream_t s);
 
 // per-width instantiation units (k_page_w*.hip)
 template <int W>
 hipError_t page_int_w(int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, hipStream_t s);
+template <int W>
+hipError_t page_patch_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err,
+                        hipStream_t s);
 template <int W>
 hipError_t page_rle_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err,
                       hipStream_t s);

@@ -219,6 +219,123 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restric
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Patched FFOR pages, 32- and 64-bit (include/fls/cor/exp/ffor/ffor_exp.hpp:56-130): per vector
+// [exceptions T][positions][bw][base][packed]. The vector is FFOR-decoded into the wave's LDS staging, then the
+// exceptions overwrite their transposed positions k (held untransposed at orig_of(k)) before the copy-out.
+// VAR 3 (S_PATCH): u16 count + u16 positions (src/cor/prm/patch/s_patch/s_patch_decompress.cpp:13-30).
+// VAR 4 (B_PATCH): 16 u64 bitmap words, exceptions in bit order (src/cor/prm/patch/patch.cpp:71-83).
+// VAR 2 (LL_PATCH): u16 count + u16 first position; a patched slot's FFOR value minus base plus one is the distance
+// to the next exception (src/cor/prm/patch/ll_patch/ll_patch_decompress.cpp:13-42). The chain is serial: one thread
+// walks it. A step that leaves the vector sets err bit 8 and ends the chain (the host validated S positions).
+// ---------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned orig_of(unsigned k) {
+    const unsigned b = (k >> 4) & 7;
+    return (k >> 7) + 8 * (((b & 1) << 2) | (b & 2) | (b >> 2)) + 64 * (k & 15);
+}
+
+struct PatchMeta {
+    uint64_t base;
+    uint32_t bw, ep_exc, ep_pos, ep_pk;
+};
+
+template <int W>
+__device__ __forceinline__ PatchMeta patch_meta(const uint8_t* page, const PageMeta& pm, uint64_t v) {
+    using T            = typename uint_of<W>::type;
+    const u32_ua* eps  = reinterpret_cast<const u32_ua*>(page + 8 + 4 * pm.arr_c);
+    const u32_ua* row  = eps + (v ? (v - 1) * 5 : 0);
+    PatchMeta     m {};
+    if (v) m.ep_exc = row[0], m.ep_pos = row[1], m.ep_pk = row[4];
+    if (pm.fixed) {
+        m.bw   = page[pm.off[2] + v];
+        m.base = load_t<T>(page + pm.off[3] + v * (W / 8));
+    } else {
+        const uint32_t e2 = v ? (uint32_t)row[2] : 0u, e3 = v ? (uint32_t)row[3] : 0u;
+        m.bw   = page[pm.off[2] + e2];
+        m.base = load_t<T>(page + pm.off[3] + e3);
+    }
+    return m;
+}
+
+// Exceptions of one vector into its LDS staging `vl`, by the TPV threads of its slot (t = 0 .. TPV-1).
+template <int W, int VAR, unsigned TPV>
+__device__ __forceinline__ void apply_patches(const uint8_t* exc, const uint8_t* pos, uint64_t base, unsigned t,
+                                              uint8_t* vl, uint32_t* err) {
+    using T                = typename uint_of<W>::type;
+    constexpr unsigned SZ  = W / 8;
+    auto               at  = [&](unsigned k) { return reinterpret_cast<T*>(vl + lds_pad(orig_of(k) * SZ)); };
+    if constexpr (VAR == 3) {
+        const unsigned c = *reinterpret_cast<const u16_ua*>(pos);
+        for (unsigned i = t; i < c; i += TPV) {
+            const unsigned k = *reinterpret_cast<const u16_ua*>(pos + 2 + 2 * i);
+            *at(k & 1023)    = load_t<T>(exc + i * SZ);
+        }
+    } else if constexpr (VAR == 4) {
+        // thread t < 16 owns bitmap word t; its first exception index is the popcount of the words before it
+        uint64_t unit = t < 16 ? *reinterpret_cast<const u64_ua*>(pos + 8 * t) : 0ull;
+        unsigned c    = 0;
+        for (unsigned w = 0; w < 16 && w < t; ++w) c += __builtin_popcountll(*reinterpret_cast<const u64_ua*>(pos + 8 * w));
+        while (unit) {
+            const unsigned r = __builtin_ctzll(unit);
+            *at(64 * t + r)  = load_t<T>(exc + c * SZ);
+            ++c;
+            unit &= unit - 1;
+        }
+    } else {
+        if (t == 0) {
+            const unsigned c   = *reinterpret_cast<const u16_ua*>(pos);
+            T              cur = *reinterpret_cast<const u16_ua*>(pos + 2);
+            for (unsigned i = 0; i < c; ++i) {
+                if (cur >= 1024) {
+                    atomicOr(err, 8u);
+                    break;
+                }
+                T* slot          = at((unsigned)cur);
+                const T next_off = (T)(*slot - (T)base + 1);
+                *slot            = load_t<T>(exc + i * SZ);
+                cur              = (T)(cur + next_off);
+            }
+        }
+    }
+}
+
+// Same pipeline as k_page_int (MODE 1) with a patch step between the LDS decode and the copy-out.
+template <int W, int VAR, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_page_patch(const uint8_t* __restrict__ page, PageMeta pm,
+                                                          uint32_t vec_c, uint8_t* __restrict__ out,
+                                                          uint32_t* __restrict__ err) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
+    constexpr unsigned TPV = lane_tpv<W>(), VB = 128 * W;
+    using WS               = WaveStage<TPV, VB>;
+    const unsigned lane = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
+    const unsigned t = lane % TPV, slot = lane / TPV;
+    uint8_t*       wl = fls_lds + wave * WS::BYTES;
+    uint8_t*       vl = wl + slot * WS::PVB;
+    const LdsSink  sink {vl};
+    const uint64_t N      = vec_c;
+    const uint64_t stride = (uint64_t)gridDim.x * waves * WS::VPW;
+    uint64_t       v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
+    PatchMeta      m {};
+    if (v0 + slot < N) m = patch_meta<W>(page, pm, v0 + slot);
+    for (; v0 < N; v0 += stride) {
+        const uint64_t  v   = v0 + slot;
+        const PatchMeta cur = m;
+        if (v < N) {
+            const uint8_t* pk = page + pm.off[4] + cur.ep_pk;
+            dispatch_bw<W>(cur.bw, [&](auto BW) { ffor_untransposed<W, BW, false>(pk, cur.base, 0, t, sink); });
+        }
+        if (v + stride < N) m = patch_meta<W>(page, pm, v + stride);
+        wave_sync();
+        if (v < N)
+            apply_patches<W, VAR, TPV>(page + pm.off[0] + cur.ep_exc, page + pm.off[1] + cur.ep_pos, cur.base, t, vl,
+                                       err);
+        wave_sync();
+        const uint64_t left = N - v0;
+        wave_copy_out<TPV, VB, NT>(wl, out + v0 * VB, left < WS::VPW ? (uint32_t)left : WS::VPW, lane);
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // RLE pages (include/fls/cor/exp/rle/rle_exp.hpp:37-74). VAR 1: [run values][u16 positions, transposed];
 // VAR 2: [run values][pos lane bases][bw][base u16][packed u16 deltas]. out[i] = runs[pos[i]]
 // (src/cor/lyt/vec.cpp:318-332). Positions are clamped to the vector's run count; a clamp sets err bit 1.

@@ -53,7 +53,7 @@ int kind_of(const PageEntry& pe, unsigned W) {
     const int wc = W == 8 ? 0 : W == 16 ? 1 : W == 32 ? 2 : 3;
     switch (pe.exp_t) {
     case EXP_UNCOMPRESSED: return RG_UNC8 + wc;
-    case EXP_FFOR: return RG_FFOR8 + wc;
+    case EXP_FFOR: return pe.var == 1 ? RG_FFOR8 + wc : -1;
     case EXP_DELTA: return RG_DELTA8 + wc;
     case EXP_RLE: return pe.var == 2 ? RG_RLE8 + wc : -1;
     default: return is_dict_exp(pe.exp_t) && pe.var == 2 ? RG_DICT8 + wc : -1;

@@ -116,6 +116,38 @@ def for_with_outliers(n: int, width: int = 8, outlier_frac: float = 0.01, seed: 
     return x
 
 
+def patch_column(dtype, n: int, seed: int = 8) -> np.ndarray:
+    """Data for the patched FFOR variants (32/64-bit): per 1024-value vector a body of 0-20 bits above a random base
+    with up to 120 outliers anywhere in the type's range, and every eighth vector a corner case in turn: constant,
+    all zero but three values far apart (a bit width of 0 makes LL_PATCH link every position in between), full-range
+    values (no patch pays), exactly 100 and exactly 101 outliers."""
+    dt = np.dtype(dtype)
+    bits = dt.itemsize * 8
+    udt = np.uint64 if bits == 64 else np.uint32
+    umax = (1 << bits) - 1
+    g = rng(seed)
+    nv = n // 1024
+    out = np.empty((nv, 1024), dtype=udt)
+    for v in range(nv):
+        row = out[v]
+        corner = (v // 8) % 5 if v % 8 == 7 else -1
+        if corner == 0:
+            row[:] = g.integers(0, umax, dtype=udt, endpoint=True)
+        elif corner == 1:
+            row[:] = 0
+            row[[3, 500, 1020]] = g.integers(1, umax, 3, dtype=udt, endpoint=True)
+        elif corner == 2:
+            row[:] = g.integers(0, umax, 1024, dtype=udt, endpoint=True)
+        else:
+            bw = int(g.integers(0, 21))
+            base = int(g.integers(0, umax - (1 << bw) + 1, dtype=udt, endpoint=True))
+            row[:] = (base + g.integers(0, 1 << bw, 1024, dtype=np.uint64)).astype(udt)
+            k = 100 if corner == 3 else 101 if corner == 4 else int(g.integers(0, 121))
+            pos = g.choice(1024, k, replace=False)
+            row[pos] = g.integers(0, umax, k, dtype=udt, endpoint=True)
+    return out.reshape(-1).view(dt)
+
+
 def vocab_strings_fast(n: int, seed: int = 7, min_len: int = 8, max_len: int = 64, n_words: int = 2000):
     """Vectorized variant for large n: a text of space-separated vocabulary words is cut into consecutive strings
     of length U[min_len, max_len]. Returns (offsets uint32[n+1], bytes uint8)."""

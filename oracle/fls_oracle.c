@@ -230,6 +230,9 @@ static unsigned dtype_bits(uint8_t dt) {
 typedef struct {
     uint8_t exp_t, var, arr_c, dict;
 } expr_t;
+/* FFOR var_t (include/fls/cor/exp/ffor/ffor_exp.hpp:24-30) */
+enum { FFOR_LL = 2, FFOR_S = 3, FFOR_B = 4 };
+#define EXC_LIMIT 100 /* CFG::CMPR::EXC_LIMIT_C, include/fls/cfg/cfg.hpp */
 /* Pool ids and (ExpT, var) per src/cor/exp_pool.cpp:23-65 (i64) and :66-100 (str); arr_c from the RPN tables. */
 static int expr_lookup(uint8_t dtype, uint8_t id, expr_t* e) {
     if (dtype == DT_STR) {
@@ -239,6 +242,10 @@ static int expr_lookup(uint8_t dtype, uint8_t id, expr_t* e) {
     switch (id) {
     case 0: *e = (expr_t){EXP_UNCOMPRESSED, 1, 1, 0}; return 0;
     case 1: *e = (expr_t){EXP_FFOR, 1, 3, 0}; return 0;
+    /* patched FFOR, 32/64-bit integers only (ffor_patch_analyze resolves u32/u64, ffor_analyze.cpp:145-160) */
+    case 2: *e = (expr_t){EXP_FFOR, FFOR_S, 5, 0}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE ? 0 : -1;
+    case 4: *e = (expr_t){EXP_FFOR, FFOR_B, 5, 0}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE ? 0 : -1;
+    case 5: *e = (expr_t){EXP_FFOR, FFOR_LL, 5, 0}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE ? 0 : -1;
     case 6: *e = (expr_t){EXP_RLE, 2, 5, 0}; return 0;
     case 7: *e = (expr_t){EXP_RLE, 1, 2, 0}; return 0;
     case 11: *e = (expr_t){EXP_DELTA, 1, 4, 0}; return 0;
@@ -253,7 +260,10 @@ static int expr_lookup(uint8_t dtype, uint8_t id, expr_t* e) {
 static int expr_from_page(uint8_t exp_t, uint8_t var, expr_t* e) {
     switch (exp_t) {
     case EXP_UNCOMPRESSED: *e = (expr_t){exp_t, var, 1, 0}; return var == 1 ? 0 : -1;
-    case EXP_FFOR: *e = (expr_t){exp_t, var, 3, 0}; return var == 1 ? 0 : -1;
+    case EXP_FFOR:
+        if (var == 1) { *e = (expr_t){exp_t, var, 3, 0}; return 0; }
+        if (var == FFOR_LL || var == FFOR_S || var == FFOR_B) { *e = (expr_t){exp_t, var, 5, 0}; return 0; }
+        return -1;
     case EXP_DELTA: *e = (expr_t){exp_t, var, 4, 0}; return var == 1 ? 0 : -1;
     case EXP_RLE:
         if (var == 1) { *e = (expr_t){exp_t, var, 2, 0}; return 0; }
@@ -518,6 +528,92 @@ static void enc_ffor_parts(unsigned W, const void* t, vec_parts* vp, unsigned o,
     vp->parts[o + 2] = packed, vp->s[o + 2] = 128ULL * bw;
 }
 
+/* Patched FFOR on a transposed vector t (u32/u64), parts [exceptions][positions][bw][base][packed].
+ * Choice of base/bw: ffor_patch_analyze + find_best_params (src/cor/prm/ffor/ffor_analyze.cpp:41-75, :105-140) over
+ * the histogram of distinct values (ascending): for each distinct value as base, bw from W-1 down to 0, the
+ * exceptions are the values outside [base, SafeUpperBound(base, bw)] (include/fls/utl/util.hpp:87-101, which is 0
+ * for bw 0) counted with their repetitions (src/cor/prm/patch/patch.cpp:41-68); a setting replaces the best when its
+ * size 1024*bw/8 + exc*sizeof(T) is strictly smaller and it has at most EXC_LIMIT exceptions; the start is
+ * {bw W, base 0}. Positions: values with val < base or val - base > 2^bw - 1 (ffor_analyze.cpp:19-37).
+ * S: count + positions (s_patch_compress.cpp); B: bitmap of 16 u64 (patch.cpp:22-32); LL: compulsory links where
+ * consecutive positions are more than 2^bw apart, then count + first position, each slot holding next - pos +
+ * base - 1 (ll_patch_compress.cpp:13-52; the last slot, whose link the reference takes from an unset array entry,
+ * gets distance 1). */
+static uint64_t safe_upper(unsigned W, uint64_t base, unsigned bw) {
+    if (bw == 0) return 0;
+    uint64_t range = bmask(bw), mx = wmask(W);
+    if (base > 0 && range > mx - base) return mx;
+    return base + range;
+}
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+static void enc_patch(unsigned W, uint8_t var, uint8_t* t, vec_parts* vp, uint8_t* bwb, uint8_t* baseb) {
+    const unsigned sz = W / 8;
+    uint64_t       srt[1024], val[1024], rep[1024];
+    unsigned       nd = 0;
+    for (unsigned k = 0; k < 1024; ++k) srt[k] = ld(W, t, k);
+    qsort(srt, 1024, 8, cmp_u64);
+    for (unsigned k = 0; k < 1024; ++k) {
+        if (k && srt[k] == srt[k - 1]) rep[nd - 1]++;
+        else val[nd] = srt[k], rep[nd] = 1, nd++;
+    }
+    unsigned best_bw = W;
+    uint64_t best_base = 0, best_size = 1024ULL * W / 8;
+    for (unsigned idx = 0; idx < nd; ++idx) {
+        unsigned nbw = W;
+        uint64_t nsize = 1024ULL * W / 8;
+        for (int bw = (int)W - 1; bw >= 0; --bw) {
+            uint64_t lo = val[idx], hi = safe_upper(W, lo, (unsigned)bw), exc = 0;
+            for (unsigned i = 0; i < nd; ++i)
+                if (!(val[i] <= hi && val[i] >= lo)) exc += rep[i];
+            uint64_t size = 1024ULL * (unsigned)bw / 8 + exc * sz;
+            if (size < nsize && exc <= EXC_LIMIT) nbw = (unsigned)bw, nsize = size;
+        }
+        if (nsize < best_size) best_size = nsize, best_bw = nbw, best_base = nbw == W ? 0 : val[idx];
+    }
+    uint16_t pos[2048];
+    unsigned n = 0;
+    for (unsigned k = 0; k < 1024; ++k) {
+        uint64_t x = ld(W, t, k);
+        if (best_bw != W && (x < best_base || ((x - best_base) & wmask(W)) > bmask(best_bw))) pos[n++] = (uint16_t)k;
+    }
+    if (var == FFOR_LL && n) {
+        uint16_t chain[2048];
+        unsigned m = 0;
+        uint64_t hop = 1ULL << (best_bw < 11 ? best_bw : 11), cur = pos[0];
+        chain[m++] = (uint16_t)cur;
+        for (unsigned i = 1; i < n; ++i) {
+            while (pos[i] - cur > hop) cur += hop, chain[m++] = (uint16_t)cur;
+            cur = pos[i], chain[m++] = (uint16_t)cur;
+        }
+        memcpy(pos, chain, 2 * m), n = m;
+    }
+    for (unsigned k = 0; k < n; ++k) st(W, vp->b0, k, ld(W, t, pos[k]));
+    vp->parts[0] = vp->b0, vp->s[0] = (uint64_t)n * sz;
+    uint16_t n16 = (uint16_t)n;
+    if (var == FFOR_B) {
+        memset(vp->b1, 0, 128);
+        for (unsigned k = 0; k < n; ++k) vp->b1[pos[k] / 8] |= (uint8_t)(1u << (pos[k] % 8));
+        vp->parts[1] = vp->b1, vp->s[1] = 128;
+    } else if (var == FFOR_S) {
+        memcpy(vp->b1, &n16, 2), memcpy(vp->b1 + 2, pos, 2ULL * n);
+        vp->parts[1] = vp->b1, vp->s[1] = 2 + 2ULL * n;
+    } else {
+        uint16_t first = n ? pos[0] : 0;
+        memcpy(vp->b1, &n16, 2), memcpy(vp->b1 + 2, &first, 2);
+        vp->parts[1] = vp->b1, vp->s[1] = 4;
+        for (unsigned k = 0; k < n; ++k) st(W, t, pos[k], best_base + (k + 1 < n ? (uint64_t)(pos[k + 1] - pos[k]) : 1) - 1);
+    }
+    bwb[0] = (uint8_t)best_bw;
+    memset(baseb, 0, 8), memcpy(baseb, &best_base, sz);
+    orc_ffor(W, t, vp->b2, best_bw, baseb);
+    vp->parts[2] = bwb, vp->s[2] = 1;
+    vp->parts[3] = baseb, vp->s[3] = sz;
+    vp->parts[4] = vp->b2, vp->s[4] = 128ULL * best_bw;
+}
+
 static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint32_t* codes, vec_parts* vp) {
     static __thread uint8_t t[8 * 1024], d[8 * 1024], bwb[8], baseb[8];
     const unsigned          sz = W / 8;
@@ -528,7 +624,8 @@ static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint
         break;
     case EXP_FFOR:
         orc_transpose(W, vals, t);
-        enc_ffor_parts(W, t, vp, 0, bwb, baseb, vp->b2);
+        if (e->var == 1) enc_ffor_parts(W, t, vp, 0, bwb, baseb, vp->b2);
+        else enc_patch(W, e->var, t, vp, bwb, baseb);
         break;
     case EXP_DELTA:
         orc_transpose(W, vals, t);
@@ -700,6 +797,50 @@ static uint64_t page_end(unsigned W, const page_r* pg, const expr_t* e) {
     return (uint64_t)(q - pg->p) + tail;
 }
 
+/* Patched FFOR vector into transposed t: unffor of [bw][base][packed] (buffers 2..4), then the patch step
+ * (S: s_patch_decompress.cpp:13-30; B: patch.cpp:71-83; LL: ll_patch_decompress.cpp:13-42, where each followed
+ * slot's decoded value minus base plus one is the distance to the next exception). Positions outside the vector
+ * are a format error here (the reference would write out of bounds). */
+static int decode_patched(unsigned W, const page_r* pg, const expr_t* e, unsigned v, uint8_t* t) {
+    if (unffor_parts(W, pg, v, 2, t)) return -1;
+    const uint8_t* exc = vbuf(pg, v, 0);
+    const uint8_t* p1  = vbuf(pg, v, 1);
+    const unsigned sz  = W / 8;
+    uint64_t       base = 0, x = 0;
+    memcpy(&base, vbuf(pg, v, 3), sz);
+    if (e->var == FFOR_S) {
+        uint16_t c, q;
+        memcpy(&c, p1, 2);
+        for (unsigned i = 0; i < c; ++i) {
+            memcpy(&q, p1 + 2 + 2 * i, 2);
+            if (q >= 1024) return -1;
+            memcpy(&x, exc + (size_t)i * sz, sz), st(W, t, q, x);
+        }
+    } else if (e->var == FFOR_B) {
+        unsigned c = 0;
+        for (unsigned w = 0; w < 16; ++w) {
+            uint64_t unit;
+            memcpy(&unit, p1 + 8 * w, 8);
+            while (unit) {
+                unsigned r = (unsigned)__builtin_ctzll(unit);
+                memcpy(&x, exc + (size_t)c++ * sz, sz), st(W, t, 64 * w + r, x);
+                unit &= unit - 1;
+            }
+        }
+    } else {
+        uint16_t c, first;
+        memcpy(&c, p1, 2), memcpy(&first, p1 + 2, 2);
+        uint64_t cur = first;
+        for (unsigned i = 0; i < c; ++i) {
+            if (cur >= 1024) return -1;
+            uint64_t off = (ld(W, t, cur) - base + 1) & wmask(W);
+            memcpy(&x, exc + (size_t)i * sz, sz), st(W, t, cur, x);
+            cur = (cur + off) & wmask(W);
+        }
+    }
+    return 0;
+}
+
 static int decode_vector(unsigned W, const page_r* pg, const expr_t* e, unsigned v, const uint64_t* dict,
                          uint64_t dict_n, void* out) {
     uint8_t        t[8 * 1024], d[8 * 1024];
@@ -707,7 +848,9 @@ static int decode_vector(unsigned W, const page_r* pg, const expr_t* e, unsigned
     switch (e->exp_t) {
     case EXP_UNCOMPRESSED: memcpy(t, vbuf(pg, v, 0), 1024ULL * sz); break;
     case EXP_FFOR:
-        if (unffor_parts(W, pg, v, 0, t)) return -1;
+        if (e->var == 1) {
+            if (unffor_parts(W, pg, v, 0, t)) return -1;
+        } else if (decode_patched(W, pg, e, v, t)) return -1;
         break;
     case EXP_DELTA:
         if (unffor_parts(W, pg, v, 1, d)) return -1;

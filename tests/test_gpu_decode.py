@@ -83,6 +83,8 @@ def test_config1_golden(torch_cuda, oracle):
 def _column(dtype, eid, n, seed):
     g = np.random.default_rng(seed)
     info = np.iinfo(dtype)
+    if eid in (2, 4, 5):
+        return wl.patch_column(dtype, n, seed=seed)
     if eid in (6, 7):
         return wl.geometric_runs(n, 16.0, seed=seed, value_bits=min(20, info.bits - 1)).astype(dtype)
     if eid >= 16:
@@ -112,6 +114,57 @@ def test_page_decode_matches_oracle(torch_cuda, oracle, dtype, eid):
     plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
     plan.check()
     np.testing.assert_array_equal(to_np(d_out, dtype, n), want)
+
+
+@pytest.mark.parametrize("eid", [2, 4, 5])
+@pytest.mark.parametrize("dtype", [np.uint32, np.int32, np.uint64, np.int64])
+def test_patched_ffor_matches_oracle(torch_cuda, oracle, dtype, eid):
+    """S_PATCH / B_PATCH / LL_PATCH pages from the oracle's encoder (the reference's exception choice), decoded on the
+    GPU; the column mixes narrow bodies with outliers, constant, all-zero and full-range vectors."""
+    torch = torch_cuda
+    print(f"patched dtype={np.dtype(dtype).name} expr={eid}", flush=True)
+    n = 40 * 1024
+    x = wl.patch_column(dtype, n, seed=eid + np.dtype(dtype).itemsize)
+    enc = oracle.encode_column(x, eid)
+    want = oracle.decode_column(enc, dtype, n)
+    np.testing.assert_array_equal(want, x)
+    plan = fa.Plan(enc, dtype)
+    d_in, d_out = dev(torch, enc), empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    plan.check()
+    np.testing.assert_array_equal(to_np(d_out, dtype, n), want)
+
+
+@pytest.mark.parametrize("eid", [2, 4, 5])
+def test_patched_ffor_large_round_trip(torch_cuda, eid):
+    """2^20 uint64 values through this library's encoder and the GPU decoder (round trip; the CPU encoders agree
+    byte for byte on smaller inputs)."""
+    torch = torch_cuda
+    x = wl.patch_column(np.uint64, 1 << 20, seed=40 + eid)
+    enc = fa.encode_from_memory(x, eid)
+    plan = fa.Plan(enc, np.uint64)
+    d_in, d_out = dev(torch, enc), empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    plan.check()
+    assert np.array_equal(to_np(d_out, np.uint64, x.size), x)
+
+
+def test_ll_patch_chain_leaving_the_vector_is_reported(torch_cuda, oracle):
+    """A linked list whose next hop leaves the vector: the kernel stops the chain and reports it."""
+    torch = torch_cuda
+    g = np.random.default_rng(3)
+    t = (1000 + g.integers(0, 128, 1024)).astype(np.int64)
+    t[[10, 400, 700, 1000, 1023]] = 41_000
+    x = oracle.untranspose(t)
+    enc = oracle.encode_column(x, 5)
+    off = np.frombuffer(enc[8:28].tobytes(), np.uint32)
+    bad = enc.copy()
+    bad[off[1] + 2: off[1] + 4] = np.frombuffer(np.uint16(1023).tobytes(), np.uint8)  # first position
+    plan = fa.Plan(bad, np.int64)
+    d_in, d_out = dev(torch, bad), empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    with pytest.raises(fa.FlsError):
+        plan.check()
 
 
 def test_multi_page_delta(torch_cuda, oracle):
@@ -202,7 +255,7 @@ def test_fsst_reference_table_golden(torch_cuda, oracle):
     np.testing.assert_array_equal(d_bytes.cpu().numpy()[: data.size], data)
 
 
-@pytest.mark.parametrize("eid", [0, 1, 11, 6, 16])
+@pytest.mark.parametrize("eid", [0, 1, 2, 4, 5, 11, 6, 16])
 def test_c_api_round_trip_host_pointers(torch_cuda, eid):
     x = _column(np.int64, eid, 8 * 1024, seed=3)
     enc = fa.encode_from_memory(x, eid)
@@ -221,7 +274,7 @@ def test_c_api_device_pointers(torch_cuda):
     np.testing.assert_array_equal(to_np(d_out, np.int64, x.size), x)
 
 
-@pytest.mark.parametrize("eid", [1, 11, 6, 7, 16, 17])
+@pytest.mark.parametrize("eid", [1, 2, 4, 5, 11, 6, 7, 16, 17])
 def test_random_access(torch_cuda, eid):
     import ctypes as C
     x = _column(np.int64, eid, 5 * 1024, seed=9)

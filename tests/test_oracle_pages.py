@@ -6,6 +6,7 @@ import os
 import numpy as np
 import pytest
 
+import fastlanes_amd as fa
 from fastlanes_amd import workloads as wl
 from tests.conftest import GOLDEN
 
@@ -83,7 +84,10 @@ def test_rejects_ragged_and_unknown(oracle):
     with pytest.raises(ValueError):
         oracle.encode_column(np.arange(1000, dtype=np.int64), 1)
     with pytest.raises(ValueError):
-        oracle.encode_column(np.arange(1024, dtype=np.int64), 3)  # patched variants are not supported
+        oracle.encode_column(np.arange(1024, dtype=np.int64), 3)  # S_PATCH_REC is not supported
+    for eid in (2, 4, 5):  # patched FFOR exists for 32- and 64-bit integers only
+        with pytest.raises(ValueError):
+            oracle.encode_column(np.arange(1024, dtype=np.uint16), eid)
     empty = oracle.encode_column(np.zeros(0, dtype=np.int64), 1)
     assert oracle.decode_column(empty, np.int64, 0).size == 0
 
@@ -160,3 +164,86 @@ def test_fsst_reference_decodes_oracle_tables(oracle, reference):
     o2, d2 = reference.fsst_decode(table, comp[: coff[-1]].copy(), coff, data.size)
     np.testing.assert_array_equal(o2, offs)
     np.testing.assert_array_equal(d2, data)
+
+
+# -- patched FFOR (S_PATCH 2, B_PATCH 4, LL_PATCH 5) -------------------------------------------------------------
+
+def _gen_5exp_vector(oracle):
+    """One int64 vector shaped like the reference's gen_5exp_nn_nr table: a 7-bit body and 5 exceptions about 2^15
+    above it, at transposed positions 10, 400, 700, 1000, 1023 (at bw 7, LL_PATCH needs 7 compulsory links there)."""
+    g = np.random.default_rng(3)
+    t = (1000 + g.integers(0, 128, 1024)).astype(np.int64)
+    t[0], t[1] = 1000, 1127
+    for i, p in enumerate([10, 400, 700, 1000, 1023]):
+        t[p] = 1000 + 40_000 + i
+    return oracle.untranspose(t)
+
+
+@pytest.mark.parametrize("n_vec", [1, 16])
+def test_patch_sizes_match_recorded(oracle, recorded, n_vec):
+    """Recorded sizes of test/local_result/col_result/generated/i64_pt{1,16}vec.csv rows 1, 2, 4, 5: they fix the
+    patch layouts (u16 count + positions; 128-byte bitmap; count + first position) and the chosen bw 7 with 5
+    exceptions (12 with LL's links), from both encoders, byte for byte the same."""
+    rows = {r["id"]: r["size"] for r in recorded["i64_pt1vec.csv" if n_vec == 1 else "i64_pt16vec.csv"]}
+    x = np.tile(_gen_5exp_vector(oracle), n_vec)
+    for eid in (1, 2, 4, 5):
+        enc = oracle.encode_column(x, eid)
+        assert enc.size == rows[eid], (eid, enc.size, rows[eid])
+        np.testing.assert_array_equal(fa.encode_from_memory(x, eid), enc)
+        np.testing.assert_array_equal(oracle.decode_column(enc, np.int64, x.size), x)
+
+
+def _one_vector_page(enc):
+    """(bw, base, exception count) of a one-vector patched FFOR column."""
+    off = np.frombuffer(enc[8:28].tobytes(), np.uint32)
+    sz = off[4] - off[3]  # the base buffer holds one value
+    bw = int(enc[off[2]])
+    base = int.from_bytes(enc[off[3]: off[3] + sz].tobytes(), "little")
+    return bw, base, int((off[1] - off[0]) // sz)
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.int64])
+def test_patch_choice_special_cases(oracle, dtype):
+    """Closed forms of find_best_params (src/cor/prm/ffor/ffor_analyze.cpp:41-75): a constant c != 0 takes bw 1
+    (SafeUpperBound(c, 0) is 0, include/fls/utl/util.hpp:91); all zeros take bw 0; full-range values keep bw W with
+    base 0; exactly EXC_LIMIT = 100 outliers are patched, 101 are not."""
+    W = np.dtype(dtype).itemsize * 8
+    g = np.random.default_rng(0)
+    cases = {
+        "const": (np.full(1024, 77, dtype), (1, 77, 0)),
+        "zero": (np.zeros(1024, dtype), (0, 0, 0)),
+        "full": (g.integers(np.iinfo(dtype).min, np.iinfo(dtype).max, 1024, dtype=dtype, endpoint=True), (W, 0, 0)),
+    }
+    for k, bw_want in ((100, 4), (101, None)):
+        x = (500 + g.integers(0, 16, 1024)).astype(dtype)
+        x[:2] = 500, 515
+        x[2 + np.arange(k) * 9] = 1 << (W - 2)
+        cases[f"{k} outliers"] = (x, (bw_want, 500, k) if bw_want else None)
+    for name, (x, want) in cases.items():
+        for eid in (2, 4):
+            enc = oracle.encode_column(x, eid)
+            np.testing.assert_array_equal(oracle.decode_column(enc, dtype, 1024), x)
+            got = _one_vector_page(enc)
+            if want is None:
+                assert got[2] == 0 and got[0] >= W - 2, (name, got)
+            else:
+                assert got == want, (name, eid, got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.int32, np.uint64, np.int64])
+@pytest.mark.parametrize("eid", [2, 4, 5])
+def test_patch_round_trip_and_encoders_agree(oracle, dtype, eid):
+    x = wl.patch_column(dtype, 40 * 1024, seed=eid)
+    enc = oracle.encode_column(x, eid)
+    np.testing.assert_array_equal(oracle.decode_column(enc, dtype, x.size), x)
+    np.testing.assert_array_equal(fa.encode_from_memory(x, eid), enc)
+
+
+def test_ll_patch_chain_leaving_the_vector_is_rejected(oracle):
+    x = _gen_5exp_vector(oracle)
+    enc = oracle.encode_column(x, 5)
+    off = np.frombuffer(enc[8:28].tobytes(), np.uint32)
+    bad = enc.copy()
+    bad[off[1] + 2: off[1] + 4] = np.frombuffer(np.uint16(1023).tobytes(), np.uint8)  # first position
+    with pytest.raises(ValueError):
+        oracle.decode_column(bad, np.int64, x.size)
