@@ -67,6 +67,12 @@ def pages(dev, stream, log2n=26, verbose=True):
             "rle": (6, wl.geometric_runs(m, 16.0, seed=1, value_bits=min(W - 1, 20)).astype(dt)),
             "dict_ffor": (17, rng.integers(0, 1 << min(W - 1, 12), m).astype(dt)),
         }
+        if W >= 32:  # patched FFOR: a 7-bit body with 1% outliers anywhere in the type's range
+            body = (1000 + rng.integers(0, 128, m)).astype(dt)
+            hit = rng.random(m) < 0.01
+            body[hit] = rng.integers(0, np.iinfo(dt).max, int(hit.sum()), dtype=dt, endpoint=True)
+            for name, eid in (("ffor_s_patch", 2), ("ffor_b_patch", 4), ("ffor_ll_patch", 5)):
+                cases[name] = (eid, body)
         for name, (eid, x) in cases.items():
             res[f"u{W}_{name}"] = decode_column_timed(x, eid, dev, stream)
             if verbose:
