@@ -270,10 +270,16 @@ __device__ __forceinline__ void apply_patches(const uint8_t* exc, const uint8_t*
             *at(k & 1023)    = load_t<T>(exc + i * SZ);
         }
     } else if constexpr (VAR == 4) {
-        // thread t < 16 owns bitmap word t; its first exception index is the popcount of the words before it
-        uint64_t unit = t < 16 ? *reinterpret_cast<const u64_ua*>(pos + 8 * t) : 0ull;
-        unsigned c    = 0;
-        for (unsigned w = 0; w < 16 && w < t; ++w) c += __builtin_popcountll(*reinterpret_cast<const u64_ua*>(pos + 8 * w));
+        // thread t < 16 owns bitmap word t; its first exception index is the popcount of the words before it (an
+        // exclusive scan over the slot's lanes; every lane of an active slot takes part)
+        uint64_t       unit = t < 16 ? *reinterpret_cast<const u64_ua*>(pos + 8 * t) : 0ull;
+        const unsigned own  = __builtin_popcountll(unit);
+        unsigned       incl = own;
+        static_for<4>([&](auto d) {
+            const unsigned y = __shfl_up(incl, 1u << decltype(d)::value, TPV);
+            if (t >= (1u << decltype(d)::value)) incl += y;
+        });
+        unsigned c = incl - own;
         while (unit) {
             const unsigned r = __builtin_ctzll(unit);
             *at(64 * t + r)  = load_t<T>(exc + c * SZ);
@@ -282,17 +288,27 @@ __device__ __forceinline__ void apply_patches(const uint8_t* exc, const uint8_t*
         }
     } else {
         if (t == 0) {
-            const unsigned c   = *reinterpret_cast<const u16_ua*>(pos);
-            T              cur = *reinterpret_cast<const u16_ua*>(pos + 2);
-            for (unsigned i = 0; i < c; ++i) {
-                if (cur >= 1024) {
-                    atomicOr(err, 8u);
-                    break;
-                }
-                T* slot          = at((unsigned)cur);
-                const T next_off = (T)(*slot - (T)base + 1);
-                *slot            = load_t<T>(exc + i * SZ);
-                cur              = (T)(cur + next_off);
+            // the exception values are loaded 16 at a time ahead of the walk: one memory wait per 16 links
+            constexpr unsigned K   = 16;
+            const unsigned     c   = *reinterpret_cast<const u16_ua*>(pos);
+            T                  cur = *reinterpret_cast<const u16_ua*>(pos + 2);
+            bool               ok  = true;
+            for (unsigned i0 = 0; i0 < c && ok; i0 += K) {
+                T e[K];
+                static_for<K>([&](auto k) { e[k] = i0 + k < c ? load_t<T>(exc + (i0 + k) * SZ) : (T)0; });
+                static_for<K>([&](auto k) {
+                    if (ok && i0 + k < c) {
+                        if (cur >= 1024) {
+                            atomicOr(err, 8u);
+                            ok = false;
+                        } else {
+                            T* slot          = at((unsigned)cur);
+                            const T next_off = (T)(*slot - (T)base + 1);
+                            *slot            = e[k];
+                            cur              = (T)(cur + next_off);
+                        }
+                    }
+                });
             }
         }
     }
