@@ -58,6 +58,14 @@ class Connection:
     def read_fls(self, dir_path: str) -> "Reader":
         return Reader(dir_path, self)
 
+    @staticmethod
+    def spell(columns) -> list[dict]:
+        """Connection::spell (src/connection.cpp:91-100) -> Wizard::Spell: the structural plan of a row-group's columns
+        (CONSTANT / EQUAL / one-to-one map / ZERO_COPY per column), computed on the GPU (fastlanes_amd.spell)."""
+        from fastlanes_amd import spell as sp
+
+        return sp.spell(columns)
+
 
 class Chunk:
     """One vector index of every column: device tensors of (up to) 1024 values each."""

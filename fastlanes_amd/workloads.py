@@ -258,3 +258,75 @@ def wizard_columns(n_columns: int = 512, n: int = 1 << 20, seed: int = 99):
     """The wizard data set: column c belongs to family c mod 5, with its own seed."""
     return [(WIZARD_FAMILIES[c % len(WIZARD_FAMILIES)], wizard_column(WIZARD_FAMILIES[c % len(WIZARD_FAMILIES)], n,
                                                                        seed * 100003 + c)) for c in range(n_columns)]
+
+
+# -- structural wizard tables -----------------------------------------------------------------------------------
+
+def _strs(items) -> tuple[np.ndarray, np.ndarray]:
+    """(offsets u64, bytes u8) of a list of bytes objects."""
+    lens = np.fromiter((len(b) for b in items), dtype=np.uint64, count=len(items))
+    offs = np.zeros(len(items) + 1, dtype=np.uint64)
+    np.cumsum(lens, out=offs[1:])
+    return offs, np.frombuffer(b"".join(items), dtype=np.uint8).copy()
+
+
+SPELL_EXAMPLE_1 = ["CONSTANT_INT", "CONSTANT_STR", "EQUAL_GROUP_0_COL_0", "RANGE_0_1000", "NUMBER_AS_STRING",
+                   "EQUAL_GROUP_0_COL_1", "EQUAL_GROUP_0_COL_2", "MAP1T1_GROUP_0_COL_0", "MAP1T1_GROUP_0_COL_1"]
+
+
+def spell_example_1(n: int = 8192, seed: int = 1):
+    """A table with the columns of the reference's generated example (data/generated/example_1/schema.json), filled
+    as their names say: a constant integer and string, a group of three equal integer columns, integers in
+    [0, 1000), numbers written as strings, and a string key column with an integer column that is a function of it.
+    Returns (columns, expected kinds, expected refs) -- the plan Wizard::Spell makes for such data."""
+    g = rng(seed)
+    eq = g.integers(-(1 << 62), 1 << 62, n, dtype=np.int64)
+    keys = g.integers(0, 100, n)
+    cols = [
+        np.full(n, 7, dtype=np.int64),
+        _strs([b"fastlanes"] * n),
+        eq,
+        g.integers(0, 1000, n).astype(np.int64),
+        _strs([str(v).encode() for v in g.integers(0, 1 << 40, n)]),
+        eq.copy(),
+        eq.copy(),
+        _strs([b"key_%d" % k for k in keys]),
+        (1000 + 7 * keys).astype(np.int64),
+    ]
+    kinds = ["CONSTANT", "CONSTANT", "EXPRESSION", "EXPRESSION", "EXPRESSION", "EQUAL", "EQUAL", "MAP_SOURCE",
+             "MAP_TARGET"]
+    refs = [-1, -1, -1, -1, -1, 2, 2, 8, 7]
+    return cols, kinds, refs
+
+
+def spell_edge_tables():
+    """Tables whose plan follows from their construction: the 25% distinct boundary, NaN and signed zeros, a type
+    mismatch, one first with several seconds, a second already taken. Yields (name, columns, kinds, refs)."""
+    n = 4096
+    i = np.arange(n)
+    g = rng(5)
+    c4 = g.standard_normal(n)
+    c4[::7] = 0.0
+    c4[3::7] = -0.0
+    c5 = c4.copy()
+    c5[::7] = -0.0
+    c5[3::7] = 0.0
+    c6 = ((i * 7919) % 5).astype(np.float64)
+    c6[5] = np.nan
+    yield ("boundary_nan_zero",
+           [(i % 1024).astype(np.int32), ((i % 1024) * 3 + 1).astype(np.int64), (i % 1025).astype(np.uint16),
+            (i % 1025).astype(np.float64), c4, c5, c6, c6.copy()],
+           ["MAP_SOURCE", "MAP_TARGET", "EXPRESSION", "EXPRESSION", "EXPRESSION", "EQUAL", "MAP_SOURCE", "MAP_TARGET"],
+           [1, 0, -1, -1, -1, 4, 7, 6])
+    n = 2048
+    i = np.arange(n)
+    yield ("several_seconds",
+           [(i % 16).astype(np.int64), (i % 16).astype(np.uint64), _strs([b"s%d" % v for v in i % 16]),
+            (i % 8).astype(np.int8), (i % 32).astype(np.int64), ((i % 32) // 2).astype(np.int64)],
+           ["MAP_SOURCE", "MAP_TARGET", "MAP_TARGET", "MAP_TARGET", "MAP_SOURCE", "MAP_TARGET"],
+           [1, 0, 0, 0, 5, 4])
+    yield ("constants", [np.full(n, 5, np.int64), np.full(n, 5, np.int64), _strs([b""] * n)],
+           ["CONSTANT", "CONSTANT", "CONSTANT"], [-1, -1, -1])
+    yield ("one_row", [np.array([3], np.int64), np.array([4.0]), _strs([b"x"])],
+           ["CONSTANT", "CONSTANT", "CONSTANT"], [-1, -1, -1])
+    yield ("empty", [np.zeros(0, np.int64), np.zeros(0, np.uint8)], ["CONSTANT", "CONSTANT"], [-1, -1])

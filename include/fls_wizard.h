@@ -61,6 +61,31 @@ int fls_mlp_predict(fls_mlp* m, const float* dev_features, uint64_t n, int32_t* 
 /* Logits of the last forward pass (host copy, batch x 32 float32). */
 int fls_mlp_get_logits(fls_mlp* m, float* host_logits, uint32_t batch);
 
+/* ---- Structural wizard (reference Wizard::Spell, src/wizard/wizard.cpp:19-44) -------------------------------- */
+/* Plan of each column of a row-group, in the reference's order: CONSTANT (all values equal), then EQUAL (same type
+ * and values as an earlier column), then one-to-one maps (a first column with at most 25% distinct values whose
+ * every value always meets the same value of a later column: the first becomes DICT + SUB_EXPRESSION, the second
+ * FIXME; a first may take several seconds), and EXPRESSION (ZERO_COPY) for the rest. Values compare as the
+ * reference's types do: integers bit for bit, doubles with == (-0.0 equals 0.0, a NaN equals nothing), strings byte
+ * for byte. */
+enum fls_spell_kind {
+    FLS_SPELL_EXPRESSION = 0, /* ZERO_COPY: encoded by an expression of its own */
+    FLS_SPELL_CONSTANT   = 1, /* CONSTANT */
+    FLS_SPELL_EQUAL      = 2, /* EQUAL; ref = the first earlier column with the same values */
+    FLS_SPELL_MAP_SOURCE = 3, /* DICT + SUB_EXPRESSION (uint16), once per target; ref = its first target */
+    FLS_SPELL_MAP_TARGET = 4  /* FIXME; ref = its source */
+};
+struct fls_spell_column {
+    uint8_t         type;    /* enum data_t of fastlanes.h (DOUBLE, INT8..UINT64, STR) */
+    const void*     values;  /* device: n_rows values, or the string bytes */
+    const uint64_t* offsets; /* device, strings only: n_rows + 1 byte offsets into values */
+};
+/* Plans n_columns device-resident columns of n_rows rows each (n_rows < 2^32 - 1). out_kind / out_ref are host
+ * arrays of n_columns (ref -1 where the kind has none). Runs on `stream` and returns when the plan is complete;
+ * allocates its hash tables stream-ordered and frees them before returning. */
+int fls_wizard_spell(uint32_t n_columns, const struct fls_spell_column* columns, uint64_t n_rows, int32_t* out_kind,
+                     int32_t* out_ref, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

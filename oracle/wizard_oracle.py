@@ -17,6 +17,13 @@ What it restates, and how each part is pinned:
     reference). PINNED by brute-force Python loops on small samples (tests/test_wizard_oracle.py).
   * mlp_forward_backward(...): softmax cross-entropy MLP 64-256-256-K with the GPU kernels' rounding points
     (BF16 operands, FP32 accumulation). PINNED by float64 finite differences of the loss (tests/test_wizard_oracle.py).
+  * spell(columns): the structural wizard Wizard::Spell (src/wizard/wizard.cpp:19-151) as literal loops over Python
+    values, whose == / != and dict semantics match the reference's types (ints; floats: -0.0 == 0.0, a NaN equals
+    nothing and each NaN is its own dict key, as in the reference's unordered_map; strings as bytes). PINNED by
+    tables built to a known plan: the columns of the reference's own generated example
+    (data/generated/example_1/schema.json: a constant int and string, an equal group of three, a one-to-one pair),
+    the 25% distinct-value boundary, NaN and signed-zero cases, type mismatches, a first with two seconds, and the
+    order effects of the reference's loops (tests/test_wizard_oracle.py).
 """
 from __future__ import annotations
 
@@ -262,3 +269,59 @@ def mlp_forward_backward(p: np.ndarray, X: np.ndarray, y: np.ndarray, n_classes:
 
 def mlp_loss(p, X, y, n_classes):
     return mlp_forward_backward(p, X, y, n_classes, rounding=False)[0]
+
+
+# -- structural wizard ------------------------------------------------------------------------------------------
+
+def spell(columns):
+    """columns: list of (type_name, values) with values a Python list (ints, floats or bytes). Returns (kinds, refs):
+    kinds in EXPRESSION / CONSTANT / EQUAL / MAP_SOURCE / MAP_TARGET, refs the column each kind refers to (-1)."""
+    n = len(columns)
+    rows = len(columns[0][1]) if n else 0
+    kinds, refs, det = ["EXPRESSION"] * n, [-1] * n, [False] * n
+
+    # constant_check (wizard.cpp:88-93): stats.is_constant, cleared when a value differs from the previous one
+    # (src/table/attribute.cpp:136-138)
+    for c, (_, v) in enumerate(columns):
+        if not any(v[k] != v[k - 1] for k in range(1, len(v))):
+            kinds[c], det[c] = "CONSTANT", True
+
+    # equality_check (wizard.cpp:96-111), Rowgroup::Equal (src/table/rowgroup.cpp:113-156)
+    for i in range(n):
+        for j in range(i + 1, n):
+            if det[j]:
+                continue
+            if columns[i][0] == columns[j][0] and not any(a != b for a, b in zip(columns[i][1], columns[j][1])):
+                kinds[j], refs[j], det[j] = "EQUAL", i, True
+
+    def good_for_dictionary(c):  # src/table/rowgroup.cpp:192-217
+        distinct = {}
+        for x in columns[c][1]:
+            distinct.setdefault(x, None)
+        return not (len(distinct) / rows > 0.25) if rows else True
+
+    def is_map_1t1(f, s):  # src/table/rowgroup.cpp:159-175
+        m = {}
+        for a, b in zip(columns[f][1], columns[s][1]):
+            if a in m:
+                if m[a] != b:
+                    return False
+            else:
+                m[a] = b
+        return True
+
+    # map_1t1_check (wizard.cpp:125-151): the first's own flag is tested only in the outer loop
+    for f in range(n):
+        if det[f]:
+            continue
+        for s in range(f + 1, n):
+            if det[s]:
+                continue
+            if not good_for_dictionary(f):
+                continue
+            if is_map_1t1(f, s):
+                if kinds[f] != "MAP_SOURCE":
+                    kinds[f], refs[f] = "MAP_SOURCE", s
+                det[f] = True
+                kinds[s], refs[s], det[s] = "MAP_TARGET", f, True
+    return kinds, refs

@@ -121,3 +121,41 @@ def test_bf16_rounding_is_nearest_even():
     r = wo.bf16(x)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.015625  # ties to even, above-half rounds up
     assert abs(r[3] - -2.5e-3) < 2e-5
+
+
+# -- structural wizard (Wizard::Spell) ----------------------------------------------------------------------------
+
+def oracle_columns(columns):
+    """Columns as the oracle takes them: (type name, Python values)."""
+    out = []
+    for c in columns:
+        if isinstance(c, tuple):
+            offs, data = c
+            b = data.tobytes()
+            out.append(("str", [b[int(offs[k]): int(offs[k + 1])] for k in range(len(offs) - 1)]))
+        else:
+            out.append((c.dtype.name, c.tolist()))
+    return out
+
+
+def test_spell_example_1_plan():
+    cols, kinds, refs = wl.spell_example_1()
+    assert wo.spell(oracle_columns(cols)) == (kinds, refs)
+
+
+@pytest.mark.parametrize("case", list(wl.spell_edge_tables()), ids=lambda c: c[0])
+def test_spell_edge_tables(case):
+    _, cols, kinds, refs = case
+    assert wo.spell(oracle_columns(cols)) == (kinds, refs)
+
+
+def test_spell_operator_lists():
+    from fastlanes_amd import spell
+
+    _, cols, kinds, refs = next(c for c in wl.spell_edge_tables() if c[0] == "several_seconds")
+    ops = spell.operators(kinds, refs)
+    assert ops[0] == ["DICT", "SUB_EXPRESSION"] * 3 and ops[4] == ["DICT", "SUB_EXPRESSION"]
+    assert ops[1] == ["FIXME"]
+    cols, kinds, refs = wl.spell_example_1()
+    ops = spell.operators(kinds, refs)
+    assert ops[:3] == [["CONSTANT"], ["CONSTANT"], ["ZERO_COPY"]] and ops[5] == ["EQUAL"]
