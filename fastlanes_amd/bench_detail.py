@@ -117,6 +117,42 @@ def cascade(dev, stream, log2n=26, log2_strings=24, verbose=True):
     return out
 
 
+def spell_bench(dev, log2n=20, n_blocks=16, reps=5, verbose=True):
+    """Structural wizard over a 64-column row-group of 2^20 rows (per block of four: a high-cardinality int64
+    column, its copy, an int32 key with ~1000 distinct values and an int64 function of it), device-resident."""
+    import torch
+
+    from fastlanes_amd import spell as sp
+
+    n = 1 << log2n
+    g = np.random.default_rng(4)
+    i = np.arange(n)
+    cols, keep = [], []
+    for blk in range(n_blocks):
+        hi = g.integers(0, 1 << 62, n, dtype=np.int64)
+        key = ((i * (blk + 3)) % (1000 + blk)).astype(np.int32)
+        cols += [hi, hi.copy(), key, key.astype(np.int64) * 11 + blk]
+    descs = []
+    for c in cols:
+        d = torch.from_numpy(c).to(dev)
+        keep.append(d)
+        descs.append((fa.dtype_code(c.dtype), d.data_ptr(), 0))
+    torch.cuda.synchronize(dev)
+    times = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        kinds, refs = sp.spell_device(descs, n)
+        times.append(time.perf_counter() - t0)
+    t = min(times[1:])
+    nbytes = sum(c.nbytes for c in cols)
+    out = {"columns": len(cols), "rows": n, "column_bytes": int(nbytes), "ms": round(t * 1e3, 3),
+           "gb_s_over_column_bytes": round(nbytes / t / 1e9, 1),
+           "plan": {k: kinds.count(k) for k in sp.KINDS}}
+    if verbose:
+        print(f"spell: {out}", flush=True)
+    return out
+
+
 def main():
     import argparse
     import json
@@ -124,7 +160,7 @@ def main():
     import torch
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sections", default="pages,cascade", help="pages, cascade, rowgroup, wizard")
+    ap.add_argument("--sections", default="pages,cascade", help="pages, cascade, rowgroup, wizard, spell")
     ap.add_argument("--log2n", type=int, default=26)
     a = ap.parse_args()
     dev = torch.device("cuda:0")
@@ -143,6 +179,8 @@ def main():
             from fastlanes_amd import wizard
 
             res["wizard"] = wizard.bench(dev)
+        elif sec == "spell":
+            res["spell"] = spell_bench(dev)
     print(json.dumps(res), flush=True)
 
 

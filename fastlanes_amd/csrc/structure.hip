@@ -116,8 +116,16 @@ __global__ void k_content_hash(SCol c, uint64_t n, unsigned long long* hash, uin
             has_nan |= x != x;
         }
     }
+    // one atomic per block: wave sums, then the block's waves through LDS
+    __shared__ uint64_t part[16];
     for (int d = 32; d >= 1; d >>= 1) h += __shfl_xor(h, d, 64);
-    if (threadIdx.x % 64 == 0) atomicAdd(hash, (unsigned long long)h);
+    if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (unsigned w = 0; w < blockDim.x / 64; ++w) t += part[w];
+        atomicAdd(hash, (unsigned long long)t);
+    }
     if (has_nan) nan[0] = 1u;
 }
 
@@ -130,9 +138,9 @@ __global__ void k_columns_differ(SCol a, SCol b, uint64_t n, uint32_t* flag) {
 
 // Distinct values of a column into an open-addressing set of `cap` (power of two) 64-bit slots: 0 = empty, else
 // (32 hash bits << 32) | (row + 1) of the value's first insertion. codes[i] = the slot of row i's value. count =
-// slots taken; once it passes `limit` the column is too diverse and `stop` ends the pass. The launch uses at most
-// cap / 4 threads and cap >= 4 (limit + 1), so at most limit + 1 + cap / 4 <= cap / 2 slots are ever taken and
-// every probe sequence ends at an empty or matching slot; the cap-step bound is a second guarantee.
+// slots taken (added once per wave); once it passes `limit` the column is too diverse and `stop` ends the pass.
+// cap >= 2 * rows, so the set is never more than half full and every probe sequence ends at an empty or matching
+// slot; the cap-step bound is a second guarantee.
 template <int KT>
 __global__ void k_distinct(SCol c, uint64_t n, unsigned long long* table, uint32_t cap, uint32_t* codes,
                            uint32_t* count, uint32_t limit, uint32_t* stop) {
@@ -149,7 +157,6 @@ __global__ void k_distinct(SCol c, uint64_t n, unsigned long long* table, uint32
                 w = atomicCAS(table + s, 0ull, mine);
                 if (w == 0ull) {
                     code = s;
-                    if (atomicAdd(count, 1u) + 1u > limit) __hip_atomic_store(stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
             }
@@ -160,6 +167,15 @@ __global__ void k_distinct(SCol c, uint64_t n, unsigned long long* table, uint32
         }
         if (code == 0xffffffffu) __hip_atomic_store(stop, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         codes[i] = code;
+        // new slots are counted once per wave: the lowest active lane adds the wave's total
+        const bool     fresh  = code != 0xffffffffu && (uint32_t)(table[code] & 0xffffffffull) == (uint32_t)(i + 1);
+        const uint64_t active = __ballot(1);
+        const uint64_t mask   = __ballot(fresh);
+        const int      leader = __ffsll((long long)active) - 1;
+        if ((int)__lane_id() == leader && mask) {
+            const uint32_t add = (uint32_t)__popcll(mask);
+            if (atomicAdd(count, add) + add > limit) __hip_atomic_store(stop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -214,7 +230,7 @@ using namespace fls;
 extern "C" int fls_wizard_spell(uint32_t n_columns, const struct fls_spell_column* columns, uint64_t n_rows,
                                 int32_t* out_kind, int32_t* out_ref, void* stream) {
     if (n_columns && (!columns || !out_kind || !out_ref)) return fail(FLS_E_ARG, "null argument");
-    if (n_rows >= 0xffffffffull) return fail(FLS_E_ARG, "a row-group holds fewer than 2^32 - 1 rows");
+    if (n_rows > (1ull << 30)) return fail(FLS_E_ARG, "a row-group holds at most 2^30 rows");
     std::vector<int> kt(n_columns);
     for (uint32_t c = 0; c < n_columns; ++c) {
         kt[c] = key_of(columns[c].type);
@@ -244,7 +260,8 @@ extern "C" int fls_wizard_spell(uint32_t n_columns, const struct fls_spell_colum
         const dim3 g(grid_for_rows(n)), b(256);
         KEY_SWITCH(kt[c], {
             hipLaunchKernelGGL((k_not_constant<KT>), g, b, 0, s, sc, n, &d_flags[c].not_const);
-            hipLaunchKernelGGL((k_content_hash<KT>), g, b, 0, s, sc, n, &d_flags[c].hash, &d_flags[c].nan);
+            hipLaunchKernelGGL((k_content_hash<KT>), dim3(grid_for_rows(n, 1024)), b, 0, s, sc, n, &d_flags[c].hash,
+                               &d_flags[c].nan);
         });
     }
     if ((e = hipGetLastError()) != hipSuccess) return finish(hip_fail(e, "kernel launch"));
@@ -279,7 +296,7 @@ extern "C" int fls_wizard_spell(uint32_t n_columns, const struct fls_spell_colum
     // 3) one-to-one maps, in the reference's order
     const uint32_t limit = (uint32_t)(n / 4);  // distinct / rows <= 0.25
     uint32_t       cap   = 64;
-    while (cap < 4ull * (limit + 1)) cap <<= 1;
+    while (cap < 2ull * n) cap <<= 1;
     unsigned long long* d_table = nullptr;
     uint32_t*           d_codes = nullptr;
     uint32_t*           d_cnt   = nullptr;  // [0] count, [1] stop, [2..] per-second flags
@@ -304,10 +321,10 @@ extern "C" int fls_wizard_spell(uint32_t n_columns, const struct fls_spell_colum
         if ((e = hipMemsetAsync(d_table, 0, 8ull * cap, s)) != hipSuccess ||
             (e = hipMemsetAsync(d_cnt, 0, 4ull * (2 + n_columns), s)) != hipSuccess)
             return finish2(hip_fail(e, "hipMemsetAsync"));
-        const SCol     fc      = col(f);
-        const unsigned threads = std::max(64u, cap / 4);
-        KEY_SWITCH(kt[f], hipLaunchKernelGGL((k_distinct<KT>), dim3(grid_for_rows(n, threads / 256 ? threads / 256 : 1)),
-                                             dim3(256), 0, s, fc, n, d_table, cap, d_codes, d_cnt, limit, d_cnt + 1));
+        const SCol fc = col(f);
+        // 2^18 threads: a column with more than n / 4 distinct values stops after about one row per thread
+        KEY_SWITCH(kt[f], hipLaunchKernelGGL((k_distinct<KT>), dim3(grid_for_rows(n, 1024)), dim3(256), 0, s, fc, n,
+                                             d_table, cap, d_codes, d_cnt, limit, d_cnt + 1));
         uint32_t hc[2];
         if ((e = hipGetLastError()) != hipSuccess ||
             (e = hipMemcpyAsync(hc, d_cnt, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||

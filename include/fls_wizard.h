@@ -80,7 +80,7 @@ struct fls_spell_column {
     const void*     values;  /* device: n_rows values, or the string bytes */
     const uint64_t* offsets; /* device, strings only: n_rows + 1 byte offsets into values */
 };
-/* Plans n_columns device-resident columns of n_rows rows each (n_rows < 2^32 - 1). out_kind / out_ref are host
+/* Plans n_columns device-resident columns of n_rows rows each (n_rows <= 2^30). out_kind / out_ref are host
  * arrays of n_columns (ref -1 where the kind has none). Runs on `stream` and returns when the plan is complete;
  * allocates its hash tables stream-ordered and frees them before returning. */
 int fls_wizard_spell(uint32_t n_columns, const struct fls_spell_column* columns, uint64_t n_rows, int32_t* out_kind,
