@@ -41,16 +41,18 @@ def _time(fn, stream, reps=5, warm=2):
     return float(np.median(ts))
 
 
-def _candidates(n_cu):
+def _candidates(n_cu, nts=NTS):
     for block in BLOCKS:
-        for nt in NTS:
+        for nt in nts:
             for k in CU_BLOCKS:
                 yield block, nt, k * n_cu
 
 
 def tune_family(lib, family, W, bw, fn, stream, n_cu, bytes_moved):
     best = None
-    for block, nt, maxb in _candidates(n_cu):
+    # integer pages of 32/64-bit values may also write without LDS staging (bit 1 of the store flags)
+    nts = NTS + (2, 3) if family in (FAM_UNC, FAM_FFOR, FAM_DELTA) and W >= 32 else NTS
+    for block, nt, maxb in _candidates(n_cu, nts):
         fa.check(lib.fls_gpu_tune_set(family, W, bw, block, nt, maxb), "tune_set")
         t = _time(fn, stream)
         if best is None or t < best[0]:

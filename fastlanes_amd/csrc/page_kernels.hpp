@@ -184,7 +184,9 @@ __device__ __forceinline__ IntMeta int_meta(const uint8_t* page, const PageMeta&
 // Staged loop as staged_vector_loop, software-pipelined one step deep: the metadata of the vector a thread decodes
 // next is loaded right after the current vector's values are decoded, so it arrives while the wave writes its
 // output, and each step waits on one round of loads (the packed values) instead of three dependent ones.
-template <int W, int MODE, bool NT>
+// DIRECT (32/64-bit values): no LDS staging; each thread writes its lane's contiguous segment (128 or 512 bytes,
+// whole lines) straight to HBM, so occupancy is bounded by registers instead of the staging buffers.
+template <int W, int MODE, bool NT, bool DIRECT = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
                                                         uint8_t* __restrict__ out) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
@@ -203,18 +205,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restric
     for (; v0 < N; v0 += stride) {
         const uint64_t v = v0 + slot;
         if (v < N) {
-            const uint8_t* pk = page + pm.off[P] + m.ep;
-            if constexpr (MODE == 0) {
-                ffor_untransposed<W, W, false>(pk, 0, 0, t, sink);
-            } else {
-                dispatch_bw<W>(m.bw, [&](auto BW) { ffor_untransposed<W, BW, MODE == 2>(pk, m.base, m.lb, t, sink); });
-            }
+            const uint8_t* pk     = page + pm.off[P] + m.ep;
+            auto           decode = [&](const auto& dst) {
+                if constexpr (MODE == 0) {
+                    ffor_untransposed<W, W, false>(pk, 0, 0, t, dst);
+                } else {
+                    dispatch_bw<W>(m.bw, [&](auto BW) { ffor_untransposed<W, BW, MODE == 2>(pk, m.base, m.lb, t, dst); });
+                }
+            };
+            if constexpr (DIRECT) decode(GlobalSink<NT> {out + v * VB});
+            else decode(sink);
         }
         if (v + stride < N) m = int_meta<W, MODE>(page, pm, v + stride, t);
-        wave_sync();
-        const uint64_t left = N - v0;
-        wave_copy_out<TPV, VB, NT>(wl, out + v0 * VB, left < WS::VPW ? (uint32_t)left : WS::VPW, lane);
-        wave_sync();
+        if constexpr (!DIRECT) {
+            wave_sync();
+            const uint64_t left = N - v0;
+            wave_copy_out<TPV, VB, NT>(wl, out + v0 * VB, left < WS::VPW ? (uint32_t)left : WS::VPW, lane);
+            wave_sync();
+        }
     }
 }
 

@@ -370,3 +370,34 @@ def test_ffor_page_with_strided_metadata(torch_cuda, oracle, W):
     plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(to_np(d_out, dt, x.size), x)
+
+
+@pytest.mark.parametrize("W", [32, 64])
+def test_integer_pages_direct_stores(torch_cuda, oracle, W):
+    """UNCOMPRESSED / FFOR / DELTA pages of 32/64-bit values with the launch shape that writes lane segments
+    directly (no LDS staging), several grid shapes; the default table is restored afterwards."""
+    import ctypes as C
+    import os
+
+    torch = torch_cuda
+    lib = fa.lib()
+    lib.fls_gpu_tune_set.argtypes = [C.c_int, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint]
+    lib.fls_gpu_tune_load.argtypes = [C.c_char_p]
+    dt = W2DT[W]
+    try:
+        for fam, eid in ((1, 0), (2, 1), (3, 11)):
+            x = _column(dt, eid, 37 * 1024, seed=fam + W)
+            enc = oracle.encode_column(x, eid)
+            want = oracle.decode_column(enc, dt, x.size)
+            plan = fa.Plan(enc, dt)
+            d_in = dev(torch, enc)
+            for block, nt, maxb in ((256, 2, 0), (128, 3, 0), (512, 2, 8)):
+                print(f"direct W={W} family={fam} block={block} nt={nt} max_blocks={maxb}", flush=True)
+                fa.check(lib.fls_gpu_tune_set(fam, W, 255, block, nt, maxb), "tune_set")
+                d_out = empty(torch, x.nbytes)
+                plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+                plan.check()
+                np.testing.assert_array_equal(to_np(d_out, dt, x.size), want)
+    finally:
+        path = os.path.join(os.path.dirname(os.path.abspath(fa.__file__)), "tuning.txt")
+        assert lib.fls_gpu_tune_load(path.encode()) > 0
