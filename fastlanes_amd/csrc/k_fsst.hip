@@ -20,8 +20,8 @@ namespace {
 
 constexpr unsigned kSeg      = 512;        // strings per segment (= threads per block)
 constexpr unsigned kSegPerV  = 1024 / kSeg;
-constexpr uint32_t kCompLds  = 24 * 1024;  // staged compressed bytes per segment
-constexpr uint32_t kOutLds   = 32 * 1024;  // staged decoded bytes per segment
+constexpr uint32_t kCompLds  = 20 * 1024;  // staged compressed bytes per segment
+constexpr uint32_t kOutLds   = 24 * 1024;  // staged decoded bytes per segment
 
 // transposed index of original position i (inverse of untranspose_i): i = r + 8*ORDER[b] + 64c -> 128r + 16b + c
 __device__ __forceinline__ unsigned transposed_of(unsigned i) {
@@ -163,6 +163,92 @@ struct WordWriter {
     }
 };
 
+// Decodes a staged segment (n compressed bytes in cbuf, this thread's string starting at my_start) into obuf,
+// splitting the bytes evenly over the block's threads. A byte is a literal when it follows an odd number of 255
+// bytes inside its string (a run of escapes starts with a code, so codes and literals alternate along it); a code
+// c < 255 emits symbol[c], an escape emits the byte after it (0 when the string ends first), a literal emits
+// nothing -- the same lengths k_fsst_len counts per string. Each thread's output range comes from a block scan of
+// its byte share; it assembles its bytes in a 64-bit register and stores whole aligned 32-bit words, OR-ing into
+// the (pre-zeroed) first and last words it shares with its neighbours. Returns false (nothing written) when the
+// total disagrees with the segment's offsets; every thread returns the same value.
+__device__ __forceinline__ bool decode_segment_bytes(const uint64_t* sym, const uint8_t* len, const uint8_t* cbuf,
+                                                     uint32_t n, uint32_t my_start, uint8_t* obuf, uint64_t n_out) {
+    __shared__ uint32_t sbits[kCompLds / 32 + 1];  // string starts over positions [0, n]
+    const unsigned      tid = threadIdx.x;
+    for (uint32_t k = tid; k <= n / 32; k += kSeg) sbits[k] = 0u;
+    __syncthreads();
+    atomicOr(&sbits[my_start >> 5], 1u << (my_start & 31));
+    if (tid == 0) atomicOr(&sbits[n >> 5], 1u << (n & 31));
+    __syncthreads();
+    auto is_start = [&](uint32_t p) { return ((sbits[p >> 5] >> (p & 31)) & 1u) != 0; };
+    const uint32_t C = (n + kSeg - 1) / kSeg, q0 = tid * C < n ? tid * C : n, q1 = q0 + C < n ? q0 + C : n;
+    bool lit0 = false;
+    if (q0 < q1 && !is_start(q0)) {
+        uint32_t p = q0;
+        while (!is_start(p) && cbuf[p - 1] == 255) --p;
+        lit0 = ((q0 - p) & 1u) != 0;
+    }
+    uint32_t tot = 0;
+    bool     lit = lit0;
+    for (uint32_t q = q0; q < q1; ++q) {
+        const unsigned c = cbuf[q];
+        if (lit) {
+            lit = false;
+        } else if (c == 255) {
+            tot += 1;
+            lit = !is_start(q + 1);
+        } else {
+            tot += len[c];
+        }
+    }
+    uint64_t total;
+    const uint32_t base = (uint32_t)block_exclusive_scan(tot, &total);
+    if (total != n_out) return false;
+    uint32_t* words = reinterpret_cast<uint32_t*>(obuf);
+    const uint32_t first = base >> 2, last = (base + tot - 1) >> 2;
+    if (tot) words[first] = 0u, words[last] = 0u;
+    __syncthreads();
+    if (tot) {
+        uint32_t w   = first;
+        uint64_t acc = 0;
+        unsigned ab  = base & 3u;  // pending bytes in acc (the leading ones are the neighbour's: zeros)
+        auto put_word = [&](uint32_t x) {
+            if (w == first || w == last) atomicOr(&words[w], x);
+            else words[w] = x;
+            ++w;
+        };
+        lit = lit0;
+        for (uint32_t q = q0; q < q1; ++q) {
+            const unsigned c = cbuf[q];
+            uint64_t       x;
+            unsigned       k;
+            if (lit) {
+                lit = false;
+                continue;
+            } else if (c == 255) {
+                lit = !is_start(q + 1);
+                x = lit ? cbuf[q + 1] : 0u, k = 1;
+            } else {
+                x = sym[c], k = len[c];
+            }
+            const uint64_t lo  = acc | (x << (8 * ab));
+            const uint64_t hi  = ab ? x >> (64 - 8 * ab) : 0ull;
+            const unsigned all = ab + k;
+            if (all >= 8) {
+                put_word((uint32_t)lo), put_word((uint32_t)(lo >> 32));
+                acc = hi, ab = all - 8;
+            } else if (all >= 4) {
+                put_word((uint32_t)lo);
+                acc = lo >> 32, ab = all - 4;
+            } else {
+                acc = lo, ab = all;
+            }
+        }
+        if (ab) atomicOr(&words[w], (uint32_t)acc);  // the last, partial word
+    }
+    return true;
+}
+
 __global__ __launch_bounds__(kSeg) void k_fsst_decode(const uint8_t* __restrict__ page, const FsstTableDev* __restrict__ table,
                                                       uint64_t first_string, const uint64_t* __restrict__ offsets,
                                                       uint8_t* __restrict__ out, uint64_t cap, uint32_t* err) {
@@ -179,10 +265,15 @@ __global__ __launch_bounds__(kSeg) void k_fsst_decode(const uint8_t* __restrict_
     const uint64_t  o0 = o[h * kSeg], o1 = o[h * kSeg + kSeg];
     const bool      in_lds  = stage_bytes(cb, s0, s1 > s0 ? s1 : s0, cbuf);
     const bool      out_lds = in_lds && o1 >= o0 && o1 - o0 <= kOutLds && o1 <= cap;
-    __syncthreads();
     uint32_t s = end_of(ends, (int)i - 1), e = end_of(ends, (int)i);
-    if (s > e) s = e;
     const uint64_t w0 = o[i], stop = o[i + 1];
+    // Byte-parallel form: when the segment's strings are well formed, every thread decodes an equal share of the
+    // segment's compressed bytes instead of one whole string (a wave otherwise runs as long as its longest string).
+    const bool fast = __syncthreads_and(out_lds && s0 <= s && s <= e && e <= s1 && w0 <= stop && stop <= cap) != 0;
+    if (fast && decode_segment_bytes(sym, len, cbuf, s1 - s0, s - s0, obuf, o1 - o0)) {
+        // decoded into obuf; the copy-out follows
+    } else {
+    if (s > e) s = e;
     if (stop > cap || stop < w0) {
         atomicOr(err, 4u);
     } else if (out_lds && s >= s0 && e <= s1) {
@@ -212,6 +303,7 @@ __global__ __launch_bounds__(kSeg) void k_fsst_decode(const uint8_t* __restrict_
                 q += 1;
             }
         }
+    }
     }
     if (out_lds) {
         __syncthreads();
