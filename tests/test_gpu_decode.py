@@ -214,6 +214,55 @@ def test_fsst_strings_match_oracle(torch_cuda, oracle):
     np.testing.assert_array_equal(d_bytes.cpu().numpy()[:total], data)
 
 
+def _mixed_strings(seed):
+    """Vectors of: vocabulary strings; strings of random bytes including runs of 0xFF (escape runs after
+    compression); empty strings; 100-300-byte strings (their segments exceed the LDS buffers and decode from
+    global memory)."""
+    g = np.random.default_rng(seed)
+    out = []
+    words = wl.vocabulary(300, seed=seed)
+    for v in range(8):
+        for i in range(1024):
+            kind = v % 4
+            if kind == 0:
+                out.append(b" ".join(words[int(k)] for k in g.integers(0, len(words), int(g.integers(1, 5)))))
+            elif kind == 1:
+                b = bytearray(g.integers(0, 256, int(g.integers(0, 40)), dtype=np.uint8).tobytes())
+                if i % 3 == 0 and b:
+                    r = int(g.integers(1, 6))
+                    p = int(g.integers(0, len(b)))
+                    b[p:p] = b"\xff" * r
+                out.append(bytes(b))
+            elif kind == 2:
+                out.append(b"" if i % 2 else words[int(g.integers(0, len(words)))])
+            else:
+                out.append(bytes(g.integers(97, 123, int(g.integers(100, 300)), dtype=np.uint8).tobytes()))
+    return wl._strs(out)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_fsst_escapes_empty_and_long_strings(torch_cuda, oracle, seed):
+    """Both decode forms (byte-parallel segments in LDS, and one string per thread from global memory) on strings
+    with escape runs, empty strings and segments too large for LDS, encoded by this library's FSST encoder."""
+    torch = torch_cuda
+    print(f"fsst mixed strings seed={seed}", flush=True)
+    offs, data = _mixed_strings(seed)
+    enc = fa.encode_strings(offs.astype(np.uint32), data)
+    n = offs.size - 1
+    o2, d2 = oracle.decode_str_column(enc, n, data.size + 64)
+    np.testing.assert_array_equal(d2, data)
+    plan = fa.Plan(enc, "str")
+    d_in = dev(torch, enc)
+    d_off = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
+    plan.string_offsets(d_in.data_ptr(), d_off.data_ptr())
+    np.testing.assert_array_equal(d_off.cpu().numpy(), offs.astype(np.int64))
+    d_bytes = empty(torch, data.size)
+    plan.decode_strings(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size)
+    torch.cuda.synchronize()
+    plan.check()
+    np.testing.assert_array_equal(d_bytes.cpu().numpy()[: data.size], data)
+
+
 def test_fsst_reference_table_golden(torch_cuda, oracle):
     """Strings compressed by the reference's own fsst_create/fsst_compress (fixture), packed into pages."""
     torch = torch_cuda
