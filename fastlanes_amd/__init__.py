@@ -46,6 +46,9 @@ def lib() -> C.CDLL:
     L.fls_gpu_plan_n_pages.restype = _u32
     L.fls_gpu_plan_decode.argtypes = [_vp, _vp, _vp, _u64, _vp]
     L.fls_gpu_plan_check.argtypes = [_vp, _vp]
+    L.fls_gpu_plan_select.argtypes = [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp]
+    L.fls_gpu_selection_indices.argtypes = [_vp, _u64, _vp, _vp, _vp]
+    L.fls_gpu_selection_and.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp]
     L.fls_gpu_plan_string_offsets.argtypes = [_vp, _vp, _vp, _vp]
     L.fls_gpu_plan_decode_strings.argtypes = [_vp, _vp, _vp, _vp, _u64, _vp]
     L.fls_gpu_unffor_stream.argtypes = [C.c_uint, C.c_uint, _vp, _vp, _vp, _u64, _vp]
@@ -119,6 +122,15 @@ class Plan:
     def check(self, stream: int = 0) -> None:
         check(lib().fls_gpu_plan_check(self.h, stream), "plan_check")
 
+    def select(self, dev_encoded_ptr: int, op: str, constant, dev_bitmap_ptr: int, dev_counts_ptr: int,
+               stream: int = 0) -> None:
+        """Predicate pushdown: per vector, 16 bitmap words of `value <op> constant` and the selected count
+        (op in ==, >, <, >=, <=, !=); no decoded values are written."""
+        bits = np.array([constant], dtype=np.dtype(self.dtype)).view(np.uint8)
+        c = int.from_bytes(bits.tobytes(), "little")
+        check(lib().fls_gpu_plan_select(self.h, dev_encoded_ptr, PRED_OPS[op], c, dev_bitmap_ptr, dev_counts_ptr,
+                                        stream), "plan_select")
+
     def string_offsets(self, dev_encoded_ptr: int, dev_offsets_ptr: int, stream: int = 0) -> None:
         check(lib().fls_gpu_plan_string_offsets(self.h, dev_encoded_ptr, dev_offsets_ptr, stream), "string_offsets")
 
@@ -130,6 +142,20 @@ class Plan:
         if getattr(self, "h", None) and _lib is not None:
             _lib.fls_gpu_plan_destroy(self.h)
             self.h = None
+
+
+PRED_OPS = {"==": 0, ">": 1, "<": 2, ">=": 3, "<=": 4, "!=": 5}
+
+
+def selection_indices(dev_bitmap: int, n_vec: int, dev_indices: int, dev_counts: int | None = None,
+                      stream: int = 0) -> None:
+    """Index arrays (uint16, 1024 slots per vector) of the selected positions, ascending."""
+    check(lib().fls_gpu_selection_indices(dev_bitmap, n_vec, dev_indices, dev_counts, stream), "selection_indices")
+
+
+def selection_and(dev_a: int, dev_b: int, n_vec: int, dev_out: int, dev_counts: int | None = None,
+                  stream: int = 0) -> None:
+    check(lib().fls_gpu_selection_and(dev_a, dev_b, n_vec, dev_out, dev_counts, stream), "selection_and")
 
 
 class RowGroup:

@@ -148,6 +148,48 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
     return FLS_OK;
 }
 
+int fls_gpu_plan_select(fls_gpu_plan* plan, const uint8_t* dev_encoded, int op, uint64_t constant,
+                        uint64_t* dev_bitmap, uint32_t* dev_counts, void* stream) {
+    if (!plan || !dev_encoded || ((!dev_bitmap || !dev_counts) && plan->col.n_values)) return fail(FLS_E_ARG, "null argument");
+    if (plan->col.dtype == STR) return fail(FLS_E_UNSUPPORTED, "predicates on string columns are not supported");
+    if (op < FLS_PRED_EQ || op > FLS_PRED_NE) return fail(FLS_E_ARG, "unknown predicate operator");
+    const unsigned W = plan->W;
+    const uint8_t  dt = plan->col.dtype;
+    SelArgs        sa {};
+    sa.c   = W == 64 ? constant : (constant & ((1ull << W) - 1));
+    sa.op  = (uint32_t)op;
+    sa.sgn = dt == DT_DOUBLE ? 2u : (dt == DT_I8 || dt == DT_I16 || dt == DT_I32 || dt == DT_I64) ? 1u : 0u;
+    hipStream_t s = (hipStream_t)stream;
+    for (size_t i = 0; i < plan->col.pages.size(); ++i) {
+        const PageEntry& pe   = plan->col.pages[i];
+        const uint8_t*   page = dev_encoded + pe.page_off;
+        const uint64_t   v0   = pe.out_index / 1024;
+        SelArgs          ps   = sa;
+        ps.bitmap = dev_bitmap + v0 * 16, ps.counts = dev_counts + v0;
+        hipError_t e = hipSuccess;
+        PageMeta   pm {};
+        for (unsigned b = 0; b < pe.arr_c && b < 6; ++b) pm.off[b] = pe.boff[b];
+        pm.arr_c = pe.arr_c, pm.fixed = pe.fixed_meta;
+        switch (pe.exp_t) {
+        case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pm, pe.vec_c, nullptr, s, &ps); break;
+        case EXP_FFOR:
+            e = pe.var == 1 ? launch_page_int(W, 1, page, pm, pe.vec_c, nullptr, s, &ps)
+                            : launch_page_patch(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps);
+            break;
+        case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, nullptr, s, &ps); break;
+        case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps); break;
+        default: {
+            DevDict& d = plan->dicts[i];
+            e          = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, s);
+            if (e == hipSuccess)
+                e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, nullptr, plan->d_err, s, &ps);
+        }
+        }
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    }
+    return FLS_OK;
+}
+
 int fls_gpu_plan_check(fls_gpu_plan* plan, void* stream) {
     if (!plan) return fail(FLS_E_ARG, "null plan");
     if (plan->col.dtype == STR) return strings_check(plan->strings, (hipStream_t)stream);

@@ -68,47 +68,48 @@ int tune_load(const char* path) {
 }
 
 hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                           hipStream_t s) {
+                           hipStream_t s, const SelArgs* sel) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_int_w<8>(mode, page, pm, vec_c, out, s);
-    case 16: return page_int_w<16>(mode, page, pm, vec_c, out, s);
-    case 32: return page_int_w<32>(mode, page, pm, vec_c, out, s);
-    case 64: return page_int_w<64>(mode, page, pm, vec_c, out, s);
+    case 8: return page_int_w<8>(mode, page, pm, vec_c, out, sel, s);
+    case 16: return page_int_w<16>(mode, page, pm, vec_c, out, sel, s);
+    case 32: return page_int_w<32>(mode, page, pm, vec_c, out, sel, s);
+    case 64: return page_int_w<64>(mode, page, pm, vec_c, out, sel, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_page_patch(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                             uint32_t* err, hipStream_t s) {
+                             uint32_t* err, hipStream_t s, const SelArgs* sel) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 32: return page_patch_w<32>(var, page, pm, vec_c, out, err, s);
-    case 64: return page_patch_w<64>(var, page, pm, vec_c, out, err, s);
+    case 32: return page_patch_w<32>(var, page, pm, vec_c, out, err, sel, s);
+    case 64: return page_patch_w<64>(var, page, pm, vec_c, out, err, sel, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                           uint32_t* err, hipStream_t s) {
+                           uint32_t* err, hipStream_t s, const SelArgs* sel) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_rle_w<8>(var, page, pm, vec_c, out, err, s);
-    case 16: return page_rle_w<16>(var, page, pm, vec_c, out, err, s);
-    case 32: return page_rle_w<32>(var, page, pm, vec_c, out, err, s);
-    case 64: return page_rle_w<64>(var, page, pm, vec_c, out, err, s);
+    case 8: return page_rle_w<8>(var, page, pm, vec_c, out, err, sel, s);
+    case 16: return page_rle_w<16>(var, page, pm, vec_c, out, err, sel, s);
+    case 32: return page_rle_w<32>(var, page, pm, vec_c, out, err, sel, s);
+    case 64: return page_rle_w<64>(var, page, pm, vec_c, out, err, sel, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,
-                            const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s) {
+                            const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s,
+                            const SelArgs* sel) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_dict_w<8>(var, page, pm, vec_c, dict, dict_n, out, err, s);
-    case 16: return page_dict_w<16>(var, page, pm, vec_c, dict, dict_n, out, err, s);
-    case 32: return page_dict_w<32>(var, page, pm, vec_c, dict, dict_n, out, err, s);
-    case 64: return page_dict_w<64>(var, page, pm, vec_c, dict, dict_n, out, err, s);
+    case 8: return page_dict_w<8>(var, page, pm, vec_c, dict, dict_n, out, err, sel, s);
+    case 16: return page_dict_w<16>(var, page, pm, vec_c, dict, dict_n, out, err, sel, s);
+    case 32: return page_dict_w<32>(var, page, pm, vec_c, dict, dict_n, out, err, sel, s);
+    case 64: return page_dict_w<64>(var, page, pm, vec_c, dict, dict_n, out, err, sel, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -7,6 +7,7 @@ namespace fls {
 
 struct HzlChunk;
 struct PageMeta;
+struct SelArgs;
 
 // Kernel families with separately tuned launch shapes.
 enum Family : int {
@@ -42,27 +43,29 @@ inline unsigned grid_for(uint64_t n_vec, const Tune& t, unsigned tpv) {
 hipError_t launch_unffor_stream(unsigned W, unsigned bw, const uint8_t* packed, const void* bases, uint8_t* out,
                                 uint64_t n_vec, hipStream_t s);
 hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                           hipStream_t s);
+                           hipStream_t s, const SelArgs* sel = nullptr);
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                           uint32_t* err, hipStream_t s);
+                           uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr);
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,
-                            const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s);
+                            const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s,
+                            const SelArgs* sel = nullptr);
 hipError_t launch_page_patch(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                             uint32_t* err, hipStream_t s);
+                             uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr);
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
                       hipStream_t s);
 
 // per-width instantiation units (k_page_w*.hip)
 template <int W>
-hipError_t page_int_w(int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, hipStream_t s);
+hipError_t page_int_w(int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
+                      const SelArgs* sel, hipStream_t s);
 template <int W>
 hipError_t page_patch_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err,
-                        hipStream_t s);
+                        const SelArgs* sel, hipStream_t s);
 template <int W>
 hipError_t page_rle_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err,
-                      hipStream_t s);
+                      const SelArgs* sel, hipStream_t s);
 template <int W>
 hipError_t page_dict_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, const void* dict,
-                       uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s);
+                       uint32_t dict_n, uint8_t* out, uint32_t* err, const SelArgs* sel, hipStream_t s);
 
 }  // namespace fls

@@ -96,6 +96,81 @@ __device__ __forceinline__ void wave_copy_out(const uint8_t* wl, uint8_t* dst, u
     }
 }
 
+// Predicate pushdown: instead of writing a decoded vector, compare its values with a constant and write the
+// selection as the reference's SelectionDS bitmap (include/fls/expression/selection_ds.hpp: 16 uint64 words per
+// 1024-value vector, bit i of word j = value 64j + i) plus the number of selected values (n_selected_values).
+// Operators follow src/primitive/predicate/equal.cpp (==, >, <) plus >=, <=, !=; values compare in the column's
+// own type (signed or unsigned).
+enum SelOp : uint32_t { SEL_EQ = 0, SEL_GT = 1, SEL_LT = 2, SEL_GE = 3, SEL_LE = 4, SEL_NE = 5 };
+struct SelArgs {
+    uint64_t* bitmap;  // 16 words per vector of the launch
+    uint32_t* counts;  // one per vector of the launch
+    uint64_t  c;       // the constant, in the column's width
+    uint32_t  op, sgn;  // sgn: 0 unsigned, 1 signed, 2 double
+};
+
+template <typename T>
+__device__ __forceinline__ bool sel_pred(T x, const SelArgs& sa) {
+    using S = std::make_signed_t<T>;
+    const T c = (T)sa.c;
+    bool    lt, gt, eq = x == c;
+    if constexpr (sizeof(T) == 8) {
+        if (sa.sgn == 2) {  // double: IEEE comparisons (NaN compares false, -0.0 == 0.0)
+            const double xd = __longlong_as_double((long long)x), cd = __longlong_as_double((long long)c);
+            lt = xd < cd, gt = xd > cd, eq = xd == cd;
+            switch (sa.op) {
+            case SEL_EQ: return eq;
+            case SEL_GT: return gt;
+            case SEL_LT: return lt;
+            case SEL_GE: return xd >= cd;
+            case SEL_LE: return xd <= cd;
+            default: return !eq;
+            }
+        }
+    }
+    lt = sa.sgn ? (S)x < (S)c : x < c;
+    gt = sa.sgn ? (S)x > (S)c : x > c;
+    switch (sa.op) {
+    case SEL_EQ: return eq;
+    case SEL_GT: return gt;
+    case SEL_LT: return lt;
+    case SEL_GE: return !lt;
+    case SEL_LE: return !gt;
+    default: return !eq;
+    }
+}
+
+// The wave's staged vectors (slots 0 .. n_valid-1, first vector v0) into bitmap words and counts: lane l reads
+// values 64j + l (consecutive lanes, consecutive values), a ballot forms word j, lane j keeps it, and lanes 0-15
+// store the vector's 16 words together.
+template <typename T, unsigned TPV, unsigned VB>
+__device__ __forceinline__ void wave_select(const uint8_t* wl, uint64_t v0, uint32_t n_valid, unsigned lane,
+                                            const SelArgs& sa) {
+    using WS = WaveStage<TPV, VB>;
+    for (uint32_t slot = 0; slot < n_valid; ++slot) {
+        const uint8_t* vb   = wl + slot * WS::PVB;
+        uint64_t       mine = 0;
+        uint32_t       cnt  = 0;
+        static_for<16>([&](auto j) {
+            const uint32_t idx = 64 * decltype(j)::value + lane;
+            const T        x   = *reinterpret_cast<const T*>(vb + lds_pad(idx * (uint32_t)sizeof(T)));
+            const uint64_t m   = __ballot(sel_pred<T>(x, sa));
+            if (lane == decltype(j)::value) mine = m;
+            cnt += (uint32_t)__popcll(m);
+        });
+        if (lane < 16) sa.bitmap[(v0 + slot) * 16 + lane] = mine;
+        if (lane == 0) sa.counts[v0 + slot] = cnt;
+    }
+}
+
+// End of a wave step: copy the staged vectors out, or (SEL) write their selection.
+template <typename T, unsigned TPV, unsigned VB, bool NT, bool SEL>
+__device__ __forceinline__ void wave_out(const uint8_t* wl, uint8_t* out, uint64_t v0, uint32_t n_valid, unsigned lane,
+                                         const SelArgs& sa) {
+    if constexpr (SEL) wave_select<T, TPV, VB>(wl, v0, n_valid, lane, sa);
+    else wave_copy_out<TPV, VB, NT>(wl, out + v0 * VB, n_valid, lane);
+}
+
 // Loop over the kernel's vectors wave by wave; body(v, t, sink) decodes vector v (< N) into the sink.
 template <unsigned TPV, unsigned VB, bool NT, typename F>
 __device__ __forceinline__ void staged_vector_loop(uint64_t N, uint8_t* out, F&& body) {
@@ -186,9 +261,9 @@ __device__ __forceinline__ IntMeta int_meta(const uint8_t* page, const PageMeta&
 // output, and each step waits on one round of loads (the packed values) instead of three dependent ones.
 // DIRECT (32/64-bit values): no LDS staging; each thread writes its lane's contiguous segment (128 or 512 bytes,
 // whole lines) straight to HBM, so occupancy is bounded by registers instead of the staging buffers.
-template <int W, int MODE, bool NT, bool DIRECT = false>
+template <int W, int MODE, bool NT, bool DIRECT = false, bool SEL = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
-                                                        uint8_t* __restrict__ out) {
+                                                        uint8_t* __restrict__ out, SelArgs sa) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     constexpr unsigned TPV = lane_tpv<W>(), VB = 128 * W;
     constexpr unsigned P   = MODE == 0 ? 0 : (MODE == 2 ? 3 : 2);
@@ -220,7 +295,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restric
         if constexpr (!DIRECT) {
             wave_sync();
             const uint64_t left = N - v0;
-            wave_copy_out<TPV, VB, NT>(wl, out + v0 * VB, left < WS::VPW ? (uint32_t)left : WS::VPW, lane);
+            wave_out<typename uint_of<W>::type, TPV, VB, NT, SEL>(wl, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW,
+                                                                  lane, sa);
             wave_sync();
         }
     }
@@ -323,10 +399,10 @@ __device__ __forceinline__ void apply_patches(const uint8_t* exc, const uint8_t*
 }
 
 // Same pipeline as k_page_int (MODE 1) with a patch step between the LDS decode and the copy-out.
-template <int W, int VAR, bool NT>
+template <int W, int VAR, bool NT, bool SEL = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_patch(const uint8_t* __restrict__ page, PageMeta pm,
                                                           uint32_t vec_c, uint8_t* __restrict__ out,
-                                                          uint32_t* __restrict__ err) {
+                                                          uint32_t* __restrict__ err, SelArgs sa) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     constexpr unsigned TPV = lane_tpv<W>(), VB = 128 * W;
     using WS               = WaveStage<TPV, VB>;
@@ -354,7 +430,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_patch(const uint8_t* __restr
                                        err);
         wave_sync();
         const uint64_t left = N - v0;
-        wave_copy_out<TPV, VB, NT>(wl, out + v0 * VB, left < WS::VPW ? (uint32_t)left : WS::VPW, lane);
+        wave_out<typename uint_of<W>::type, TPV, VB, NT, SEL>(wl, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW,
+                                                              lane, sa);
         wave_sync();
     }
 }
@@ -455,9 +532,9 @@ __device__ __forceinline__ RleMeta rle_meta(const uint8_t* page, const PageMeta&
 // A wave decodes two vectors per step (32 threads each), pipelined like k_page_int. Per step: each vector's runs go
 // to LDS when they fit, positions are decoded and the run values gathered into the wave's output staging, which is
 // then written out.
-template <typename T, int VAR, bool NT>
+template <typename T, int VAR, bool NT, bool SEL = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
-                                                        uint8_t* __restrict__ out, uint32_t* err) {
+                                                        uint8_t* __restrict__ out, uint32_t* err, SelArgs sa) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     using WS              = WaveStage<32, 1024 * sizeof(T)>;
     constexpr unsigned RB = kRleLds * sizeof(T);
@@ -497,8 +574,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
         if (v + stride < N) m = rle_meta<VAR>(page, pm, v + stride, N, t);
         wave_sync();
         const uint64_t left = N - v0;
-        wave_copy_out<32, 1024 * sizeof(T), NT>(wl, out + v0 * 1024 * sizeof(T), left < WS::VPW ? (uint32_t)left : WS::VPW,
-                                                lane);
+        wave_out<T, 32, 1024 * sizeof(T), NT, SEL>(wl, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW, lane, sa);
         wave_sync();
     }
 }
@@ -576,10 +652,10 @@ __device__ __forceinline__ CodeMeta code_meta(const uint8_t* page, const PageMet
 
 // Dictionary pages, pipelined like k_page_int. IN_LDS: the decoded dictionary is first copied into LDS (after the
 // waves' staging areas) and the code gathers read LDS instead of L2; the host picks it when the dictionary fits.
-template <typename T, typename D, int VAR, bool NT, bool IN_LDS>
+template <typename T, typename D, int VAR, bool NT, bool IN_LDS, bool SEL = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
                                                          const D* __restrict__ dict, uint32_t dict_n,
-                                                         uint8_t* __restrict__ out, uint32_t* err) {
+                                                         uint8_t* __restrict__ out, uint32_t* err, SelArgs sa) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     if (dict_n == 0) return;  // the host rejects such pages
     constexpr unsigned VB = 1024 * sizeof(T);
@@ -616,7 +692,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restri
         if (v + stride < N) m = code_meta<VAR>(page, pm, v + stride, t);
         wave_sync();
         const uint64_t left = N - v0;
-        wave_copy_out<32, VB, NT>(wl, out + v0 * VB, left < WS::VPW ? (uint32_t)left : WS::VPW, lane);
+        wave_out<T, 32, VB, NT, SEL>(wl, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW, lane, sa);
         wave_sync();
     }
 }

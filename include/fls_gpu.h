@@ -60,6 +60,24 @@ int fls_gpu_plan_string_offsets(fls_gpu_plan* plan, const uint8_t* dev_encoded, 
 int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, const uint64_t* dev_offsets,
                                 uint8_t* dev_bytes, uint64_t bytes_capacity, void* stream);
 
+/* ---- predicate pushdown (reference src/primitive/predicate/equal.cpp, include/fls/expression/predicate_operator.hpp,
+ * include/fls/expression/selection_ds.hpp) ----------------------------------------------------------------------
+ * fls_gpu_plan_select decodes every vector and compares its values with `constant` in the same launch, writing no
+ * decoded values: per vector, 16 uint64 bitmap words (bit i of word j set when value 64j + i satisfies
+ * `value OP constant`; the SelectionDS bitmap) and the selected count (n_selected_values). Values compare in the
+ * column's type: signed for INT*, unsigned for UINT*, IEEE for DOUBLE; `constant` holds the value's bit pattern in
+ * its low W bits. dev_bitmap: 16 * n_vec words; dev_counts: n_vec. Not for string columns. */
+enum fls_predicate_op { FLS_PRED_EQ = 0, FLS_PRED_GT = 1, FLS_PRED_LT = 2, FLS_PRED_GE = 3, FLS_PRED_LE = 4, FLS_PRED_NE = 5 };
+int fls_gpu_plan_select(fls_gpu_plan* plan, const uint8_t* dev_encoded, int op, uint64_t constant,
+                        uint64_t* dev_bitmap, uint32_t* dev_counts, void* stream);
+/* Index-array form of a selection (SelectionDS idx_arr): vector v's selected positions, ascending, at
+ * dev_indices[v * 1024 ...]; entries past the count are unspecified. dev_counts may be NULL. */
+int fls_gpu_selection_indices(const uint64_t* dev_bitmap, uint64_t n_vec, uint16_t* dev_indices, uint32_t* dev_counts,
+                              void* stream);
+/* AND of two selections (and_selection_ds_func, src/primitive/predicate/equal.cpp:96-118); dev_counts may be NULL. */
+int fls_gpu_selection_and(const uint64_t* dev_a, const uint64_t* dev_b, uint64_t n_vec, uint64_t* dev_out,
+                          uint32_t* dev_counts, void* stream);
+
 /* Encode a string column with FSST (1024-string vectors; n must be a multiple of 1024): strings i are
  * bytes[offsets[i], offsets[i+1]). Writes *out_size bytes. No counterpart in the reference's C API, which accepts
  * INT64 and DOUBLE only (reference src/encoder/single_col_encoder.cpp:13-33); the layout is the reference's FSST

@@ -450,3 +450,79 @@ def test_integer_pages_direct_stores(torch_cuda, oracle, W):
     finally:
         path = os.path.join(os.path.dirname(os.path.abspath(fa.__file__)), "tuning.txt")
         assert lib.fls_gpu_tune_load(path.encode()) > 0
+
+
+def _bitmap_words(mask: np.ndarray) -> np.ndarray:
+    """SelectionDS bitmap words of a boolean mask: per 1024 values, 16 uint64 words, bit i of word j = value 64j+i."""
+    bits = np.packbits(mask.reshape(-1, 64), axis=1, bitorder="little")
+    return bits.view(np.uint64).reshape(-1)
+
+
+PRED = {"==": np.equal, ">": np.greater, "<": np.less, ">=": np.greater_equal, "<=": np.less_equal,
+        "!=": np.not_equal}
+
+
+@pytest.mark.parametrize("eid", [0, 1, 2, 5, 6, 7, 11, 16, 17, 22, 29])
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint32, np.int32, np.int64, np.uint64])
+def test_predicate_pushdown_matches_numpy(torch_cuda, oracle, dtype, eid):
+    """Fused decode + compare (every page kind) against numpy comparisons of the oracle's decode: bitmaps, counts,
+    index arrays, and the AND of two selections."""
+    if eid in (2, 5) and np.dtype(dtype).itemsize < 4:
+        pytest.skip("patched FFOR exists for 32/64-bit values only")
+    torch = torch_cuda
+    print(f"select dtype={np.dtype(dtype).name} expr={eid}", flush=True)
+    n = 24 * 1024
+    x = _column(dtype, eid, n, seed=eid + 3 * np.dtype(dtype).itemsize)
+    enc = oracle.encode_column(x, eid)
+    want_x = oracle.decode_column(enc, dtype, n)
+    plan = fa.Plan(enc, dtype)
+    d_in = dev(torch, enc)
+    n_vec = n // 1024
+    g = np.random.default_rng(eid)
+    consts = [x[int(g.integers(0, n))], x[int(g.integers(0, n))], np.iinfo(dtype).min, np.iinfo(dtype).max]
+    bms = []
+    for k, (op, fn) in enumerate(PRED.items()):
+        c = dtype(consts[k % len(consts)])
+        d_bm = torch.zeros(16 * n_vec, dtype=torch.int64, device="cuda")
+        d_cnt = torch.zeros(n_vec, dtype=torch.int32, device="cuda")
+        plan.select(d_in.data_ptr(), op, c, d_bm.data_ptr(), d_cnt.data_ptr())
+        plan.check()
+        mask = fn(want_x, c)
+        got_bm = d_bm.cpu().numpy().view(np.uint64)
+        np.testing.assert_array_equal(got_bm, _bitmap_words(mask), err_msg=f"{op} {c}")
+        np.testing.assert_array_equal(d_cnt.cpu().numpy(), mask.reshape(-1, 1024).sum(axis=1), err_msg=op)
+        bms.append((d_bm, mask))
+    # index arrays (the reference's idx_arr form) and AND of two selections
+    d_idx = torch.zeros(n_vec * 1024, dtype=torch.int16, device="cuda")
+    d_cnt = torch.zeros(n_vec, dtype=torch.int32, device="cuda")
+    fa.selection_indices(bms[1][0].data_ptr(), n_vec, d_idx.data_ptr(), d_cnt.data_ptr())
+    idx = d_idx.cpu().numpy().view(np.uint16).reshape(n_vec, 1024)
+    cnt = d_cnt.cpu().numpy()
+    for v in range(n_vec):
+        np.testing.assert_array_equal(idx[v, : cnt[v]], np.nonzero(bms[1][1][v * 1024:(v + 1) * 1024])[0])
+    d_and = torch.zeros(16 * n_vec, dtype=torch.int64, device="cuda")
+    fa.selection_and(bms[1][0].data_ptr(), bms[3][0].data_ptr(), n_vec, d_and.data_ptr(), d_cnt.data_ptr())
+    both = bms[1][1] & bms[3][1]
+    np.testing.assert_array_equal(d_and.cpu().numpy().view(np.uint64), _bitmap_words(both))
+    np.testing.assert_array_equal(d_cnt.cpu().numpy(), both.reshape(-1, 1024).sum(axis=1))
+
+
+def test_predicate_on_doubles(torch_cuda):
+    """DOUBLE columns (UNCOMPRESSED) compare as IEEE values: NaN selects nothing but !=, -0.0 equals 0.0."""
+    torch = torch_cuda
+    g = np.random.default_rng(2)
+    x = np.round(g.standard_normal(4096), 1)
+    x[::97] = np.nan
+    x[5::31] = -0.0
+    x[7::29] = 0.0
+    enc = fa.encode_from_memory(x, 0)
+    plan = fa.Plan(enc, np.float64)
+    d_in = dev(torch, enc)
+    for op, fn in PRED.items():
+        for c in (0.0, -0.0, 0.5, np.nan):
+            d_bm = torch.zeros(64, dtype=torch.int64, device="cuda")
+            d_cnt = torch.zeros(4, dtype=torch.int32, device="cuda")
+            plan.select(d_in.data_ptr(), op, c, d_bm.data_ptr(), d_cnt.data_ptr())
+            with np.errstate(invalid="ignore"):
+                mask = fn(x, c)
+            np.testing.assert_array_equal(d_bm.cpu().numpy().view(np.uint64), _bitmap_words(mask), err_msg=f"{op} {c}")
