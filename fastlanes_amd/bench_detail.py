@@ -117,6 +117,49 @@ def cascade(dev, stream, log2n=26, log2_strings=24, verbose=True):
     return out
 
 
+def select_bench(dev, stream, log2n=26, verbose=True):
+    """Predicate pushdown against full decode on the same columns: `value < median` fused into the decode kernels
+    (bitmap + counts written instead of values). Bytes are the algorithmic ones: encoded bytes read plus bitmap
+    (1 bit per value) and counts written."""
+    import torch
+
+    n = 1 << log2n
+    rng = np.random.default_rng(7)
+    cases = {
+        "u32_ffor_w7": (1, rng.integers(0, 128, n, dtype=np.uint32)),
+        "i64_timestamps_delta": (11, wl.timestamps(n)),
+        "u32_runs_rle": (6, wl.geometric_runs(n, 16.0, seed=1, value_bits=20).astype(np.uint32)),
+        "i32_zipf_dict": (17, wl.zipf_codes(n, 4096, 1.1, seed=5, dtype=np.int32)),
+    }
+    out = {}
+    sp = int(stream.cuda_stream)
+    for name, (eid, x) in cases.items():
+        enc = fa.encode_from_memory(x, eid)
+        plan = fa.Plan(enc, x.dtype)
+        d_in = torch.from_numpy(enc).to(dev)
+        d_out = torch.empty(x.nbytes + 16, dtype=torch.uint8, device=dev)
+        n_vec = n // 1024
+        d_bm = torch.empty(16 * n_vec, dtype=torch.int64, device=dev)
+        d_cnt = torch.empty(n_vec, dtype=torch.int32, device=dev)
+        samp = np.sort(x[:: max(1, n // 4096)])
+        c = samp[samp.size // 2]  # about half the values pass
+
+        t_dec = _events_time(lambda: plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes, sp), stream)
+        t_sel = _events_time(lambda: plan.select(d_in.data_ptr(), "<", c, d_bm.data_ptr(), d_cnt.data_ptr(), sp), stream)
+        plan.check(sp)
+        mask = x < c
+        ok = bool(np.array_equal(d_cnt.cpu().numpy(), mask.reshape(-1, 1024).sum(axis=1)))
+        sel_bytes = enc.size + n // 8 + 4 * n_vec
+        out[name] = {"values": n, "decode_ms": round(t_dec * 1e3, 4), "select_ms": round(t_sel * 1e3, 4),
+                     "select_gvalues_s": round(n / t_sel / 1e9, 1), "decode_gvalues_s": round(n / t_dec / 1e9, 1),
+                     "select_hbm_frac": round(sel_bytes / t_sel / 1e9 / HBM, 4), "selected": int(mask.sum()),
+                     "counts_exact": ok}
+        if verbose:
+            print(f"select {name}: {out[name]}", flush=True)
+        del plan, d_in, d_out, d_bm, d_cnt
+    return out
+
+
 def spell_bench(dev, log2n=20, n_blocks=16, reps=5, verbose=True):
     """Structural wizard over a 64-column row-group of 2^20 rows (per block of four: a high-cardinality int64
     column, its copy, an int32 key with ~1000 distinct values and an int64 function of it), device-resident."""
@@ -160,7 +203,7 @@ def main():
     import torch
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sections", default="pages,cascade", help="pages, cascade, rowgroup, wizard, spell")
+    ap.add_argument("--sections", default="pages,cascade", help="pages, cascade, rowgroup, wizard, spell, select")
     ap.add_argument("--log2n", type=int, default=26)
     a = ap.parse_args()
     dev = torch.device("cuda:0")
@@ -181,6 +224,8 @@ def main():
             res["wizard"] = wizard.bench(dev)
         elif sec == "spell":
             res["spell"] = spell_bench(dev)
+        elif sec == "select":
+            res["select"] = select_bench(dev, stream, a.log2n)
     print(json.dumps(res), flush=True)
 
 

@@ -156,9 +156,36 @@ int fls_gpu_plan_select(fls_gpu_plan* plan, const uint8_t* dev_encoded, int op, 
     const unsigned W = plan->W;
     const uint8_t  dt = plan->col.dtype;
     SelArgs        sa {};
-    sa.c   = W == 64 ? constant : (constant & ((1ull << W) - 1));
-    sa.op  = (uint32_t)op;
-    sa.sgn = dt == DT_DOUBLE ? 2u : (dt == DT_I8 || dt == DT_I16 || dt == DT_I32 || dt == DT_I64) ? 1u : 0u;
+    // one range test on an order-preserving key (see SelArgs)
+    const uint64_t maxk = W == 64 ? ~0ull : ((1ull << W) - 1);
+    uint64_t       key  = constant & maxk;
+    if (dt == DT_DOUBLE) {
+        sa.dbl     = 1;
+        sa.nan_val = op == FLS_PRED_NE;
+        const uint64_t mag = constant & 0x7fffffffffffffffull;
+        if (mag > 0x7ff0000000000000ull) {  // NaN constant: every comparison false, != true
+            sa.none = 1, sa.none_val = op == FLS_PRED_NE;
+        }
+        const uint64_t z = mag == 0 ? 0ull : constant;
+        key              = (z >> 63) ? ~z : (z | 0x8000000000000000ull);
+    } else if (dt == DT_I8 || dt == DT_I16 || dt == DT_I32 || dt == DT_I64) {
+        sa.flip = 1ull << (W - 1);
+        key ^= sa.flip;
+    }
+    switch (op) {
+    case FLS_PRED_EQ: sa.lo = key, sa.span = 0; break;
+    case FLS_PRED_NE: sa.lo = key, sa.span = 0, sa.inv = 1; break;
+    case FLS_PRED_GT:
+        if (key == maxk) sa.none = 1;
+        else sa.lo = key + 1, sa.span = maxk - key - 1;
+        break;
+    case FLS_PRED_LT:
+        if (key == 0) sa.none = 1;
+        else sa.lo = 0, sa.span = key - 1;
+        break;
+    case FLS_PRED_GE: sa.lo = key, sa.span = maxk - key; break;
+    default: sa.lo = 0, sa.span = key; break;  // FLS_PRED_LE
+    }
     hipStream_t s = (hipStream_t)stream;
     for (size_t i = 0; i < plan->col.pages.size(); ++i) {
         const PageEntry& pe   = plan->col.pages[i];

@@ -102,50 +102,76 @@ __device__ __forceinline__ void wave_copy_out(const uint8_t* wl, uint8_t* dst, u
 // Operators follow src/primitive/predicate/equal.cpp (==, >, <) plus >=, <=, !=; values compare in the column's
 // own type (signed or unsigned).
 enum SelOp : uint32_t { SEL_EQ = 0, SEL_GT = 1, SEL_LT = 2, SEL_GE = 3, SEL_LE = 4, SEL_NE = 5 };
+// The host reduces every (operator, constant, type) to one test: map the value to an order-preserving unsigned key
+// (xor `flip`: the sign bit for signed types; doubles: the IEEE total-order key with -0.0 folded onto 0.0), then
+// `key - lo <= span` in the type's width, inverted by `inv` (!=); `none` fixes the result to `none_val` (> the
+// maximum, < the minimum, a NaN constant), and a NaN value yields `nan_val` (true only for !=).
 struct SelArgs {
     uint64_t* bitmap;  // 16 words per vector of the launch
     uint32_t* counts;  // one per vector of the launch
-    uint64_t  c;       // the constant, in the column's width
-    uint32_t  op, sgn;  // sgn: 0 unsigned, 1 signed, 2 double
+    uint64_t  lo, span, flip;
+    uint32_t  inv, none, none_val, nan_val, dbl;
 };
 
-template <typename T>
+template <typename T, bool DBL>
 __device__ __forceinline__ bool sel_pred(T x, const SelArgs& sa) {
-    using S = std::make_signed_t<T>;
-    const T c = (T)sa.c;
-    bool    lt, gt, eq = x == c;
-    if constexpr (sizeof(T) == 8) {
-        if (sa.sgn == 2) {  // double: IEEE comparisons (NaN compares false, -0.0 == 0.0)
-            const double xd = __longlong_as_double((long long)x), cd = __longlong_as_double((long long)c);
-            lt = xd < cd, gt = xd > cd, eq = xd == cd;
-            switch (sa.op) {
-            case SEL_EQ: return eq;
-            case SEL_GT: return gt;
-            case SEL_LT: return lt;
-            case SEL_GE: return xd >= cd;
-            case SEL_LE: return xd <= cd;
-            default: return !eq;
-            }
-        }
-    }
-    lt = sa.sgn ? (S)x < (S)c : x < c;
-    gt = sa.sgn ? (S)x > (S)c : x > c;
-    switch (sa.op) {
-    case SEL_EQ: return eq;
-    case SEL_GT: return gt;
-    case SEL_LT: return lt;
-    case SEL_GE: return !lt;
-    case SEL_LE: return !gt;
-    default: return !eq;
+    if constexpr (DBL && sizeof(T) == 8) {
+        const uint64_t mag = x & 0x7fffffffffffffffull;
+        if (mag > 0x7ff0000000000000ull) return sa.nan_val != 0;  // NaN
+        const uint64_t z   = mag == 0 ? 0ull : x;                   // -0.0 -> 0.0
+        const uint64_t key = (z >> 63) ? ~z : (z | 0x8000000000000000ull);
+        const bool     in  = key - sa.lo <= sa.span;
+        return sa.none ? sa.none_val != 0 : (in != (sa.inv != 0));
+    } else {
+        const T    key = (T)(x ^ (T)sa.flip);
+        const bool in  = (T)(key - (T)sa.lo) <= (T)sa.span;
+        return sa.none ? sa.none_val != 0 : (in != (sa.inv != 0));
     }
 }
+
+// Sink that turns decoded values straight into selection bits: every value the decode body stores is tested, its
+// bit gathered in a register per 32-value bitmap word, and the word OR-ed into the vector's LDS bitmap when the
+// body moves on to another word (a lane segment is contiguous, so that happens once per 32 values).
+template <typename T, bool DBL>
+struct PredSink {
+    uint32_t*        bits;  // LDS: the vector's 32 bitmap words, cleared beforehand
+    const SelArgs*   sa;
+    mutable uint32_t acc = 0;
+    mutable int32_t  cur = -1;
+    __device__ __forceinline__ void add(uint32_t idx, T x) const {
+        const int32_t w = (int32_t)(idx >> 5);
+        if (w != cur) {
+            if (cur >= 0 && acc) atomicOr(&bits[cur], acc);
+            acc = 0, cur = w;
+        }
+        acc |= (uint32_t)sel_pred<T, DBL>(x, *sa) << (idx & 31);
+    }
+    __device__ __forceinline__ void flush() const {
+        if (cur >= 0 && acc) atomicOr(&bits[cur], acc);
+        acc = 0, cur = -1;
+    }
+    template <typename V>
+    __device__ __forceinline__ void put_n(uint32_t off, V x) const {
+        constexpr unsigned per = sizeof(V) / sizeof(T);
+        union {
+            V u;
+            T e[per];
+        } b;
+        b.u = x;
+        static_for<per>([&](auto k) { add(off / (uint32_t)sizeof(T) + decltype(k)::value, b.e[k]); });
+    }
+    __device__ __forceinline__ void put16(uint32_t off, v4u x) const { put_n(off, x); }
+    __device__ __forceinline__ void put8(uint32_t off, v2u x) const { put_n(off, x); }
+    template <typename U>
+    __device__ __forceinline__ void put(uint32_t off, U x) const { put_n(off, x); }
+};
 
 // The wave's staged vectors (slots 0 .. n_valid-1, first vector v0) into bitmap words and counts: lane l reads
 // values 64j + l (consecutive lanes, consecutive values), a ballot forms word j, lane j keeps it, and lanes 0-15
 // store the vector's 16 words together.
-template <typename T, unsigned TPV, unsigned VB>
-__device__ __forceinline__ void wave_select(const uint8_t* wl, uint64_t v0, uint32_t n_valid, unsigned lane,
-                                            const SelArgs& sa) {
+template <typename T, unsigned TPV, unsigned VB, bool DBL>
+__device__ __forceinline__ void wave_select_k(const uint8_t* wl, uint64_t v0, uint32_t n_valid, unsigned lane,
+                                              const SelArgs& sa) {
     using WS = WaveStage<TPV, VB>;
     for (uint32_t slot = 0; slot < n_valid; ++slot) {
         const uint8_t* vb   = wl + slot * WS::PVB;
@@ -154,13 +180,19 @@ __device__ __forceinline__ void wave_select(const uint8_t* wl, uint64_t v0, uint
         static_for<16>([&](auto j) {
             const uint32_t idx = 64 * decltype(j)::value + lane;
             const T        x   = *reinterpret_cast<const T*>(vb + lds_pad(idx * (uint32_t)sizeof(T)));
-            const uint64_t m   = __ballot(sel_pred<T>(x, sa));
+            const uint64_t m   = __ballot(sel_pred<T, DBL>(x, sa));
             if (lane == decltype(j)::value) mine = m;
             cnt += (uint32_t)__popcll(m);
         });
         if (lane < 16) sa.bitmap[(v0 + slot) * 16 + lane] = mine;
         if (lane == 0) sa.counts[v0 + slot] = cnt;
     }
+}
+template <typename T, unsigned TPV, unsigned VB>
+__device__ __forceinline__ void wave_select(const uint8_t* wl, uint64_t v0, uint32_t n_valid, unsigned lane,
+                                            const SelArgs& sa) {
+    if (sizeof(T) == 8 && sa.dbl) wave_select_k<T, TPV, VB, true>(wl, v0, n_valid, lane, sa);
+    else wave_select_k<T, TPV, VB, false>(wl, v0, n_valid, lane, sa);
 }
 
 // End of a wave step: copy the staged vectors out, or (SEL) write their selection.
@@ -277,6 +309,41 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_int(const uint8_t* __restric
     uint64_t       v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
     IntMeta        m {};
     if (v0 + slot < N) m = int_meta<W, MODE>(page, pm, v0 + slot, t);
+    if constexpr (SEL && W <= 32) {
+        // predicate pushdown: values become bits in registers; LDS holds only the wave's bitmaps (128 B per vector).
+        // (64-bit values take the staged path below: their register form compiled to slower code.)
+        using T         = typename uint_of<W>::type;
+        uint32_t* wbits = reinterpret_cast<uint32_t*>(fls_lds) + wave * (WS::VPW * 32);
+        for (; v0 < N; v0 += stride) {
+            const uint64_t v = v0 + slot;
+            for (unsigned k = t; k < 32; k += TPV) wbits[slot * 32 + k] = 0u;
+            wave_sync();
+            if (v < N) {
+                const uint8_t* pk     = page + pm.off[P] + m.ep;
+                auto           decode = [&](const auto& dst) {
+                    if constexpr (MODE == 0) {
+                        ffor_untransposed<W, W, false>(pk, 0, 0, t, dst);
+                    } else {
+                        dispatch_bw<W>(m.bw, [&](auto BW) { ffor_untransposed<W, BW, MODE == 2>(pk, m.base, m.lb, t, dst); });
+                    }
+                    dst.flush();
+                };
+                decode(PredSink<T, false> {wbits + slot * 32, &sa});
+            }
+            if (v + stride < N) m = int_meta<W, MODE>(page, pm, v + stride, t);
+            wave_sync();
+            // lane = 16 q + j writes word j of the wave's vector q, and the 16 lanes of a vector sum its count
+            const unsigned q = lane / 16, j = lane % 16;
+            const bool     ok = q < WS::VPW && v0 + q < N;
+            const uint64_t w  = ok ? ((uint64_t)wbits[q * 32 + 2 * j + 1] << 32) | wbits[q * 32 + 2 * j] : 0ull;
+            if (ok) sa.bitmap[(v0 + q) * 16 + j] = w;
+            unsigned c = (unsigned)__popcll(w);
+            for (unsigned d = 8; d >= 1; d >>= 1) c += __shfl_xor(c, d, 16);
+            if (ok && j == 0) sa.counts[v0 + q] = c;
+            wave_sync();
+        }
+        return;
+    }
     for (; v0 < N; v0 += stride) {
         const uint64_t v = v0 + slot;
         if (v < N) {
