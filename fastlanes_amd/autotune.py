@@ -51,7 +51,8 @@ def _candidates(n_cu, nts=NTS):
 def tune_family(lib, family, W, bw, fn, stream, n_cu, bytes_moved):
     best = None
     # integer pages of 32/64-bit values may also write without LDS staging (bit 1 of the store flags)
-    nts = NTS + (2, 3) if family in (FAM_UNC, FAM_FFOR, FAM_DELTA) and W >= 32 else NTS
+    nts = NTS + (2, 3) if (family in (FAM_UNC, FAM_FFOR, FAM_DELTA) and W >= 32) or \
+        (family in (FAM_RLE, FAM_DICT) and W == 64) else NTS
     for block, nt, maxb in _candidates(n_cu, nts):
         fa.check(lib.fls_gpu_tune_set(family, W, bw, block, nt, maxb), "tune_set")
         t = _time(fn, stream)

@@ -599,18 +599,21 @@ __device__ __forceinline__ RleMeta rle_meta(const uint8_t* page, const PageMeta&
 // A wave decodes two vectors per step (32 threads each), pipelined like k_page_int. Per step: each vector's runs go
 // to LDS when they fit, positions are decoded and the run values gathered into the wave's output staging, which is
 // then written out.
-template <typename T, int VAR, bool NT, bool SEL = false>
+// DIRECT (64-bit values): no output staging; a thread's 16-value run segments (128 bytes, whole lines) are stored
+// straight to HBM, and LDS holds only the run values.
+template <typename T, int VAR, bool NT, bool SEL = false, bool DIRECT = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
                                                         uint8_t* __restrict__ out, uint32_t* err, SelArgs sa) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     using WS              = WaveStage<32, 1024 * sizeof(T)>;
     constexpr unsigned RB = kRleLds * sizeof(T);
     constexpr unsigned P  = VAR == 1 ? 1 : 4;
+    constexpr unsigned SB = DIRECT ? 0u : WS::BYTES;  // staging bytes per wave
     const unsigned     lane = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
     const unsigned     t = lane % 32, slot = lane / 32;
-    uint8_t*           wl    = fls_lds + wave * (WS::BYTES + WS::VPW * RB);
-    T*                 lruns = reinterpret_cast<T*>(wl + WS::BYTES + slot * RB);
-    const LdsSink      sink {wl + slot * WS::PVB};
+    uint8_t*           wl    = fls_lds + wave * (SB + WS::VPW * RB);
+    T*                 lruns = reinterpret_cast<T*>(wl + SB + slot * RB);
+    const LdsSink      lsink {wl + slot * WS::PVB};
     const uint64_t     N      = vec_c;
     const uint64_t     stride = (uint64_t)gridDim.x * waves * WS::VPW;
     uint64_t           v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
@@ -626,13 +629,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
         if (v < N) {
             const uint8_t* pk = page + pm.off[P] + m.ep;
             auto positions    = [&](auto runs_src) {
-                if constexpr (VAR == 1) {
-                    rle_vector<T, 16, false>(pk, 0, 0, t, runs_src, n_runs, sink, err);
-                } else {
-                    dispatch_bw<16>(m.bw, [&](auto BW) {
-                        rle_vector<T, BW, true>(pk, m.base, m.lb, t, runs_src, n_runs, sink, err);
-                    });
-                }
+                auto body = [&](const auto& sink) {
+                    if constexpr (VAR == 1) {
+                        rle_vector<T, 16, false>(pk, 0, 0, t, runs_src, n_runs, sink, err);
+                    } else {
+                        dispatch_bw<16>(m.bw, [&](auto BW) {
+                            rle_vector<T, BW, true>(pk, m.base, m.lb, t, runs_src, n_runs, sink, err);
+                        });
+                    }
+                };
+                if constexpr (DIRECT) body(GlobalSink<NT> {out + v * (1024 * sizeof(T))});
+                else body(lsink);
             };
             if (n_runs == 0) atomicOr(err, 1u);  // the host rejects such pages; never read an empty run buffer
             else if (n_runs <= kRleLds) positions(static_cast<const T*>(lruns));
@@ -640,9 +647,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
         }
         if (v + stride < N) m = rle_meta<VAR>(page, pm, v + stride, N, t);
         wave_sync();
-        const uint64_t left = N - v0;
-        wave_out<T, 32, 1024 * sizeof(T), NT, SEL>(wl, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW, lane, sa);
-        wave_sync();
+        if constexpr (!DIRECT) {
+            const uint64_t left = N - v0;
+            wave_out<T, 32, 1024 * sizeof(T), NT, SEL>(wl, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW, lane,
+                                                       sa);
+            wave_sync();
+        }
     }
 }
 
@@ -719,7 +729,9 @@ __device__ __forceinline__ CodeMeta code_meta(const uint8_t* page, const PageMet
 
 // Dictionary pages, pipelined like k_page_int. IN_LDS: the decoded dictionary is first copied into LDS (after the
 // waves' staging areas) and the code gathers read LDS instead of L2; the host picks it when the dictionary fits.
-template <typename T, typename D, int VAR, bool NT, bool IN_LDS, bool SEL = false>
+// DIRECT (64-bit values): no output staging, segments stored straight to HBM (as k_page_rle); LDS holds only the
+// dictionary.
+template <typename T, typename D, int VAR, bool NT, bool IN_LDS, bool SEL = false, bool DIRECT = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
                                                          const D* __restrict__ dict, uint32_t dict_n,
                                                          uint8_t* __restrict__ out, uint32_t* err, SelArgs sa) {
@@ -728,17 +740,18 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restri
     constexpr unsigned VB = 1024 * sizeof(T);
     constexpr unsigned P  = VAR == 1 ? 0 : (VAR == 7 ? 3 : 2);
     using WS              = WaveStage<32, VB>;
+    constexpr unsigned SB = DIRECT ? 0u : WS::BYTES;  // staging bytes per wave
     const D* dp           = dict;
     if constexpr (IN_LDS) {
-        D* ldict = reinterpret_cast<D*>(fls_lds + (blockDim.x / 64) * WS::BYTES);
+        D* ldict = reinterpret_cast<D*>(fls_lds + (blockDim.x / 64) * SB);
         for (uint32_t i = threadIdx.x; i < dict_n; i += blockDim.x) ldict[i] = dict[i];
         __syncthreads();
         dp = ldict;
     }
     const unsigned lane = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
     const unsigned t = lane % 32, slot = lane / 32;
-    uint8_t*       wl = fls_lds + wave * WS::BYTES;
-    const LdsSink  sink {wl + slot * WS::PVB};
+    uint8_t*       wl = fls_lds + wave * SB;
+    const LdsSink  lsink {wl + slot * WS::PVB};
     const uint64_t N      = vec_c;
     const uint64_t stride = (uint64_t)gridDim.x * waves * WS::VPW;
     uint64_t       v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
@@ -747,20 +760,26 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restri
     for (; v0 < N; v0 += stride) {
         const uint64_t v = v0 + slot;
         if (v < N) {
-            const uint8_t* pk = page + pm.off[P] + m.ep;
-            if constexpr (VAR == 1) {
-                dict_vector<T, D, 32, false>(pk, 0, 0, t, dp, dict_n, sink, err);
-            } else {
-                dispatch_bw<32>(m.bw, [&](auto BW) {
-                    dict_vector<T, D, BW, VAR == 7>(pk, m.base, m.lb, t, dp, dict_n, sink, err);
-                });
-            }
+            const uint8_t* pk   = page + pm.off[P] + m.ep;
+            auto           body = [&](const auto& sink) {
+                if constexpr (VAR == 1) {
+                    dict_vector<T, D, 32, false>(pk, 0, 0, t, dp, dict_n, sink, err);
+                } else {
+                    dispatch_bw<32>(m.bw, [&](auto BW) {
+                        dict_vector<T, D, BW, VAR == 7>(pk, m.base, m.lb, t, dp, dict_n, sink, err);
+                    });
+                }
+            };
+            if constexpr (DIRECT) body(GlobalSink<NT> {out + v * VB});
+            else body(lsink);
         }
         if (v + stride < N) m = code_meta<VAR>(page, pm, v + stride, t);
-        wave_sync();
-        const uint64_t left = N - v0;
-        wave_out<T, 32, VB, NT, SEL>(wl, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW, lane, sa);
-        wave_sync();
+        if constexpr (!DIRECT) {
+            wave_sync();
+            const uint64_t left = N - v0;
+            wave_out<T, 32, VB, NT, SEL>(wl, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW, lane, sa);
+            wave_sync();
+        }
     }
 }
 

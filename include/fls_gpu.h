@@ -97,8 +97,8 @@ int fls_gpu_unffor_stream(unsigned width, unsigned bw, const void* dev_packed, c
  * The decoder reads a table of launch shapes (threads per block, non-temporal stores, grid cap) from tuning.txt
  * next to the library (or the file named by FLS_TUNING) on first use; fastlanes_amd/autotune.py measures and writes
  * it. family: 0 raw streams, 1 UNCOMPRESSED, 2 FFOR, 3 DELTA, 4 RLE, 5 dictionary pages; bw 255 = all bit widths.
- * nontemporal: bit 0 = non-temporal stores; bit 1 = (families 1-3, 32/64-bit values) direct stores of each lane's
- * segment instead of staging the output through LDS. */
+ * nontemporal: bit 0 = non-temporal stores; bit 1 = direct stores of each lane's segment instead of staging the
+ * output through LDS (families 1-3 with 32/64-bit values, families 4-5 with 64-bit values). */
 int fls_gpu_tune_set(int family, unsigned width, unsigned bw, unsigned block, unsigned nontemporal,
                      unsigned max_blocks);
 int fls_gpu_tune_load(const char* path);

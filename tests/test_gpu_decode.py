@@ -423,8 +423,9 @@ def test_ffor_page_with_strided_metadata(torch_cuda, oracle, W):
 
 @pytest.mark.parametrize("W", [32, 64])
 def test_integer_pages_direct_stores(torch_cuda, oracle, W):
-    """UNCOMPRESSED / FFOR / DELTA pages of 32/64-bit values with the launch shape that writes lane segments
-    directly (no LDS staging), several grid shapes; the default table is restored afterwards."""
+    """UNCOMPRESSED / FFOR / DELTA pages of 32/64-bit values (and RLE / dictionary pages of 64-bit values) with the
+    launch shape that writes lane segments directly (no LDS staging), several grid shapes; the default table is
+    restored afterwards."""
     import ctypes as C
     import os
 
@@ -434,7 +435,8 @@ def test_integer_pages_direct_stores(torch_cuda, oracle, W):
     lib.fls_gpu_tune_load.argtypes = [C.c_char_p]
     dt = W2DT[W]
     try:
-        for fam, eid in ((1, 0), (2, 1), (3, 11)):
+        fams = ((1, 0), (2, 1), (3, 11)) + (((4, 6), (4, 7), (5, 16), (5, 17), (5, 22)) if W == 64 else ())
+        for fam, eid in fams:
             x = _column(dt, eid, 37 * 1024, seed=fam + W)
             enc = oracle.encode_column(x, eid)
             want = oracle.decode_column(enc, dt, x.size)
