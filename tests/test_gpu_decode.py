@@ -180,6 +180,25 @@ def test_multi_page_delta(torch_cuda, oracle):
     assert np.array_equal(got, x)
 
 
+def test_predicate_on_multi_page_column(torch_cuda, oracle):
+    """Selections of a two-page column land at each page's own vector offsets."""
+    torch = torch_cuda
+    x = wl.timestamps(65_600 * 1024)
+    enc = fa.encode_from_memory(x, 11)
+    plan = fa.Plan(enc, np.int64)
+    assert plan.n_pages == 2
+    n_vec = x.size // 1024
+    d_in = dev(torch, enc)
+    d_bm = torch.zeros(16 * n_vec, dtype=torch.int64, device="cuda")
+    d_cnt = torch.zeros(n_vec, dtype=torch.int32, device="cuda")
+    c = x[x.size // 2 + 12345]
+    plan.select(d_in.data_ptr(), ">=", c, d_bm.data_ptr(), d_cnt.data_ptr())
+    plan.check()
+    mask = x >= c
+    np.testing.assert_array_equal(d_cnt.cpu().numpy(), mask.reshape(-1, 1024).sum(axis=1))
+    np.testing.assert_array_equal(d_bm.cpu().numpy().view(np.uint64), _bitmap_words(mask))
+
+
 def test_out_of_range_codes_are_clamped_and_reported(torch_cuda, oracle):
     torch = torch_cuda
     x = np.tile(np.arange(4, dtype=np.int64), 1024)
