@@ -49,6 +49,7 @@ def lib() -> C.CDLL:
     L.fls_gpu_plan_select.argtypes = [_vp, _vp, C.c_int, _u64, _vp, _vp, _vp]
     L.fls_gpu_selection_indices.argtypes = [_vp, _u64, _vp, _vp, _vp]
     L.fls_gpu_selection_and.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp]
+    L.fls_gpu_encode.argtypes = [_vp, _u64, C.c_uint8, C.c_uint8, _vp, _u64, C.POINTER(_u64), _vp]
     L.fls_gpu_plan_string_offsets.argtypes = [_vp, _vp, _vp, _vp]
     L.fls_gpu_plan_decode_strings.argtypes = [_vp, _vp, _vp, _vp, _u64, _vp]
     L.fls_gpu_unffor_stream.argtypes = [C.c_uint, C.c_uint, _vp, _vp, _vp, _u64, _vp]
@@ -142,6 +143,14 @@ class Plan:
         if getattr(self, "h", None) and _lib is not None:
             _lib.fls_gpu_plan_destroy(self.h)
             self.h = None
+
+
+def gpu_encode(dev_values: int, n: int, dtype, expr_id: int, dev_out: int, capacity: int, stream: int = 0) -> int:
+    """Encodes n device-resident values on the GPU (expressions 0, 1, 11); returns the encoded size."""
+    size = _u64(0)
+    check(lib().fls_gpu_encode(dev_values, n, dtype_code(dtype), expr_id, dev_out, capacity, C.byref(size), stream),
+          "gpu_encode")
+    return size.value
 
 
 PRED_OPS = {"==": 0, ">": 1, "<": 2, ">=": 3, "<=": 4, "!=": 5}

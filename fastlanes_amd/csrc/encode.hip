@@ -1,0 +1,287 @@
+// encode.hip -- GPU encoder for integer columns: UNCOMPRESSED (pool id 0), FFOR (1) and DELTA (11) pages, byte for
+// byte what the host encoder (encode_from_memory) writes.
+//
+// Encode RPNs: UNCOMPRESSED {TRS, MEM_CPY} (include/fls/cor/exp/uncompressed/uncompressed_exp.hpp:32-47), FFOR
+// {TRS, ANALYZE, FFOR} (include/fls/cor/exp/ffor/ffor_exp.hpp:39-46), DELTA {TRS, RSUM, ANALYZE, FFOR}
+// (include/fls/cor/exp/delta/delta_exp.hpp:39-46); ANALYZE takes base = min and bw = the bit length of max - min
+// (src/cor/prm/ffor/ffor_analyze.cpp:78-102, exact); pages as TCompressor::FlushPage / Page::Absorb
+// (src/cor/eng/compressor.cpp:41-86, src/cor/lyt/page/page.cpp:87-130).
+//
+// Two passes: k_enc_analyze finds each vector's base and bit width (one wave per vector); the host reads the bit
+// widths back, lays the pages out exactly as the host encoder does (65535 vectors and 128 MiB per page at most) and
+// writes the page headers; k_enc_pack writes every vector's buffers and entry points (one block per vector: the
+// vector is staged in LDS, thread i < L packs lane i).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/fls_gpu.h"
+#include "decode_core.hpp"
+#include "gpu_internal.hpp"
+
+namespace fls {
+namespace {
+
+// original position of transposed index k (k = 128r + 16b + c -> r + 8 ORDER[b] + 64c)
+__device__ __forceinline__ unsigned enc_orig(unsigned k) {
+    const unsigned b = (k >> 4) & 7;
+    return (k >> 7) + 8 * (((b & 1) << 2) | (b & 2) | (b >> 2)) + 64 * (k & 15);
+}
+
+// The value FFOR packs at original position o: the value itself, or (DELTA) its difference from the previous value
+// inside the aligned W-value segment (0 at a segment start) -- the unrsum of the transposed lane order.
+template <typename T, int MODE>
+__device__ __forceinline__ T enc_value(const T* xs, unsigned o) {
+    constexpr unsigned W = 8 * sizeof(T);
+    if constexpr (MODE == 2) return (o % W) ? (T)(xs[o] - xs[o - 1]) : (T)0;
+    else return xs[o];
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T x) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        const T y = __shfl_xor(x, d, 64);
+        x         = y < x ? y : x;
+    }
+    return x;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T x) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        const T y = __shfl_xor(x, d, 64);
+        x         = y > x ? y : x;
+    }
+    return x;
+}
+
+// One wave per vector: base = min, bw = bit length of max - min over the values FFOR will pack.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_enc_analyze(const T* __restrict__ x, uint64_t n_vec, uint8_t* __restrict__ bw,
+                                                     uint64_t* __restrict__ base) {
+    using C              = std::conditional_t<sizeof(T) == 8, unsigned long long, unsigned>;  // shuffle type
+    const unsigned lane  = threadIdx.x % 64;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; v < n_vec; v += waves) {
+        const T* xv = x + v * 1024;
+        C        mn = (C)(T)~(T)0, mx = 0;
+        for (unsigned o = lane; o < 1024; o += 64) {
+            const C val = (C)enc_value<T, MODE>(xv, o);
+            mn          = val < mn ? val : mn;
+            mx          = val > mx ? val : mx;
+        }
+        mn = wave_min<C>(mn), mx = wave_max<C>(mx);
+        if (lane == 0) {
+            const uint64_t d = (uint64_t)(T)(mx - mn);
+            bw[v]            = (uint8_t)(d ? 64 - __clzll((long long)d) : 0);
+            base[v]          = (uint64_t)mn;
+        }
+    }
+}
+
+struct EncPage {
+    uint64_t start, v0;  // byte offset of the page in the output, its first vector
+    uint32_t off[4];     // buffer offsets from the page start
+};
+
+template <typename T>
+__device__ __forceinline__ void st_ua(uint8_t* p, T x) {
+    if constexpr (sizeof(T) == 1) *p = x;
+    else if constexpr (sizeof(T) == 2) *reinterpret_cast<u16_ua*>(p) = x;
+    else if constexpr (sizeof(T) == 4) *reinterpret_cast<u32_ua*>(p) = x;
+    else *reinterpret_cast<u64_ua*>(p) = x;
+}
+
+// One block of 128 threads per vector.
+template <typename T, int MODE>
+__global__ __launch_bounds__(128) void k_enc_pack(const T* __restrict__ x, const uint8_t* __restrict__ bw_in,
+                                                  const uint64_t* __restrict__ base_in,
+                                                  const uint64_t* __restrict__ pk_rel, const uint32_t* __restrict__ vpage,
+                                                  const EncPage* __restrict__ pages, uint32_t arr_c,
+                                                  uint8_t* __restrict__ out) {
+    constexpr unsigned W = 8 * sizeof(T), L = 1024 / W, SZ = sizeof(T);
+    __shared__ T       xs[1024];
+    const uint64_t     v   = blockIdx.x;
+    const unsigned     tid = threadIdx.x;
+    for (unsigned k = tid; k < 1024; k += 128) xs[k] = x[v * 1024 + k];
+    __syncthreads();
+    const EncPage  pg = pages[vpage[v]];
+    const uint32_t r  = (uint32_t)(v - pg.v0);
+    uint8_t*       p  = out + pg.start;
+    uint8_t*       ep = p + 8 + 4 * arr_c + 4ull * arr_c * (r ? r - 1 : 0);  // this vector's entry-point row
+    if constexpr (MODE == 0) {
+        uint8_t* dst = p + pg.off[0] + (uint64_t)r * 1024 * SZ;
+        for (unsigned k = tid; k < 1024; k += 128) st_ua<T>(dst + k * SZ, xs[enc_orig(k)]);
+        if (r && tid == 0) st_ua<uint32_t>(ep, r * 1024 * SZ);
+    } else {
+        const unsigned bw   = bw_in[v];
+        const T        base = (T)base_in[v];
+        constexpr unsigned o = MODE == 2 ? 1 : 0;  // bw buffer
+        if (tid == 0) {
+            p[pg.off[o] + r] = (uint8_t)bw;
+            st_ua<T>(p + pg.off[o + 1] + (uint64_t)r * SZ, base);
+            if (r) {
+                if constexpr (MODE == 2) st_ua<uint32_t>(ep, r * 128), ep += 4;
+                st_ua<uint32_t>(ep, r), st_ua<uint32_t>(ep + 4, r * SZ), st_ua<uint32_t>(ep + 8, (uint32_t)pk_rel[v]);
+            }
+        }
+        if constexpr (MODE == 2)  // lane bases: the first L transposed values
+            if (tid < L) st_ua<T>(p + pg.off[0] + (uint64_t)r * 128 + tid * SZ, xs[enc_orig(tid)]);
+        if (tid < L && bw) {
+            uint8_t*       pk   = p + pg.off[o + 2] + pk_rel[v];
+            const uint64_t mask = bw >= 64 ? ~0ull : ((1ull << bw) - 1);
+            using A             = std::conditional_t<sizeof(T) == 8, unsigned __int128, uint64_t>;
+            A        acc = 0;
+            unsigned nb = 0, w = 0;
+            for (unsigned j = 0; j < W; ++j) {
+                const uint64_t val = (uint64_t)(T)(enc_value<T, MODE>(xs, enc_orig(j * L + tid)) - base) & mask;
+                acc |= (A)val << nb;
+                nb += bw;
+                while (nb >= W) {
+                    st_ua<T>(pk + ((uint64_t)w * L + tid) * SZ, (T)acc);
+                    acc >>= W, nb -= W, ++w;
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_enc(int mode, const void* x, uint64_t n_vec, uint8_t* bw, uint64_t* base, hipStream_t s, bool analyze,
+                      const uint64_t* pk_rel, const uint32_t* vpage, const EncPage* pages, uint32_t arr_c,
+                      uint8_t* out) {
+    const T* xv = static_cast<const T*>(x);
+    if (analyze) {
+        const unsigned blocks = (unsigned)std::min<uint64_t>((n_vec + 3) / 4, 65536);
+        if (mode == 1) hipLaunchKernelGGL((k_enc_analyze<T, 1>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base);
+        else hipLaunchKernelGGL((k_enc_analyze<T, 2>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base);
+    } else {
+#define P(M) hipLaunchKernelGGL((k_enc_pack<T, M>), dim3((unsigned)n_vec), dim3(128), 0, s, xv, bw, base, pk_rel, vpage, pages, arr_c, out)
+        if (mode == 0) P(0);
+        else if (mode == 1) P(1);
+        else P(2);
+#undef P
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_enc_w(unsigned W, int mode, const void* x, uint64_t n_vec, uint8_t* bw, uint64_t* base, hipStream_t s,
+                        bool analyze, const uint64_t* pk_rel = nullptr, const uint32_t* vpage = nullptr,
+                        const EncPage* pages = nullptr, uint32_t arr_c = 0, uint8_t* out = nullptr) {
+    switch (W) {
+    case 8: return launch_enc<uint8_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out);
+    case 16: return launch_enc<uint16_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out);
+    case 32: return launch_enc<uint32_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out);
+    default: return launch_enc<uint64_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out);
+    }
+}
+
+}  // namespace
+}  // namespace fls
+
+using namespace fls;
+
+extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t type, uint8_t expr_id, uint8_t* dev_out,
+                              uint64_t capacity, uint64_t* out_size, void* stream) {
+    if (!out_size || !dev_out || (n && !dev_values)) return fail(FLS_E_ARG, "null argument");
+    Expr e;
+    if (type == STR || !expr_from_id((uint8_t)type, expr_id, e)) return fail(FLS_E_UNSUPPORTED, "unknown expression");
+    const int mode = e.exp_t == EXP_UNCOMPRESSED ? 0 : (e.exp_t == EXP_FFOR && e.var == 1) ? 1 : e.exp_t == EXP_DELTA ? 2 : -1;
+    if (mode < 0) return fail(FLS_E_UNSUPPORTED, "the GPU encoder writes UNCOMPRESSED, FFOR and DELTA pages");
+    if (type == DOUBLE && mode) return fail(FLS_E_UNSUPPORTED, "double columns support UNCOMPRESSED only");
+    if (n % 1024) return fail(FLS_E_UNSUPPORTED, "value count must be a multiple of 1024");
+    const unsigned    W = dtype_bits((uint8_t)type), SZ = W / 8, arr_c = e.arr_c;
+    const uint64_t    n_vec = n / 1024;
+    hipStream_t       s     = (hipStream_t)stream;
+    hipError_t        err;
+    std::vector<uint8_t> bw(n_vec, 0);
+    uint8_t*          d_bw   = nullptr;
+    uint64_t*         d_base = nullptr;
+    uint64_t*         d_pk   = nullptr;
+    uint32_t*         d_vp   = nullptr;
+    EncPage*          d_pg   = nullptr;
+    auto              release = [&](int rc) {
+        for (void* q : {(void*)d_bw, (void*)d_base, (void*)d_pk, (void*)d_vp, (void*)d_pg})
+            if (q) (void)hipFreeAsync(q, s);
+        (void)hipStreamSynchronize(s);
+        return rc;
+    };
+    if (n_vec) {
+        if ((err = hipMallocAsync((void**)&d_bw, n_vec, s)) != hipSuccess ||
+            (err = hipMallocAsync((void**)&d_base, 8 * n_vec, s)) != hipSuccess)
+            return release(hip_fail(err, "hipMallocAsync"));
+        if (mode) {
+            if ((err = launch_enc_w(W, mode, dev_values, n_vec, d_bw, d_base, s, true)) != hipSuccess ||
+                (err = hipMemcpyAsync(bw.data(), d_bw, n_vec, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+                (err = hipStreamSynchronize(s)) != hipSuccess)
+                return release(hip_fail(err, "analyze"));
+        }
+    }
+    // page layout, as the host encoder's assembly loop (format.cpp encode_t step 2)
+    auto vec_bytes = [&](uint64_t v, unsigned b) -> uint64_t {
+        if (mode == 0) return 1024ull * SZ;
+        const unsigned o = mode == 2 ? 1 : 0;
+        if (mode == 2 && b == 0) return 128;
+        if (b == o) return 1;
+        if (b == o + 1) return SZ;
+        return 128ull * bw[v];
+    };
+    std::vector<EncPage>  pages;
+    std::vector<uint8_t>  heads;  // per page: 8-byte header + arr_c offsets
+    std::vector<uint32_t> vpage(n_vec);
+    std::vector<uint64_t> pk_rel(n_vec);
+    const uint64_t        page_c = n_vec ? (n_vec + kMaxVecPerPage - 1) / kMaxVecPerPage : 1;
+    const uint64_t        per_pg = n_vec ? (n_vec + page_c - 1) / page_c : 0;
+    uint64_t              v = 0, pos = 0;
+    for (;;) {
+        const uint64_t v0 = v, stop = std::min(n_vec, v + per_pg);
+        uint64_t       bufs[4] = {0, 0, 0, 0}, eps = 0, vec_c = 0;
+        for (; v < stop; ++v) {
+            uint64_t add = 4ull * arr_c, cur = 8 + 4ull * arr_c + eps;
+            for (unsigned b = 0; b < arr_c; ++b) add += vec_bytes(v, b), cur += bufs[b];
+            if (vec_c && cur + add >= kDataPageMax) break;  // TryFlushVec limit (compressor.cpp:47)
+            if (vec_c) eps += 4ull * arr_c;
+            pk_rel[v] = mode == 0 ? 0 : bufs[arr_c - 1];
+            vpage[v]  = (uint32_t)pages.size();
+            for (unsigned b = 0; b < arr_c; ++b) bufs[b] += vec_bytes(v, b);
+            ++vec_c;
+        }
+        const bool more = v < n_vec;
+        EncPage    pg {};
+        pg.start = pos, pg.v0 = v0;
+        uint64_t off = 8 + 4ull * arr_c + eps;
+        const uint8_t hdr[8] = {e.exp_t, e.var, 0, 0, (uint8_t)(vec_c & 0xff), (uint8_t)(vec_c >> 8), (uint8_t)more, 0};
+        heads.insert(heads.end(), hdr, hdr + 8);
+        for (unsigned b = 0; b < arr_c; ++b) {
+            pg.off[b]      = (uint32_t)off;
+            const uint32_t o32 = (uint32_t)off;
+            heads.insert(heads.end(), (const uint8_t*)&o32, (const uint8_t*)&o32 + 4);
+            off += bufs[b];
+        }
+        pages.push_back(pg);
+        pos += off;
+        if (!more) break;
+    }
+    *out_size = pos;
+    if (pos > capacity) return release(fail(FLS_E_CAPACITY, "encoded column exceeds capacity"));
+    // headers and buffer offsets (host -> device), then every vector's buffers
+    const uint64_t hb = 8 + 4ull * arr_c;
+    for (size_t q = 0; q < pages.size(); ++q)
+        if ((err = hipMemcpyAsync(dev_out + pages[q].start, heads.data() + q * hb, hb, hipMemcpyHostToDevice, s)) != hipSuccess)
+            return release(hip_fail(err, "hipMemcpyAsync"));
+    if (n_vec) {
+        if ((err = hipMallocAsync((void**)&d_pk, 8 * n_vec, s)) != hipSuccess ||
+            (err = hipMallocAsync((void**)&d_vp, 4 * n_vec, s)) != hipSuccess ||
+            (err = hipMallocAsync((void**)&d_pg, sizeof(EncPage) * pages.size(), s)) != hipSuccess)
+            return release(hip_fail(err, "hipMallocAsync"));
+        if ((err = hipMemcpyAsync(d_pk, pk_rel.data(), 8 * n_vec, hipMemcpyHostToDevice, s)) != hipSuccess ||
+            (err = hipMemcpyAsync(d_vp, vpage.data(), 4 * n_vec, hipMemcpyHostToDevice, s)) != hipSuccess ||
+            (err = hipMemcpyAsync(d_pg, pages.data(), sizeof(EncPage) * pages.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+            return release(hip_fail(err, "hipMemcpyAsync"));
+        if ((err = launch_enc_w(W, mode, dev_values, n_vec, d_bw, d_base, s, false, d_pk, d_vp, d_pg, arr_c, dev_out)) !=
+            hipSuccess)
+            return release(hip_fail(err, "kernel launch"));
+    }
+    return release(FLS_OK);
+}

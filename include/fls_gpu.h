@@ -78,6 +78,15 @@ int fls_gpu_selection_indices(const uint64_t* dev_bitmap, uint64_t n_vec, uint16
 int fls_gpu_selection_and(const uint64_t* dev_a, const uint64_t* dev_b, uint64_t n_vec, uint64_t* dev_out,
                           uint32_t* dev_counts, void* stream);
 
+/* GPU encoder (reference encode RPNs of include/fls/cor/exp/{uncompressed,ffor,delta}/ and the page writer
+ * of src/cor/eng/compressor.cpp:41-86): encodes n device-resident values (n a multiple of 1024) of `type` with
+ * pool expression 0 (UNCOMPRESSED), 1 (FFOR) or 11 (DELTA) into dev_out, the same bytes encode_from_memory writes
+ * for those values. *out_size receives the encoded size (also when it exceeds `capacity`, which returns
+ * FLS_E_CAPACITY and writes nothing). Reads the per-vector bit widths back to lay out the pages, so it returns
+ * after `stream` has completed. */
+int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t type, uint8_t expr_id, uint8_t* dev_out,
+                   uint64_t capacity, uint64_t* out_size, void* stream);
+
 /* Encode a string column with FSST (1024-string vectors; n must be a multiple of 1024): strings i are
  * bytes[offsets[i], offsets[i+1]). Writes *out_size bytes. No counterpart in the reference's C API, which accepts
  * INT64 and DOUBLE only (reference src/encoder/single_col_encoder.cpp:13-33); the layout is the reference's FSST

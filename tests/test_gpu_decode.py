@@ -547,3 +547,38 @@ def test_predicate_on_doubles(torch_cuda):
             with np.errstate(invalid="ignore"):
                 mask = fn(x, c)
             np.testing.assert_array_equal(d_bm.cpu().numpy().view(np.uint64), _bitmap_words(mask), err_msg=f"{op} {c}")
+
+
+@pytest.mark.parametrize("eid", [0, 1, 11])
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint32, np.int32, np.int64, np.uint64])
+def test_gpu_encoder_matches_host_encoder(torch_cuda, dtype, eid):
+    """fls_gpu_encode writes the same bytes as the host encoder (itself pinned to the oracle and the reference's
+    recorded sizes), and the result decodes back to the input."""
+    torch = torch_cuda
+    print(f"gpu encode dtype={np.dtype(dtype).name} expr={eid}", flush=True)
+    for n in (0, 1024, 37 * 1024):
+        x = _column(dtype, eid, n, seed=n + eid) if n else np.zeros(0, dtype)
+        want = fa.encode_from_memory(x, eid)
+        d_x = dev(torch, x)
+        cap = want.size + 4096
+        d_out = torch.zeros(cap, dtype=torch.uint8, device="cuda")
+        size = fa.gpu_encode(d_x.data_ptr(), n, dtype, eid, d_out.data_ptr(), cap)
+        assert size == want.size
+        np.testing.assert_array_equal(d_out[:size].cpu().numpy(), want, err_msg=f"n={n}")
+
+
+def test_gpu_encoder_multi_page_and_capacity(torch_cuda):
+    """A column over 65535 vectors (two pages) and an 8-byte column whose pages hit the 128 MiB limit first; a too
+    small capacity is reported with the needed size."""
+    torch = torch_cuda
+    x = wl.timestamps(65_600 * 1024)
+    for eid in (11, 0):
+        want = fa.encode_from_memory(x, eid)
+        d_x = dev(torch, x)
+        d_out = torch.zeros(want.size + 64, dtype=torch.uint8, device="cuda")
+        size = fa.gpu_encode(d_x.data_ptr(), x.size, np.int64, eid, d_out.data_ptr(), want.size + 64)
+        assert size == want.size
+        assert torch.equal(d_out[:size].cpu(), torch.from_numpy(want)), eid
+        del d_out
+    with pytest.raises(fa.FlsError):
+        fa.gpu_encode(d_x.data_ptr(), x.size, np.int64, 11, d_x.data_ptr(), 100)
