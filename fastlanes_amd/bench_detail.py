@@ -160,6 +160,41 @@ def select_bench(dev, stream, log2n=26, verbose=True):
     return out
 
 
+def encode_bench(dev, stream, log2n=26, verbose=True):
+    """GPU encoder (fls_gpu_encode) against the host encoder (encode_from_memory, all host threads) on the same
+    columns; bytes are the input values read plus the encoded bytes written."""
+    import torch
+
+    n = 1 << log2n
+    rng = np.random.default_rng(9)
+    cases = {"u32_ffor_w7": (1, rng.integers(0, 128, n, dtype=np.uint32)),
+             "i64_timestamps_delta": (11, wl.timestamps(n)),
+             "u64_uncompressed": (0, rng.integers(0, 1 << 62, n, dtype=np.uint64))}
+    out = {}
+    sp = int(stream.cuda_stream)
+    for name, (eid, x) in cases.items():
+        t0 = time.perf_counter()
+        want = fa.encode_from_memory(x, eid)
+        t_host = time.perf_counter() - t0
+        d_x = torch.from_numpy(x).to(dev)
+        d_out = torch.empty(want.size + 64, dtype=torch.uint8, device=dev)
+        torch.cuda.synchronize(dev)
+        ts = []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            size = fa.gpu_encode(d_x.data_ptr(), n, x.dtype, eid, d_out.data_ptr(), want.size + 64, sp)
+            ts.append(time.perf_counter() - t0)
+        t = min(ts[1:])
+        same = size == want.size and bool(torch.equal(d_out[:size].cpu(), torch.from_numpy(want)))
+        out[name] = {"values": n, "encoded_bytes": int(size), "gpu_ms": round(t * 1e3, 3),
+                     "gpu_gvalues_s": round(n / t / 1e9, 1), "gpu_gb_s": round((x.nbytes + size) / t / 1e9, 1),
+                     "host_ms": round(t_host * 1e3, 1), "same_bytes": same}
+        if verbose:
+            print(f"encode {name}: {out[name]}", flush=True)
+        del d_x, d_out
+    return out
+
+
 def spell_bench(dev, log2n=20, n_blocks=16, reps=5, verbose=True):
     """Structural wizard over a 64-column row-group of 2^20 rows (per block of four: a high-cardinality int64
     column, its copy, an int32 key with ~1000 distinct values and an int64 function of it), device-resident."""
@@ -203,7 +238,7 @@ def main():
     import torch
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sections", default="pages,cascade", help="pages, cascade, rowgroup, wizard, spell, select")
+    ap.add_argument("--sections", default="pages,cascade", help="pages, cascade, rowgroup, wizard, spell, select, encode")
     ap.add_argument("--log2n", type=int, default=26)
     a = ap.parse_args()
     dev = torch.device("cuda:0")
@@ -226,6 +261,8 @@ def main():
             res["spell"] = spell_bench(dev)
         elif sec == "select":
             res["select"] = select_bench(dev, stream, a.log2n)
+        elif sec == "encode":
+            res["encode"] = encode_bench(dev, stream, a.log2n)
     print(json.dumps(res), flush=True)
 
 
