@@ -31,6 +31,7 @@ struct DevDict {
     uint8_t*  values = nullptr;  // decoded dictionary (DW-bit values)
     HzlChunk* chunks = nullptr;
     uint32_t  n_chunks = 0;
+    bool      exc      = false;  // some chunk carries exceptions
 };
 
 struct fls_gpu_plan {
@@ -51,7 +52,7 @@ int plan_decode_dictionaries(fls_gpu_plan* p, const uint8_t* dev_encoded, hipStr
     for (size_t i = 0; i < p->col.pages.size(); ++i) {
         const DevDict& d = p->dicts[i];
         if (!d.n_chunks) continue;
-        hipError_t e = launch_hzl(p->DW, dev_encoded, d.chunks, d.n_chunks, d.values, s);
+        hipError_t e = launch_hzl(p->DW, dev_encoded, d.chunks, d.n_chunks, d.values, d.exc, s);
         if (e != hipSuccess) return hip_fail(e, "dictionary launch");
     }
     return FLS_OK;
@@ -82,6 +83,7 @@ int fls_gpu_plan_create(const uint8_t* host_encoded, uint64_t encoded_size, enum
         std::vector<HzlChunk> ch(pe.hzl.size());
         for (size_t k = 0; k < ch.size(); ++k)
             ch[k] = {pe.hzl[k].src, pe.hzl[k].dst, pe.hzl[k].base, pe.hzl[k].bw, pe.hzl[k].exc_c, pe.hzl[k].n, 0};
+        for (const auto& c : pe.hzl) d.exc |= c.exc_c != 0;
         d.n_chunks          = (uint32_t)ch.size();
         const uint64_t vals = (uint64_t)pe.dict_n + 1024;  // chunk writes are whole 1024-value vectors
         if ((e = hipMalloc(&d.values, vals * (plan->DW / 8))) != hipSuccess) return hip_fail(e, "hipMalloc");
@@ -138,7 +140,7 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s); break;
         default: {
             DevDict& d = plan->dicts[i];
-            e          = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, s);
+            e          = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, d.exc, s);
             if (e == hipSuccess)
                 e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, out, plan->d_err, s);
         }
@@ -207,7 +209,7 @@ int fls_gpu_plan_select(fls_gpu_plan* plan, const uint8_t* dev_encoded, int op, 
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps); break;
         default: {
             DevDict& d = plan->dicts[i];
-            e          = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, s);
+            e          = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, d.exc, s);
             if (e == hipSuccess)
                 e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, nullptr, plan->d_err, s, &ps);
         }
@@ -260,7 +262,7 @@ int fls_gpu_tune_set(int family, unsigned width, unsigned bw, unsigned block, un
         return fail(FLS_E_ARG, "block must be a multiple of 64, <= 512");
     if (family == FAM_ROWGROUP && block == 0) return fail(FLS_E_ARG, "tile size must be positive");
     (void)tune_for(family, width, bw);  // make sure the default table is loaded first
-    tune_set(family, width, bw, Tune {(uint16_t)block, (uint16_t)(nontemporal != 0), max_blocks});
+    tune_set(family, width, bw, Tune {(uint16_t)block, (uint16_t)(nontemporal & 3), max_blocks});
     return FLS_OK;
 }
 

@@ -60,7 +60,7 @@ int tune_load(const char* path) {
         if (std::sscanf(line, "%d %u %u %u %u %u", &fam, &W, &bw, &block, &nt, &maxb) != 6) continue;
         if (fam < 0 || fam >= FAM_COUNT || block == 0) continue;
         if (fam != FAM_ROWGROUP && (block % 64 || block > kMaxBlock)) continue;
-        tune_set(fam, W, bw, Tune {(uint16_t)block, (uint16_t)(nt != 0), maxb});
+        tune_set(fam, W, bw, Tune {(uint16_t)block, (uint16_t)(nt & 3), maxb});  // bit 0 non-temporal, bit 1 direct
         ++n;
     }
     std::fclose(f);
@@ -115,15 +115,15 @@ hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const Page
 }
 
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
-                      hipStream_t s) {
+                      bool exceptions, hipStream_t s) {
     if (!n_chunks) return hipSuccess;
     const dim3 g(n_chunks < 65535 ? n_chunks : 65535), b(256), ge(n_chunks < 1024 ? n_chunks : 1024);
     if (DW == 32) {
         hipLaunchKernelGGL((k_hzl_chunks<32>), g, b, 0, s, enc, chunks, n_chunks, dict);
-        hipLaunchKernelGGL((k_hzl_exceptions<32>), ge, b, 0, s, enc, chunks, n_chunks, dict);
+        if (exceptions) hipLaunchKernelGGL((k_hzl_exceptions<32>), ge, b, 0, s, enc, chunks, n_chunks, dict);
     } else {
         hipLaunchKernelGGL((k_hzl_chunks<64>), g, b, 0, s, enc, chunks, n_chunks, dict);
-        hipLaunchKernelGGL((k_hzl_exceptions<64>), ge, b, 0, s, enc, chunks, n_chunks, dict);
+        if (exceptions) hipLaunchKernelGGL((k_hzl_exceptions<64>), ge, b, 0, s, enc, chunks, n_chunks, dict);
     }
     return hipGetLastError();
 }

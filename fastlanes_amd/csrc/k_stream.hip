@@ -23,10 +23,10 @@ hipError_t launch_unffor_stream(unsigned W, unsigned bw, const uint8_t* packed, 
     const Tune t = tune_for(FAM_STREAM, W, bw);
     StreamFn   f = nullptr;
     switch (W) {
-    case 8: f = t.nt ? kTab<8, true>[bw] : kTab<8, false>[bw]; break;
-    case 16: f = t.nt ? kTab<16, true>[bw] : kTab<16, false>[bw]; break;
-    case 32: f = t.nt ? kTab<32, true>[bw] : kTab<32, false>[bw]; break;
-    case 64: f = t.nt ? kTab<64, true>[bw] : kTab<64, false>[bw]; break;
+    case 8: f = (t.nt & 1) ? kTab<8, true>[bw] : kTab<8, false>[bw]; break;
+    case 16: f = (t.nt & 1) ? kTab<16, true>[bw] : kTab<16, false>[bw]; break;
+    case 32: f = (t.nt & 1) ? kTab<32, true>[bw] : kTab<32, false>[bw]; break;
+    case 64: f = (t.nt & 1) ? kTab<64, true>[bw] : kTab<64, false>[bw]; break;
     default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(f, dim3(grid_for(n_vec, t, kTPV)), dim3(t.block), 0, s, packed, bases, out, n_vec);

@@ -52,7 +52,7 @@ hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const Page
 hipError_t launch_page_patch(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                              uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr);
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
-                      hipStream_t s);
+                      bool exceptions, hipStream_t s);  // exceptions: some chunk has them (else that pass is skipped)
 
 // per-width instantiation units (k_page_w*.hip)
 template <int W>
