@@ -146,7 +146,7 @@ class Plan:
 
 
 def gpu_encode(dev_values: int, n: int, dtype, expr_id: int, dev_out: int, capacity: int, stream: int = 0) -> int:
-    """Encodes n device-resident values on the GPU (expressions 0, 1, 11); returns the encoded size."""
+    """Encodes n device-resident values on the GPU (expressions 0, 1, 11, 6, 7); returns the encoded size."""
     size = _u64(0)
     check(lib().fls_gpu_encode(dev_values, n, dtype_code(dtype), expr_id, dev_out, capacity, C.byref(size), stream),
           "gpu_encode")

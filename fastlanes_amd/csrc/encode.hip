@@ -1,5 +1,5 @@
-// encode.hip -- GPU encoder for integer columns: UNCOMPRESSED (pool id 0), FFOR (1) and DELTA (11) pages, byte for
-// byte what the host encoder (encode_from_memory) writes.
+// encode.hip -- GPU encoder for integer columns: UNCOMPRESSED (pool id 0), FFOR (1), DELTA (11) and RLE (6, 7)
+// pages, byte for byte what the host encoder (encode_from_memory) writes.
 //
 // Encode RPNs: UNCOMPRESSED {TRS, MEM_CPY} (include/fls/cor/exp/uncompressed/uncompressed_exp.hpp:32-47), FFOR
 // {TRS, ANALYZE, FFOR} (include/fls/cor/exp/ffor/ffor_exp.hpp:39-46), DELTA {TRS, RSUM, ANALYZE, FFOR}
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_enc_analyze(const T* __restrict__ x, ui
 
 struct EncPage {
     uint64_t start, v0;  // byte offset of the page in the output, its first vector
-    uint32_t off[4];     // buffer offsets from the page start
+    uint32_t off[6];     // buffer offsets from the page start
 };
 
 template <typename T>
@@ -147,6 +147,98 @@ __global__ __launch_bounds__(128) void k_enc_pack(const T* __restrict__ x, const
     }
 }
 
+// RLE (include/fls/cor/exp/rle/rle_exp.hpp:37-74, split of src/cor/lyt/vec.cpp:50-71): a vector's run values
+// (one per change of value), and per value its u16 run index, transposed; VAR 1 stores the indexes, VAR 2 their
+// 64 lane bases and the FFOR of their in-segment differences (16-value segments of u16), which are the run-start
+// flags themselves: 0 at a segment start, else 1 where a run starts -- so base 0 and bw 0 or 1.
+// Analysis, one wave per vector: runs[v] = number of runs, bw[v] = the difference FFOR's bit width (VAR 2).
+template <typename T>
+__global__ __launch_bounds__(256) void k_enc_rle_analyze(const T* __restrict__ x, uint64_t n_vec, uint8_t* __restrict__ bw,
+                                                         uint64_t* __restrict__ runs) {
+    const unsigned lane  = threadIdx.x % 64;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; v < n_vec; v += waves) {
+        const T* xv  = x + v * 1024;
+        unsigned n   = 0;
+        bool     one = false;
+        for (unsigned o = lane; o < 1024; o += 64) {
+            const bool f = o == 0 || xv[o] != xv[o - 1];
+            n += f;
+            one |= f && (o % 16);
+        }
+        for (int d = 32; d >= 1; d >>= 1) n += __shfl_xor(n, d, 64);
+        const bool any = __ballot(one) != 0;
+        if (lane == 0) runs[v] = n, bw[v] = any ? 1 : 0;
+    }
+}
+
+// One block of 128 threads per vector: run-start flags, run indexes by a block scan, then the page buffers.
+template <typename T, int VAR>
+__global__ __launch_bounds__(128) void k_enc_rle_pack(const T* __restrict__ x, const uint8_t* __restrict__ bw_in,
+                                                      const uint64_t* __restrict__ rv_rel,
+                                                      const uint64_t* __restrict__ pk_rel, const uint32_t* __restrict__ vpage,
+                                                      const EncPage* __restrict__ pages, uint32_t arr_c,
+                                                      uint8_t* __restrict__ out) {
+    constexpr unsigned SZ = sizeof(T);
+    __shared__ T        xs[1024];
+    __shared__ uint16_t ps[1024];
+    __shared__ unsigned part[2];
+    const uint64_t      v   = blockIdx.x;
+    const unsigned      tid = threadIdx.x, lane = tid % 64;
+    for (unsigned k = tid; k < 1024; k += 128) xs[k] = x[v * 1024 + k];
+    __syncthreads();
+    // thread t owns values 8t .. 8t+7: local flags, then an exclusive scan over the block
+    unsigned f8 = 0, c = 0;
+    for (unsigned q = 0; q < 8; ++q) {
+        const unsigned o = 8 * tid + q;
+        const bool     f = o == 0 || xs[o] != xs[o - 1];
+        f8 |= (unsigned)f << q;
+        c += f;
+    }
+    unsigned inc = c;
+    for (unsigned d = 1; d < 64; d <<= 1) {
+        const unsigned y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+    }
+    if (lane == 63) part[tid / 64] = inc;
+    __syncthreads();
+    unsigned run = inc - c + (tid >= 64 ? part[0] : 0);  // runs started before value 8t
+    const EncPage  pg = pages[vpage[v]];
+    const uint32_t r  = (uint32_t)(v - pg.v0);
+    uint8_t*       p  = out + pg.start;
+    uint8_t*       rv = p + pg.off[0] + rv_rel[v];
+    for (unsigned q = 0; q < 8; ++q) {
+        const unsigned o = 8 * tid + q;
+        if ((f8 >> q) & 1u) st_ua<T>(rv + (uint64_t)run * SZ, xs[o]), ++run;
+        ps[o] = (uint16_t)(run - 1);
+    }
+    __syncthreads();
+    uint8_t* ep = p + 8 + 4 * arr_c + 4ull * arr_c * (r ? r - 1 : 0);
+    if constexpr (VAR == 1) {
+        for (unsigned k = tid; k < 1024; k += 128) st_ua<uint16_t>(p + pg.off[1] + (uint64_t)r * 2048 + 2 * k, ps[enc_orig(k)]);
+        if (r && tid == 0) st_ua<uint32_t>(ep, (uint32_t)rv_rel[v]), st_ua<uint32_t>(ep + 4, r * 2048);
+    } else {
+        const unsigned bw = bw_in[v];
+        if (tid < 64) st_ua<uint16_t>(p + pg.off[1] + (uint64_t)r * 128 + 2 * tid, ps[enc_orig(tid)]);
+        if (tid == 0) {
+            p[pg.off[2] + r] = (uint8_t)bw;
+            st_ua<uint16_t>(p + pg.off[3] + (uint64_t)r * 2, 0);  // base: the minimum difference is 0
+            if (r) {
+                st_ua<uint32_t>(ep, (uint32_t)rv_rel[v]), st_ua<uint32_t>(ep + 4, r * 128);
+                st_ua<uint32_t>(ep + 8, r), st_ua<uint32_t>(ep + 12, r * 2), st_ua<uint32_t>(ep + 16, (uint32_t)pk_rel[v]);
+            }
+        }
+        if (bw && tid < 64) {  // bw 1: lane i's 16 differences (transposed j * 64 + i) form one u16 word
+            unsigned word = 0;
+            for (unsigned j = 0; j < 16; ++j) {
+                const unsigned o = enc_orig(j * 64 + tid);
+                word |= (unsigned)((o % 16) ? (ps[o] - ps[o - 1]) : 0) << j;
+            }
+            st_ua<uint16_t>(p + pg.off[4] + pk_rel[v] + 2 * tid, (uint16_t)word);
+        }
+    }
+}
+
 template <typename T>
 hipError_t launch_enc(int mode, const void* x, uint64_t n_vec, uint8_t* bw, uint64_t* base, hipStream_t s, bool analyze,
                       const uint64_t* pk_rel, const uint32_t* vpage, const EncPage* pages, uint32_t arr_c,
@@ -155,7 +247,13 @@ hipError_t launch_enc(int mode, const void* x, uint64_t n_vec, uint8_t* bw, uint
     if (analyze) {
         const unsigned blocks = (unsigned)std::min<uint64_t>((n_vec + 3) / 4, 65536);
         if (mode == 1) hipLaunchKernelGGL((k_enc_analyze<T, 1>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base);
-        else hipLaunchKernelGGL((k_enc_analyze<T, 2>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base);
+        else if (mode == 2) hipLaunchKernelGGL((k_enc_analyze<T, 2>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base);
+        else hipLaunchKernelGGL((k_enc_rle_analyze<T>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base);
+    } else if (mode >= 3) {  // RLE: `base` carries the run-value offsets
+#define R(V) hipLaunchKernelGGL((k_enc_rle_pack<T, V>), dim3((unsigned)n_vec), dim3(128), 0, s, xv, bw, base, pk_rel, vpage, pages, arr_c, out)
+        if (mode == 3) R(1);
+        else R(2);
+#undef R
     } else {
 #define P(M) hipLaunchKernelGGL((k_enc_pack<T, M>), dim3((unsigned)n_vec), dim3(128), 0, s, xv, bw, base, pk_rel, vpage, pages, arr_c, out)
         if (mode == 0) P(0);
@@ -187,22 +285,28 @@ extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t ty
     if (!out_size || !dev_out || (n && !dev_values)) return fail(FLS_E_ARG, "null argument");
     Expr e;
     if (type == STR || !expr_from_id((uint8_t)type, expr_id, e)) return fail(FLS_E_UNSUPPORTED, "unknown expression");
-    const int mode = e.exp_t == EXP_UNCOMPRESSED ? 0 : (e.exp_t == EXP_FFOR && e.var == 1) ? 1 : e.exp_t == EXP_DELTA ? 2 : -1;
-    if (mode < 0) return fail(FLS_E_UNSUPPORTED, "the GPU encoder writes UNCOMPRESSED, FFOR and DELTA pages");
+    const int mode = e.exp_t == EXP_UNCOMPRESSED           ? 0
+                     : (e.exp_t == EXP_FFOR && e.var == 1) ? 1
+                     : e.exp_t == EXP_DELTA                ? 2
+                     : e.exp_t == EXP_RLE                  ? (e.var == 1 ? 3 : 4)
+                                                           : -1;
+    if (mode < 0) return fail(FLS_E_UNSUPPORTED, "the GPU encoder writes UNCOMPRESSED, FFOR, DELTA and RLE pages");
     if (type == DOUBLE && mode) return fail(FLS_E_UNSUPPORTED, "double columns support UNCOMPRESSED only");
     if (n % 1024) return fail(FLS_E_UNSUPPORTED, "value count must be a multiple of 1024");
     const unsigned    W = dtype_bits((uint8_t)type), SZ = W / 8, arr_c = e.arr_c;
     const uint64_t    n_vec = n / 1024;
     hipStream_t       s     = (hipStream_t)stream;
     hipError_t        err;
-    std::vector<uint8_t> bw(n_vec, 0);
+    std::vector<uint8_t>  bw(n_vec, 0);
+    std::vector<uint64_t> runs(mode >= 3 ? n_vec : 0);
     uint8_t*          d_bw   = nullptr;
     uint64_t*         d_base = nullptr;
     uint64_t*         d_pk   = nullptr;
     uint32_t*         d_vp   = nullptr;
+    uint64_t*         d_rv   = nullptr;
     EncPage*          d_pg   = nullptr;
     auto              release = [&](int rc) {
-        for (void* q : {(void*)d_bw, (void*)d_base, (void*)d_pk, (void*)d_vp, (void*)d_pg})
+        for (void* q : {(void*)d_bw, (void*)d_base, (void*)d_pk, (void*)d_vp, (void*)d_pg, (void*)d_rv})
             if (q) (void)hipFreeAsync(q, s);
         (void)hipStreamSynchronize(s);
         return rc;
@@ -214,6 +318,7 @@ extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t ty
         if (mode) {
             if ((err = launch_enc_w(W, mode, dev_values, n_vec, d_bw, d_base, s, true)) != hipSuccess ||
                 (err = hipMemcpyAsync(bw.data(), d_bw, n_vec, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+                (mode >= 3 && (err = hipMemcpyAsync(runs.data(), d_base, 8 * n_vec, hipMemcpyDeviceToHost, s)) != hipSuccess) ||
                 (err = hipStreamSynchronize(s)) != hipSuccess)
                 return release(hip_fail(err, "analyze"));
         }
@@ -221,6 +326,11 @@ extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t ty
     // page layout, as the host encoder's assembly loop (format.cpp encode_t step 2)
     auto vec_bytes = [&](uint64_t v, unsigned b) -> uint64_t {
         if (mode == 0) return 1024ull * SZ;
+        if (mode >= 3) {  // RLE: [run values][positions] or [run values][lane bases][bw][base u16][packed]
+            if (b == 0) return runs[v] * SZ;
+            if (mode == 3) return 2048;
+            return b == 1 ? 128 : b == 2 ? 1 : b == 3 ? 2 : 128ull * bw[v];
+        }
         const unsigned o = mode == 2 ? 1 : 0;
         if (mode == 2 && b == 0) return 128;
         if (b == o) return 1;
@@ -230,19 +340,20 @@ extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t ty
     std::vector<EncPage>  pages;
     std::vector<uint8_t>  heads;  // per page: 8-byte header + arr_c offsets
     std::vector<uint32_t> vpage(n_vec);
-    std::vector<uint64_t> pk_rel(n_vec);
+    std::vector<uint64_t> pk_rel(n_vec), rv_rel(mode >= 3 ? n_vec : 0);
     const uint64_t        page_c = n_vec ? (n_vec + kMaxVecPerPage - 1) / kMaxVecPerPage : 1;
     const uint64_t        per_pg = n_vec ? (n_vec + page_c - 1) / page_c : 0;
     uint64_t              v = 0, pos = 0;
     for (;;) {
         const uint64_t v0 = v, stop = std::min(n_vec, v + per_pg);
-        uint64_t       bufs[4] = {0, 0, 0, 0}, eps = 0, vec_c = 0;
+        uint64_t       bufs[6] = {0, 0, 0, 0, 0, 0}, eps = 0, vec_c = 0;
         for (; v < stop; ++v) {
             uint64_t add = 4ull * arr_c, cur = 8 + 4ull * arr_c + eps;
             for (unsigned b = 0; b < arr_c; ++b) add += vec_bytes(v, b), cur += bufs[b];
             if (vec_c && cur + add >= kDataPageMax) break;  // TryFlushVec limit (compressor.cpp:47)
             if (vec_c) eps += 4ull * arr_c;
             pk_rel[v] = mode == 0 ? 0 : bufs[arr_c - 1];
+            if (mode >= 3) rv_rel[v] = bufs[0];
             vpage[v]  = (uint32_t)pages.size();
             for (unsigned b = 0; b < arr_c; ++b) bufs[b] += vec_bytes(v, b);
             ++vec_c;
@@ -279,8 +390,11 @@ extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t ty
             (err = hipMemcpyAsync(d_vp, vpage.data(), 4 * n_vec, hipMemcpyHostToDevice, s)) != hipSuccess ||
             (err = hipMemcpyAsync(d_pg, pages.data(), sizeof(EncPage) * pages.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
             return release(hip_fail(err, "hipMemcpyAsync"));
-        if ((err = launch_enc_w(W, mode, dev_values, n_vec, d_bw, d_base, s, false, d_pk, d_vp, d_pg, arr_c, dev_out)) !=
-            hipSuccess)
+        if (mode >= 3 && ((err = hipMallocAsync((void**)&d_rv, 8 * n_vec, s)) != hipSuccess ||
+                          (err = hipMemcpyAsync(d_rv, rv_rel.data(), 8 * n_vec, hipMemcpyHostToDevice, s)) != hipSuccess))
+            return release(hip_fail(err, "run-value offsets"));
+        if ((err = launch_enc_w(W, mode, dev_values, n_vec, d_bw, mode >= 3 ? d_rv : d_base, s, false, d_pk, d_vp, d_pg,
+                                arr_c, dev_out)) != hipSuccess)
             return release(hip_fail(err, "kernel launch"));
     }
     return release(FLS_OK);

@@ -80,7 +80,7 @@ int fls_gpu_selection_and(const uint64_t* dev_a, const uint64_t* dev_b, uint64_t
 
 /* GPU encoder (reference encode RPNs of include/fls/cor/exp/{uncompressed,ffor,delta}/ and the page writer
  * of src/cor/eng/compressor.cpp:41-86): encodes n device-resident values (n a multiple of 1024) of `type` with
- * pool expression 0 (UNCOMPRESSED), 1 (FFOR) or 11 (DELTA) into dev_out, the same bytes encode_from_memory writes
+ * pool expression 0 (UNCOMPRESSED), 1 (FFOR), 11 (DELTA), 6 or 7 (RLE) into dev_out, the same bytes encode_from_memory writes
  * for those values. *out_size receives the encoded size (also when it exceeds `capacity`, which returns
  * FLS_E_CAPACITY and writes nothing). Reads the per-vector bit widths back to lay out the pages, so it returns
  * after `stream` has completed. */

@@ -549,7 +549,7 @@ def test_predicate_on_doubles(torch_cuda):
             np.testing.assert_array_equal(d_bm.cpu().numpy().view(np.uint64), _bitmap_words(mask), err_msg=f"{op} {c}")
 
 
-@pytest.mark.parametrize("eid", [0, 1, 11])
+@pytest.mark.parametrize("eid", [0, 1, 11, 6, 7])
 @pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.uint32, np.int32, np.int64, np.uint64])
 def test_gpu_encoder_matches_host_encoder(torch_cuda, dtype, eid):
     """fls_gpu_encode writes the same bytes as the host encoder (itself pinned to the oracle and the reference's
@@ -572,7 +572,8 @@ def test_gpu_encoder_multi_page_and_capacity(torch_cuda):
     small capacity is reported with the needed size."""
     torch = torch_cuda
     x = wl.timestamps(65_600 * 1024)
-    for eid in (11, 0):
+    x[: 50_000 * 1024] //= 1000  # runs in the first part, for the RLE pages
+    for eid in (11, 0, 6):
         want = fa.encode_from_memory(x, eid)
         d_x = dev(torch, x)
         d_out = torch.zeros(want.size + 64, dtype=torch.uint8, device="cuda")
