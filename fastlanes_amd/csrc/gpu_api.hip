@@ -139,10 +139,9 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
         case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, out, s); break;
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s); break;
         default: {
-            DevDict& d = plan->dicts[i];
-            e          = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, d.exc, s);
-            if (e == hipSuccess)
-                e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, out, plan->d_err, s);
+            DevDict&     d = plan->dicts[i];
+            const HzlSrc hz {dev_encoded, d.chunks, d.n_chunks, d.exc ? 1u : 0u};
+            e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, out, plan->d_err, s, nullptr, &hz);
         }
         }
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
@@ -208,10 +207,9 @@ int fls_gpu_plan_select(fls_gpu_plan* plan, const uint8_t* dev_encoded, int op, 
         case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, nullptr, s, &ps); break;
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps); break;
         default: {
-            DevDict& d = plan->dicts[i];
-            e          = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, d.exc, s);
-            if (e == hipSuccess)
-                e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, nullptr, plan->d_err, s, &ps);
+            DevDict&     d = plan->dicts[i];
+            const HzlSrc hz {dev_encoded, d.chunks, d.n_chunks, d.exc ? 1u : 0u};
+            e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, nullptr, plan->d_err, s, &ps, &hz);
         }
         }
         if (e != hipSuccess) return hip_fail(e, "kernel launch");

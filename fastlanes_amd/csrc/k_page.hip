@@ -103,13 +103,13 @@ hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageM
 
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,
                             const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s,
-                            const SelArgs* sel) {
+                            const SelArgs* sel, const HzlSrc* hz) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_dict_w<8>(var, page, pm, vec_c, dict, dict_n, out, err, sel, s);
-    case 16: return page_dict_w<16>(var, page, pm, vec_c, dict, dict_n, out, err, sel, s);
-    case 32: return page_dict_w<32>(var, page, pm, vec_c, dict, dict_n, out, err, sel, s);
-    case 64: return page_dict_w<64>(var, page, pm, vec_c, dict, dict_n, out, err, sel, s);
+    case 8: return page_dict_w<8>(var, page, pm, vec_c, dict, dict_n, out, err, sel, hz, s);
+    case 16: return page_dict_w<16>(var, page, pm, vec_c, dict, dict_n, out, err, sel, hz, s);
+    case 32: return page_dict_w<32>(var, page, pm, vec_c, dict, dict_n, out, err, sel, hz, s);
+    case 64: return page_dict_w<64>(var, page, pm, vec_c, dict, dict_n, out, err, sel, hz, s);
     default: return hipErrorInvalidValue;
     }
 }

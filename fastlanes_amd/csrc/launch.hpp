@@ -8,6 +8,7 @@ namespace fls {
 struct HzlChunk;
 struct PageMeta;
 struct SelArgs;
+struct HzlSrc;
 
 // Kernel families with separately tuned launch shapes.
 enum Family : int {
@@ -46,9 +47,11 @@ hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const Page
                            hipStream_t s, const SelArgs* sel = nullptr);
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                            uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr);
+// hz: the page's HZL dictionary payload; the launcher decodes it into `dict` (or, when it fits LDS, lets the page
+// kernel decode it per block). nullptr: `dict` already holds the decoded dictionary.
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,
                             const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s,
-                            const SelArgs* sel = nullptr);
+                            const SelArgs* sel = nullptr, const HzlSrc* hz = nullptr);
 hipError_t launch_page_patch(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                              uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr);
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
@@ -66,6 +69,7 @@ hipError_t page_rle_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t
                       const SelArgs* sel, hipStream_t s);
 template <int W>
 hipError_t page_dict_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, const void* dict,
-                       uint32_t dict_n, uint8_t* out, uint32_t* err, const SelArgs* sel, hipStream_t s);
+                       uint32_t dict_n, uint8_t* out, uint32_t* err, const SelArgs* sel, const HzlSrc* hz,
+                       hipStream_t s);
 
 }  // namespace fls

@@ -727,14 +727,54 @@ __device__ __forceinline__ CodeMeta code_meta(const uint8_t* page, const PageMet
     return m;
 }
 
+struct HzlChunk {
+    uint64_t src;    // byte offset of the packed data (FFOR) or raw bytes (tail) in the encoded buffer
+    uint64_t dst;    // first value index in the decoded dictionary
+    uint64_t base;   // FFOR base
+    uint32_t bw;     // 255 = raw tail
+    uint32_t exc_c;  // exceptions following the packed data
+    uint32_t n;      // values in this chunk (1024, or the tail's count)
+    uint32_t pad;
+};
+
+// Value k of an HZL chunk (src/cor/eng/hzl_de_cmp.cpp:28-77): a raw tail value, or FFOR value j of lane k % L
+// (k = j * L + lane, the chunk's lane-major order) plus the chunk base.
+template <typename D>
+__device__ __forceinline__ D hzl_value(const uint8_t* in, const HzlChunk& ch, uint32_t k) {
+    constexpr unsigned DW = 8 * sizeof(D), L = 1024 / DW;
+    if (ch.bw == 255) return load_t<D>(in + (uint64_t)k * sizeof(D));
+    const unsigned bw = ch.bw;
+    D              v  = 0;
+    if (bw) {
+        const D        mask = bw >= DW ? (D)~(D)0 : (D)(((D)1 << bw) - 1);
+        const unsigned j = k / L, lane = k % L, bit = j * bw, w = bit / DW, sh = bit % DW;
+        v = load_t<D>(in + ((uint64_t)w * L + lane) * sizeof(D)) >> sh;
+        if (sh + bw > DW) v |= load_t<D>(in + ((uint64_t)(w + 1) * L + lane) * sizeof(D)) << (DW - sh);
+        v &= mask;
+    }
+    return (D)(v + (D)ch.base);
+}
+
+// The HZL payload of a page's dictionary, for kernels that decode it themselves.
+struct HzlSrc {
+    const uint8_t*  enc;     // the encoded buffer the chunk offsets refer to
+    const HzlChunk* chunks;  // device table
+    uint32_t        n_chunks;
+    uint32_t        exc;     // some chunk carries exceptions (then the separate HZL kernels decode it)
+};
+
 // Dictionary pages, pipelined like k_page_int. IN_LDS: the decoded dictionary is first copied into LDS (after the
 // waves' staging areas) and the code gathers read LDS instead of L2; the host picks it when the dictionary fits.
 // DIRECT (64-bit values): no output staging, segments stored straight to HBM (as k_page_rle); LDS holds only the
 // dictionary.
-template <typename T, typename D, int VAR, bool NT, bool IN_LDS, bool SEL = false, bool DIRECT = false>
+// FROM_HZL (with IN_LDS): every block decodes the dictionary's HZL chunks straight into LDS, so no separate
+// dictionary pass runs before the page (chunks without exceptions only).
+template <typename T, typename D, int VAR, bool NT, bool IN_LDS, bool SEL = false, bool DIRECT = false,
+          bool FROM_HZL = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
                                                          const D* __restrict__ dict, uint32_t dict_n,
-                                                         uint8_t* __restrict__ out, uint32_t* err, SelArgs sa) {
+                                                         uint8_t* __restrict__ out, uint32_t* err, SelArgs sa,
+                                                         HzlSrc hz) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     if (dict_n == 0) return;  // the host rejects such pages
     constexpr unsigned VB = 1024 * sizeof(T);
@@ -744,7 +784,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restri
     const D* dp           = dict;
     if constexpr (IN_LDS) {
         D* ldict = reinterpret_cast<D*>(fls_lds + (blockDim.x / 64) * SB);
-        for (uint32_t i = threadIdx.x; i < dict_n; i += blockDim.x) ldict[i] = dict[i];
+        if constexpr (FROM_HZL) {
+            for (uint32_t c = 0; c < hz.n_chunks; ++c) {
+                const HzlChunk ch  = hz.chunks[c];
+                const uint8_t* in  = hz.enc + ch.src;
+                const uint32_t cnt = ch.bw == 255 ? ch.n : 1024u;
+                for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x)
+                    if (ch.dst + k < dict_n) ldict[ch.dst + k] = hzl_value<D>(in, ch, k);
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i < dict_n; i += blockDim.x) ldict[i] = dict[i];
+        }
         __syncthreads();
         dp = ldict;
     }
@@ -787,15 +837,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_dict(const uint8_t* __restri
 // HZL dictionary payload chunks (src/cor/eng/hzl_de_cmp.cpp:28-77): an FFOR chunk decodes to 1024 values in array
 // order (the lane-major order of unffor); the raw tail is copied; exceptions are patched by a second launch.
 // ---------------------------------------------------------------------------------------------------------------
-struct HzlChunk {
-    uint64_t src;    // byte offset of the packed data (FFOR) or raw bytes (tail) in the encoded buffer
-    uint64_t dst;    // first value index in the decoded dictionary
-    uint64_t base;   // FFOR base
-    uint32_t bw;     // 255 = raw tail
-    uint32_t exc_c;  // exceptions following the packed data
-    uint32_t n;      // values in this chunk (1024, or the tail's count)
-    uint32_t pad;
-};
 
 // One block per chunk, one thread per (value row j, lane): every thread extracts its value straight from the two
 // packed words that hold it (word floor(j*bw/DW) and the next of its lane), so a chunk decodes with one round of
