@@ -66,6 +66,7 @@ def pages(dev, stream, log2n=26, verbose=True):
             "delta": (11, np.cumsum(rng.integers(0, 100, m)).astype(dt)),
             "rle": (6, wl.geometric_runs(m, 16.0, seed=1, value_bits=min(W - 1, 20)).astype(dt)),
             "dict_ffor": (17, rng.integers(0, 1 << min(W - 1, 12), m).astype(dt)),
+            "dict_rle": (27, wl.dict_runs(m, dt, n_keys=4096 if W > 8 else 200, mean=16.0, seed=2)),
         }
         if W >= 32:  # patched FFOR: a 7-bit body with 1% outliers anywhere in the type's range
             body = (1000 + rng.integers(0, 128, m)).astype(dt)

@@ -39,6 +39,7 @@ bool expr_from_id(uint8_t dtype, uint8_t id, Expr& e) {
     case 16: e = {EXP_DICT, 1, 1, true}; return true;
     case 17: e = {EXP_DICT, 2, 3, true}; return true;
     case 22: e = {EXP_DICT, 7, 4, true}; return true;
+    case 27: e = {EXP_DICT_RLE, kDictRleRec, 5, true}; return dtype != DT_DOUBLE;
     case 28: e = {EXP_DICT_FRQ, 1, 1, true}; return true;
     case 29: e = {EXP_DICT_FRQ, 2, 3, true}; return true;
     default: return false;
@@ -64,6 +65,7 @@ bool expr_from_page(uint8_t exp_t, uint8_t var, Expr& e) {
         if (var == 2) return e = {exp_t, var, 3, true}, true;
         if (var == 7) return e = {exp_t, var, 4, true}, true;
         return false;
+    case EXP_DICT_RLE: e = {exp_t, var, 5, true}; return var == kDictRleRec;
     case EXP_FSST: e = {exp_t, var, 2, true}; return var == 1;
     default: return false;
     }
@@ -109,7 +111,7 @@ std::string parse_hzl(const Reader& r, uint64_t off, unsigned dw, uint32_t entry
         n += 1024;
         if (n > (1ull << 32)) return "dictionary too large";
     }
-    if (n != entry_c) return "dictionary entry count mismatch";
+    if (entry_c != ~0u && n != entry_c) return "dictionary entry count mismatch";
     used = i - off;
     return "";
 }
@@ -168,7 +170,19 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
     // per-vector extents
     const unsigned sz = W / 8;
     auto           ep = [&](unsigned v, unsigned b) { return ep_of(pg, arr_c, v, b); };
-    auto           cap = [&](unsigned b) { return (b + 1 < arr_c ? boff[b + 1] : avail) - boff[b]; };
+    // DICT_RLE: buffer 0 is an HZL chain of u32 run codes whose entry points address the decoded codes
+    const bool drle = e.exp_t == EXP_DICT_RLE;
+    if (drle) {
+        uint64_t used = 0;
+        auto     err  = parse_hzl(r, off + boff[0], 32, ~0u, pe.run_hzl, used);
+        if (!err.empty()) return "run codes: " + err;
+        if (used != boff[1] - boff[0]) return "run codes do not fill their buffer";
+        for (const auto& c : pe.run_hzl) pe.run_codes += c.n;
+        if (4 * pe.run_codes > 0xffffffffull) return "run codes exceed 32-bit entry points";
+    }
+    auto cap = [&](unsigned b) {
+        return drle && b == 0 ? 4 * pe.run_codes : (b + 1 < arr_c ? boff[b + 1] : avail) - boff[b];
+    };
     auto           need = [&](unsigned v, unsigned b, uint64_t n) -> bool {
         const uint64_t s = ep(v, b);
         const uint64_t lim = v + 1 < pe.vec_c ? ep(v + 1, b) : cap(b);
@@ -231,6 +245,13 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
             break;
         }
         case EXP_FSST: ok = need(v, 1, 4096) && need(v, 0, 0); break;
+        case EXP_DICT_RLE: {  // RLE DELTA_NO_PATCH positions over u32 run codes
+            ok = ep(v, 0) % 4 == 0 && need(v, 0, 4) && need(v, 1, 128) && need(v, 2, 1) && need(v, 3, 2);
+            if (!ok) break;
+            const unsigned bw = bwc(v, 2);
+            ok = bw <= 16 && need(v, 4, 128ull * bw);
+            break;
+        }
         default: {  // dictionary codes
             if (e.var == 1) ok = need(v, 0, 4096);
             else {
@@ -260,6 +281,7 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
             else fsz[1] = 128, fsz[2] = 1, fsz[3] = 2;
             break;
         case EXP_FSST: fsz[1] = 4096; break;
+        case EXP_DICT_RLE: fsz[1] = 128, fsz[2] = 1, fsz[3] = 2; break;
         default:
             if (e.var == 1) fsz[0] = 4096;
             else if (e.var == 2) fsz[0] = 1, fsz[1] = 4;
@@ -327,6 +349,7 @@ std::string parse_column(const uint8_t* p, uint64_t size, uint8_t dtype, Column&
         if (is_dict_exp(pe.exp_t) != (pe.dict_off != ~0ull && p[pe.dict_off] != EXP_FSST) ||
             (pe.exp_t == EXP_FSST) != (pe.dict_off != ~0ull && p[pe.dict_off] == EXP_FSST))
             return "dictionary page does not match the data page's expression";
+        if (is_dict_exp(pe.exp_t) && pe.vec_c && pe.dict_n == 0) return "empty dictionary";
         if (pe.exp_t == EXP_FSST) {
             // every vector's compressed bytes must lie inside buffer 0's slice: end offsets are monotone and
             // bounded by the slice; checked here on the host copy.
@@ -672,6 +695,25 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
                 }
                 break;
             }
+            case EXP_DICT_RLE: {  // Vec::Init split of the codes (src/cor/lyt/vec.cpp:72-91), then RLE DELTA_NO_PATCH
+                const uint32_t* c = codes.data() + v * 1024;
+                uint16_t        pos[1024], posT[1024], dl[1024];
+                uint64_t        r = 0;
+                for (unsigned i = 0; i < 1024; ++i) {
+                    if (i == 0 || c[i] != c[i - 1]) {
+                        if (i) r++;
+                        std::memcpy(b0.data() + r * 4, &c[i], 4);
+                    }
+                    pos[i] = (uint16_t)r;
+                }
+                parts[0] = {b0.data(), (r + 1) * 4};
+                transpose<uint16_t>(pos, posT);
+                std::memcpy(b1.data(), posT, 128);
+                parts[1] = {b1.data(), 128};
+                unrsum<uint16_t>(posT, dl);
+                ffor_parts(dl, 2, b4.data());
+                break;
+            }
             default: {
                 transpose<uint32_t>(codes.data() + v * 1024, ct);
                 if (e.var == 1) {
@@ -728,6 +770,13 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
             } else {
                 hzl_encode<uint64_t>(dvals, out);
             }
+        }
+        if (e.exp_t == EXP_DICT_RLE) {  // the page's run codes as one HZL chain (THzlCmp<idx_t>, hzl_cmp.cpp:43-176)
+            std::vector<uint32_t> rc(pg.bufs[0].b.size() / 4);
+            if (!rc.empty()) std::memcpy(rc.data(), pg.bufs[0].b.data(), rc.size() * 4);
+            Buf h;
+            hzl_encode<uint32_t>(rc, h);
+            pg.bufs[0].b.swap(h.b);
         }
         if (first_page && meta) *meta = out.b.size() + 8 + 4ull * e.arr_c * std::max(1u, pg.vec_c);
         first_page = false;

@@ -27,6 +27,7 @@ enum : uint8_t {
     EXP_DICT         = 8,
     EXP_DICT_FRQ     = 9,
     EXP_DICT_VAL     = 10,
+    EXP_DICT_RLE     = 11,
 };
 // data_t (include/fastlanes.h)
 enum : uint8_t {
@@ -49,7 +50,12 @@ constexpr unsigned kExcLimit      = 100;                      // CFG::CMPR::EXC_
 constexpr uint8_t kFforLL = 2, kFforS = 3, kFforB = 4;
 
 unsigned dtype_bits(uint8_t dt);
-inline bool is_dict_exp(uint8_t e) { return e == EXP_DICT || e == EXP_DICT_FRQ || e == EXP_DICT_VAL; }
+inline bool is_dict_exp(uint8_t e) {
+    return e == EXP_DICT || e == EXP_DICT_FRQ || e == EXP_DICT_VAL || e == EXP_DICT_RLE;
+}
+// DICT_RLE var DELTA_NO_PATCH_REC (include/fls/cor/exp/dict_rle/dict_rle_exp.hpp:23-29): the only DICT_RLE variant
+// in the integer pools (src/cor/exp_pool.cpp:53, id 27)
+constexpr uint8_t kDictRleRec = 4;
 
 struct Expr {
     uint8_t exp_t, var, arr_c;
@@ -81,6 +87,9 @@ struct PageEntry {
     // dictionary (integer columns)
     uint32_t               dict_n = 0;
     std::vector<HzlChunkH> hzl;
+    // DICT_RLE: buffer 0 (the page's run codes, u32) is one HZL chain; its chunks and decoded code count
+    std::vector<HzlChunkH> run_hzl;
+    uint64_t               run_codes = 0;
     // FSST (string columns)
     FsstTableH fsst;
 };

@@ -32,7 +32,22 @@ struct DevDict {
     HzlChunk* chunks = nullptr;
     uint32_t  n_chunks = 0;
     bool      exc      = false;  // some chunk carries exceptions
+    // DICT_RLE: the page's run codes (HZL chunks), decoded codes and their dictionary values
+    HzlChunk* run_chunks   = nullptr;
+    uint32_t  n_run_chunks = 0;
+    bool      run_exc      = false;
+    uint32_t* codes        = nullptr;
+    uint8_t*  run_values   = nullptr;
 };
+
+namespace {
+std::vector<HzlChunk> device_chunks(const std::vector<HzlChunkH>& h, bool& exc) {
+    std::vector<HzlChunk> ch(h.size());
+    for (size_t k = 0; k < ch.size(); ++k) ch[k] = {h[k].src, h[k].dst, h[k].base, h[k].bw, h[k].exc_c, h[k].n, 0};
+    for (const auto& c : h) exc |= c.exc_c != 0;
+    return ch;
+}
+}  // namespace
 
 struct fls_gpu_plan {
     Column               col;
@@ -59,6 +74,24 @@ int plan_decode_dictionaries(fls_gpu_plan* p, const uint8_t* dev_encoded, hipStr
 }
 }  // namespace fls
 
+namespace {
+// DICT_RLE page: dictionary and run codes from their HZL chains, run values = dict[code], then the RLE kernel
+// (positions DELTA_NO_PATCH) gathers from those values.
+hipError_t drle_page(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded, const uint8_t* page, const PageMeta& pm,
+                     uint8_t* out, const SelArgs* sel, hipStream_t s) {
+    const PageEntry& pe = plan->col.pages[i];
+    DevDict&         d  = plan->dicts[i];
+    if (!pe.vec_c) return hipSuccess;
+    hipError_t e = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, d.exc, s);
+    if (e == hipSuccess) e = launch_hzl(32, dev_encoded, d.run_chunks, d.n_run_chunks, (uint8_t*)d.codes, d.run_exc, s);
+    if (e == hipSuccess)
+        e = launch_drle_values(plan->W, d.codes, pe.run_codes, d.values, pe.dict_n, d.run_values, plan->d_err, s);
+    const RunSrc rs {d.run_values, (uint32_t)(4 * pe.run_codes), 2u};
+    if (e == hipSuccess) e = launch_page_rle(plan->W, 2, page, pm, pe.vec_c, out, plan->d_err, s, sel, &rs);
+    return e;
+}
+}  // namespace
+
 extern "C" {
 
 const char* fls_gpu_error_string(void) { return g_err.c_str(); }
@@ -79,17 +112,25 @@ int fls_gpu_plan_create(const uint8_t* host_encoded, uint64_t encoded_size, enum
     for (size_t i = 0; i < plan->col.pages.size(); ++i) {
         const PageEntry& pe = plan->col.pages[i];
         if (pe.hzl.empty()) continue;
-        DevDict&              d = plan->dicts[i];
-        std::vector<HzlChunk> ch(pe.hzl.size());
-        for (size_t k = 0; k < ch.size(); ++k)
-            ch[k] = {pe.hzl[k].src, pe.hzl[k].dst, pe.hzl[k].base, pe.hzl[k].bw, pe.hzl[k].exc_c, pe.hzl[k].n, 0};
-        for (const auto& c : pe.hzl) d.exc |= c.exc_c != 0;
-        d.n_chunks          = (uint32_t)ch.size();
+        DevDict&              d  = plan->dicts[i];
+        std::vector<HzlChunk> ch = device_chunks(pe.hzl, d.exc);
+        d.n_chunks               = (uint32_t)ch.size();
         const uint64_t vals = (uint64_t)pe.dict_n + 1024;  // chunk writes are whole 1024-value vectors
         if ((e = hipMalloc(&d.values, vals * (plan->DW / 8))) != hipSuccess) return hip_fail(e, "hipMalloc");
         if ((e = hipMalloc(&d.chunks, ch.size() * sizeof(HzlChunk))) != hipSuccess) return hip_fail(e, "hipMalloc");
         if ((e = hipMemcpy(d.chunks, ch.data(), ch.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
             return hip_fail(e, "hipMemcpy");
+        if (pe.exp_t == EXP_DICT_RLE && !pe.run_hzl.empty()) {
+            std::vector<HzlChunk> rc = device_chunks(pe.run_hzl, d.run_exc);
+            d.n_run_chunks           = (uint32_t)rc.size();
+            if ((e = hipMalloc(&d.run_chunks, rc.size() * sizeof(HzlChunk))) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipMemcpy(d.run_chunks, rc.data(), rc.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
+                return hip_fail(e, "hipMemcpy");
+            // chunk writes are whole 1024-code vectors
+            if ((e = hipMalloc(&d.codes, (pe.run_codes + 1024) * 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipMalloc(&d.run_values, pe.run_codes * (plan->W / 8) + 16)) != hipSuccess)
+                return hip_fail(e, "hipMalloc");
+        }
     }
     if (type == STR) {
         int rc = strings_prepare(plan->col, host_encoded, plan->strings);
@@ -104,6 +145,9 @@ void fls_gpu_plan_destroy(fls_gpu_plan* plan) {
     for (auto& d : plan->dicts) {
         if (d.values) (void)hipFree(d.values);
         if (d.chunks) (void)hipFree(d.chunks);
+        if (d.run_chunks) (void)hipFree(d.run_chunks);
+        if (d.codes) (void)hipFree(d.codes);
+        if (d.run_values) (void)hipFree(d.run_values);
     }
     strings_release(plan->strings);
     if (plan->d_err) (void)hipFree(plan->d_err);
@@ -138,6 +182,7 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
             break;
         case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, out, s); break;
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s); break;
+        case EXP_DICT_RLE: e = drle_page(plan, i, dev_encoded, page, pm, out, nullptr, s); break;
         default: {
             DevDict&     d = plan->dicts[i];
             const HzlSrc hz {dev_encoded, d.chunks, d.n_chunks, d.exc ? 1u : 0u};
@@ -206,6 +251,7 @@ int fls_gpu_plan_select(fls_gpu_plan* plan, const uint8_t* dev_encoded, int op, 
             break;
         case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, nullptr, s, &ps); break;
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps); break;
+        case EXP_DICT_RLE: e = drle_page(plan, i, dev_encoded, page, pm, nullptr, &ps, s); break;
         default: {
             DevDict&     d = plan->dicts[i];
             const HzlSrc hz {dev_encoded, d.chunks, d.n_chunks, d.exc ? 1u : 0u};

@@ -90,15 +90,41 @@ hipError_t launch_page_patch(unsigned W, int var, const uint8_t* page, const Pag
 }
 
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                           uint32_t* err, hipStream_t s, const SelArgs* sel) {
+                           uint32_t* err, hipStream_t s, const SelArgs* sel, const RunSrc* runs) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_rle_w<8>(var, page, pm, vec_c, out, err, sel, s);
-    case 16: return page_rle_w<16>(var, page, pm, vec_c, out, err, sel, s);
-    case 32: return page_rle_w<32>(var, page, pm, vec_c, out, err, sel, s);
-    case 64: return page_rle_w<64>(var, page, pm, vec_c, out, err, sel, s);
+    case 8: return page_rle_w<8>(var, page, pm, vec_c, out, err, sel, runs, s);
+    case 16: return page_rle_w<16>(var, page, pm, vec_c, out, err, sel, runs, s);
+    case 32: return page_rle_w<32>(var, page, pm, vec_c, out, err, sel, runs, s);
+    case 64: return page_rle_w<64>(var, page, pm, vec_c, out, err, sel, runs, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_drle_values(unsigned W, const uint32_t* codes, uint64_t n, const void* dict, uint32_t dict_n,
+                              uint8_t* rv, uint32_t* err, hipStream_t s) {
+    if (!n) return hipSuccess;
+    const uint64_t blocks = (n + 255) / 256;
+    const dim3     g((unsigned)(blocks < 4096 ? blocks : 4096)), b(256);
+    const auto*    d32 = static_cast<const uint32_t*>(dict);
+    const auto*    d64 = static_cast<const uint64_t*>(dict);
+    switch (W) {
+    case 8: hipLaunchKernelGGL((k_drle_values<uint8_t, uint32_t>), g, b, 0, s, codes, n, d32, dict_n, rv, err); break;
+    case 16:
+        hipLaunchKernelGGL((k_drle_values<uint16_t, uint32_t>), g, b, 0, s, codes, n, d32, dict_n,
+                           reinterpret_cast<uint16_t*>(rv), err);
+        break;
+    case 32:
+        hipLaunchKernelGGL((k_drle_values<uint32_t, uint32_t>), g, b, 0, s, codes, n, d32, dict_n,
+                           reinterpret_cast<uint32_t*>(rv), err);
+        break;
+    case 64:
+        hipLaunchKernelGGL((k_drle_values<uint64_t, uint64_t>), g, b, 0, s, codes, n, d64, dict_n,
+                           reinterpret_cast<uint64_t*>(rv), err);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,

@@ -9,6 +9,7 @@ struct HzlChunk;
 struct PageMeta;
 struct SelArgs;
 struct HzlSrc;
+struct RunSrc;
 
 // Kernel families with separately tuned launch shapes.
 enum Family : int {
@@ -45,8 +46,12 @@ hipError_t launch_unffor_stream(unsigned W, unsigned bw, const uint8_t* packed, 
                                 uint64_t n_vec, hipStream_t s);
 hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                            hipStream_t s, const SelArgs* sel = nullptr);
+// runs: where the run values are (DICT_RLE pages); nullptr: the page's buffer 0
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                           uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr);
+                           uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr, const RunSrc* runs = nullptr);
+// DICT_RLE: rv[i] = dict[codes[i]] (W-bit values from a max(32, W)-bit dictionary)
+hipError_t launch_drle_values(unsigned W, const uint32_t* codes, uint64_t n, const void* dict, uint32_t dict_n,
+                              uint8_t* rv, uint32_t* err, hipStream_t s);
 // hz: the page's HZL dictionary payload; the launcher decodes it into `dict` (or, when it fits LDS, lets the page
 // kernel decode it per block). nullptr: `dict` already holds the decoded dictionary.
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,
@@ -66,7 +71,7 @@ hipError_t page_patch_w(int var, const uint8_t* page, const PageMeta& pm, uint32
                         const SelArgs* sel, hipStream_t s);
 template <int W>
 hipError_t page_rle_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err,
-                      const SelArgs* sel, hipStream_t s);
+                      const SelArgs* sel, const RunSrc* runs, hipStream_t s);
 template <int W>
 hipError_t page_dict_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, const void* dict,
                        uint32_t dict_n, uint8_t* out, uint32_t* err, const SelArgs* sel, const HzlSrc* hz,

@@ -573,12 +573,21 @@ struct RleMeta {
     uint32_t r0, r1, lb, bw, base, ep;
 };
 
+// Where a page's run values are: `base` holds them (T-typed), entry points of buffer 0 count 2^esh bytes per run and
+// `bytes` is buffer 0's extent in those units. RLE pages: the page's own buffer 0 (esh = log2 sizeof(T)). DICT_RLE
+// pages: run values looked up from the page's decoded run codes beforehand (esh = 2, u32 codes).
+struct RunSrc {
+    const uint8_t* base;
+    uint32_t       bytes, esh;
+};
+
 template <int VAR>
-__device__ __forceinline__ RleMeta rle_meta(const uint8_t* page, const PageMeta& pm, uint64_t v, uint64_t N, unsigned t) {
+__device__ __forceinline__ RleMeta rle_meta(const uint8_t* page, const PageMeta& pm, uint64_t v, uint64_t N, unsigned t,
+                                            uint32_t run_bytes) {
     const u32_ua* eps = reinterpret_cast<const u32_ua*>(page + 8 + 4 * pm.arr_c);
     RleMeta       m {};
     m.r0 = v ? eps[(v - 1) * pm.arr_c] : 0u;
-    m.r1 = v + 1 < N ? eps[v * pm.arr_c] : pm.off[1] - pm.off[0];
+    m.r1 = v + 1 < N ? eps[v * pm.arr_c] : run_bytes;
     if constexpr (VAR == 1) {
         m.ep = pm.fixed ? (uint32_t)(v * 2048) : (v ? eps[(v - 1) * pm.arr_c + 1] : 0u);
     } else {
@@ -603,7 +612,8 @@ __device__ __forceinline__ RleMeta rle_meta(const uint8_t* page, const PageMeta&
 // straight to HBM, and LDS holds only the run values.
 template <typename T, int VAR, bool NT, bool SEL = false, bool DIRECT = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restrict__ page, PageMeta pm, uint32_t vec_c,
-                                                        uint8_t* __restrict__ out, uint32_t* err, SelArgs sa) {
+                                                        uint8_t* __restrict__ out, uint32_t* err, SelArgs sa,
+                                                        RunSrc rs) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     using WS              = WaveStage<32, 1024 * sizeof(T)>;
     constexpr unsigned RB = kRleLds * sizeof(T);
@@ -618,11 +628,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
     const uint64_t     stride = (uint64_t)gridDim.x * waves * WS::VPW;
     uint64_t           v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
     RleMeta            m {};
-    if (v0 + slot < N) m = rle_meta<VAR>(page, pm, v0 + slot, N, t);
+    if (v0 + slot < N) m = rle_meta<VAR>(page, pm, v0 + slot, N, t, rs.bytes);
     for (; v0 < N; v0 += stride) {
         const uint64_t v      = v0 + slot;
-        const uint8_t* runs   = page + pm.off[0] + m.r0;
-        const uint32_t n_runs = (uint32_t)((m.r1 - m.r0) / sizeof(T));
+        const uint8_t* runs   = rs.base + (uint64_t)(m.r0 >> rs.esh) * sizeof(T);
+        const uint32_t n_runs = (m.r1 - m.r0) >> rs.esh;
         if (v < N && n_runs <= kRleLds)
             for (uint32_t i = t; i < n_runs; i += 32) lruns[i] = load_t<T>(runs + (uint64_t)i * sizeof(T));
         wave_sync();
@@ -645,7 +655,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
             else if (n_runs <= kRleLds) positions(static_cast<const T*>(lruns));
             else positions(runs);
         }
-        if (v + stride < N) m = rle_meta<VAR>(page, pm, v + stride, N, t);
+        if (v + stride < N) m = rle_meta<VAR>(page, pm, v + stride, N, t, rs.bytes);
         wave_sync();
         if constexpr (!DIRECT) {
             const uint64_t left = N - v0;
@@ -654,6 +664,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
             wave_sync();
         }
     }
+}
+
+// DICT_RLE run values: rv[i] = dict[codes[i]] over a page's decoded run codes, so that the RLE kernel gathers
+// values (include/fls/encoder/assert_eq.hpp:329-383 flattens the codes first and looks them up after; the lookup
+// commutes with the gather). Codes >= dict_n are clamped (err bit 2).
+template <typename T, typename D>
+__global__ __launch_bounds__(256) void k_drle_values(const uint32_t* __restrict__ codes, uint64_t n,
+                                                     const D* __restrict__ dict, uint32_t dict_n, T* __restrict__ rv,
+                                                     uint32_t* err) {
+    if (dict_n == 0) return;  // the host rejects such pages
+    bool bad = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const uint32_t c = codes[i];
+        bad |= c >= dict_n;
+        rv[i] = (T)dict[c < dict_n ? c : dict_n - 1];
+    }
+    if (bad) atomicOr(err, 2u);
 }
 
 // ---------------------------------------------------------------------------------------------------------------

@@ -330,3 +330,11 @@ def spell_edge_tables():
     yield ("one_row", [np.array([3], np.int64), np.array([4.0]), _strs([b"x"])],
            ["CONSTANT", "CONSTANT", "CONSTANT"], [-1, -1, -1])
     yield ("empty", [np.zeros(0, np.int64), np.zeros(0, np.uint8)], ["CONSTANT", "CONSTANT"], [-1, -1])
+
+
+def dict_runs(n: int, dtype=np.int64, n_keys: int = 300, mean: float = 12.0, seed: int = 6) -> np.ndarray:
+    """Runs (Geometric(1/mean) lengths) over n_keys random keys of the type: the shape DICT_RLE compresses."""
+    info = np.iinfo(dtype)
+    keys = rng(seed).integers(info.min, info.max, n_keys, dtype=dtype, endpoint=True)
+    idx = geometric_runs(n, mean, seed=seed + 1, value_bits=16, dtype=np.int64) % n_keys
+    return keys[idx]

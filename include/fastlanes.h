@@ -50,7 +50,8 @@ enum data_t
 
 /* i64 expression ids; reference include/fastlanes.h:37-41 (ids index src/cor/exp_pool.cpp:23-65). The same ids
  * select the same expressions for every integer type here; the library also accepts 6 (RLE+DELTA), 7 (RLE),
- * 16/17/22 (DICT, DICT+FFOR, DICT+DELTA) and 28/29 (DICT_FRQ, DICT_FRQ+FFOR). */
+ * 16/17/22 (DICT, DICT+FFOR, DICT+DELTA), 27 (DICT_RLE: run codes compressed page-wide, RLE+DELTA positions) and
+ * 28/29 (DICT_FRQ, DICT_FRQ+FFOR). init_random_access reports no buffers for a DICT_RLE page. */
 enum i64_expr_encoding_t
 #ifdef __cplusplus
     : uint8_t

@@ -23,6 +23,10 @@
  *   HZL decode                 closed form of src/cor/eng/hzl_de_cmp.cpp:28-77 (rec_hdr chain + exceptions + tail),
  *                              round trip; the encoder's exception search is simplified (no exceptions) -- the
  *                              decoder handles exceptions and is tested on hand-built exception chunks.
+ *   DICT_RLE pages             parity unpinned against the reference: no reference test or fixture holds a
+ *                              DICT_RLE page. Checked by round trips on run-structured inputs, by a hand-computed
+ *                              page (run codes, positions and HZL header) and by byte equality with the library's
+ *                              own encoder; built from the same RLE and HZL steps the pinned rows above cover.
  *   FSST decode                byte-identical to the compiled reference fsst_decompress on tables and strings
  *                              produced by the compiled reference fsst_create/fsst_compress (tests/golden fixtures),
  *                              and round trip with this file's own table builder.
@@ -210,7 +214,10 @@ enum { DT_DOUBLE = 1, DT_I8 = 2, DT_I16 = 3, DT_I32 = 4, DT_I64 = 5, DT_U8 = 6, 
        DT_STR = 10 };
 /* ExpT (include/fls/cor/exp/exp_type.hpp:7-28) */
 enum { EXP_UNCOMPRESSED = 2, EXP_FFOR = 4, EXP_DELTA = 5, EXP_RLE = 6, EXP_FSST = 7, EXP_DICT = 8, EXP_DICT_FRQ = 9,
-       EXP_DICT_VAL = 10 };
+       EXP_DICT_VAL = 10, EXP_DICT_RLE = 11 };
+/* DICT_RLE var DELTA_NO_PATCH_REC (include/fls/cor/exp/dict_rle/dict_rle_exp.hpp:23-29, RPN :80-90): the run codes */
+/* (u32) and RLE DELTA_NO_PATCH positions; the page's run-code buffer is one HZL chain (is_compressed, RPN :87) */
+#define DICT_RLE_REC 4
 
 static unsigned dtype_bits(uint8_t dt) {
     switch (dt) {
@@ -252,6 +259,7 @@ static int expr_lookup(uint8_t dtype, uint8_t id, expr_t* e) {
     case 16: *e = (expr_t){EXP_DICT, 1, 1, 1}; return 0;
     case 17: *e = (expr_t){EXP_DICT, 2, 3, 1}; return 0;
     case 22: *e = (expr_t){EXP_DICT, 7, 4, 1}; return 0;
+    case 27: *e = (expr_t){EXP_DICT_RLE, DICT_RLE_REC, 5, 1}; return dtype != DT_DOUBLE ? 0 : -1; /* exp_pool.cpp:53 */
     case 28: *e = (expr_t){EXP_DICT_FRQ, 1, 1, 1}; return 0;
     case 29: *e = (expr_t){EXP_DICT_FRQ, 2, 3, 1}; return 0;
     default: return -1;
@@ -276,6 +284,7 @@ static int expr_from_page(uint8_t exp_t, uint8_t var, expr_t* e) {
         if (var == 2) { *e = (expr_t){exp_t, var, 3, 1}; return 0; }
         if (var == 7) { *e = (expr_t){exp_t, var, 4, 1}; return 0; }
         return -1;
+    case EXP_DICT_RLE: *e = (expr_t){exp_t, var, 5, 1}; return var == DICT_RLE_REC ? 0 : -1;
     case EXP_FSST: *e = (expr_t){exp_t, var, 2, 1}; return var == 1 ? 0 : -1;
     default: return -1;
     }
@@ -658,6 +667,24 @@ static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint
         }
         break;
     }
+    case EXP_DICT_RLE: { /* Vec::Init's run split of the codes (src/cor/lyt/vec.cpp:72-91), then RLE positions */
+        uint16_t pos[1024], posT[1024], dl[1024];
+        uint64_t r = 0;
+        for (unsigned i = 0; i < 1024; ++i) {
+            if (i == 0 || codes[i] != codes[i - 1]) {
+                if (i) r++;
+                st(32, vp->b0, r, codes[i]);
+            }
+            pos[i] = (uint16_t)r;
+        }
+        vp->parts[0] = vp->b0, vp->s[0] = (r + 1) * 4;
+        orc_transpose(16, pos, posT);
+        memcpy(vp->b1, posT, 128);
+        vp->parts[1] = vp->b1, vp->s[1] = 128;
+        orc_unrsum(16, posT, dl);
+        enc_ffor_parts(16, dl, vp, 2, bwb, baseb, vp->b4);
+        break;
+    }
     default: { /* dictionary: codes are u32 */
         orc_transpose(32, codes, t);
         if (e->var == 1) {
@@ -714,6 +741,15 @@ static int64_t encode_impl(uint8_t dtype, uint8_t expr_id, const void* in, uint6
         }
         int more = v < n_vec;
         if (!rc && e.dict) rc = dict_page_write(W, dvals, (uint64_t)ne, out);
+        if (!rc && e.exp_t == EXP_DICT_RLE) { /* THzlCmp<idx_t>::compress_buf + finalize_buf of buffer 0 */
+            uint64_t nc  = pg.bufs[0].n / 4;
+            int64_t  hsz = orc_hzl_encode(32, pg.bufs[0].p, nc, NULL);
+            buf_t    h   = {0};
+            h.p = (uint8_t*)malloc((size_t)hsz + 8), h.n = (uint64_t)hsz, h.cap = (uint64_t)hsz + 8;
+            orc_hzl_encode(32, pg.bufs[0].p, nc, h.p);
+            buf_free(&pg.bufs[0]);
+            pg.bufs[0] = h;
+        }
         if (!rc) rc = page_flush(&pg, e.exp_t, e.var, more, out);
         page_free(&pg);
         if (!more) break;
@@ -749,11 +785,14 @@ typedef struct {
     unsigned       vec_c, arr_c;
     uint32_t       off[MAX_ARR];
     const uint8_t* eps;
+    uint8_t*       runs;   /* DICT_RLE: decoded run codes of the page (u32) */
+    uint64_t       runs_n; /* their count */
 } page_r;
 
 static int page_open(const uint8_t* p, uint64_t avail, page_r* pg, expr_t* e) {
     if (avail < 8) return -1;
     pg->p     = p;
+    pg->runs  = NULL, pg->runs_n = 0;
     pg->exp_t = p[0], pg->var = p[1];
     pg->vec_c = (unsigned)p[4] | ((unsigned)p[5] << 8);
     pg->more  = p[6];
@@ -867,6 +906,25 @@ static int decode_vector(unsigned W, const page_r* pg, const expr_t* e, unsigned
         for (unsigned k = 0; k < 1024; ++k) memcpy(t + (size_t)k * sz, runs + (size_t)posT[k] * sz, sz); /* vec.cpp:318-332 */
         break;
     }
+    case EXP_DICT_RLE: { /* RLE positions, then dict[run code] (include/fls/encoder/assert_eq.hpp:329-383) */
+        uint16_t posT[1024], dl[1024];
+        uint32_t r0 = 0, r1;
+        if (v) memcpy(&r0, pg->eps + 4ULL * ((uint64_t)(v - 1) * pg->arr_c), 4);
+        if (v + 1 < pg->vec_c) memcpy(&r1, pg->eps + 4ULL * ((uint64_t)v * pg->arr_c), 4);
+        else r1 = (uint32_t)(pg->runs_n * 4);
+        if (r0 % 4 || r1 < r0 + 4 || r1 > pg->runs_n * 4) return -1;
+        if (unffor_parts(16, pg, v, 2, dl)) return -1;
+        orc_rsum(16, dl, posT, vbuf(pg, v, 1));
+        const uint32_t n_runs = (r1 - r0) / 4;
+        for (unsigned k = 0; k < 1024; ++k) {
+            if (posT[k] >= n_runs) return -1;
+            uint32_t code;
+            memcpy(&code, pg->runs + r0 + 4ULL * posT[k], 4);
+            if (code >= dict_n) return -1;
+            st(W, t, k, dict[code]);
+        }
+        break;
+    }
     default: { /* dictionary codes */
         uint32_t c[1024], cd[1024];
         if (e->var == 1) memcpy(c, vbuf(pg, v, 0), 4096);
@@ -913,7 +971,7 @@ int64_t orc_decode_column(uint8_t dtype, const uint8_t* in, uint64_t in_size, vo
         uint64_t  dn   = 0;
         if (pos >= in_size) return -1;
         uint8_t t0 = in[pos];
-        if (t0 == EXP_DICT || t0 == EXP_DICT_FRQ || t0 == EXP_DICT_VAL) {
+        if (t0 == EXP_DICT || t0 == EXP_DICT_FRQ || t0 == EXP_DICT_VAL || t0 == EXP_DICT_RLE) {
             int64_t c = dict_page_read(W, in + pos, in_size - pos, &dict, &dn);
             if (c < 0) return -1;
             pos += (uint64_t)c;
@@ -922,11 +980,19 @@ int64_t orc_decode_column(uint8_t dtype, const uint8_t* in, uint64_t in_size, vo
         expr_t e;
         if (page_open(in + pos, in_size - pos, &pg, &e)) { free(dict); return -1; }
         if ((n + 1024ULL * pg.vec_c) * (W / 8) > cap) { free(dict); return -2; }
+        if (e.exp_t == EXP_DICT_RLE && pg.vec_c) { /* THzlDeCmp::decompress_buf of buffer 0 (hzl_de_cmp.cpp:28-77) */
+            uint64_t bytes = pg.off[1] - pg.off[0], consumed = 0;
+            pg.runs        = (uint8_t*)malloc((bytes / 12 + 1) * 4096 + bytes + 8);
+            int64_t nr     = orc_hzl_decode(32, pg.p + pg.off[0], pg.runs, &consumed);
+            if (nr <= 0 || consumed != bytes) { free(pg.runs), free(dict); return -1; }
+            pg.runs_n = (uint64_t)nr;
+        }
         for (unsigned v = 0; v < pg.vec_c; ++v)
             if (decode_vector(W, &pg, &e, v, dict, dn, (uint8_t*)out + (n + 1024ULL * v) * (W / 8))) {
-                free(dict);
+                free(pg.runs), free(dict);
                 return -1;
             }
+        free(pg.runs);
         n += 1024ULL * pg.vec_c;
         free(dict);
         if (!pg.more) break;
