@@ -82,10 +82,16 @@ hipError_t drle_page(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded, c
     const PageEntry& pe = plan->col.pages[i];
     DevDict&         d  = plan->dicts[i];
     if (!pe.vec_c) return hipSuccess;
-    hipError_t e = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, d.exc, s);
-    if (e == hipSuccess) e = launch_hzl(32, dev_encoded, d.run_chunks, d.n_run_chunks, (uint8_t*)d.codes, d.run_exc, s);
-    if (e == hipSuccess)
-        e = launch_drle_values(plan->W, d.codes, pe.run_codes, d.values, pe.dict_n, d.run_values, plan->d_err, s);
+    hipError_t e = hipSuccess;
+    if (!d.exc && !d.run_exc) {  // one pass: codes and their values straight from the two chains
+        e = launch_drle_fused(plan->W, dev_encoded, d.run_chunks, d.n_run_chunks, d.chunks, d.n_chunks, pe.dict_n,
+                              d.run_values, plan->d_err, s);
+    } else {
+        e = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, d.exc, s);
+        if (e == hipSuccess) e = launch_hzl(32, dev_encoded, d.run_chunks, d.n_run_chunks, (uint8_t*)d.codes, d.run_exc, s);
+        if (e == hipSuccess)
+            e = launch_drle_values(plan->W, d.codes, pe.run_codes, d.values, pe.dict_n, d.run_values, plan->d_err, s);
+    }
     const RunSrc rs {d.run_values, (uint32_t)(4 * pe.run_codes), 2u};
     if (e == hipSuccess) e = launch_page_rle(plan->W, 2, page, pm, pe.vec_c, out, plan->d_err, s, sel, &rs);
     return e;

@@ -140,6 +140,24 @@ hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const Page
     }
 }
 
+hipError_t launch_drle_fused(unsigned W, const uint8_t* enc, const HzlChunk* runs, uint32_t n_runs,
+                             const HzlChunk* dict, uint32_t n_dict, uint32_t dict_n, uint8_t* rv, uint32_t* err,
+                             hipStream_t s) {
+    if (!n_runs) return hipSuccess;
+    const dim3 g(n_runs < 65535 ? n_runs : 65535), b(256);
+#define L(T, D) \
+    hipLaunchKernelGGL((k_drle_fused<T, D>), g, b, 0, s, enc, runs, n_runs, dict, n_dict, dict_n, reinterpret_cast<T*>(rv), err)
+    switch (W) {
+    case 8: L(uint8_t, uint32_t); break;
+    case 16: L(uint16_t, uint32_t); break;
+    case 32: L(uint32_t, uint32_t); break;
+    case 64: L(uint64_t, uint64_t); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef L
+    return hipGetLastError();
+}
+
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
                       bool exceptions, hipStream_t s) {
     if (!n_chunks) return hipSuccess;

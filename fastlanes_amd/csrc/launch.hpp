@@ -52,6 +52,10 @@ hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageM
 // DICT_RLE: rv[i] = dict[codes[i]] (W-bit values from a max(32, W)-bit dictionary)
 hipError_t launch_drle_values(unsigned W, const uint32_t* codes, uint64_t n, const void* dict, uint32_t dict_n,
                               uint8_t* rv, uint32_t* err, hipStream_t s);
+// DICT_RLE run values straight from the run-code and dictionary HZL chains (neither has exceptions)
+hipError_t launch_drle_fused(unsigned W, const uint8_t* enc, const HzlChunk* runs, uint32_t n_runs,
+                             const HzlChunk* dict, uint32_t n_dict, uint32_t dict_n, uint8_t* rv, uint32_t* err,
+                             hipStream_t s);
 // hz: the page's HZL dictionary payload; the launcher decodes it into `dict` (or, when it fits LDS, lets the page
 // kernel decode it per block). nullptr: `dict` already holds the decoded dictionary.
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,

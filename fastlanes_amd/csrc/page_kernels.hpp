@@ -790,6 +790,31 @@ struct HzlSrc {
     uint32_t        exc;     // some chunk carries exceptions (then the separate HZL kernels decode it)
 };
 
+// DICT_RLE run values in one pass over the page's run-code chunks: each code is unpacked from its HZL chunk and its
+// value read straight from the packed dictionary chunk that holds it (code / 1024; the tail chunk follows the full
+// ones). For chains without exceptions; codes >= dict_n are clamped (err bit 2).
+template <typename T, typename D>
+__global__ __launch_bounds__(256) void k_drle_fused(const uint8_t* __restrict__ enc, const HzlChunk* runs,
+                                                    uint32_t n_runs, const HzlChunk* dict, uint32_t n_dict,
+                                                    uint32_t dict_n, T* __restrict__ rv, uint32_t* err) {
+    if (dict_n == 0 || n_dict == 0) return;  // the host rejects such pages
+    bool bad = false;
+    for (uint32_t c = blockIdx.x; c < n_runs; c += gridDim.x) {
+        const HzlChunk ch  = runs[c];
+        const uint8_t* in  = enc + ch.src;
+        const uint32_t cnt = ch.bw == 255 ? ch.n : 1024u;
+        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+            uint32_t code = hzl_value<uint32_t>(in, ch, k);
+            bad |= code >= dict_n;
+            code              = code < dict_n ? code : dict_n - 1;
+            const uint32_t di = (code >> 10) < n_dict ? (code >> 10) : n_dict - 1;
+            const HzlChunk dc = dict[di];
+            rv[ch.dst + k]    = (T)hzl_value<D>(enc + dc.src, dc, (uint32_t)(code - dc.dst));
+        }
+    }
+    if (bad) atomicOr(err, 2u);
+}
+
 // Dictionary pages, pipelined like k_page_int. IN_LDS: the decoded dictionary is first copied into LDS (after the
 // waves' staging areas) and the code gathers read LDS instead of L2; the host picks it when the dictionary fits.
 // DIRECT (64-bit values): no output staging, segments stored straight to HBM (as k_page_rle); LDS holds only the

@@ -71,3 +71,40 @@ def test_out_of_range_run_code_is_reported(torch_cuda, oracle):
     plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
     with pytest.raises(fa.FlsError):
         plan.check()
+
+
+@pytest.mark.parametrize("dtype", [np.int64, np.uint32])
+def test_dictionary_with_hzl_exceptions(torch_cuda, oracle, dtype):
+    """A dictionary chain with exceptions (as the reference's HZL compressor writes them, src/cor/eng/hzl_cmp.cpp;
+    layout src/cor/eng/hzl_de_cmp.cpp:40-58) spliced before a DICT_RLE page: the dictionary is decoded by the
+    HZL kernels first, then the run values looked up."""
+    torch = torch_cuda
+    DW = 64 if np.dtype(dtype).itemsize == 8 else 32
+    ddt = np.uint64 if DW == 64 else np.uint32
+    K = 1000
+    v = (np.arange(1024, dtype=np.uint64) * K + 5).astype(dtype)
+    x = np.tile(v, 8)  # every value its own run; codes 0..1023 in every vector
+    enc = fa.encode_from_memory(x, 27)
+    bw0 = int(1023 * K).bit_length()
+    dict_len = 8 + 12 + 128 * bw0 + 12
+    assert enc[0] == 8 and enc[dict_len] == 11  # dictionary page, then the DICT_RLE page
+    want = (100 + np.arange(1024) % 8).astype(ddt)
+    packed = oracle.ffor(want, 3, ddt(100))
+    hdr = np.zeros(12, dtype=np.uint8)
+    hdr[0] = 3
+    hdr[2:4] = np.frombuffer(np.uint16(2).tobytes(), np.uint8)
+    hdr[4:12] = np.frombuffer(np.uint64(100).tobytes(), np.uint8)
+    pos = np.array([7, 1000], dtype=np.uint16)
+    exc = np.array([123456, 42], dtype=ddt)
+    tail = np.zeros(12, dtype=np.uint8)
+    tail[0] = 255
+    blob = np.concatenate([enc[:8], hdr, packed.view(np.uint8), pos.view(np.uint8), exc.view(np.uint8), tail,
+                           enc[dict_len:]])
+    want[7], want[1000] = 123456, 42
+    expect = np.tile(want.astype(dtype), 8)
+    np.testing.assert_array_equal(oracle.decode_column(blob, dtype, x.size), expect)
+    plan = fa.Plan(blob, dtype)
+    d_in, d_out = dev(torch, blob), empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    plan.check()
+    np.testing.assert_array_equal(to_np(d_out, dtype, x.size), expect)
