@@ -24,8 +24,32 @@ def main():
     ap.add_argument("--copy-log2-bytes", type=int, default=30)
     ap.add_argument("--fsst-log2-strings", type=int, default=0,
                     help="instead: decode an FSST string column of 2^k vocabulary strings (offsets + bytes)")
+    ap.add_argument("--dict-rle", action="store_true",
+                    help="instead: decode a DICT_RLE column (pool id 27) of 2^log2n values of --width bits, runs of "
+                         "mean length 16 over 4096 keys")
     a = ap.parse_args()
     import torch
+
+    if a.dict_rle:
+        import numpy as np
+
+        from fastlanes_amd import workloads as wl
+
+        dt = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[a.width]
+        n = 1 << a.log2n
+        x = wl.dict_runs(n, dt, n_keys=4096 if a.width > 8 else 200, mean=16.0, seed=2)
+        enc = fa.encode_from_memory(x, 27)
+        plan = fa.Plan(enc, dt)
+        d_in = torch.from_numpy(enc).cuda()
+        d_out = torch.empty(x.nbytes + 16, dtype=torch.uint8, device="cuda")
+        for _ in range(a.reps):
+            plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+        torch.cuda.synchronize()
+        plan.check()
+        ok = bool(np.array_equal(d_out[: x.nbytes].cpu().numpy().view(dt), x))
+        print(json.dumps({"values": n, "width": a.width, "encoded_bytes": int(enc.size), "pages": plan.n_pages,
+                          "reps": a.reps, "bit_exact": ok}), flush=True)
+        return
 
     if a.fsst_log2_strings:
         from fastlanes_amd import workloads as wl
