@@ -108,3 +108,12 @@ def test_dictionary_with_hzl_exceptions(torch_cuda, oracle, dtype):
     plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
     plan.check()
     np.testing.assert_array_equal(to_np(d_out, dtype, x.size), expect)
+
+
+def test_c_api_round_trip_host_pointers(torch_cuda):
+    """decode_to_memory (the reference's C entry point) on a DICT_RLE column in host memory."""
+    x = wl.dict_runs(8 * 1024, np.int64, seed=11)
+    enc = fa.encode_from_memory(x, 27)
+    out = np.zeros_like(x)
+    fa.decode_to_memory(enc.ctypes.data, out.ctypes.data, np.int64)
+    np.testing.assert_array_equal(out, x)

@@ -111,3 +111,15 @@ def test_rejected_for_doubles_and_bad_codes(oracle):
     bad[dsz + off[0] + 12: dsz + off[0] + 16] = np.frombuffer(np.uint32(9).tobytes(), np.uint8)  # code 9 of 2
     with pytest.raises(ValueError):
         oracle.decode_column(bad, np.int64, x.size)
+
+
+def test_random_access_reports_no_buffers():
+    """No vector owns a slice of the page-wide run-code chain: init_random_access reports zero buffers."""
+    import ctypes as C
+
+    x = wl.dict_runs(4 * 1024, np.int64, seed=12)
+    enc = fa.encode_from_memory(x, 27)
+    nb = C.c_uint64(99)
+    pairs = (C.c_uint64 * 16)()
+    fa.lib().init_random_access(enc.ctypes.data, 1, C.byref(nb), pairs)
+    assert nb.value == 0
