@@ -771,7 +771,7 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
                 hzl_encode<uint64_t>(dvals, out);
             }
         }
-        if (e.exp_t == EXP_DICT_RLE) {  // the page's run codes as one HZL chain (THzlCmp<idx_t>, hzl_cmp.cpp:43-176)
+        if (e.exp_t == EXP_DICT_RLE && pg.vec_c) {  // the page's run codes as one HZL chain (THzlCmp<idx_t>, hzl_cmp.cpp:43-176)
             std::vector<uint32_t> rc(pg.bufs[0].b.size() / 4);
             if (!rc.empty()) std::memcpy(rc.data(), pg.bufs[0].b.data(), rc.size() * 4);
             Buf h;

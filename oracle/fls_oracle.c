@@ -741,7 +741,7 @@ static int64_t encode_impl(uint8_t dtype, uint8_t expr_id, const void* in, uint6
         }
         int more = v < n_vec;
         if (!rc && e.dict) rc = dict_page_write(W, dvals, (uint64_t)ne, out);
-        if (!rc && e.exp_t == EXP_DICT_RLE) { /* THzlCmp<idx_t>::compress_buf + finalize_buf of buffer 0 */
+        if (!rc && e.exp_t == EXP_DICT_RLE && pg.vec_c) { /* THzlCmp<idx_t>::compress_buf + finalize_buf of buffer 0 */
             uint64_t nc  = pg.bufs[0].n / 4;
             int64_t  hsz = orc_hzl_encode(32, pg.bufs[0].p, nc, NULL);
             buf_t    h   = {0};

@@ -100,6 +100,15 @@ def test_every_value_its_own_run(oracle):
     np.testing.assert_array_equal(fa.encode_from_memory(x, 27), enc)
 
 
+def test_empty_column(oracle):
+    """No vectors: an empty dictionary page and a data page whose buffers are all empty."""
+    x = np.zeros(0, np.int64)
+    enc = oracle.encode_column(x, 27)
+    assert enc.size == 20 + 8 + 4 * 5  # dictionary page (tail chunk of 0 bytes), page header and offsets
+    np.testing.assert_array_equal(fa.encode_from_memory(x, 27), enc)
+    assert oracle.decode_column(enc, np.int64, 0).size == 0
+
+
 def test_rejected_for_doubles_and_bad_codes(oracle):
     with pytest.raises(ValueError):
         oracle.encode_column(np.zeros(1024, np.float64), 27)
