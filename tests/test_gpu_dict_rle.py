@@ -117,3 +117,16 @@ def test_c_api_round_trip_host_pointers(torch_cuda):
     out = np.zeros_like(x)
     fa.decode_to_memory(enc.ctypes.data, out.ctypes.data, np.int64)
     np.testing.assert_array_equal(out, x)
+
+
+def test_empty_column_and_encoded_size(torch_cuda):
+    """The plan's encoded size is the encoder's output size, for an empty column and a multi-chunk one."""
+    torch = torch_cuda
+    for x in (np.zeros(0, np.int64), wl.dict_runs(40 * 1024, np.int64, seed=13)):
+        enc = fa.encode_from_memory(x, 27)
+        plan = fa.Plan(enc, np.int64)
+        assert plan.n_values == x.size and plan.encoded_size == enc.size
+        d_in, d_out = dev(torch, enc), empty(torch, x.nbytes)
+        plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+        plan.check()
+        np.testing.assert_array_equal(to_np(d_out, np.int64, x.size), x)
