@@ -15,8 +15,9 @@ def test_reader_get_chunk(tmp_path):
     n = 40 * 1024
     cols = [wl.rowgroup_column("bitpack", 8, n, 1), wl.rowgroup_column("delta", 64, n, 2),
             wl.rowgroup_column("rle", 32, n, 3), wl.rowgroup_column("dict", 16, n, 4),
-            wl.geometric_runs(n, 6.0, seed=5, value_bits=10).astype(np.int64), np.linspace(0, 1, n)]
-    exprs = [1, 11, 6, 17, 7, 0]
+            wl.geometric_runs(n, 6.0, seed=5, value_bits=10).astype(np.int64), np.linspace(0, 1, n),
+            wl.dict_runs(n, np.uint16, seed=6)]
+    exprs = [1, 11, 6, 17, 7, 0, 27]
     Connection.to_fls(str(tmp_path), cols, exprs)
     rd = Connection().read_fls(str(tmp_path))
     assert rd.n_vec == 40
