@@ -131,6 +131,7 @@ def select_bench(dev, stream, log2n=26, verbose=True):
         "i64_timestamps_delta": (11, wl.timestamps(n)),
         "u32_runs_rle": (6, wl.geometric_runs(n, 16.0, seed=1, value_bits=20).astype(np.uint32)),
         "i32_zipf_dict": (17, wl.zipf_codes(n, 4096, 1.1, seed=5, dtype=np.int32)),
+        "i64_dict_runs_dict_rle": (27, wl.dict_runs(n, np.int64, n_keys=4096, mean=16.0, seed=2)),
     }
     out = {}
     sp = int(stream.cuda_stream)
