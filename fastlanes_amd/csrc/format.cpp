@@ -695,7 +695,7 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
                 }
                 break;
             }
-            case EXP_DICT_RLE: {  // Vec::Init split of the codes (src/cor/lyt/vec.cpp:72-91), then RLE DELTA_NO_PATCH
+            case EXP_DICT_RLE: {  // Vec::Init split of the codes (src/cor/lyt/vec.cpp:72-96), then RLE DELTA_NO_PATCH
                 const uint32_t* c = codes.data() + v * 1024;
                 uint16_t        pos[1024], posT[1024], dl[1024];
                 uint64_t        r = 0;

@@ -53,7 +53,7 @@ unsigned dtype_bits(uint8_t dt);
 inline bool is_dict_exp(uint8_t e) {
     return e == EXP_DICT || e == EXP_DICT_FRQ || e == EXP_DICT_VAL || e == EXP_DICT_RLE;
 }
-// DICT_RLE var DELTA_NO_PATCH_REC (include/fls/cor/exp/dict_rle/dict_rle_exp.hpp:23-29): the only DICT_RLE variant
+// DICT_RLE var DELTA_NO_PATCH_REC (include/fls/cor/exp/dict_rle/dict_rle_exp.hpp:24-30): the only DICT_RLE variant
 // in the integer pools (src/cor/exp_pool.cpp:53, id 27)
 constexpr uint8_t kDictRleRec = 4;
 

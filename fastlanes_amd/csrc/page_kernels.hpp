@@ -667,7 +667,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
 }
 
 // DICT_RLE run values: rv[i] = dict[codes[i]] over a page's decoded run codes, so that the RLE kernel gathers
-// values (include/fls/encoder/assert_eq.hpp:329-383 flattens the codes first and looks them up after; the lookup
+// values (include/fls/encoder/assert_eq.hpp:329-400 flattens the codes first and looks them up after; the lookup
 // commutes with the gather). Codes >= dict_n are clamped (err bit 2).
 template <typename T, typename D>
 __global__ __launch_bounds__(256) void k_drle_values(const uint32_t* __restrict__ codes, uint64_t n,

@@ -215,8 +215,8 @@ enum { DT_DOUBLE = 1, DT_I8 = 2, DT_I16 = 3, DT_I32 = 4, DT_I64 = 5, DT_U8 = 6, 
 /* ExpT (include/fls/cor/exp/exp_type.hpp:7-28) */
 enum { EXP_UNCOMPRESSED = 2, EXP_FFOR = 4, EXP_DELTA = 5, EXP_RLE = 6, EXP_FSST = 7, EXP_DICT = 8, EXP_DICT_FRQ = 9,
        EXP_DICT_VAL = 10, EXP_DICT_RLE = 11 };
-/* DICT_RLE var DELTA_NO_PATCH_REC (include/fls/cor/exp/dict_rle/dict_rle_exp.hpp:23-29, RPN :80-90): the run codes */
-/* (u32) and RLE DELTA_NO_PATCH positions; the page's run-code buffer is one HZL chain (is_compressed, RPN :87) */
+/* DICT_RLE var DELTA_NO_PATCH_REC (include/fls/cor/exp/dict_rle/dict_rle_exp.hpp:24-30, RPN :86-96): the run codes */
+/* (u32) and RLE DELTA_NO_PATCH positions; the page's run-code buffer is one HZL chain (is_compressed, RPN :93-95) */
 #define DICT_RLE_REC 4
 
 static unsigned dtype_bits(uint8_t dt) {
@@ -667,7 +667,7 @@ static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint
         }
         break;
     }
-    case EXP_DICT_RLE: { /* Vec::Init's run split of the codes (src/cor/lyt/vec.cpp:72-91), then RLE positions */
+    case EXP_DICT_RLE: { /* Vec::Init's run split of the codes (src/cor/lyt/vec.cpp:72-96), then RLE positions */
         uint16_t pos[1024], posT[1024], dl[1024];
         uint64_t r = 0;
         for (unsigned i = 0; i < 1024; ++i) {
@@ -906,7 +906,7 @@ static int decode_vector(unsigned W, const page_r* pg, const expr_t* e, unsigned
         for (unsigned k = 0; k < 1024; ++k) memcpy(t + (size_t)k * sz, runs + (size_t)posT[k] * sz, sz); /* vec.cpp:318-332 */
         break;
     }
-    case EXP_DICT_RLE: { /* RLE positions, then dict[run code] (include/fls/encoder/assert_eq.hpp:329-383) */
+    case EXP_DICT_RLE: { /* RLE positions, then dict[run code] (include/fls/encoder/assert_eq.hpp:329-400) */
         uint16_t posT[1024], dl[1024];
         uint32_t r0 = 0, r1;
         if (v) memcpy(&r0, pg->eps + 4ULL * ((uint64_t)(v - 1) * pg->arr_c), 4);
