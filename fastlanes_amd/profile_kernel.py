@@ -27,8 +27,30 @@ def main():
     ap.add_argument("--dict-rle", action="store_true",
                     help="instead: decode a DICT_RLE column (pool id 27) of 2^log2n values of --width bits, runs of "
                          "mean length 16 over 4096 keys")
+    ap.add_argument("--page-ffor", action="store_true",
+                    help="instead: decode an FFOR column (pool id 1) of 2^log2n values of --width bits, values in "
+                         "[0, 128) (bit width 7)")
     a = ap.parse_args()
     import torch
+
+    if a.page_ffor:
+        import numpy as np
+
+        dt = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[a.width]
+        n = 1 << a.log2n
+        x = np.random.default_rng(1).integers(0, 1 << min(a.width - 1, 7), n).astype(dt)
+        enc = fa.encode_from_memory(x, 1)
+        plan = fa.Plan(enc, dt)
+        d_in = torch.from_numpy(enc).cuda()
+        d_out = torch.empty(x.nbytes + 16, dtype=torch.uint8, device="cuda")
+        for _ in range(a.reps):
+            plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+        torch.cuda.synchronize()
+        plan.check()
+        ok = bool(np.array_equal(d_out[: x.nbytes].cpu().numpy().view(dt), x))
+        print(json.dumps({"values": n, "width": a.width, "encoded_bytes": int(enc.size), "pages": plan.n_pages,
+                          "reps": a.reps, "bit_exact": ok}), flush=True)
+        return
 
     if a.dict_rle:
         import numpy as np
