@@ -156,8 +156,8 @@ void init_random_access(uint8_t* in, uint64_t vec_id, uint64_t* n_buf, uint64_t*
     const uint8_t* pg  = in + pos;
     Expr           e;
     if (!expr_from_page(pg[0], pg[1], e)) return;
-    // DICT_RLE compresses its run codes page-wide (one HZL chain), so no vector owns a slice of that buffer
-    if (e.exp_t == EXP_DICT_RLE) return;
+    // DICT_RLE and S_PATCH_REC compress buffer 0 page-wide (one HZL chain): no vector owns a slice of it
+    if (e.exp_t == EXP_DICT_RLE || (e.exp_t == EXP_FFOR && e.var == kFforSRec)) return;
     const unsigned vec_c = (unsigned)pg[4] | ((unsigned)pg[5] << 8);
     if (vec_id >= vec_c) return;
     const unsigned v = (unsigned)vec_id;

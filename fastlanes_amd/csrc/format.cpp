@@ -31,6 +31,7 @@ bool expr_from_id(uint8_t dtype, uint8_t id, Expr& e) {
     case 0: e = {EXP_UNCOMPRESSED, 1, 1, false}; return true;
     case 1: e = {EXP_FFOR, 1, 3, false}; return true;
     case 2: e = {EXP_FFOR, kFforS, 5, false}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE;   // S_PATCH
+    case 3: e = {EXP_FFOR, kFforSRec, 5, false}; return dtype_bits(dtype) == 64 && dtype != DT_DOUBLE;  // S_PATCH_REC
     case 4: e = {EXP_FFOR, kFforB, 5, false}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE;   // B_PATCH
     case 5: e = {EXP_FFOR, kFforLL, 5, false}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE;  // LL_PATCH
     case 6: e = {EXP_RLE, 2, 5, false}; return true;
@@ -51,7 +52,7 @@ bool expr_from_page(uint8_t exp_t, uint8_t var, Expr& e) {
     case EXP_UNCOMPRESSED: e = {exp_t, var, 1, false}; return var == 1;
     case EXP_FFOR:
         if (var == 1) return e = {exp_t, var, 3, false}, true;
-        if (var == kFforLL || var == kFforS || var == kFforB) return e = {exp_t, var, 5, false}, true;
+        if (var == kFforLL || var == kFforS || var == kFforB || var == kFforSRec) return e = {exp_t, var, 5, false}, true;
         return false;
     case EXP_DELTA: e = {exp_t, var, 4, false}; return var == 1;
     case EXP_RLE:
@@ -170,18 +171,22 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
     // per-vector extents
     const unsigned sz = W / 8;
     auto           ep = [&](unsigned v, unsigned b) { return ep_of(pg, arr_c, v, b); };
-    // DICT_RLE: buffer 0 is an HZL chain of u32 run codes whose entry points address the decoded codes
-    const bool drle = e.exp_t == EXP_DICT_RLE;
+    // DICT_RLE / S_PATCH_REC: buffer 0 is an HZL chain (u32 run codes / u64 exception values) whose entry points
+    // address the decoded values
+    const bool     srec = e.exp_t == EXP_FFOR && e.var == kFforSRec;
+    const bool     drle = e.exp_t == EXP_DICT_RLE || srec;
+    const unsigned rdw  = srec ? 64 : 32;
+    if (srec && W != 64) return "S_PATCH_REC needs 64-bit values";
     if (drle) {
         uint64_t used = 0;
-        auto     err  = parse_hzl(r, off + boff[0], 32, ~0u, pe.run_hzl, used);
+        auto     err  = parse_hzl(r, off + boff[0], rdw, ~0u, pe.run_hzl, used);
         if (!err.empty()) return "run codes: " + err;
         if (used != boff[1] - boff[0]) return "run codes do not fill their buffer";
         for (const auto& c : pe.run_hzl) pe.run_codes += c.n;
-        if (4 * pe.run_codes > 0xffffffffull) return "run codes exceed 32-bit entry points";
+        if (rdw / 8 * pe.run_codes > 0xffffffffull) return "recursive buffer exceeds 32-bit entry points";
     }
     auto cap = [&](unsigned b) {
-        return drle && b == 0 ? 4 * pe.run_codes : (b + 1 < arr_c ? boff[b + 1] : avail) - boff[b];
+        return drle && b == 0 ? rdw / 8 * pe.run_codes : (b + 1 < arr_c ? boff[b + 1] : avail) - boff[b];
     };
     auto           need = [&](unsigned v, unsigned b, uint64_t n) -> bool {
         const uint64_t s = ep(v, b);
@@ -204,15 +209,16 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
                 if (!ok) break;
                 const uint8_t* b1  = pg + boff[1] + ep(v, 1);
                 uint32_t       exc = 0;
-                if (e.var == kFforB) {
+                const uint8_t  pv  = srec ? kFforS : e.var;
+                if (pv == kFforB) {
                     ok = need(v, 1, 128);
                     if (!ok) break;
                     for (unsigned k = 0; k < 128; ++k) exc += (uint32_t)__builtin_popcount(b1[k]);
                 } else {
-                    ok = need(v, 1, e.var == kFforS ? 2 : 4);
+                    ok = need(v, 1, pv == kFforS ? 2 : 4);
                     if (!ok) break;
                     exc = (uint32_t)b1[0] | ((uint32_t)b1[1] << 8);
-                    if (e.var == kFforS) {
+                    if (pv == kFforS) {
                         ok = exc <= 1024 && need(v, 1, 2 + 2ull * exc);
                         for (uint32_t k = 0; ok && k < exc; ++k) ok = ((uint32_t)b1[2 + 2 * k] | ((uint32_t)b1[3 + 2 * k] << 8)) < 1024;
                     } else {
@@ -273,7 +279,7 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
         case EXP_UNCOMPRESSED: fsz[0] = 1024 * sz; break;
         case EXP_FFOR:
             if (e.var == 1) fsz[0] = 1, fsz[1] = sz;
-            else fsz[2] = 1, fsz[3] = sz, fsz[1] = e.var == kFforB ? 128 : e.var == kFforLL ? 4 : 0;
+            else fsz[2] = 1, fsz[3] = sz, fsz[1] = e.var == kFforB ? 128 : e.var == kFforLL ? 4 : 0;  // S, S_REC: 0
             break;
         case EXP_DELTA: fsz[0] = 128, fsz[1] = 1, fsz[2] = sz; break;
         case EXP_RLE:
@@ -631,15 +637,16 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
                 if (e.var == 1) {
                     ffor_parts(t, 0, b2.data());
                 } else if constexpr (sizeof(T) >= 4) {
-                    const PatchPlan<T> pp = plan_patch<T>(t, e.var);
+                    const uint8_t      pv = e.var == kFforSRec ? kFforS : e.var;  // S_PATCH_REC vectors are S_PATCH
+                    const PatchPlan<T> pp = plan_patch<T>(t, pv);
                     const uint16_t     n  = (uint16_t)pp.pos.size();
                     for (uint16_t k = 0; k < n; ++k) std::memcpy(b0.data() + k * sizeof(T), &t[pp.pos[k]], sizeof(T));
                     parts[0] = {b0.data(), (uint64_t)n * sizeof(T)};
-                    if (e.var == kFforB) {
+                    if (pv == kFforB) {
                         std::memset(b1.data(), 0, 128);
                         for (uint16_t p : pp.pos) b1[p >> 3] |= (uint8_t)(1u << (p & 7));
                         parts[1] = {b1.data(), 128};
-                    } else if (e.var == kFforS) {
+                    } else if (pv == kFforS) {
                         std::memcpy(b1.data(), &n, 2);
                         for (uint16_t k = 0; k < n; ++k) std::memcpy(b1.data() + 2 + 2 * k, &pp.pos[k], 2);
                         parts[1] = {b1.data(), 2ull + 2ull * n};
@@ -651,7 +658,7 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
                     // S and B leave the exceptions in place (FFOR keeps their low bw bits of value - base); LL
                     // stores in each exception's slot the distance to the next one, minus one, plus base (the last
                     // slot's link is never followed: it gets distance 1)
-                    if (e.var == kFforLL)
+                    if (pv == kFforLL)
                         for (uint16_t k = 0; k < n; ++k) {
                             const uint64_t gap = k + 1 < n ? (uint64_t)(pp.pos[k + 1] - pp.pos[k]) : 1;
                             t[pp.pos[k]]       = (T)(pp.base + (T)(gap - 1));
@@ -776,6 +783,13 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
             if (!rc.empty()) std::memcpy(rc.data(), pg.bufs[0].b.data(), rc.size() * 4);
             Buf h;
             hzl_encode<uint32_t>(rc, h);
+            pg.bufs[0].b.swap(h.b);
+        }
+        if (e.exp_t == EXP_FFOR && e.var == kFforSRec && pg.vec_c) {  // exception values, THzlCmp<uint64_t>
+            std::vector<uint64_t> xv(pg.bufs[0].b.size() / 8);
+            if (!xv.empty()) std::memcpy(xv.data(), pg.bufs[0].b.data(), xv.size() * 8);
+            Buf h;
+            hzl_encode<uint64_t>(xv, h);
             pg.bufs[0].b.swap(h.b);
         }
         if (first_page && meta) *meta = out.b.size() + 8 + 4ull * e.arr_c * std::max(1u, pg.vec_c);

@@ -48,6 +48,9 @@ constexpr unsigned kMaxVecPerPage = 65535;                    // u16 PageHdr::ve
 constexpr unsigned kExcLimit      = 100;                      // CFG::CMPR::EXC_LIMIT_C, include/fls/cfg/cfg.hpp:67
 // FFOR variants (include/fls/cor/exp/ffor/ffor_exp.hpp var_t): 1 NO_PATCH, 2 LL_PATCH, 3 S_PATCH, 4 B_PATCH
 constexpr uint8_t kFforLL = 2, kFforS = 3, kFforB = 4;
+// S_PATCH_REC (var 5): S_PATCH whose exception values are one HZL chain of u64 values per page
+// (include/fls/cor/exp/ffor/ffor_exp.hpp:87-94, src/cor/eng/hzl_cmp_fac.cpp:24-31); 64-bit integer columns
+constexpr uint8_t kFforSRec = 5;
 
 unsigned dtype_bits(uint8_t dt);
 inline bool is_dict_exp(uint8_t e) {
@@ -87,7 +90,8 @@ struct PageEntry {
     // dictionary (integer columns)
     uint32_t               dict_n = 0;
     std::vector<HzlChunkH> hzl;
-    // DICT_RLE: buffer 0 (the page's run codes, u32) is one HZL chain; its chunks and decoded code count
+    // DICT_RLE / S_PATCH_REC: buffer 0 (the page's run codes, u32 / exception values, u64) is one HZL chain; its
+    // chunks and decoded value count
     std::vector<HzlChunkH> run_hzl;
     uint64_t               run_codes = 0;
     // FSST (string columns)

@@ -96,6 +96,17 @@ hipError_t drle_page(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded, c
     if (e == hipSuccess) e = launch_page_rle(plan->W, 2, page, pm, pe.vec_c, out, plan->d_err, s, sel, &rs);
     return e;
 }
+
+// S_PATCH_REC page: the exception values from the page's HZL chain, then the S_PATCH kernel reads them there.
+hipError_t srec_page(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded, const uint8_t* page, const PageMeta& pm,
+                     uint8_t* out, const SelArgs* sel, hipStream_t s) {
+    const PageEntry& pe = plan->col.pages[i];
+    DevDict&         d  = plan->dicts[i];
+    if (!pe.vec_c) return hipSuccess;
+    hipError_t e = launch_hzl(64, dev_encoded, d.run_chunks, d.n_run_chunks, d.run_values, d.run_exc, s);
+    if (e == hipSuccess) e = launch_page_patch(plan->W, kFforS, page, pm, pe.vec_c, out, plan->d_err, s, sel, d.run_values);
+    return e;
+}
 }  // namespace
 
 extern "C" {
@@ -117,25 +128,30 @@ int fls_gpu_plan_create(const uint8_t* host_encoded, uint64_t encoded_size, enum
     plan->dicts.resize(plan->col.pages.size());
     for (size_t i = 0; i < plan->col.pages.size(); ++i) {
         const PageEntry& pe = plan->col.pages[i];
-        if (pe.hzl.empty()) continue;
-        DevDict&              d  = plan->dicts[i];
-        std::vector<HzlChunk> ch = device_chunks(pe.hzl, d.exc);
-        d.n_chunks               = (uint32_t)ch.size();
-        const uint64_t vals = (uint64_t)pe.dict_n + 1024;  // chunk writes are whole 1024-value vectors
-        if ((e = hipMalloc(&d.values, vals * (plan->DW / 8))) != hipSuccess) return hip_fail(e, "hipMalloc");
-        if ((e = hipMalloc(&d.chunks, ch.size() * sizeof(HzlChunk))) != hipSuccess) return hip_fail(e, "hipMalloc");
-        if ((e = hipMemcpy(d.chunks, ch.data(), ch.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
-            return hip_fail(e, "hipMemcpy");
-        if (pe.exp_t == EXP_DICT_RLE && !pe.run_hzl.empty()) {
+        DevDict&         d  = plan->dicts[i];
+        if (!pe.hzl.empty()) {
+            std::vector<HzlChunk> ch = device_chunks(pe.hzl, d.exc);
+            d.n_chunks               = (uint32_t)ch.size();
+            const uint64_t vals = (uint64_t)pe.dict_n + 1024;  // chunk writes are whole 1024-value vectors
+            if ((e = hipMalloc(&d.values, vals * (plan->DW / 8))) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipMalloc(&d.chunks, ch.size() * sizeof(HzlChunk))) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipMemcpy(d.chunks, ch.data(), ch.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
+                return hip_fail(e, "hipMemcpy");
+        }
+        if (!pe.run_hzl.empty()) {  // DICT_RLE run codes / S_PATCH_REC exception values
             std::vector<HzlChunk> rc = device_chunks(pe.run_hzl, d.run_exc);
             d.n_run_chunks           = (uint32_t)rc.size();
             if ((e = hipMalloc(&d.run_chunks, rc.size() * sizeof(HzlChunk))) != hipSuccess) return hip_fail(e, "hipMalloc");
             if ((e = hipMemcpy(d.run_chunks, rc.data(), rc.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
                 return hip_fail(e, "hipMemcpy");
-            // chunk writes are whole 1024-code vectors
-            if ((e = hipMalloc(&d.codes, (pe.run_codes + 1024) * 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
-            if ((e = hipMalloc(&d.run_values, pe.run_codes * (plan->W / 8) + 16)) != hipSuccess)
-                return hip_fail(e, "hipMalloc");
+            if (pe.exp_t == EXP_DICT_RLE) {
+                // chunk writes are whole 1024-code vectors
+                if ((e = hipMalloc(&d.codes, (pe.run_codes + 1024) * 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
+                if ((e = hipMalloc(&d.run_values, pe.run_codes * (plan->W / 8) + 16)) != hipSuccess)
+                    return hip_fail(e, "hipMalloc");
+            } else if ((e = hipMalloc(&d.run_values, (pe.run_codes + 1024) * 8)) != hipSuccess) {
+                return hip_fail(e, "hipMalloc");  // S_PATCH_REC: decoded u64 exception values
+            }
         }
     }
     if (type == STR) {
@@ -183,8 +199,9 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
         switch (pe.exp_t) {
         case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pm, pe.vec_c, out, s); break;
         case EXP_FFOR:
-            e = pe.var == 1 ? launch_page_int(W, 1, page, pm, pe.vec_c, out, s)
-                            : launch_page_patch(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s);
+            e = pe.var == 1           ? launch_page_int(W, 1, page, pm, pe.vec_c, out, s)
+                : pe.var == kFforSRec ? srec_page(plan, i, dev_encoded, page, pm, out, nullptr, s)
+                                      : launch_page_patch(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s);
             break;
         case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, out, s); break;
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s); break;
@@ -252,8 +269,9 @@ int fls_gpu_plan_select(fls_gpu_plan* plan, const uint8_t* dev_encoded, int op, 
         switch (pe.exp_t) {
         case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pm, pe.vec_c, nullptr, s, &ps); break;
         case EXP_FFOR:
-            e = pe.var == 1 ? launch_page_int(W, 1, page, pm, pe.vec_c, nullptr, s, &ps)
-                            : launch_page_patch(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps);
+            e = pe.var == 1           ? launch_page_int(W, 1, page, pm, pe.vec_c, nullptr, s, &ps)
+                : pe.var == kFforSRec ? srec_page(plan, i, dev_encoded, page, pm, nullptr, &ps, s)
+                                      : launch_page_patch(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps);
             break;
         case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, nullptr, s, &ps); break;
         case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps); break;

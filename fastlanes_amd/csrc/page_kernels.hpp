@@ -469,7 +469,8 @@ __device__ __forceinline__ void apply_patches(const uint8_t* exc, const uint8_t*
 template <int W, int VAR, bool NT, bool SEL = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_patch(const uint8_t* __restrict__ page, PageMeta pm,
                                                           uint32_t vec_c, uint8_t* __restrict__ out,
-                                                          uint32_t* __restrict__ err, SelArgs sa) {
+                                                          uint32_t* __restrict__ err, SelArgs sa,
+                                                          const uint8_t* __restrict__ exc_base) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
     constexpr unsigned TPV = lane_tpv<W>(), VB = 128 * W;
     using WS               = WaveStage<TPV, VB>;
@@ -493,7 +494,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_patch(const uint8_t* __restr
         if (v + stride < N) m = patch_meta<W>(page, pm, v + stride);
         wave_sync();
         if (v < N)
-            apply_patches<W, VAR, TPV>(page + pm.off[0] + cur.ep_exc, page + pm.off[1] + cur.ep_pos, cur.base, t, vl,
+            apply_patches<W, VAR, TPV>(exc_base + cur.ep_exc, page + pm.off[1] + cur.ep_pos, cur.base, t, vl,
                                        err);
         wave_sync();
         const uint64_t left = N - v0;

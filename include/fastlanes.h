@@ -50,8 +50,9 @@ enum data_t
 
 /* i64 expression ids; reference include/fastlanes.h:37-41 (ids index src/cor/exp_pool.cpp:23-65). The same ids
  * select the same expressions for every integer type here; the library also accepts 6 (RLE+DELTA), 7 (RLE),
- * 16/17/22 (DICT, DICT+FFOR, DICT+DELTA), 27 (DICT_RLE: run codes compressed page-wide, RLE+DELTA positions) and
- * 28/29 (DICT_FRQ, DICT_FRQ+FFOR). init_random_access reports no buffers for a DICT_RLE page. */
+ * 16/17/22 (DICT, DICT+FFOR, DICT+DELTA), 27 (DICT_RLE: run codes compressed page-wide, RLE+DELTA positions),
+ * 28/29 (DICT_FRQ, DICT_FRQ+FFOR), 2/4/5 (FFOR S_PATCH, B_PATCH, LL_PATCH) and 3 (S_PATCH_REC: exception values
+ * compressed page-wide; 64-bit columns). init_random_access reports no buffers for DICT_RLE and S_PATCH_REC pages. */
 enum i64_expr_encoding_t
 #ifdef __cplusplus
     : uint8_t

@@ -238,7 +238,9 @@ typedef struct {
     uint8_t exp_t, var, arr_c, dict;
 } expr_t;
 /* FFOR var_t (include/fls/cor/exp/ffor/ffor_exp.hpp:24-30) */
-enum { FFOR_LL = 2, FFOR_S = 3, FFOR_B = 4 };
+enum { FFOR_LL = 2, FFOR_S = 3, FFOR_B = 4, FFOR_S_REC = 5 };
+/* S_PATCH_REC: S_PATCH with the page's exception values as one HZL chain of u64 values                          */
+/* (include/fls/cor/exp/ffor/ffor_exp.hpp:87-94, src/cor/eng/hzl_cmp_fac.cpp:24-31); 64-bit integer columns     */
 #define EXC_LIMIT 100 /* CFG::CMPR::EXC_LIMIT_C, include/fls/cfg/cfg.hpp */
 /* Pool ids and (ExpT, var) per src/cor/exp_pool.cpp:23-65 (i64) and :66-100 (str); arr_c from the RPN tables. */
 static int expr_lookup(uint8_t dtype, uint8_t id, expr_t* e) {
@@ -251,6 +253,7 @@ static int expr_lookup(uint8_t dtype, uint8_t id, expr_t* e) {
     case 1: *e = (expr_t){EXP_FFOR, 1, 3, 0}; return 0;
     /* patched FFOR, 32/64-bit integers only (ffor_patch_analyze resolves u32/u64, ffor_analyze.cpp:145-160) */
     case 2: *e = (expr_t){EXP_FFOR, FFOR_S, 5, 0}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE ? 0 : -1;
+    case 3: *e = (expr_t){EXP_FFOR, FFOR_S_REC, 5, 0}; return dtype_bits(dtype) == 64 && dtype != DT_DOUBLE ? 0 : -1;
     case 4: *e = (expr_t){EXP_FFOR, FFOR_B, 5, 0}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE ? 0 : -1;
     case 5: *e = (expr_t){EXP_FFOR, FFOR_LL, 5, 0}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE ? 0 : -1;
     case 6: *e = (expr_t){EXP_RLE, 2, 5, 0}; return 0;
@@ -270,7 +273,7 @@ static int expr_from_page(uint8_t exp_t, uint8_t var, expr_t* e) {
     case EXP_UNCOMPRESSED: *e = (expr_t){exp_t, var, 1, 0}; return var == 1 ? 0 : -1;
     case EXP_FFOR:
         if (var == 1) { *e = (expr_t){exp_t, var, 3, 0}; return 0; }
-        if (var == FFOR_LL || var == FFOR_S || var == FFOR_B) { *e = (expr_t){exp_t, var, 5, 0}; return 0; }
+        if (var == FFOR_LL || var == FFOR_S || var == FFOR_B || var == FFOR_S_REC) { *e = (expr_t){exp_t, var, 5, 0}; return 0; }
         return -1;
     case EXP_DELTA: *e = (expr_t){exp_t, var, 4, 0}; return var == 1 ? 0 : -1;
     case EXP_RLE:
@@ -634,7 +637,7 @@ static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint
     case EXP_FFOR:
         orc_transpose(W, vals, t);
         if (e->var == 1) enc_ffor_parts(W, t, vp, 0, bwb, baseb, vp->b2);
-        else enc_patch(W, e->var, t, vp, bwb, baseb);
+        else enc_patch(W, e->var == FFOR_S_REC ? FFOR_S : e->var, t, vp, bwb, baseb);
         break;
     case EXP_DELTA:
         orc_transpose(W, vals, t);
@@ -741,12 +744,15 @@ static int64_t encode_impl(uint8_t dtype, uint8_t expr_id, const void* in, uint6
         }
         int more = v < n_vec;
         if (!rc && e.dict) rc = dict_page_write(W, dvals, (uint64_t)ne, out);
-        if (!rc && e.exp_t == EXP_DICT_RLE && pg.vec_c) { /* THzlCmp<idx_t>::compress_buf + finalize_buf of buffer 0 */
-            uint64_t nc  = pg.bufs[0].n / 4;
-            int64_t  hsz = orc_hzl_encode(32, pg.bufs[0].p, nc, NULL);
+        if (!rc && (e.exp_t == EXP_DICT_RLE || (e.exp_t == EXP_FFOR && e.var == FFOR_S_REC)) && pg.vec_c) {
+            /* THzlCmp<idx_t> (DICT_RLE run codes) / THzlCmp<uint64_t> (exception values): compress_buf +
+             * finalize_buf of buffer 0 */
+            unsigned rw  = e.exp_t == EXP_DICT_RLE ? 32 : 64;
+            uint64_t nc  = pg.bufs[0].n / (rw / 8);
+            int64_t  hsz = orc_hzl_encode(rw, pg.bufs[0].p, nc, NULL);
             buf_t    h   = {0};
             h.p = (uint8_t*)malloc((size_t)hsz + 8), h.n = (uint64_t)hsz, h.cap = (uint64_t)hsz + 8;
-            orc_hzl_encode(32, pg.bufs[0].p, nc, h.p);
+            orc_hzl_encode(rw, pg.bufs[0].p, nc, h.p);
             buf_free(&pg.bufs[0]);
             pg.bufs[0] = h;
         }
@@ -843,11 +849,19 @@ static uint64_t page_end(unsigned W, const page_r* pg, const expr_t* e) {
 static int decode_patched(unsigned W, const page_r* pg, const expr_t* e, unsigned v, uint8_t* t) {
     if (unffor_parts(W, pg, v, 2, t)) return -1;
     const uint8_t* exc = vbuf(pg, v, 0);
+    uint8_t        var = e->var;
+    if (var == FFOR_S_REC) { /* exception values from the page's decoded chain, at the vector's entry point */
+        uint32_t x0 = 0;
+        if (v) memcpy(&x0, pg->eps + 4ULL * ((uint64_t)(v - 1) * pg->arr_c), 4);
+        if (x0 > pg->runs_n * 8) return -1;
+        exc = pg->runs + x0;
+        var = FFOR_S;
+    }
     const uint8_t* p1  = vbuf(pg, v, 1);
     const unsigned sz  = W / 8;
     uint64_t       base = 0, x = 0;
     memcpy(&base, vbuf(pg, v, 3), sz);
-    if (e->var == FFOR_S) {
+    if (var == FFOR_S) {
         uint16_t c, q;
         memcpy(&c, p1, 2);
         for (unsigned i = 0; i < c; ++i) {
@@ -855,7 +869,7 @@ static int decode_patched(unsigned W, const page_r* pg, const expr_t* e, unsigne
             if (q >= 1024) return -1;
             memcpy(&x, exc + (size_t)i * sz, sz), st(W, t, q, x);
         }
-    } else if (e->var == FFOR_B) {
+    } else if (var == FFOR_B) {
         unsigned c = 0;
         for (unsigned w = 0; w < 16; ++w) {
             uint64_t unit;
@@ -980,11 +994,13 @@ int64_t orc_decode_column(uint8_t dtype, const uint8_t* in, uint64_t in_size, vo
         expr_t e;
         if (page_open(in + pos, in_size - pos, &pg, &e)) { free(dict); return -1; }
         if ((n + 1024ULL * pg.vec_c) * (W / 8) > cap) { free(dict); return -2; }
-        if (e.exp_t == EXP_DICT_RLE && pg.vec_c) { /* THzlDeCmp::decompress_buf of buffer 0 (hzl_de_cmp.cpp:28-77) */
+        if ((e.exp_t == EXP_DICT_RLE || (e.exp_t == EXP_FFOR && e.var == FFOR_S_REC)) && pg.vec_c) {
+            /* THzlDeCmp::decompress_buf of buffer 0 (hzl_de_cmp.cpp:28-77) */
+            unsigned rw    = e.exp_t == EXP_DICT_RLE ? 32 : 64;
             uint64_t bytes = pg.off[1] - pg.off[0], consumed = 0;
-            pg.runs        = (uint8_t*)malloc((bytes / 12 + 1) * 4096 + bytes + 8);
-            int64_t nr     = orc_hzl_decode(32, pg.p + pg.off[0], pg.runs, &consumed);
-            if (nr <= 0 || consumed != bytes) { free(pg.runs), free(dict); return -1; }
+            pg.runs        = (uint8_t*)malloc((bytes / 12 + 1) * 1024 * (rw / 8) + bytes + 8);
+            int64_t nr     = orc_hzl_decode(rw, pg.p + pg.off[0], pg.runs, &consumed);
+            if (nr < 0 || consumed != bytes) { free(pg.runs), free(dict); return -1; }
             pg.runs_n = (uint64_t)nr;
         }
         for (unsigned v = 0; v < pg.vec_c; ++v)

@@ -59,7 +59,7 @@ def test_encode_errors():
     with pytest.raises(fa.FlsError):
         fa.encode_from_memory(np.arange(1000, dtype=np.int64), 1)  # not a multiple of 1024
     with pytest.raises(fa.FlsError):
-        fa.encode_from_memory(np.arange(1024, dtype=np.int64), 3)  # S_PATCH_REC: not supported
+        fa.encode_from_memory(np.arange(1024, dtype=np.uint32), 3)  # S_PATCH_REC: 64-bit only
     with pytest.raises(fa.FlsError):
         fa.encode_from_memory(np.arange(1024, dtype=np.uint16), 2)  # patched FFOR: 32/64-bit only
     with pytest.raises(fa.FlsError):

@@ -84,7 +84,7 @@ def test_rejects_ragged_and_unknown(oracle):
     with pytest.raises(ValueError):
         oracle.encode_column(np.arange(1000, dtype=np.int64), 1)
     with pytest.raises(ValueError):
-        oracle.encode_column(np.arange(1024, dtype=np.int64), 3)  # S_PATCH_REC is not supported
+        oracle.encode_column(np.arange(1024, dtype=np.uint16), 3)  # S_PATCH_REC exists for 64-bit values only
     for eid in (2, 4, 5):  # patched FFOR exists for 32- and 64-bit integers only
         with pytest.raises(ValueError):
             oracle.encode_column(np.arange(1024, dtype=np.uint16), eid)
