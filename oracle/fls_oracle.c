@@ -27,6 +27,9 @@
  *                              DICT_RLE page. Checked by round trips on run-structured inputs, by a hand-computed
  *                              page (run codes, positions and HZL header) and by byte equality with the library's
  *                              own encoder; built from the same RLE and HZL steps the pinned rows above cover.
+ *   S_PATCH_REC pages          parity unpinned against the reference (no fixture): identity with the pinned S_PATCH
+ *                              page (buffers 1-4, entry points; buffer 0 HZL-decodes to its exception values), round
+ *                              trips, a page without exceptions, byte equality with the library's encoder.
  *   FSST decode                byte-identical to the compiled reference fsst_decompress on tables and strings
  *                              produced by the compiled reference fsst_create/fsst_compress (tests/golden fixtures),
  *                              and round trip with this file's own table builder.
