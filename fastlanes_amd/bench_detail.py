@@ -74,6 +74,8 @@ def pages(dev, stream, log2n=26, verbose=True):
             body[hit] = rng.integers(0, np.iinfo(dt).max, int(hit.sum()), dtype=dt, endpoint=True)
             for name, eid in (("ffor_s_patch", 2), ("ffor_b_patch", 4), ("ffor_ll_patch", 5)):
                 cases[name] = (eid, body)
+            if W == 64:
+                cases["ffor_s_patch_rec"] = (3, body)
         for name, (eid, x) in cases.items():
             res[f"u{W}_{name}"] = decode_column_timed(x, eid, dev, stream)
             if verbose:

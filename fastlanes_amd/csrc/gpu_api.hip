@@ -41,10 +41,24 @@ struct DevDict {
 };
 
 namespace {
-std::vector<HzlChunk> device_chunks(const std::vector<HzlChunkH>& h, bool& exc) {
-    std::vector<HzlChunk> ch(h.size());
-    for (size_t k = 0; k < ch.size(); ++k) ch[k] = {h[k].src, h[k].dst, h[k].base, h[k].bw, h[k].exc_c, h[k].n, 0};
-    for (const auto& c : h) exc |= c.exc_c != 0;
+// Device chunk table of an HZL chain of dw-bit values. The raw tail is split into pieces of 1024 values, so that a
+// long tail (a chain whose values need the full width) is copied by many blocks; every entry keeps dst = 1024 * index.
+std::vector<HzlChunk> device_chunks(const std::vector<HzlChunkH>& h, bool& exc, unsigned dw) {
+    std::vector<HzlChunk> ch;
+    ch.reserve(h.size());
+    for (const auto& c : h) {
+        exc |= c.exc_c != 0;
+        if (c.bw != 255) {
+            ch.push_back({c.src, c.dst, c.base, c.bw, c.exc_c, c.n, 0});
+            continue;
+        }
+        uint32_t k = 0;
+        do {
+            const uint32_t n = c.n - k < 1024 ? c.n - k : 1024;
+            ch.push_back({c.src + (uint64_t)k * (dw / 8), c.dst + k, c.base, 255u, 0u, n, 0});
+            k += n;
+        } while (k < c.n);
+    }
     return ch;
 }
 }  // namespace
@@ -130,7 +144,7 @@ int fls_gpu_plan_create(const uint8_t* host_encoded, uint64_t encoded_size, enum
         const PageEntry& pe = plan->col.pages[i];
         DevDict&         d  = plan->dicts[i];
         if (!pe.hzl.empty()) {
-            std::vector<HzlChunk> ch = device_chunks(pe.hzl, d.exc);
+            std::vector<HzlChunk> ch = device_chunks(pe.hzl, d.exc, plan->DW);
             d.n_chunks               = (uint32_t)ch.size();
             const uint64_t vals = (uint64_t)pe.dict_n + 1024;  // chunk writes are whole 1024-value vectors
             if ((e = hipMalloc(&d.values, vals * (plan->DW / 8))) != hipSuccess) return hip_fail(e, "hipMalloc");
@@ -139,7 +153,7 @@ int fls_gpu_plan_create(const uint8_t* host_encoded, uint64_t encoded_size, enum
                 return hip_fail(e, "hipMemcpy");
         }
         if (!pe.run_hzl.empty()) {  // DICT_RLE run codes / S_PATCH_REC exception values
-            std::vector<HzlChunk> rc = device_chunks(pe.run_hzl, d.run_exc);
+            std::vector<HzlChunk> rc = device_chunks(pe.run_hzl, d.run_exc, pe.exp_t == EXP_DICT_RLE ? 32u : 64u);
             d.n_run_chunks           = (uint32_t)rc.size();
             if ((e = hipMalloc(&d.run_chunks, rc.size() * sizeof(HzlChunk))) != hipSuccess) return hip_fail(e, "hipMalloc");
             if ((e = hipMemcpy(d.run_chunks, rc.data(), rc.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
