@@ -23,13 +23,13 @@
  *   HZL decode                 closed form of src/cor/eng/hzl_de_cmp.cpp:28-77 (rec_hdr chain + exceptions + tail),
  *                              round trip; the encoder's exception search is simplified (no exceptions) -- the
  *                              decoder handles exceptions and is tested on hand-built exception chunks.
- *   DICT_RLE pages             parity unpinned against the reference: no reference test or fixture holds a
- *                              DICT_RLE page. Checked by round trips on run-structured inputs, by a hand-computed
- *                              page (run codes, positions and HZL header) and by byte equality with the library's
- *                              own encoder; built from the same RLE and HZL steps the pinned rows above cover.
- *   S_PATCH_REC pages          parity unpinned against the reference (no fixture): identity with the pinned S_PATCH
- *                              page (buffers 1-4, entry points; buffer 0 HZL-decodes to its exception values), round
- *                              trips, a page without exceptions, byte equality with the library's encoder.
+ *   DICT_RLE pages             recorded size 1915 B (i64_pt1vec.csv row 27) on the 133-run vector that pins the
+ *                              RLE/DICT rows; a hand-computed page (run codes, positions and HZL header); round
+ *                              trips on run-structured inputs; byte equality with the library's own encoder.
+ *   S_PATCH_REC pages          recorded sizes 997 B / 15652 B (i64_pt1vec.csv and i64_pt16vec.csv row 3) on the
+ *                              vector that pins S_PATCH; identity with the pinned S_PATCH page (buffers 1-4, entry
+ *                              points; buffer 0 HZL-decodes to its exception values), round trips, a page without
+ *                              exceptions, byte equality with the library's encoder.
  *   FSST decode                byte-identical to the compiled reference fsst_decompress on tables and strings
  *                              produced by the compiled reference fsst_create/fsst_compress (tests/golden fixtures),
  *                              and round trip with this file's own table builder.

@@ -45,11 +45,14 @@ def test_page_sizes_match_recorded(oracle, recorded, n_vec):
     runs = np.tile(_runs_vector(), n_vec)
     cases = {0: runs, 7: runs, 6: runs, 1: np.tile(_ffor16_vector(), n_vec), 11: np.tile(_delta17_vector(), n_vec)}
     if n_vec == 1:
-        cases.update({16: runs, 17: runs, 22: runs, 28: runs, 29: runs})
+        # DICT_RLE (27): the dictionary page of the run values plus the RLE page of their codes, with the run codes
+        # as one HZL chain; 1915 B at i64_pt1vec.csv row 27
+        cases.update({16: runs, 17: runs, 22: runs, 27: runs, 28: runs, 29: runs})
     for eid, x in cases.items():
         enc = oracle.encode_column(x, eid)
         assert enc.size == rows[eid], (eid, enc.size, rows[eid])
         np.testing.assert_array_equal(oracle.decode_column(enc, np.int64, x.size), x)
+        np.testing.assert_array_equal(fa.encode_from_memory(x, eid), enc)
 
 
 DTYPES = [np.uint8, np.uint16, np.uint32, np.uint64, np.int8, np.int16, np.int32, np.int64]
@@ -181,12 +184,14 @@ def _gen_5exp_vector(oracle):
 
 @pytest.mark.parametrize("n_vec", [1, 16])
 def test_patch_sizes_match_recorded(oracle, recorded, n_vec):
-    """Recorded sizes of test/local_result/col_result/generated/i64_pt{1,16}vec.csv rows 1, 2, 4, 5: they fix the
-    patch layouts (u16 count + positions; 128-byte bitmap; count + first position) and the chosen bw 7 with 5
-    exceptions (12 with LL's links), from both encoders, byte for byte the same."""
+    """Recorded sizes of test/local_result/col_result/generated/i64_pt{1,16}vec.csv rows 1-5: they fix the
+    patch layouts (u16 count + positions; 128-byte bitmap; count + first position), the chosen bw 7 with 5
+    exceptions (12 with LL's links), and S_PATCH_REC's page-wide HZL chain of the exception values (id 3: 997 and
+    15652 B, 12 B more than S_PATCH's 985 and 15640: the exception values become one raw HZL tail, a 12-byte
+    chunk header before the same value bytes), from both encoders, byte for byte the same."""
     rows = {r["id"]: r["size"] for r in recorded["i64_pt1vec.csv" if n_vec == 1 else "i64_pt16vec.csv"]}
     x = np.tile(_gen_5exp_vector(oracle), n_vec)
-    for eid in (1, 2, 4, 5):
+    for eid in (1, 2, 3, 4, 5):
         enc = oracle.encode_column(x, eid)
         assert enc.size == rows[eid], (eid, enc.size, rows[eid])
         np.testing.assert_array_equal(fa.encode_from_memory(x, eid), enc)
