@@ -61,6 +61,10 @@ def lib() -> C.CDLL:
     L.fls_gpu_rowgroup_decode.argtypes = [_vp, _vp, _vp, _int, _vp]
     L.fls_gpu_rowgroup_decode_graph.argtypes = [_vp, _vp, _vp, _vp]
     L.fls_gpu_rowgroup_check.argtypes = [_vp, _vp]
+    L.fls_gpu_tune_set.argtypes = [_int, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint]
+    L.fls_gpu_rowgroup_work_list.argtypes = [_vp, C.POINTER(_u32), _vp, _u32, C.POINTER(_u32), C.POINTER(_u32)]
+    L.fls_rowgroup_plan_work_list.argtypes = [_u32, C.POINTER(_vp), C.POINTER(_u64), C.POINTER(_u64),
+                                              C.POINTER(C.c_uint8), C.POINTER(_u32), _vp, _u32, C.POINTER(_u32)]
     # include/fastlanes.h
     L.fls_cpi_connection_open.restype = _vp
     L.fls_cpi_connection_close.argtypes = [_vp]
@@ -167,6 +171,35 @@ def selection_and(dev_a: int, dev_b: int, n_vec: int, dev_out: int, dev_counts: 
     check(lib().fls_gpu_selection_and(dev_a, dev_b, n_vec, dev_out, dev_counts, stream), "selection_and")
 
 
+# One tile of the persistent row-group launch (struct RgTile, fastlanes_amd/csrc/rowgroup.hpp): vectors
+# [v0, v0 + nv) of the page at byte `page` of the encoded buffer, decoded to byte `out` of the output buffer.
+RG_TILE = np.dtype([("page", "<u8"), ("out", "<u8"), ("dict", "<u8"), ("v0", "<u4"), ("dict_n", "<u4"),
+                    ("nv", "<u2"), ("kind", "<u2"), ("arr_c", "<u2"), ("pad", "<u2"), ("off", "<u4", (5,)),
+                    ("vec_c", "<u4")])
+assert RG_TILE.itemsize == 64
+# Tile kinds (enum RgKind): 4 element widths each of UNCOMPRESSED, FFOR, DELTA, RLE and dictionary codes
+# (UNCOMPRESSED, FFOR, DELTA).
+RG_KIND_NAMES = [f"{k}{w}" for k in ("unc", "ffor", "delta", "rle", "dict1_", "dict2_", "dict7_") for w in (8, 16, 32, 64)]
+
+
+def plan_work_list(host_columns, dtypes, device_offsets):
+    """The persistent row-group launch's work list planned on the host alone (fls_rowgroup_plan_work_list):
+    (tiles, LDS bytes per wave). Dictionary tiles carry a placeholder instead of a device address."""
+    n = len(host_columns)
+    cols = [np.ascontiguousarray(c, dtype=np.uint8) for c in host_columns]
+    ptrs = (_vp * max(n, 1))(*[c.ctypes.data for c in cols])
+    sizes = (_u64 * max(n, 1))(*[c.size for c in cols])
+    offs = (_u64 * max(n, 1))(*[int(o) for o in device_offsets])
+    types = (C.c_uint8 * max(n, 1))(*[dtype_code(d) for d in dtypes])
+    cnt, wlds = _u32(0), _u32(0)
+    check(lib().fls_rowgroup_plan_work_list(n, ptrs, sizes, offs, types, C.byref(cnt), None, 0, C.byref(wlds)),
+          "plan_work_list")
+    tiles = np.zeros(cnt.value, dtype=RG_TILE)
+    check(lib().fls_rowgroup_plan_work_list(n, ptrs, sizes, offs, types, C.byref(cnt), tiles.ctypes.data, cnt.value,
+                                            C.byref(wlds)), "plan_work_list")
+    return tiles, int(wlds.value)
+
+
 class RowGroup:
     """Many encoded columns decoded together (fls_gpu_rowgroup_*). The encoded columns live in one device buffer at
     `device_offsets`; decoded columns land in one output buffer at `offsets` (256-byte aligned)."""
@@ -189,13 +222,22 @@ class RowGroup:
         self.total_bytes = int(total.value)
 
     def decode(self, dev_encoded: int, dev_out: int, mode: int = 0, stream: int = 0) -> None:
-        """mode 1: one launch per page in column order; mode 2: the same launches over 4 parallel stream branches;
-        mode 0 (one persistent launch over the tile list) is disabled in this build."""
+        """mode 0: one persistent launch over the work list of (page, vector range) tiles; mode 1: one launch per page
+        in column order; mode 2: the same launches over 4 parallel stream branches."""
         check(lib().fls_gpu_rowgroup_decode(self.h, dev_encoded, dev_out, mode, stream), "rowgroup_decode")
 
     def decode_graph(self, dev_encoded: int, dev_out: int, stream: int = 0) -> None:
-        """The persistent decode captured once in a HIP graph, then replayed."""
+        """The persistent decode (mode 0) captured once in a HIP graph, then replayed."""
         check(lib().fls_gpu_rowgroup_decode_graph(self.h, dev_encoded, dev_out, stream), "rowgroup_decode_graph")
+
+    def work_list(self):
+        """(tiles as a RG_TILE structured array, blocks, LDS bytes per wave) of the persistent launch."""
+        n, blocks, wlds = _u32(0), _u32(0), _u32(0)
+        check(lib().fls_gpu_rowgroup_work_list(self.h, C.byref(n), None, 0, None, None), "rowgroup_work_list")
+        tiles = np.zeros(n.value, dtype=RG_TILE)
+        check(lib().fls_gpu_rowgroup_work_list(self.h, C.byref(n), tiles.ctypes.data, n.value, C.byref(blocks),
+                                               C.byref(wlds)), "rowgroup_work_list")
+        return tiles, int(blocks.value), int(wlds.value)
 
     def check(self, stream: int = 0) -> None:
         check(lib().fls_gpu_rowgroup_check(self.h, stream), "rowgroup_check")

@@ -5,6 +5,8 @@
 // tiles decoded by one persistent kernel (k_rowgroup.hip); mode 1 keeps the one-launch-per-page form for
 // comparison. The persistent launch can be captured in a HIP graph and replayed.
 #include <algorithm>
+#include <cstring>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -34,7 +36,7 @@ struct fls_gpu_rowgroup {
     RgTile*                    d_tiles   = nullptr;
     uint32_t*                  d_counter = nullptr;
     uint32_t*                  d_err     = nullptr;
-    unsigned                   lds = 0, blocks = 0;
+    unsigned                   wave_lds = 0, blocks = 0;
     // captured graph
     hipGraph_t     graph = nullptr;
     hipGraphExec_t exec  = nullptr;
@@ -47,16 +49,55 @@ struct fls_gpu_rowgroup {
     std::vector<size_t>      order;  // columns, largest first
 };
 
+namespace fls {
+
+std::string rg_validate_tiles(const std::vector<RgTile>& tiles, const std::vector<RgPageExtent>& pages,
+                              uint64_t enc_size, uint64_t out_size) {
+    // buffer count of each kind family: int UNC 1 / FFOR 3 / DELTA 4, RLE 5, dictionary codes 1 / 3 / 4
+    static const unsigned kBuffers[7] = {1, 3, 4, 5, 1, 3, 4};
+    for (size_t n = 0; n < tiles.size(); ++n) {
+        const RgTile& t   = tiles[n];
+        const auto    why = [&](const char* what) { return "tile " + std::to_string(n) + ": " + what; };
+        if (t.kind >= RG_KIND_COUNT) return why("unknown kind");
+        if (t.nv == 0 || t.nv > kRgMaxTileVec) return why("vector count outside 1..64");
+        if ((uint64_t)t.v0 + t.nv > t.vec_c) return why("vector range outside its page");
+        const RgPageExtent* pe = nullptr;
+        for (const auto& e : pages)
+            if (e.page == t.page) pe = &e;
+        if (!pe) return why("page not in the row-group");
+        if (pe->page + pe->page_size > enc_size) return why("page outside the encoded buffer");
+        if (pe->page_size < 16) return why("page shorter than 16 bytes");  // the kernel's stand-in load address
+        if (t.arr_c != kBuffers[t.kind / 4]) return why("buffer count does not match the kind");
+        for (unsigned b = 0; b < t.arr_c; ++b)
+            if (t.off[b] >= pe->page_size) return why("buffer offset outside its page");
+        const uint64_t vb = 128ull * rg_kind_bits(t.kind);  // bytes of one decoded vector
+        if (t.out != pe->out || (uint64_t)t.vec_c * vb != pe->out_bytes) return why("output range differs from its page");
+        if (t.out + (uint64_t)(t.v0 + t.nv) * vb > out_size) return why("output past the row-group's output");
+        if (t.kind >= RG_DICT1_8 && (!t.dict || !t.dict_n)) return why("dictionary kind without a dictionary");
+    }
+    return "";
+}
+
+}  // namespace fls
+
 namespace {
 
+// Persistent-kernel kind of a page (-1: the page decodes through its per-page kernel). The kernel reads per-vector
+// metadata at fixed strides, which is how the encoders lay pages out (PageEntry::fixed_meta).
 int kind_of(const PageEntry& pe, unsigned W) {
+    if (!pe.fixed_meta || !pe.vec_c) return -1;
     const int wc = W == 8 ? 0 : W == 16 ? 1 : W == 32 ? 2 : 3;
     switch (pe.exp_t) {
     case EXP_UNCOMPRESSED: return RG_UNC8 + wc;
     case EXP_FFOR: return pe.var == 1 ? RG_FFOR8 + wc : -1;
     case EXP_DELTA: return RG_DELTA8 + wc;
     case EXP_RLE: return pe.var == 2 ? RG_RLE8 + wc : -1;
-    default: return is_dict_exp(pe.exp_t) && pe.var == 2 ? RG_DICT8 + wc : -1;
+    case EXP_DICT:
+    case EXP_DICT_FRQ:
+    case EXP_DICT_VAL:
+        if (!pe.dict_n) return -1;
+        return pe.var == 1 ? RG_DICT1_8 + wc : pe.var == 2 ? RG_DICT2_8 + wc : pe.var == 7 ? RG_DICT7_8 + wc : -1;
+    default: return -1;
     }
 }
 
@@ -72,10 +113,6 @@ void free_rg(fls_gpu_rowgroup* rg) {
     if (rg->fork) (void)hipEventDestroy(rg->fork);
     delete rg;
 }
-
-// The persistent kernel (k_rowgroup) is switched off: its first GPU run did not finish and the cause has not been
-// found. Mode 0 reports FLS_E_UNSUPPORTED; decode_graph captures the per-page launches instead.
-constexpr bool kPersistentEnabled = false;
 
 int decode_pages(fls_gpu_rowgroup* rg, const uint8_t* enc, uint8_t* out, hipStream_t s) {
     for (size_t c = 0; c < rg->plans.size(); ++c) {
@@ -109,8 +146,92 @@ int decode_pages_branched(fls_gpu_rowgroup* rg, const uint8_t* enc, uint8_t* out
     return e == hipSuccess ? FLS_OK : hip_fail(e, "join");
 }
 
+// The persistent launch's work list for parsed columns: tiles of about `tile_kib` KiB of decoded values per page
+// of every fused column (a column is fused when all its pages have a kind), the LDS bytes per wave the launch
+// needs, and the columns' output offsets. dict_of(c, i): device address of column c's page-i dictionary. The list
+// is checked (rg_validate_tiles) before it is returned.
+struct WorkList {
+    std::vector<RgTile>   tiles;
+    std::vector<bool>     fused;
+    std::vector<uint64_t> out_off;
+    uint64_t              out_total = 0;
+    unsigned              wave_lds  = 0;
+};
+template <typename DictOf>
+std::string plan_work_list(const std::vector<const Column*>& cols, const uint8_t* types, const uint64_t* device_offsets,
+                           const uint64_t* column_sizes, unsigned tile_kib, DictOf&& dict_of, WorkList& wl) {
+    std::vector<RgPageExtent> extents;
+    uint64_t                  enc_size = 0;
+    for (size_t c = 0; c < cols.size(); ++c) {
+        const Column&  col   = *cols[c];
+        const unsigned W     = dtype_bits(types[c]);
+        const uint64_t bytes = col.n_values * (W / 8);
+        enc_size             = std::max(enc_size, device_offsets[c] + column_sizes[c]);
+        wl.out_off.push_back(wl.out_total);
+        bool fused = true;
+        for (const auto& pe : col.pages) fused &= kind_of(pe, W) >= 0;
+        wl.fused.push_back(fused);
+        if (fused) {
+            const unsigned tile_vec =
+                std::max(1u, std::min(kRgMaxTileVec, (unsigned)((tile_kib * 1024ull) / (1024ull * (W / 8)))));
+            for (size_t i = 0; i < col.pages.size(); ++i) {
+                const PageEntry& pe   = col.pages[i];
+                const int        kind = kind_of(pe, W);
+                wl.wave_lds           = std::max(wl.wave_lds, rg_wave_lds(kind));
+                extents.push_back({device_offsets[c] + pe.page_off, pe.page_size, wl.out_total + pe.out_index * (W / 8),
+                                   (uint64_t)pe.vec_c * 1024 * (W / 8)});
+                for (uint32_t v0 = 0; v0 < pe.vec_c; v0 += tile_vec) {
+                    RgTile t {};
+                    t.page   = device_offsets[c] + pe.page_off;
+                    t.out    = wl.out_total + pe.out_index * (W / 8);
+                    t.dict   = dict_of(c, i);
+                    t.dict_n = pe.dict_n;
+                    t.v0     = v0;
+                    t.nv     = (uint16_t)std::min(tile_vec, pe.vec_c - v0);
+                    t.vec_c  = pe.vec_c;
+                    t.kind   = (uint16_t)kind;
+                    t.arr_c  = (uint16_t)pe.arr_c;
+                    for (unsigned k = 0; k < 5 && k < pe.arr_c; ++k) t.off[k] = pe.boff[k];
+                    wl.tiles.push_back(t);
+                }
+            }
+        }
+        wl.out_total += (bytes + 255) & ~255ull;
+    }
+    // widest values first, each kind's tiles together: the long tiles start early and the short 8-bit ones fill
+    // the tail, and a compute unit's consecutive tiles mostly share one code path
+    std::stable_sort(wl.tiles.begin(), wl.tiles.end(), [](const RgTile& a, const RgTile& b) {
+        const unsigned wa = rg_kind_bits(a.kind), wb = rg_kind_bits(b.kind);
+        return wa != wb ? wa > wb : a.kind < b.kind;
+    });
+    std::string bad = rg_validate_tiles(wl.tiles, extents, enc_size, wl.out_total);
+    if (!bad.empty()) return bad;
+    for (const RgTile& t : wl.tiles)
+        if (rg_wave_lds(t.kind) > wl.wave_lds) return "LDS size";
+    return "";
+}
+
+unsigned tile_kib_tuned() {
+    const Tune tn = tune_for(FAM_ROWGROUP, 0, 255);  // block = KiB of decoded values per tile
+    return tn.block ? tn.block : 64;
+}
+
+// Plans with the tuned tile size, then with smaller tiles while the list has fewer than `min_tiles` tiles (a small
+// row-group would otherwise leave most waves without work), down to 16 KiB per tile.
+template <typename DictOf>
+std::string plan_work_list_sized(const std::vector<const Column*>& cols, const uint8_t* types,
+                                 const uint64_t* device_offsets, const uint64_t* column_sizes, uint64_t min_tiles,
+                                 DictOf&& dict_of, WorkList& wl) {
+    unsigned tile_kib = tile_kib_tuned();
+    for (;;) {
+        wl = WorkList {};
+        std::string bad = plan_work_list(cols, types, device_offsets, column_sizes, tile_kib, dict_of, wl);
+        if (!bad.empty() || wl.tiles.size() >= min_tiles || tile_kib <= 16) return bad;
+        tile_kib = std::max(16u, std::min(tile_kib / 2, (unsigned)(tile_kib * wl.tiles.size() / min_tiles)));
+    }
+}
+
 int decode_fused(fls_gpu_rowgroup* rg, const uint8_t* enc, uint8_t* out, hipStream_t s) {
-    if (!kPersistentEnabled) return fail(FLS_E_UNSUPPORTED, "the persistent row-group kernel is disabled in this build");
     for (size_t c = 0; c < rg->plans.size(); ++c) {
         if (!rg->fused[c]) {
             const uint64_t bytes = fls_gpu_plan_n_values(rg->plans[c]) * (dtype_bits(rg->types[c]) / 8);
@@ -122,8 +243,8 @@ int decode_fused(fls_gpu_rowgroup* rg, const uint8_t* enc, uint8_t* out, hipStre
         }
     }
     if (rg->tiles.empty()) return FLS_OK;
-    hipError_t e = launch_rowgroup(enc, out, rg->d_tiles, (uint32_t)rg->tiles.size(), rg->d_counter, rg->d_err, rg->lds,
-                                   rg->blocks, s);
+    hipError_t e = launch_rowgroup(enc, out, rg->d_tiles, (uint32_t)rg->tiles.size(), rg->d_counter, rg->d_err,
+                                   rg->wave_lds, rg->blocks, s);
     return e == hipSuccess ? FLS_OK : hip_fail(e, "row-group launch");
 }
 
@@ -137,48 +258,34 @@ int fls_gpu_rowgroup_create(uint32_t n_columns, const uint8_t* const* host_colum
         return fail(FLS_E_ARG, "null argument");
     *out    = nullptr;
     auto* rg = new fls_gpu_rowgroup();
-    const Tune tn = tune_for(FAM_ROWGROUP, 0, 255);
-    const unsigned tile_vecs = tn.block ? tn.block : 64;
+    std::vector<const Column*> cols;
     for (uint32_t c = 0; c < n_columns; ++c) {
+        if (types[c] == STR) return free_rg(rg), fail(FLS_E_UNSUPPORTED, "string columns use fls_gpu_plan_decode_strings");
         fls_gpu_plan* p  = nullptr;
         int           rc = fls_gpu_plan_create(host_columns[c], column_sizes[c], (enum data_t)types[c], &p);
         if (rc) return free_rg(rg), rc;
         rg->plans.push_back(p);
-        if (types[c] == STR) return free_rg(rg), fail(FLS_E_UNSUPPORTED, "string columns use fls_gpu_plan_decode_strings");
         rg->dev_off.push_back(device_offsets[c]);
         rg->types.push_back((uint8_t)types[c]);
-        rg->out_off.push_back(rg->out_total);
-        const unsigned W     = dtype_bits((uint8_t)types[c]);
-        const uint64_t bytes = fls_gpu_plan_n_values(p) * (W / 8);
-        const Column&  col   = plan_column(p);
-        bool           fused = true;
-        for (const auto& pe : col.pages) fused &= kind_of(pe, W) >= 0;
-        rg->fused.push_back(fused);
-        if (fused) {
-            for (size_t i = 0; i < col.pages.size(); ++i) {
-                const PageEntry& pe   = col.pages[i];
-                const int        kind = kind_of(pe, W);
-                rg->lds               = std::max(rg->lds, rg_stage_bytes(kind));
-                for (uint32_t v0 = 0; v0 < pe.vec_c; v0 += tile_vecs) {
-                    RgTile t {};
-                    t.page   = device_offsets[c] + pe.page_off;
-                    t.out    = rg->out_total + pe.out_index * (W / 8);
-                    t.dict   = (uint64_t)(uintptr_t)plan_dictionary(p, i);
-                    t.dict_n = pe.dict_n;
-                    t.v0     = v0;
-                    t.nv     = std::min(tile_vecs, pe.vec_c - v0);
-                    t.vec_c  = pe.vec_c;
-                    t.kind   = (uint16_t)kind;
-                    t.arr_c  = (uint16_t)pe.arr_c;
-                    rg->tiles.push_back(t);
-                }
-            }
-        }
-        rg->out_total += (bytes + 255) & ~255ull;
+        cols.push_back(&plan_column(p));
     }
-    // group tiles by kind so a compute unit's consecutive tiles share one code path
-    std::stable_sort(rg->tiles.begin(), rg->tiles.end(), [](const RgTile& a, const RgTile& b) { return a.kind < b.kind; });
-    hipError_t e;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    hipError_t      e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return free_rg(rg), hip_fail(e, "device properties");
+    // at least 8 tiles per wave slot of the device (16 waves per compute unit)
+    const uint64_t    min_tiles = 8ull * 16 * (uint64_t)prop.multiProcessorCount;
+    WorkList          wl;
+    const std::string bad = plan_work_list_sized(
+        cols, types, device_offsets, column_sizes, min_tiles,
+        [&](size_t c, size_t i) { return (uint64_t)(uintptr_t)plan_dictionary(rg->plans[c], i); }, wl);
+    if (!bad.empty()) return free_rg(rg), fail(FLS_E_FORMAT, "row-group work list: " + bad);
+    rg->tiles     = std::move(wl.tiles);
+    rg->fused     = std::move(wl.fused);
+    rg->out_off   = std::move(wl.out_off);
+    rg->out_total = wl.out_total;
+    rg->wave_lds  = wl.wave_lds;
     if ((e = hipMalloc(&rg->d_counter, 4)) != hipSuccess) return free_rg(rg), hip_fail(e, "hipMalloc");
     if ((e = hipMalloc(&rg->d_err, 4)) != hipSuccess) return free_rg(rg), hip_fail(e, "hipMalloc");
     if ((e = hipMemset(rg->d_err, 0, 4)) != hipSuccess) return free_rg(rg), hip_fail(e, "hipMemset");
@@ -206,14 +313,13 @@ int fls_gpu_rowgroup_create(uint32_t n_columns, const uint8_t* const* host_colum
         };
         return work(a) > work(b);
     });
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    hipDeviceProp_t prop;
-    if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return free_rg(rg), hip_fail(e, "device properties");
-    const unsigned lds_cu  = prop.maxSharedMemoryPerMultiProcessor ? (unsigned)prop.maxSharedMemoryPerMultiProcessor : 65536u;
-    const unsigned per_cu  = tn.max_blocks ? tn.max_blocks : 8;
-    const unsigned fit     = rg->lds ? std::max(1u, lds_cu / rg->lds) : per_cu;
-    rg->blocks             = (unsigned)prop.multiProcessorCount * std::min(per_cu, fit);
+    // blocks of kRgWaves waves: as many per compute unit as the LDS allows (tuning max_blocks caps it)
+    const unsigned lds_cu    = prop.maxSharedMemoryPerMultiProcessor ? (unsigned)prop.maxSharedMemoryPerMultiProcessor : 65536u;
+    const unsigned per_block = rg->wave_lds * kRgWaves;
+    const unsigned fit       = per_block ? std::max(1u, lds_cu / per_block) : 8u;
+    const Tune     tn        = tune_for(FAM_ROWGROUP, 0, 255);
+    const unsigned per_cu    = std::min(tn.max_blocks ? tn.max_blocks : 8u, fit);
+    rg->blocks               = (unsigned)prop.multiProcessorCount * per_cu;
     *out                   = rg;
     return FLS_OK;
 }
@@ -254,8 +360,7 @@ int fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encod
             (void)hipStreamDestroy(cs);
             return hip_fail(e, "begin capture");
         }
-        int rc = kPersistentEnabled ? decode_fused(rg, dev_encoded, dev_out, cs)
-                                    : decode_pages_branched(rg, dev_encoded, dev_out, cs);
+        int rc = decode_fused(rg, dev_encoded, dev_out, cs);
         e      = hipStreamEndCapture(cs, &rg->graph);
         (void)hipStreamDestroy(cs);
         if (rc) return rc;
@@ -266,6 +371,39 @@ int fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encod
     }
     e = hipGraphLaunch(rg->exec, s);
     return e == hipSuccess ? FLS_OK : hip_fail(e, "graph launch");
+}
+
+int fls_rowgroup_plan_work_list(uint32_t n_columns, const uint8_t* const* host_columns, const uint64_t* column_sizes,
+                                const uint64_t* device_offsets, const uint8_t* types, uint32_t* n_tiles, void* tiles,
+                                uint32_t capacity, uint32_t* wave_lds) {
+    if (n_columns && (!host_columns || !column_sizes || !device_offsets || !types)) return fail(FLS_E_ARG, "null argument");
+    std::vector<Column>        parsed(n_columns);
+    std::vector<const Column*> cols;
+    for (uint32_t c = 0; c < n_columns; ++c) {
+        if (types[c] == STR) return fail(FLS_E_UNSUPPORTED, "string columns use fls_gpu_plan_decode_strings");
+        const std::string err = parse_column(host_columns[c], column_sizes[c], types[c], parsed[c]);
+        if (!err.empty()) return fail(FLS_E_FORMAT, err);
+        cols.push_back(&parsed[c]);
+    }
+    // no device dictionaries exist here: dictionary tiles name the page's dictionary by a nonzero placeholder
+    WorkList          wl;
+    const std::string bad = plan_work_list_sized(cols, types, device_offsets, column_sizes, 8ull * 16 * 256,
+                                                 [&](size_t c, size_t i) { return (uint64_t)(c * 65536 + i + 1); }, wl);
+    if (!bad.empty()) return fail(FLS_E_FORMAT, "row-group work list: " + bad);
+    if (n_tiles) *n_tiles = (uint32_t)wl.tiles.size();
+    if (tiles) std::memcpy(tiles, wl.tiles.data(), std::min<size_t>(capacity, wl.tiles.size()) * sizeof(RgTile));
+    if (wave_lds) *wave_lds = wl.wave_lds;
+    return FLS_OK;
+}
+
+int fls_gpu_rowgroup_work_list(const fls_gpu_rowgroup* rg, uint32_t* n_tiles, void* tiles, uint32_t capacity,
+                               uint32_t* blocks, uint32_t* wave_lds) {
+    if (!rg) return fail(FLS_E_ARG, "null argument");
+    if (n_tiles) *n_tiles = (uint32_t)rg->tiles.size();
+    if (tiles) std::memcpy(tiles, rg->tiles.data(), std::min<size_t>(capacity, rg->tiles.size()) * sizeof(RgTile));
+    if (blocks) *blocks = rg->blocks;
+    if (wave_lds) *wave_lds = rg->wave_lds;
+    return FLS_OK;
 }
 
 int fls_gpu_rowgroup_check(fls_gpu_rowgroup* rg, void* stream) {

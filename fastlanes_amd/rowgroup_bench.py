@@ -1,8 +1,8 @@
 """Mixed row-group decode benchmark: 64 columns of uint8/16/32/64 with lengths uniform in [2^20, 2^26] values
 (multiples of 1024) and encodings drawn from bit-packing (w in 1..32), FFOR, delta, RLE and dictionary. The same
-device-resident row-group is decoded three ways -- one launch per page in column order (the per-column form), the
-same launches spread over 4 parallel stream branches, and those branches captured once in a HIP graph and replayed
--- and every column is compared with its input values. (The persistent row-group kernel is disabled in this build.)
+device-resident row-group is decoded four ways -- the persistent launch over the row-group's work list, that launch
+captured once in a HIP graph and replayed, one launch per page in column order (the per-column form), and those
+launches spread over 4 parallel stream branches -- and every column is compared with its input values.
 
     python -m fastlanes_amd.rowgroup_bench [--min-log2 20] [--max-log2 26] [--columns 64]
 """
@@ -79,7 +79,10 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
 
     res = {"columns": n_columns, "values": n_values, "decoded_bytes": dec_bytes, "encoded_bytes": enc_bytes,
            "host_build_s": round(t_build, 1)}
-    forms = {"graph": lambda: rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), sp),
+    tiles, blocks, wave_lds = rg.work_list()
+    res["work_list"] = {"tiles": int(tiles.size), "blocks": blocks, "lds_per_wave": wave_lds}
+    forms = {"persistent": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, sp),
+             "graph": lambda: rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), sp),
              "branched": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 2, sp),
              "per_page": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, sp)}
     for name, fn in forms.items():
@@ -91,8 +94,8 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
         res[name] = {"ms": round(t * 1e3, 3), "gvalues_s": round(n_values / t / 1e9, 2), "gb_s": round(gbs, 1),
                      "hbm_frac": round(gbs / HBM, 4)}
     res["graph_vs_per_page"] = round(res["per_page"]["ms"] / res["graph"]["ms"], 3)
+    res["persistent_vs_per_page"] = round(res["per_page"]["ms"] / res["persistent"]["ms"], 3)
     res["branched_vs_per_page"] = round(res["per_page"]["ms"] / res["branched"]["ms"], 3)
-    res["persistent"] = "disabled in this build"
     # where the per-column time goes: each column decoded alone, grouped by (encoding, width)
     per_kind = {}
     for c, (W, kind, n, _) in enumerate(spec):
@@ -104,6 +107,16 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
         k["columns"] += 1
         k["ms"] += t * 1e3
         k["bytes"] += vals[c].nbytes + encs[c].size
+    # the same groups of columns as row-groups of their own, decoded by the persistent launch
+    groups = {}
+    for c, (W, kind, n, _) in enumerate(spec):
+        groups.setdefault(f"{kind}{W}", []).append(c)
+    for key, cs in groups.items():
+        sub = fa.RowGroup([encs[c] for c in cs], [vals[c].dtype for c in cs], [offs[c] for c in cs])
+        t = _time(lambda: sub.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, sp), stream, 3, warm=1)
+        per_kind[key]["persistent_ms"] = round(t * 1e3, 3)
+        per_kind[key]["persistent_hbm_frac"] = round(per_kind[key]["bytes"] / t / 1e9 / HBM, 3)
+        del sub
     for k in per_kind.values():
         k["hbm_frac"] = round(k["bytes"] / (k["ms"] * 1e-3) / 1e9 / HBM, 3)
         k["ms"] = round(k["ms"], 3)

@@ -126,16 +126,25 @@ void fls_gpu_rowgroup_destroy(fls_gpu_rowgroup* rg);
 /* Output offsets (bytes, 256-aligned) of each column's values in the row-group's output buffer, and its size.
  * Each column occupies n_values rounded up to whole 1024-value vectors. */
 int  fls_gpu_rowgroup_layout(const fls_gpu_rowgroup* rg, uint64_t* out_offsets, uint64_t* out_total_bytes);
-/* mode 0: one persistent launch over the work list (dictionary pages are expanded first; expressions outside the
- * persistent kernel's set decode with their per-page kernels) -- disabled in this build, returns FLS_E_UNSUPPORTED;
- * mode 1: one launch per column page, in column order on `stream` (baseline form); mode 2: the same launches spread
- * over parallel branches (4 streams forked from and joined back into `stream`). dev_out must be 256-byte aligned and
- * hold out_total_bytes. */
+/* mode 0: one persistent launch over the work list (dictionaries of the fused pages are decoded first; pages
+ * outside the persistent kernel's set -- patched FFOR, RLE with plain positions, DICT_RLE, S_PATCH_REC -- decode
+ * with their per-page kernels before it); mode 1: one launch per column page, in column order on `stream` (the
+ * per-column form); mode 2: the same launches spread over parallel branches (4 streams forked from and joined back
+ * into `stream`). dev_out must be 256-byte aligned and hold out_total_bytes. */
 int  fls_gpu_rowgroup_decode(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, int mode,
                              void* stream);
-/* Captures the row-group's decode once in a HIP graph and replays it (one graph launch per call). With the
- * persistent kernel disabled, the graph holds the per-page launches of mode 2 as parallel graph branches. */
+/* Captures mode 0 once in a HIP graph and replays it (one graph launch per call; recaptured when the buffers
+ * change). */
 int  fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, void* stream);
+/* The persistent launch's work list: tile count, then up to `capacity` tiles of 64 bytes (the layout of RgTile in
+ * fastlanes_amd/csrc/rowgroup.hpp), and the launch shape (blocks, LDS bytes per wave). Any pointer may be NULL. */
+int  fls_gpu_rowgroup_work_list(const fls_gpu_rowgroup* rg, uint32_t* n_tiles, void* tiles, uint32_t capacity,
+                                uint32_t* blocks, uint32_t* wave_lds);
+/* The same work list planned on the host alone (no device memory is touched; dictionary tiles carry a nonzero
+ * placeholder instead of a dictionary address), with the same validation, for inspecting a row-group's tiles. */
+int  fls_rowgroup_plan_work_list(uint32_t n_columns, const uint8_t* const* host_columns, const uint64_t* column_sizes,
+                                 const uint64_t* device_offsets, const uint8_t* types, uint32_t* n_tiles, void* tiles,
+                                 uint32_t capacity, uint32_t* wave_lds);
 int  fls_gpu_rowgroup_check(fls_gpu_rowgroup* rg, void* stream);
 
 #ifdef __cplusplus

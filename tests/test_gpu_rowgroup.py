@@ -1,8 +1,7 @@
-"""Row-group decode parity: many columns of mixed widths and encodings through fls_gpu_rowgroup_* (per-page
-launches, and the same launches captured once in a HIP graph and replayed) against the CPU oracle, bit-exact.
-
-The persistent row-group kernel (mode 0) is disabled in this build: its first GPU run did not finish and the cause
-has not been found, so mode 0 must report FLS_E_UNSUPPORTED and its parity test is skipped."""
+"""Row-group decode parity: many columns of mixed widths and encodings through fls_gpu_rowgroup_* -- the
+persistent launch over the work list (mode 0), one launch per page (mode 1), the per-page launches over parallel
+branches (mode 2), and the persistent launch captured once in a HIP graph and replayed -- against the CPU oracle,
+bit-exact."""
 import numpy as np
 import pytest
 
@@ -70,8 +69,8 @@ def test_rowgroup_mixed_all_modes(torch_cuda, oracle):
     rg, d_enc, want = build(torch, oracle, mixed_columns())
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
     assert d_out.data_ptr() % 256 == 0
-    for mode in (1, 2):
-        print(f"rowgroup mixed: per-page launches, mode {mode}", flush=True)
+    for mode in (0, 1, 2):
+        print(f"rowgroup mixed: mode {mode}", flush=True)
         d_out.fill_(0xA5)
         rg.decode(d_enc.data_ptr(), d_out.data_ptr(), mode, fa.stream_ptr())
         verify(torch, rg, d_out, want, f"mode {mode}")
@@ -85,23 +84,25 @@ def test_rowgroup_mixed_all_modes(torch_cuda, oracle):
 
 
 def test_rowgroup_fallback_expressions(torch_cuda, oracle):
-    """Expressions outside the persistent kernel's set (RLE with plain positions, dictionaries with plain or
-    delta-coded codes, frequency-ordered dictionaries) decode through their per-page kernels in the same call."""
+    """Expressions outside the persistent kernel's set (RLE with plain positions, DICT_RLE) decode through their
+    per-page kernels in the same call; dictionaries with plain, FFOR or delta-coded codes and frequency-ordered
+    dictionaries are tiles of the persistent launch."""
     torch = torch_cuda
     g = np.random.default_rng(9)
     cols = []
     for dt in (np.uint8, np.uint32, np.int64):
         cols.append((wl.geometric_runs(20480, 8.0, seed=2, value_bits=7).astype(dt), 7))
         keys = g.integers(0, 100, 40).astype(dt)
-        for eid in (16, 22, 28, 29):
+        for eid in (16, 22, 28, 29, 27):
             cols.append((keys[g.integers(0, 40, 17408)], eid))
         cols.append((wl.rowgroup_column("dict", 8 * np.dtype(dt).itemsize, 9216, 5).view(dt), 17))
     rg, d_enc, want = build(torch, oracle, cols)
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
-    print("rowgroup fallback: per-page launches and graph", flush=True)
-    d_out.fill_(0x5A)
-    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, fa.stream_ptr())
-    verify(torch, rg, d_out, want, "fallback mode 1")
+    print("rowgroup fallback: per-page launches, persistent launch and graph", flush=True)
+    for mode in (1, 0):
+        d_out.fill_(0x5A)
+        rg.decode(d_enc.data_ptr(), d_out.data_ptr(), mode, fa.stream_ptr())
+        verify(torch, rg, d_out, want, f"fallback mode {mode}")
     d_out.fill_(0x5A)
     rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), fa.stream_ptr())
     verify(torch, rg, d_out, want, "fallback graph")
@@ -114,9 +115,12 @@ def test_rowgroup_multi_page_columns(torch_cuda, oracle):
     cols = [(wl.rowgroup_column("ffor", 8, n, 21), 1), (wl.rowgroup_column("delta", 16, n, 22), 11)]
     rg, d_enc, want = build(torch, oracle, cols)
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
-    print("rowgroup multi-page: graph", flush=True)
+    print("rowgroup multi-page: persistent graph and per-page launches", flush=True)
     rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), fa.stream_ptr())
-    verify(torch, rg, d_out, want, "multi-page")
+    verify(torch, rg, d_out, want, "multi-page graph")
+    d_out.zero_()
+    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, fa.stream_ptr())
+    verify(torch, rg, d_out, want, "multi-page mode 1")
 
 
 def test_rowgroup_spec_sample(torch_cuda, oracle):
@@ -127,9 +131,11 @@ def test_rowgroup_spec_sample(torch_cuda, oracle):
         cols.append((wl.rowgroup_column(kind, W, n, seed), wl.ROWGROUP_EXPRS[kind]))
     rg, d_enc, want = build(torch, oracle, cols)
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
-    print("rowgroup spec sample: per-page + graph", flush=True)
-    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, fa.stream_ptr())
-    verify(torch, rg, d_out, want, "spec per-page")
+    print("rowgroup spec sample: per-page, persistent, graph", flush=True)
+    for mode in (1, 0):
+        d_out.zero_()
+        rg.decode(d_enc.data_ptr(), d_out.data_ptr(), mode, fa.stream_ptr())
+        verify(torch, rg, d_out, want, f"spec mode {mode}")
     d_out.zero_()
     rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), fa.stream_ptr())
     verify(torch, rg, d_out, want, "spec graph")
@@ -149,15 +155,42 @@ def test_rowgroup_empty_and_errors(torch_cuda):
         rg.decode(buf.data_ptr(), buf.data_ptr() + 4, 0)  # output not 256-byte aligned
     with pytest.raises(fa.FlsError):
         rg.decode(buf.data_ptr(), buf.data_ptr(), 3)  # unknown mode
-    with pytest.raises(fa.FlsError, match="disabled"):
-        rg.decode(buf.data_ptr(), buf.data_ptr(), 0)  # persistent kernel switched off: error, no launch
 
 
-@pytest.mark.skip(reason="persistent row-group kernel disabled: its first GPU run did not finish and the cause has "
-                         "not been found; it stays off the GPU until one is shown")
 def test_rowgroup_persistent_kernel(torch_cuda, oracle):
+    """The persistent launch alone (every page of the mixed row-group is a tile of the work list), then replays of
+    its graph with other work on the stream between them."""
     torch = torch_cuda
-    rg, d_enc, want = build(torch, oracle, mixed_columns())
+    cols = mixed_columns()
+    rg, d_enc, want = build(torch, oracle, cols)
+    tiles, blocks, wave_lds = rg.work_list()
+    print(f"rowgroup persistent: {tiles.size} tiles, {blocks} blocks, {wave_lds} B LDS per wave", flush=True)
+    assert tiles.size > 0 and blocks > 0
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
+    d_out.fill_(0x3C)
     rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, fa.stream_ptr())
     verify(torch, rg, d_out, want, "mode 0")
+    for rep in range(2):
+        d_out.fill_(0xC3)
+        scratch = torch.randn(1 << 20, device="cuda").sum()  # other work between the replays
+        rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), fa.stream_ptr())
+        verify(torch, rg, d_out, want, f"persistent graph replay {rep}")
+        del scratch
+
+
+def test_rowgroup_persistent_grid_stride(torch_cuda, oracle):
+    """Tiles of one vector and a grid of one block: every wave claims many tiles one after another, through every
+    kind of the mixed row-group."""
+    torch = torch_cuda
+    lib = fa.lib()
+    fa.check(lib.fls_gpu_tune_set(6, 0, 255, 1, 1, 1), "tune_set")  # 1 KiB tiles (one vector), one block per CU
+    try:
+        rg, d_enc, want = build(torch, oracle, mixed_columns(seed=5, n_hi=1 << 14))
+    finally:
+        fa.check(lib.fls_gpu_tune_set(6, 0, 255, 64, 1, 0), "tune_set")
+    tiles, blocks, _ = rg.work_list()
+    assert (tiles["nv"] == 1).all()
+    d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
+    print(f"rowgroup persistent, one-vector tiles: {tiles.size} tiles over {blocks} blocks", flush=True)
+    rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, fa.stream_ptr())
+    verify(torch, rg, d_out, want, "one-vector tiles")
