@@ -62,9 +62,10 @@ def lib() -> C.CDLL:
     L.fls_gpu_rowgroup_decode_graph.argtypes = [_vp, _vp, _vp, _vp]
     L.fls_gpu_rowgroup_check.argtypes = [_vp, _vp]
     L.fls_gpu_tune_set.argtypes = [_int, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint]
-    L.fls_gpu_rowgroup_work_list.argtypes = [_vp, C.POINTER(_u32), _vp, _u32, C.POINTER(_u32), C.POINTER(_u32)]
+    L.fls_gpu_rowgroup_work_list.argtypes = [_vp, C.POINTER(_u32), _vp, _u32, C.POINTER(_u32), C.POINTER(_u32), _vp]
     L.fls_rowgroup_plan_work_list.argtypes = [_u32, C.POINTER(_vp), C.POINTER(_u64), C.POINTER(_u64),
-                                              C.POINTER(C.c_uint8), C.POINTER(_u32), _vp, _u32, C.POINTER(_u32)]
+                                              C.POINTER(C.c_uint8), _u32, C.POINTER(_u32), _vp, _u32, _vp,
+                                              C.POINTER(_u32)]
     # include/fastlanes.h
     L.fls_cpi_connection_open.restype = _vp
     L.fls_cpi_connection_close.argtypes = [_vp]
@@ -172,19 +173,22 @@ def selection_and(dev_a: int, dev_b: int, n_vec: int, dev_out: int, dev_counts: 
 
 
 # One tile of the persistent row-group launch (struct RgTile, fastlanes_amd/csrc/rowgroup.hpp): vectors
-# [v0, v0 + nv) of the page at byte `page` of the encoded buffer, decoded to byte `out` of the output buffer.
+# [v0, v0 + nv) of the page at byte `page` of the encoded buffer, decoded to byte `out` of the output buffer; its
+# input is `n_rng` byte ranges of the page (rsrc, rlen) staged at rlds in the wave's LDS slot.
 RG_TILE = np.dtype([("page", "<u8"), ("out", "<u8"), ("dict", "<u8"), ("v0", "<u4"), ("dict_n", "<u4"),
-                    ("nv", "<u2"), ("kind", "<u2"), ("arr_c", "<u2"), ("pad", "<u2"), ("off", "<u4", (5,)),
-                    ("vec_c", "<u4")])
-assert RG_TILE.itemsize == 64
+                    ("vec_c", "<u4"), ("nv", "<u2"), ("kind", "<u2"), ("arr_c", "<u2"), ("n_rng", "<u2"),
+                    ("off", "<u4", (5,)), ("rsrc", "<u4", (3,)), ("rlen", "<u4", (3,)), ("rlds", "<u4", (3,)),
+                    ("chunks", "<u4"), ("sdelta", "<i4", (3,)), ("ldelta", "<i4", (3,))])
+assert RG_TILE.itemsize == 128
 # Tile kinds (enum RgKind): 4 element widths each of UNCOMPRESSED, FFOR, DELTA, RLE and dictionary codes
 # (UNCOMPRESSED, FFOR, DELTA).
 RG_KIND_NAMES = [f"{k}{w}" for k in ("unc", "ffor", "delta", "rle", "dict1_", "dict2_", "dict7_") for w in (8, 16, 32, 64)]
 
 
-def plan_work_list(host_columns, dtypes, device_offsets):
+def plan_work_list(host_columns, dtypes, device_offsets, waves: int = 0):
     """The persistent row-group launch's work list planned on the host alone (fls_rowgroup_plan_work_list):
-    (tiles, LDS bytes per wave). Dictionary tiles carry a placeholder instead of a device address."""
+    (tiles, LDS bytes per wave, wave runs). Dictionary tiles carry a placeholder instead of a device address. With
+    waves > 0 the tiles come in launch order and wave w's run is tiles[runs[w]:runs[w + 1]]."""
     n = len(host_columns)
     cols = [np.ascontiguousarray(c, dtype=np.uint8) for c in host_columns]
     ptrs = (_vp * max(n, 1))(*[c.ctypes.data for c in cols])
@@ -192,12 +196,14 @@ def plan_work_list(host_columns, dtypes, device_offsets):
     offs = (_u64 * max(n, 1))(*[int(o) for o in device_offsets])
     types = (C.c_uint8 * max(n, 1))(*[dtype_code(d) for d in dtypes])
     cnt, wlds = _u32(0), _u32(0)
-    check(lib().fls_rowgroup_plan_work_list(n, ptrs, sizes, offs, types, C.byref(cnt), None, 0, C.byref(wlds)),
-          "plan_work_list")
-    tiles = np.zeros(cnt.value, dtype=RG_TILE)
-    check(lib().fls_rowgroup_plan_work_list(n, ptrs, sizes, offs, types, C.byref(cnt), tiles.ctypes.data, cnt.value,
+    check(lib().fls_rowgroup_plan_work_list(n, ptrs, sizes, offs, types, 0, C.byref(cnt), None, 0, None,
                                             C.byref(wlds)), "plan_work_list")
-    return tiles, int(wlds.value)
+    tiles = np.zeros(cnt.value, dtype=RG_TILE)
+    runs = np.zeros(waves + 1, dtype=np.uint32)
+    check(lib().fls_rowgroup_plan_work_list(n, ptrs, sizes, offs, types, waves, C.byref(cnt), tiles.ctypes.data,
+                                            cnt.value, runs.ctypes.data if waves else None, C.byref(wlds)),
+          "plan_work_list")
+    return tiles, int(wlds.value), runs
 
 
 class RowGroup:
@@ -231,13 +237,16 @@ class RowGroup:
         check(lib().fls_gpu_rowgroup_decode_graph(self.h, dev_encoded, dev_out, stream), "rowgroup_decode_graph")
 
     def work_list(self):
-        """(tiles as a RG_TILE structured array, blocks, LDS bytes per wave) of the persistent launch."""
+        """(tiles as a RG_TILE structured array in launch order, blocks, LDS bytes per wave, wave runs) of the
+        persistent launch: wave w decodes tiles[runs[w]:runs[w + 1]]."""
         n, blocks, wlds = _u32(0), _u32(0), _u32(0)
-        check(lib().fls_gpu_rowgroup_work_list(self.h, C.byref(n), None, 0, None, None), "rowgroup_work_list")
+        check(lib().fls_gpu_rowgroup_work_list(self.h, C.byref(n), None, 0, C.byref(blocks), None, None),
+              "rowgroup_work_list")
         tiles = np.zeros(n.value, dtype=RG_TILE)
+        runs = np.zeros(4 * blocks.value + 1, dtype=np.uint32)
         check(lib().fls_gpu_rowgroup_work_list(self.h, C.byref(n), tiles.ctypes.data, n.value, C.byref(blocks),
-                                               C.byref(wlds)), "rowgroup_work_list")
-        return tiles, int(blocks.value), int(wlds.value)
+                                               C.byref(wlds), runs.ctypes.data), "rowgroup_work_list")
+        return tiles, int(blocks.value), int(wlds.value), runs
 
     def check(self, stream: int = 0) -> None:
         check(lib().fls_gpu_rowgroup_check(self.h, stream), "rowgroup_check")

@@ -1,16 +1,14 @@
 // k_rowgroup.hip -- one persistent launch that decodes a whole row-group from a device-side work list.
 //
-// The host cuts every page of every column into tiles of up to 64 consecutive vectors (RgTile, rowgroup.hpp).
-// Every wave takes tiles from one atomic counter until the list is exhausted and decodes a tile one vector at a
-// time:
-//   * the tile's per-vector metadata (bit width, FFOR base, entry point of the packed buffer, RLE run extents) is
-//     loaded once, one vector per lane, and read per vector with v_readlane into scalar registers;
-//   * the packed rows of vector k + 1 are loaded into registers (16-byte loads, all 64 lanes) while vector k is
-//     decoded, then written to the wave's LDS slot;
-//   * each thread produces a contiguous run of untransposed values (16 values; 2 x 8 for 8-bit) from the staged
-//     rows with the vector's bit width as a run-time value (a 64/128-bit funnel shift per value), so one body
-//     serves every bit width, and stores it with 16-byte stores.
-// Every body is inlined into the kernel: the kernel makes no calls.
+// The host cuts every page of every column into tiles of up to 64 consecutive vectors whose input bytes (packed
+// rows, lane bases, run values) are up to three contiguous ranges of the page, about 8 KiB in all (RgTile,
+// rowgroup.hpp), and gives every wave of the launch a contiguous run of the list, balanced by bytes (a shared
+// claim counter would serialize on its device-wide atomic: about 14 ns per claim). While a wave decodes a tile
+// from its LDS slot, the next tile's ranges are already in flight into registers (8 x 16 bytes per lane) and
+// its per-vector metadata (bit width, FFOR base, entry point, RLE run extent; one vector per lane) into a second
+// set of registers, so each wave keeps a whole tile of input in flight. Values are produced from the staged rows
+// with the vector's bit width as a run-time value (a 64/128-bit funnel shift per value), so one body serves every
+// bit width, and stored with 16-byte stores. Every body is inlined into the kernel: the kernel makes no calls.
 //
 // Thread mappings (t = lane of the wave; W-bit values, L = 1024/W lanes of W values; value j of lane i sits at
 // untransposed position seg_start<W>(i) + p with j = j_of_p<W>(p), fls_lanes.hpp):
@@ -51,40 +49,111 @@ __device__ __forceinline__ T readlane_t(T x, uint32_t k) {
     }
 }
 
-// 16-byte chunks of a vector's packed rows, loaded into registers ahead of their use.
-template <unsigned N>
+// 16-byte chunks of a tile's staged ranges, loaded into registers ahead of its decode.
 struct Prefetch {
-    v4u c[N];
+    v4u c[kRgPrefetch];
 };
-// Every lane issues exactly N loads (chunks past n16 re-read the vector's last chunk, or the page's first 16 bytes
-// when the vector has none), so the number of memory operations in flight is the same on every path and the
-// compiler waits for these loads without also waiting for the stores issued after them.
-template <unsigned N>
-__device__ __forceinline__ void prefetch_rows(Prefetch<N>& pf, const uint8_t* src, uint32_t n16, unsigned lane,
-                                              const uint8_t* safe) {
-    static_for<N>([&](auto k) {
+
+// Page and LDS-slot byte offsets of the tile's 16-byte chunk c: 16 c plus the delta of the range holding it. The
+// deltas are added as differences under a condition (no branch, and no select between fields, which the compiler
+// would turn into an indexed load from a stack copy of the tile).
+__device__ __forceinline__ void chunk_at(const RgTile& t, uint32_t c, int32_t& src, int32_t& dst) {
+    const uint32_t n0 = (t.rlen[0] + 15) / 16, n01 = n0 + (t.rlen[1] + 15) / 16;
+    const bool     in1 = c >= n0, in2 = c >= n01;
+    const int32_t  s0 = t.sdelta[0], s1 = t.sdelta[1] - s0, s2 = t.sdelta[2] - t.sdelta[1];
+    const int32_t  l0 = t.ldelta[0], l1 = t.ldelta[1] - l0, l2 = t.ldelta[2] - t.ldelta[1];
+    src = 16 * (int32_t)c + s0 + (in1 ? s1 : 0) + (in2 ? s2 : 0);
+    dst = 16 * (int32_t)c + l0 + (in1 ? l1 : 0) + (in2 ? l2 : 0);
+}
+
+// Every lane issues exactly kRgPrefetch loads (lanes without a chunk re-read the page's first 16 bytes), so the
+// number of memory operations in flight is the same on every path.
+__device__ __forceinline__ void load_chunks(const uint8_t* page, const RgTile& t, unsigned lane, Prefetch& pf) {
+    static_for<kRgPrefetch>([&](auto k) {
         const uint32_t c = lane + 64u * decltype(k)::value;
-        const uint8_t* a = n16 ? src + 16ull * (c < n16 ? c : n16 - 1) : safe;
-        pf.c[k]          = *reinterpret_cast<const v4u_ua*>(a);
+        int32_t        src, dst;
+        chunk_at(t, c, src, dst);
+        pf.c[k] = *reinterpret_cast<const v4u_ua*>(page + (c < t.chunks ? src : 0));
     });
 }
-// Writes all N prefetched chunks of every lane to LDS (chunks past n16 land in the wave's slot past the vector's
-// rows, which nothing reads before they are overwritten): no branch separates the loads from these writes, so
-// the compiler waits for each chunk's own load only, not for the stores issued after the loads. Rows past the N
-// chunks per lane (64-bit vectors wider than 32 bits, src != nullptr) are copied from `src` now.
-template <unsigned N>
-__device__ __forceinline__ void stage_rows(const Prefetch<N>& pf, uint8_t* lds, uint32_t n16, unsigned lane,
-                                           const uint8_t* src = nullptr) {
-    static_for<N>([&](auto k) {
-        *reinterpret_cast<v4u*>(lds + 16u * (lane + 64u * decltype(k)::value)) = pf.c[k];
+// Writes the prefetched chunks to their places in the wave's LDS slot (lanes without a chunk write the slot's dump
+// word: no branch separates the loads from these writes, so the compiler waits for the chunks' own loads only).
+// Chunks past the registers' share (tiles larger than kRgStage bytes: single wide vectors) are copied now.
+__device__ __forceinline__ void stage_chunks(const uint8_t* page, const RgTile& t, unsigned lane, const Prefetch& pf,
+                                             uint8_t* lds, uint32_t wave_lds) {
+    static_for<kRgPrefetch>([&](auto k) {
+        const uint32_t c = lane + 64u * decltype(k)::value;
+        int32_t        src, dst;
+        chunk_at(t, c, src, dst);
+        *reinterpret_cast<v4u*>(lds + (c < t.chunks ? (uint32_t)dst : wave_lds - kRgDump)) = pf.c[k];
     });
-    if (src)
-        for (uint32_t c = lane + 64u * N; c < n16; c += 64u)
-            *reinterpret_cast<v4u*>(lds + 16u * c) = *reinterpret_cast<const v4u_ua*>(src + 16ull * c);
+    if (t.chunks > 64u * kRgPrefetch)
+        for (uint32_t c = lane + 64u * kRgPrefetch; c < t.chunks; c += 64u) {
+            int32_t src, dst;
+            chunk_at(t, c, src, dst);
+            *reinterpret_cast<v4u*>(lds + dst) = *reinterpret_cast<const v4u_ua*>(page + src);
+        }
+}
+
+// Tile `id` of the list, read through the constant address space: the address is uniform over the wave, so the
+// 128 bytes arrive in scalar registers (s_load) and every field derived from them stays scalar.
+__device__ __forceinline__ RgTile load_tile(const RgTile* tiles, uint32_t id) {
+    const auto* p = reinterpret_cast<const __attribute__((address_space(4))) uint32_t*>(
+        reinterpret_cast<uintptr_t>(tiles + id));
+    union {
+        RgTile   t;
+        uint32_t w[sizeof(RgTile) / 4];
+    } u;
+    static_for<sizeof(RgTile) / 4>([&](auto k) { u.w[k] = p[k]; });
+    return u.t;
 }
 
 __device__ __forceinline__ uint32_t ep_at(const uint8_t* page, uint32_t arr_c, uint32_t v, uint32_t b) {
     return v ? reinterpret_cast<const u32_ua*>(page + 8 + 4 * arr_c)[(v - 1) * arr_c + b] : 0u;
+}
+
+// Per-vector metadata of a tile, vector v0 + k in lane k (k < nv): bit width, entry point of range 0's rows
+// (UNCOMPRESSED values / codes: their fixed offset), FFOR base, and for RLE the run values' byte extent.
+struct TileMeta {
+    uint32_t bw, ep, r0, r1;
+    uint64_t base;
+};
+__device__ __forceinline__ uint64_t load_sz(const uint8_t* p, unsigned sz) {
+    return sz == 8 ? (uint64_t)load_t<uint64_t>(p) : sz == 4 ? (uint64_t)load_t<uint32_t>(p)
+                   : sz == 2 ? (uint64_t)load_t<uint16_t>(p) : (uint64_t)*p;
+}
+__device__ __forceinline__ void load_meta(const uint8_t* page, const RgTile& t, unsigned lane, TileMeta& m) {
+    const unsigned fam = t.kind / 4, W = 8u << (t.kind % 4), SZ = W / 8;
+    m = TileMeta {};
+    if (lane >= t.nv) return;
+    const uint32_t v = t.v0 + lane;
+    switch (fam) {
+    case 0: m.bw = W, m.ep = v * 128u * W; break;  // UNCOMPRESSED
+    case 1:                                         // FFOR [bw][base][packed]
+        m.bw = min((uint32_t)page[t.off[0] + v], W), m.base = load_sz(page + t.off[1] + (uint64_t)v * SZ, SZ);
+        m.ep = ep_at(page, t.arr_c, v, 2);
+        break;
+    case 2:  // DELTA [lane bases][bw][base][packed]
+        m.bw = min((uint32_t)page[t.off[1] + v], W), m.base = load_sz(page + t.off[2] + (uint64_t)v * SZ, SZ);
+        m.ep = ep_at(page, t.arr_c, v, 3);
+        break;
+    case 3:  // RLE [runs][pos lane bases][bw][base u16][packed positions]
+        m.bw   = min((uint32_t)page[t.off[2] + v], 16u);
+        m.base = *reinterpret_cast<const u16_ua*>(page + t.off[3] + 2ull * v);
+        m.ep   = ep_at(page, t.arr_c, v, 4);
+        m.r0   = ep_at(page, t.arr_c, v, 0);
+        m.r1   = v + 1 < t.vec_c ? ep_at(page, t.arr_c, v + 1, 0) : t.off[1] - t.off[0];
+        break;
+    case 4: m.bw = 32, m.ep = v * 4096u; break;  // dictionary, UNCOMPRESSED codes
+    case 5:                                      // dictionary [bw][base][packed codes]
+        m.bw = min((uint32_t)page[t.off[0] + v], 32u), m.base = *reinterpret_cast<const u32_ua*>(page + t.off[1] + 4ull * v);
+        m.ep = ep_at(page, t.arr_c, v, 2);
+        break;
+    default:  // dictionary [lane bases][bw][base][packed codes]
+        m.bw = min((uint32_t)page[t.off[1] + v], 32u), m.base = *reinterpret_cast<const u32_ua*>(page + t.off[2] + 4ull * v);
+        m.ep = ep_at(page, t.arr_c, v, 3);
+        break;
+    }
 }
 
 // Accumulates values of T into 16-byte words and stores each word as soon as it is full, so a thread holds one
@@ -120,18 +189,6 @@ __device__ __forceinline__ unsigned opaque_lane(unsigned lane) {
     return lane;
 }
 
-// --------------------------------------------------------------------------------------------------------------
-// Integer pages: MODE 0 UNCOMPRESSED [values], 1 FFOR [bw][base][packed], 2 DELTA [lane bases][bw][base][packed]
-// (include/fls/cor/exp/{uncompressed,ffor,delta}/*_exp.hpp), fixed-stride metadata.
-// --------------------------------------------------------------------------------------------------------------
-template <int W>
-struct IntMap {
-    static constexpr unsigned SZ = W / 8;
-    // 16-byte chunks per lane prefetched ahead: all of a vector up to bw = 32 (128 * bw bytes); wider 64-bit
-    // vectors load their remaining rows when they are staged
-    static constexpr unsigned PF = W == 64 ? 4 : W / 8;
-};
-
 // The two staged words a value needs: row w and row w + 1 of lane i (row w of lane i at lds + (w * L + i) * W/8,
 // w = bit / W). Reading them is one LDS instruction for 32/64-bit rows; values are fetched in batches and extracted
 // afterwards.
@@ -165,12 +222,6 @@ __device__ __forceinline__ uint64_t lane_extract(const RowPair<W>& p, uint32_t b
     else if constexpr (W == 32) return funnel32(p.hi, p.lo, s) & (uint32_t)mask;
     else return (((p.hi << W) | p.lo) >> s) & (uint32_t)mask;
 }
-// Value of lane i at bit offset `bit`, fetched and extracted at once (for loops that use it right away).
-template <int W>
-__device__ __forceinline__ uint64_t lane_value(const uint8_t* lds, unsigned i, uint32_t bit, uint64_t mask) {
-    return lane_extract<W>(lane_fetch<W>(lds, i, bit), bit, mask);
-}
-
 template <int W>
 __device__ __forceinline__ uint64_t bw_mask(uint32_t bw) {
     return bw >= 64 ? ~0ull : ((1ull << bw) - 1);
@@ -257,7 +308,7 @@ __device__ __forceinline__ void int_vector(const uint8_t* lds, uint32_t bw, uint
     }
 }
 
-// Lane base(s) of the thread's lane(s) of vector v (DELTA pages: buffer 0 holds L lane bases of W bits).
+// Lane base(s) of the thread's lane(s) (DELTA pages: L lane bases of W bits per vector).
 template <int W>
 __device__ __forceinline__ void load_lane_bases(const uint8_t* lbuf, unsigned t, uint64_t& lb0, uint64_t& lb1) {
     using T = typename uint_of<W>::type;
@@ -267,125 +318,68 @@ __device__ __forceinline__ void load_lane_bases(const uint8_t* lbuf, unsigned t,
     else lb0 = lbuf[t], lb1 = lbuf[t + 64];
 }
 
+// --------------------------------------------------------------------------------------------------------------
+// Integer pages: MODE 0 UNCOMPRESSED [values], 1 FFOR [bw][base][packed], 2 DELTA [lane bases][bw][base][packed]
+// (include/fls/cor/exp/{uncompressed,ffor,delta}/*_exp.hpp), fixed-stride metadata. Range 0 holds the tile's
+// packed rows (vector k at ep_k - ep_0), range 1 the DELTA lane bases (128 bytes per vector).
+// --------------------------------------------------------------------------------------------------------------
 template <int W, int MODE>
-__device__ __forceinline__ void rg_int(const uint8_t* page, const RgTile& tl, uint8_t* obase, uint8_t* lds,
+__device__ __forceinline__ void rg_int(const RgTile& t, const TileMeta& m, uint8_t* obase, const uint8_t* lds,
                                        unsigned lane_in) {
-    const unsigned lane = opaque_lane(lane_in);
-    using T                = typename uint_of<W>::type;
-    constexpr unsigned SZ  = W / 8;
-    constexpr unsigned O   = MODE == 2 ? 1 : 0;                        // bit-width buffer
-    constexpr unsigned P   = MODE == 0 ? 0 : (MODE == 1 ? 2 : 3);      // packed buffer
-    constexpr unsigned NPF = IntMap<W>::PF;
-    const uint32_t     nv  = tl.nv;
-    // per-vector metadata, vector tl.v0 + lane in lane `lane`
-    uint32_t m_bw = W, m_ep = 0;
-    uint64_t m_base = 0;
-    if (lane < nv) {
-        const uint32_t v = tl.v0 + lane;
-        if constexpr (MODE == 0) {
-            m_ep = v * (128u * W);
-        } else {
-            const uint32_t b = page[tl.off[O] + v];
-            m_bw             = b < (unsigned)W ? b : (unsigned)W;  // the host validated bw <= W
-            m_base           = load_t<T>(page + tl.off[O + 1] + (uint64_t)v * SZ);
-            m_ep             = ep_at(page, tl.arr_c, v, P);
-        }
-    }
-    const uint8_t* pk = page + tl.off[P];
-    Prefetch<NPF>  pf;
-    uint64_t       lb0 = 0, lb1 = 0;
-    {
-        const uint32_t bw = __builtin_amdgcn_readfirstlane(m_bw), ep = __builtin_amdgcn_readfirstlane(m_ep);
-        prefetch_rows(pf, pk + ep, 8 * bw, lane, page);
-        if constexpr (MODE == 2) load_lane_bases<W>(page + tl.off[0] + 128ull * tl.v0, lane, lb0, lb1);
-    }
-    for (uint32_t k = 0; k < nv; ++k) {
-        const uint32_t bw   = readlane_t(m_bw, k);
-        const T        base = readlane_t((T)m_base, k);
-        stage_rows(pf, lds, 8 * bw, lane, W == 64 && bw > 32 ? pk + readlane_t(m_ep, k) : nullptr);
-        const uint64_t cb0 = lb0, cb1 = lb1;
-        if (k + 1 < nv) {
-            const uint32_t nbw = readlane_t(m_bw, k + 1), nep = readlane_t(m_ep, k + 1);
-            prefetch_rows(pf, pk + nep, 8 * nbw, lane, page);
-            if constexpr (MODE == 2) load_lane_bases<W>(page + tl.off[0] + 128ull * (tl.v0 + k + 1), lane, lb0, lb1);
-        }
-        wave_sync();
-        uint8_t* dst = obase + (uint64_t)(tl.v0 + k) * (1024 * SZ);
-        int_vector<W, MODE == 2>(lds, bw, base, cb0, cb1, lane, dst);
-        wave_sync();
+    using T               = typename uint_of<W>::type;
+    constexpr unsigned SZ = W / 8;
+    const unsigned     lane = opaque_lane(lane_in);
+    const uint32_t     ep0  = readlane_t(m.ep, 0);
+    for (uint32_t k = 0; k < t.nv; ++k) {
+        const uint32_t bw   = readlane_t(m.bw, k);
+        const T        base = readlane_t((T)m.base, k);
+        uint64_t       lb0 = 0, lb1 = 0;
+        if constexpr (MODE == 2) load_lane_bases<W>(lds + t.rlds[1] + 128 * k, lane, lb0, lb1);
+        int_vector<W, MODE == 2>(lds + (readlane_t(m.ep, k) - ep0), bw, base, lb0, lb1, lane,
+                                 obase + (uint64_t)(t.v0 + k) * (1024 * SZ));
     }
 }
 
 // --------------------------------------------------------------------------------------------------------------
 // RLE pages, DELTA_NO_PATCH positions (include/fls/cor/exp/rle/rle_exp.hpp:56-74): per vector [run values T]
 // [pos lane bases u16 x 64][bw][base u16][packed u16 deltas]; out[i] = runs[pos[i]] (src/cor/lyt/vec.cpp:318-332).
-// Positions past the vector's last run are clamped (err bit 1).
+// Range 0 holds the position rows, range 1 the run values, range 2 the position lane bases. Positions past the
+// vector's last run are clamped (err bit 1).
 // --------------------------------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void rg_rle(const uint8_t* page, const RgTile& tl, uint8_t* obase, uint8_t* lds,
+__device__ __forceinline__ void rg_rle(const RgTile& t, const TileMeta& m, uint8_t* obase, const uint8_t* lds,
                                        unsigned lane_in, uint32_t* err) {
-    const unsigned lane = opaque_lane(lane_in);
-    constexpr unsigned SZ = sizeof(T);
-    const uint32_t     nv = tl.nv;
-    uint32_t           m_bw = 0, m_ep = 0, m_r0 = 0, m_r1 = 0, m_base = 0;
-    const uint32_t     run_bytes = tl.off[1] - tl.off[0];
-    if (lane < nv) {
-        const uint32_t v = tl.v0 + lane;
-        const uint32_t b = page[tl.off[2] + v];
-        m_bw             = b < 16u ? b : 16u;
-        m_base           = *reinterpret_cast<const u16_ua*>(page + tl.off[3] + 2ull * v);
-        m_ep             = ep_at(page, tl.arr_c, v, 4);
-        m_r0             = ep_at(page, tl.arr_c, v, 0);
-        m_r1             = v + 1 < tl.vec_c ? ep_at(page, tl.arr_c, v + 1, 0) : run_bytes;
-    }
-    uint8_t* const       runs_lds = lds + 128 * 17;
-    const uint8_t* const pk       = page + tl.off[4];
-    const uint8_t* const rv       = page + tl.off[0];
-    Prefetch<2>          pf, pr;
-    uint32_t             lb = 0;
-    auto                 fetch = [&](uint32_t k) {
-        const uint32_t bw = readlane_t(m_bw, k), ep = readlane_t(m_ep, k);
-        const uint32_t r0 = readlane_t(m_r0, k), r1 = readlane_t(m_r1, k);
-        prefetch_rows(pf, pk + ep, 8 * bw, lane, page);
-        const uint32_t nb = r1 - r0;
-        prefetch_rows(pr, rv + r0, nb <= kRgRunsLds ? (nb + 15) / 16 : 0u, lane, page);
-        lb = *reinterpret_cast<const u16_ua*>(page + tl.off[1] + 128ull * (tl.v0 + k) + 2 * lane);
-    };
-    fetch(0);
-    bool bad = false;
-    for (uint32_t k = 0; k < nv; ++k) {
-        const uint32_t bw = readlane_t(m_bw, k), base = readlane_t(m_base, k);
-        const uint32_t r0 = readlane_t(m_r0, k), r1 = readlane_t(m_r1, k);
-        const uint32_t nb = r1 - r0, n_runs = nb / SZ;
-        const bool     in_lds = nb <= kRgRunsLds;
-        stage_rows(pf, lds, 8 * bw, lane);
-        stage_rows(pr, runs_lds, (nb + 15) / 16, lane);
-        const uint32_t clb = lb;
-        if (k + 1 < nv) fetch(k + 1);
-        wave_sync();
-        const uint32_t mask = (uint32_t)bw_mask<16>(bw), last = n_runs ? n_runs - 1 : 0;
-        WordOut<T>     o {obase + (uint64_t)(tl.v0 + k) * (1024 * SZ) + SZ * seg_start<16>(lane), {}};
+    constexpr unsigned SZ   = sizeof(T);
+    const unsigned     lane = opaque_lane(lane_in);
+    const uint32_t     ep0 = readlane_t(m.ep, 0), r00 = readlane_t(m.r0, 0);
+    bool               bad = false;
+    for (uint32_t k = 0; k < t.nv; ++k) {
+        const uint32_t bw = readlane_t(m.bw, k), base = readlane_t((uint32_t)m.base, k);
+        const uint32_t r0 = readlane_t(m.r0, k), r1 = readlane_t(m.r1, k);
+        const uint32_t n_runs = (r1 - r0) / SZ, last = n_runs ? n_runs - 1 : 0;
+        const uint8_t* rows   = lds + (readlane_t(m.ep, k) - ep0);
+        const T*       runs   = reinterpret_cast<const T*>(lds + t.rlds[1] + (r0 - r00));
+        const uint32_t mask   = (uint32_t)bw_mask<16>(bw);
+        WordOut<T>     o {obase + (uint64_t)(t.v0 + k) * (1024 * SZ) + SZ * seg_start<16>(lane), {}};
         RowPair<16>    pr[16];
-        static_for<16>([&](auto r) { pr[r] = lane_fetch<16>(lds, lane, j_of_p<16>(decltype(r)::value) * bw); });
+        static_for<16>([&](auto r) { pr[r] = lane_fetch<16>(rows, lane, j_of_p<16>(decltype(r)::value) * bw); });
         batch_fence();
         uint32_t idx[16];
-        uint32_t acc = clb;
+        uint32_t acc = *reinterpret_cast<const uint16_t*>(lds + t.rlds[2] + 128 * k + 2 * lane);
         static_for<16>([&](auto r) {
             constexpr unsigned R = decltype(r)::value;
             acc                  = (acc + (uint32_t)lane_extract<16>(pr[R], j_of_p<16>(R) * bw, mask) + base) & 0xffffu;
             bad |= acc > last;
             idx[R] = acc > last ? last : acc;
         });
-        static_for<2>([&](auto hh) {  // gathers in two batches of 8 values
+        static_for<2>([&](auto hh) {  // run values in two batches of 8
             constexpr unsigned H = 8 * decltype(hh)::value;
             T                  x[8];
-            if (in_lds) static_for<8>([&](auto r) { x[r] = reinterpret_cast<const T*>(runs_lds)[idx[H + r]]; });
-            else static_for<8>([&](auto r) { x[r] = load_t<T>(rv + r0 + (uint64_t)idx[H + r] * SZ); });
+            static_for<8>([&](auto r) { x[r] = runs[idx[H + r]]; });
             batch_fence();
             static_for<8>([&](auto r) { o.template put<H + decltype(r)::value>(x[r]); });
         });
         bad |= n_runs == 0;  // the host rejects such pages
-        wave_sync();
     }
     if (bad) atomicOr(err, 1u);
 }
@@ -393,54 +387,27 @@ __device__ __forceinline__ void rg_rle(const uint8_t* page, const RgTile& tl, ui
 // --------------------------------------------------------------------------------------------------------------
 // Dictionary pages (include/fls/cor/exp/dict/dict_exp.hpp): u32 codes, VAR 1 [codes], VAR 2 [bw][base][packed],
 // VAR 7 [lane bases][bw][base][packed]; out[i] = dict[code[i]] from the page's decoded dictionary (D-typed values,
-// D = max(32, bits of T)). Codes past the dictionary are clamped (err bit 2).
+// D = max(32, bits of T)). Range 0 holds the code rows, range 1 the DELTA lane bases. Codes past the dictionary
+// are clamped (err bit 2).
 // --------------------------------------------------------------------------------------------------------------
 template <typename T, int VAR>
-__device__ __forceinline__ void rg_dict(const uint8_t* page, const RgTile& tl, uint8_t* obase, uint8_t* lds,
+__device__ __forceinline__ void rg_dict(const RgTile& t, const TileMeta& m, uint8_t* obase, const uint8_t* lds,
                                         unsigned lane_in, uint32_t* err) {
-    const unsigned lane = opaque_lane(lane_in);
     using D               = typename uint_of<(sizeof(T) < 4 ? 32 : 8 * sizeof(T))>::type;
     constexpr unsigned SZ = sizeof(T);
-    constexpr unsigned O  = VAR == 7 ? 1 : 0;
-    constexpr unsigned P  = VAR == 1 ? 0 : (VAR == 7 ? 3 : 2);
-    const uint32_t     nv = tl.nv;
-    const D*           dict = reinterpret_cast<const D*>(tl.dict);
-    const uint32_t     last = tl.dict_n - 1;  // the host only builds dictionary tiles with dict_n > 0
-    uint32_t           m_bw = 32, m_ep = 0, m_base = 0;
-    if (lane < nv) {
-        const uint32_t v = tl.v0 + lane;
-        if constexpr (VAR == 1) {
-            m_ep = v * 4096u;
-        } else {
-            const uint32_t b = page[tl.off[O] + v];
-            m_bw             = b < 32u ? b : 32u;
-            m_base           = *reinterpret_cast<const u32_ua*>(page + tl.off[O + 1] + 4ull * v);
-            m_ep             = ep_at(page, tl.arr_c, v, P);
-        }
-    }
-    const uint8_t* const pk = page + tl.off[P];
-    Prefetch<4>          pf;
-    uint64_t             lb = 0, unused = 0;
-    {
-        const uint32_t bw = __builtin_amdgcn_readfirstlane(m_bw), ep = __builtin_amdgcn_readfirstlane(m_ep);
-        prefetch_rows(pf, pk + ep, 8 * bw, lane, page);
-        if constexpr (VAR == 7) load_lane_bases<32>(page + tl.off[0] + 128ull * tl.v0, lane, lb, unused);
-    }
-    const unsigned i = lane % 32, h = lane / 32;
-    bool           bad = false;
-    for (uint32_t k = 0; k < nv; ++k) {
-        const uint32_t bw = readlane_t(m_bw, k), base = readlane_t(m_base, k);
-        stage_rows(pf, lds, 8 * bw, lane);
-        const uint32_t clb = (uint32_t)lb;
-        if (k + 1 < nv) {
-            prefetch_rows(pf, pk + readlane_t(m_ep, k + 1), 8 * readlane_t(m_bw, k + 1), lane, page);
-            if constexpr (VAR == 7) load_lane_bases<32>(page + tl.off[0] + 128ull * (tl.v0 + k + 1), lane, lb, unused);
-        }
-        wave_sync();
+    const unsigned     lane = opaque_lane(lane_in);
+    const auto*        dict = reinterpret_cast<const __attribute__((address_space(1))) D*>(t.dict);  // global
+    const uint32_t     last = t.dict_n - 1;  // the host only builds dictionary tiles with dict_n > 0
+    const uint32_t     ep0  = readlane_t(m.ep, 0);
+    const unsigned     i = lane % 32, h = lane / 32;
+    bool               bad = false;
+    for (uint32_t k = 0; k < t.nv; ++k) {
+        const uint32_t bw = readlane_t(m.bw, k), base = readlane_t((uint32_t)m.base, k);
+        const uint8_t* rows = lds + (readlane_t(m.ep, k) - ep0);
         const uint32_t mask = (uint32_t)bw_mask<32>(bw), hb = h * bw;
         auto           bit_of = [&](unsigned R) { return (4 * (R % 8) + 2 * (R / 8)) * bw + hb; };
         RowPair<32>    pr[16];
-        static_for<16>([&](auto r) { pr[r] = lane_fetch<32>(lds, i, bit_of(decltype(r)::value)); });
+        static_for<16>([&](auto r) { pr[r] = lane_fetch<32>(rows, i, bit_of(decltype(r)::value)); });
         batch_fence();
         uint32_t code[16];
         uint32_t acc = 0;
@@ -451,8 +418,9 @@ __device__ __forceinline__ void rg_dict(const uint8_t* page, const RgTile& tl, u
             code[R] = c;
         });
         if constexpr (VAR == 7) {  // DELTA codes: the second half of a lane adds the first half's total
+            const uint32_t lb  = *reinterpret_cast<const uint32_t*>(lds + t.rlds[1] + 128 * k + 4 * i);
             const uint32_t y   = __shfl_up(acc, 32, 64);
-            const uint32_t add = clb + (h ? y : 0u);
+            const uint32_t add = lb + (h ? y : 0u);
             static_for<16>([&](auto r) { code[r] += add; });
         }
         T x[16];
@@ -462,92 +430,100 @@ __device__ __forceinline__ void rg_dict(const uint8_t* page, const RgTile& tl, u
             x[r] = (T)dict[c > last ? last : c];
         });
         batch_fence();
-        WordOut<T> o {obase + (uint64_t)(tl.v0 + k) * (1024 * SZ) + SZ * (seg_start<32>(i) + 16 * h), {}};
+        WordOut<T> o {obase + (uint64_t)(t.v0 + k) * (1024 * SZ) + SZ * (seg_start<32>(i) + 16 * h), {}};
         static_for<16>([&](auto r) { o.template put<decltype(r)::value>(x[r]); });
-        wave_sync();
     }
     if (bad) atomicOr(err, 2u);
 }
 
 }  // namespace
 
-__global__ __launch_bounds__(64 * kRgWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_rowgroup(const uint8_t* __restrict__ enc, uint8_t* __restrict__ out,
-                                                            const RgTile* __restrict__ tiles, uint32_t n_tiles,
-                                                            uint32_t* __restrict__ counter, uint32_t* err,
-                                                            uint32_t wave_lds) {
+__global__ __launch_bounds__(64 * kRgWaves) void k_rowgroup(
+    const uint8_t* __restrict__ enc, uint8_t* __restrict__ out, const RgTile* __restrict__ tiles,
+    const uint32_t* __restrict__ wave_begin, uint32_t* __restrict__ err, uint32_t wave_lds) {
     extern __shared__ __attribute__((aligned(16))) uint8_t rg_lds[];
-    const unsigned lane = threadIdx.x % 64;
-    uint8_t* const lds  = rg_lds + (threadIdx.x / 64) * wave_lds;
-    // Each wave claims its next tile while it decodes the current one, so the claim's round trip overlaps work.
-    auto claim = [&]() {
-        uint32_t id = 0;
-        if (lane == 0) id = atomicAdd(counter, 1u);
-        return __builtin_amdgcn_readfirstlane(id);  // lane 0's claim, in a scalar register: the loop stays uniform
-    };
-    uint32_t next = claim();
-    for (;;) {
-        const uint32_t id = next;
-        if (id >= n_tiles) break;
-        next = claim();
-        const RgTile   tl   = tiles[id];
-        const uint8_t* page = enc + tl.page;
-        uint8_t*       ob   = out + tl.out;
-        switch (tl.kind) {
-        case RG_UNC8: rg_int<8, 0>(page, tl, ob, lds, lane); break;
-        case RG_UNC16: rg_int<16, 0>(page, tl, ob, lds, lane); break;
-        case RG_UNC32: rg_int<32, 0>(page, tl, ob, lds, lane); break;
-        case RG_UNC64: rg_int<64, 0>(page, tl, ob, lds, lane); break;
-        case RG_FFOR8: rg_int<8, 1>(page, tl, ob, lds, lane); break;
-        case RG_FFOR16: rg_int<16, 1>(page, tl, ob, lds, lane); break;
-        case RG_FFOR32: rg_int<32, 1>(page, tl, ob, lds, lane); break;
-        case RG_FFOR64: rg_int<64, 1>(page, tl, ob, lds, lane); break;
-        case RG_DELTA8: rg_int<8, 2>(page, tl, ob, lds, lane); break;
-        case RG_DELTA16: rg_int<16, 2>(page, tl, ob, lds, lane); break;
-        case RG_DELTA32: rg_int<32, 2>(page, tl, ob, lds, lane); break;
-        case RG_DELTA64: rg_int<64, 2>(page, tl, ob, lds, lane); break;
-        case RG_RLE8: rg_rle<uint8_t>(page, tl, ob, lds, lane, err); break;
-        case RG_RLE16: rg_rle<uint16_t>(page, tl, ob, lds, lane, err); break;
-        case RG_RLE32: rg_rle<uint32_t>(page, tl, ob, lds, lane, err); break;
-        case RG_RLE64: rg_rle<uint64_t>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT1_8: rg_dict<uint8_t, 1>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT1_16: rg_dict<uint16_t, 1>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT1_32: rg_dict<uint32_t, 1>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT1_64: rg_dict<uint64_t, 1>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT2_8: rg_dict<uint8_t, 2>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT2_16: rg_dict<uint16_t, 2>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT2_32: rg_dict<uint32_t, 2>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT2_64: rg_dict<uint64_t, 2>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT7_8: rg_dict<uint8_t, 7>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT7_16: rg_dict<uint16_t, 7>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT7_32: rg_dict<uint32_t, 7>(page, tl, ob, lds, lane, err); break;
-        case RG_DICT7_64: rg_dict<uint64_t, 7>(page, tl, ob, lds, lane, err); break;
+    // the wave index is uniform over the wave: readfirstlane tells the compiler so, and everything derived from it
+    // (the run bounds, the tile structs, the switch over the tile kind) lives in scalar registers
+    const unsigned lane = threadIdx.x % 64, wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    uint8_t* const lds  = rg_lds + wave * wave_lds;
+    // this wave's run of the work list (the host balanced the runs by bytes; no tile is shared between waves)
+    const uint32_t gw = blockIdx.x * kRgWaves + wave;
+    const uint32_t end = wave_begin[gw + 1];
+    uint32_t       id  = wave_begin[gw];
+    // software pipeline over the run: tile id + 1's struct, metadata and staged ranges are in flight while tile id
+    // decodes from LDS
+    RgTile   tn {};
+    TileMeta mn {};
+    Prefetch pf;
+    if (id < end) {
+        tn = load_tile(tiles, id);
+        load_meta(enc + tn.page, tn, lane, mn);
+        load_chunks(enc + tn.page, tn, lane, pf);
+    }
+    for (; id < end; ++id) {
+        const RgTile   tc = tn;
+        const TileMeta mc = mn;
+        stage_chunks(enc + tc.page, tc, lane, pf, lds, wave_lds);
+        if (id + 1 < end) {
+            tn = load_tile(tiles, id + 1);
+            load_meta(enc + tn.page, tn, lane, mn);
+            load_chunks(enc + tn.page, tn, lane, pf);
+        }
+        wave_sync();
+        uint8_t* ob = out + tc.out;
+        switch (tc.kind) {
+        case RG_UNC8: rg_int<8, 0>(tc, mc, ob, lds, lane); break;
+        case RG_UNC16: rg_int<16, 0>(tc, mc, ob, lds, lane); break;
+        case RG_UNC32: rg_int<32, 0>(tc, mc, ob, lds, lane); break;
+        case RG_UNC64: rg_int<64, 0>(tc, mc, ob, lds, lane); break;
+        case RG_FFOR8: rg_int<8, 1>(tc, mc, ob, lds, lane); break;
+        case RG_FFOR16: rg_int<16, 1>(tc, mc, ob, lds, lane); break;
+        case RG_FFOR32: rg_int<32, 1>(tc, mc, ob, lds, lane); break;
+        case RG_FFOR64: rg_int<64, 1>(tc, mc, ob, lds, lane); break;
+        case RG_DELTA8: rg_int<8, 2>(tc, mc, ob, lds, lane); break;
+        case RG_DELTA16: rg_int<16, 2>(tc, mc, ob, lds, lane); break;
+        case RG_DELTA32: rg_int<32, 2>(tc, mc, ob, lds, lane); break;
+        case RG_DELTA64: rg_int<64, 2>(tc, mc, ob, lds, lane); break;
+        case RG_RLE8: rg_rle<uint8_t>(tc, mc, ob, lds, lane, err); break;
+        case RG_RLE16: rg_rle<uint16_t>(tc, mc, ob, lds, lane, err); break;
+        case RG_RLE32: rg_rle<uint32_t>(tc, mc, ob, lds, lane, err); break;
+        case RG_RLE64: rg_rle<uint64_t>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT1_8: rg_dict<uint8_t, 1>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT1_16: rg_dict<uint16_t, 1>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT1_32: rg_dict<uint32_t, 1>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT1_64: rg_dict<uint64_t, 1>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT2_8: rg_dict<uint8_t, 2>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT2_16: rg_dict<uint16_t, 2>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT2_32: rg_dict<uint32_t, 2>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT2_64: rg_dict<uint64_t, 2>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT7_8: rg_dict<uint8_t, 7>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT7_16: rg_dict<uint16_t, 7>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT7_32: rg_dict<uint32_t, 7>(tc, mc, ob, lds, lane, err); break;
+        case RG_DICT7_64: rg_dict<uint64_t, 7>(tc, mc, ob, lds, lane, err); break;
         default:
             if (lane == 0) atomicOr(err, 8u);
             break;
         }
+        wave_sync();
     }
 }
-
-__global__ void k_rowgroup_reset(uint32_t* counter) { *counter = 0u; }
 
 unsigned rg_kind_bits(int kind) {
     if (kind < 0 || kind >= RG_KIND_COUNT) return 0;
     return 8u << (kind % 4);
 }
 
-unsigned rg_wave_lds(int kind) {
-    if (kind < 0 || kind >= RG_KIND_COUNT) return 0;
-    const unsigned W = rg_kind_bits(kind);
-    if (kind <= RG_DELTA64) return 128 * W + 128;            // packed rows (bw <= W) + pad row
-    if (kind <= RG_RLE64) return 128 * 17 + kRgRunsLds + 16;  // u16 positions (bw <= 16) + pad row, run values
-    return 128 * 33;                                          // u32 codes (bw <= 32) + pad row
+unsigned rg_blocks_per_cu(unsigned wave_lds) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_rowgroup, 64 * kRgWaves, wave_lds * kRgWaves) != hipSuccess)
+        return 1;
+    return n > 0 ? (unsigned)n : 1u;
 }
 
-hipError_t launch_rowgroup(const uint8_t* enc, uint8_t* out, const RgTile* tiles, uint32_t n_tiles, uint32_t* counter,
+hipError_t launch_rowgroup(const uint8_t* enc, uint8_t* out, const RgTile* tiles, const uint32_t* wave_begin,
                            uint32_t* err, unsigned wave_lds, unsigned blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_rowgroup_reset, dim3(1), dim3(1), 0, s, counter);
-    hipLaunchKernelGGL(k_rowgroup, dim3(blocks), dim3(64 * kRgWaves), wave_lds * kRgWaves, s, enc, out, tiles, n_tiles,
-                       counter, err, wave_lds);
+    hipLaunchKernelGGL(k_rowgroup, dim3(blocks), dim3(64 * kRgWaves), wave_lds * kRgWaves, s, enc, out, tiles,
+                       wave_begin, err, wave_lds);
     return hipGetLastError();
 }
 

@@ -23,43 +23,67 @@ enum RgKind : uint16_t {
 };
 
 // Vectors [v0, v0 + nv) of one page (nv <= 64: a wave loads the tile's per-vector metadata one vector per lane).
-// 64 bytes, read by the kernel with scalar loads.
+// The tile's input bytes are up to three contiguous ranges of its page, staged whole into the wave's LDS slot:
+//   integer pages   0: packed rows (UNCOMPRESSED: the values)      1: DELTA lane bases
+//   RLE pages       0: packed u16 position rows                    1: run values        2: position lane bases
+//   dictionary      0: packed u32 code rows (UNCOMPRESSED: codes)  1: DELTA lane bases
+// Range 0 is followed in LDS by one pad row of 128 bytes (the funnel shift of a vector's last value reads the row
+// after its packed rows; those bits are masked off). 128 bytes, read by the kernel with scalar loads.
 struct RgTile {
-    uint64_t page;    // byte offset of the page in the row-group's encoded buffer
-    uint64_t out;     // byte offset of the page's first value in the output buffer
-    uint64_t dict;    // device address of the page's decoded dictionary (dictionary kinds), else 0
-    uint32_t v0;      // first vector of the tile, page-relative
-    uint32_t dict_n;  // dictionary entries
-    uint16_t nv;      // vectors in the tile, 1..64
-    uint16_t kind;    // RgKind
-    uint16_t arr_c;   // buffers of the page
-    uint16_t pad;
-    uint32_t off[5];  // buffer offsets from the page start
-    uint32_t vec_c;   // vectors in the page
+    uint64_t page;     // byte offset of the page in the row-group's encoded buffer
+    uint64_t out;      // byte offset of the page's first value in the output buffer
+    uint64_t dict;     // device address of the page's decoded dictionary (dictionary kinds), else 0
+    uint32_t v0;       // first vector of the tile, page-relative
+    uint32_t dict_n;   // dictionary entries
+    uint32_t vec_c;    // vectors in the page
+    uint16_t nv;       // vectors in the tile, 1..64
+    uint16_t kind;     // RgKind
+    uint16_t arr_c;    // buffers of the page
+    uint16_t n_rng;    // staged ranges, 1..3
+    uint32_t off[5];   // buffer offsets from the page start
+    uint32_t rsrc[3];  // staged ranges: byte offset from the page start
+    uint32_t rlen[3];  //                bytes
+    uint32_t rlds[3];  //                byte offset in the wave's LDS slot (16-aligned)
+    uint32_t chunks;   // 16-byte chunks over all ranges
+    // chunk c (0 <= c < chunks) of range r is page byte 16 c + sdelta[r] and slot byte 16 c + ldelta[r]; range 1
+    // starts at chunk ceil(rlen[0] / 16), range 2 at ceil(rlen[0] / 16) + ceil(rlen[1] / 16)
+    int32_t  sdelta[3];
+    int32_t  ldelta[3];
 };
-static_assert(sizeof(RgTile) == 64, "RgTile is read as 64 bytes");
+static_assert(sizeof(RgTile) == 128, "RgTile is read as 128 bytes");
+
 
 constexpr unsigned kRgMaxTileVec = 64;    // vectors per tile (one metadata lane each)
-constexpr unsigned kRgRunsLds    = 2048;  // RLE run values staged in LDS when they fit (bytes per vector)
 constexpr unsigned kRgWaves      = 4;     // waves per block
+constexpr unsigned kRgPrefetch   = 8;     // 16-byte chunks per lane held in registers ahead of a tile's decode
+constexpr unsigned kRgStage      = 1024 * kRgPrefetch;  // staged bytes the registers carry (larger tiles copy the
+                                                        // rest when they are staged)
+constexpr unsigned kRgDump       = 16;    // LDS bytes at the end of a wave slot for the lanes without a chunk
 
-// LDS bytes one wave needs for a kind: the packed rows of one vector plus one pad row (the funnel shift of the
-// last value reads the row after the packed data; its bits are masked off), and for RLE the run values.
-unsigned rg_wave_lds(int kind);
 // Element width (bits) of a kind's decoded values.
 unsigned rg_kind_bits(int kind);
 
 // Host check of a tile list against the pages it refers to (run before every upload): every tile names a known
-// kind, 1..64 vectors inside its page, buffer offsets inside the page, an output range inside the row-group's
-// output, and a dictionary when its kind needs one. Returns "" or the first violation.
+// kind, 1..64 vectors inside its page, buffer offsets and staged ranges inside the page, ranges that fit the
+// wave's LDS slot, an output range inside the row-group's output, and a dictionary when its kind needs one.
+// Returns "" or the first violation.
 struct RgPageExtent {
     uint64_t page, page_size;  // byte range of the page in the encoded buffer
     uint64_t out, out_bytes;   // byte range of the page's decoded values
 };
 std::string rg_validate_tiles(const std::vector<RgTile>& tiles, const std::vector<RgPageExtent>& pages,
-                              uint64_t enc_size, uint64_t out_size);
+                              uint64_t enc_size, uint64_t out_size, unsigned wave_lds);
 
-hipError_t launch_rowgroup(const uint8_t* enc, uint8_t* out, const RgTile* tiles, uint32_t n_tiles, uint32_t* counter,
+// Orders the tiles into one contiguous run per wave of a launch of `waves` waves and returns the runs' bounds
+// (waves + 1 entries): longest tiles first, each to the wave with the least estimated work so far (decoded plus
+// staged bytes, plus a fixed cost per vector); within a run, tiles keep their list order.
+std::vector<uint32_t> rg_assign_waves(std::vector<RgTile>& tiles, uint32_t waves);
+
+// Blocks of the persistent kernel one compute unit holds at once with `wave_lds` LDS bytes per wave.
+unsigned rg_blocks_per_cu(unsigned wave_lds);
+
+// One persistent launch: `blocks` blocks of kRgWaves waves, wave w decoding tiles [wave_begin[w], wave_begin[w+1]).
+hipError_t launch_rowgroup(const uint8_t* enc, uint8_t* out, const RgTile* tiles, const uint32_t* wave_begin,
                            uint32_t* err, unsigned wave_lds, unsigned blocks, hipStream_t s);
 
 }  // namespace fls

@@ -79,7 +79,7 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
 
     res = {"columns": n_columns, "values": n_values, "decoded_bytes": dec_bytes, "encoded_bytes": enc_bytes,
            "host_build_s": round(t_build, 1)}
-    tiles, blocks, wave_lds = rg.work_list()
+    tiles, blocks, wave_lds, _ = rg.work_list()
     res["work_list"] = {"tiles": int(tiles.size), "blocks": blocks, "lds_per_wave": wave_lds}
     forms = {"persistent": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, sp),
              "graph": lambda: rg.decode_graph(d_enc.data_ptr(), d_out.data_ptr(), sp),

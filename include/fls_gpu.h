@@ -136,15 +136,19 @@ int  fls_gpu_rowgroup_decode(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, u
 /* Captures mode 0 once in a HIP graph and replays it (one graph launch per call; recaptured when the buffers
  * change). */
 int  fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, void* stream);
-/* The persistent launch's work list: tile count, then up to `capacity` tiles of 64 bytes (the layout of RgTile in
- * fastlanes_amd/csrc/rowgroup.hpp), and the launch shape (blocks, LDS bytes per wave). Any pointer may be NULL. */
+/* The persistent launch's work list: tile count, then up to `capacity` tiles of 128 bytes (the layout of RgTile in
+ * fastlanes_amd/csrc/rowgroup.hpp) in launch order, the launch shape (blocks of 4 waves, LDS bytes per wave) and
+ * each wave's run of the list (blocks * 4 + 1 bounds: wave w decodes tiles [wave_begin[w], wave_begin[w + 1])).
+ * Any pointer may be NULL. */
 int  fls_gpu_rowgroup_work_list(const fls_gpu_rowgroup* rg, uint32_t* n_tiles, void* tiles, uint32_t capacity,
-                                uint32_t* blocks, uint32_t* wave_lds);
+                                uint32_t* blocks, uint32_t* wave_lds, uint32_t* wave_begin);
 /* The same work list planned on the host alone (no device memory is touched; dictionary tiles carry a nonzero
- * placeholder instead of a dictionary address), with the same validation, for inspecting a row-group's tiles. */
+ * placeholder instead of a dictionary address), with the same validation, for inspecting a row-group's tiles.
+ * waves > 0 also assigns the tiles to that many waves (tiles in launch order, waves + 1 bounds in wave_begin). */
 int  fls_rowgroup_plan_work_list(uint32_t n_columns, const uint8_t* const* host_columns, const uint64_t* column_sizes,
-                                 const uint64_t* device_offsets, const uint8_t* types, uint32_t* n_tiles, void* tiles,
-                                 uint32_t capacity, uint32_t* wave_lds);
+                                 const uint64_t* device_offsets, const uint8_t* types, uint32_t waves,
+                                 uint32_t* n_tiles, void* tiles, uint32_t capacity, uint32_t* wave_begin,
+                                 uint32_t* wave_lds);
 int  fls_gpu_rowgroup_check(fls_gpu_rowgroup* rg, void* stream);
 
 #ifdef __cplusplus

@@ -39,23 +39,15 @@ def kind_parts(kind):
     return fam, 8 << (kind % 4)
 
 
-def claim_loop(n_tiles, n_waves, rng):
-    """The kernel's dispatch: every wave takes ids from one counter until an id is past the list. Waves run in a
-    random interleaving; returns the ids each wave decoded and the number of counter increments."""
-    counter, done, claims = 0, [False] * n_waves, [[] for _ in range(n_waves)]
-    steps = 0
-    while not all(done):
-        w = int(rng.integers(0, n_waves))
-        if done[w]:
-            continue
-        steps += 1
-        assert steps <= 10 * (n_tiles + n_waves), "claim loop did not end"
-        tid, counter = counter, counter + 1
-        if tid >= n_tiles:
-            done[w] = True
-        else:
-            claims[w].append(tid)
-    return claims, counter
+def wave_runs(runs):
+    """The kernel's dispatch: wave w loops over tiles [runs[w], runs[w + 1]) of the launch-order list; returns the
+    tile ids each wave decodes."""
+    return [list(range(int(runs[w]), int(runs[w + 1]))) for w in range(len(runs) - 1)]
+
+
+def tile_cost(t):
+    """The host's estimate of a tile's work (rg_assign_waves): decoded and staged bytes plus fixed costs."""
+    return int(t["nv"]) * (128 * kind_parts(int(t["kind"]))[1] + 2048) + 16 * int(t["chunks"]) + 4096
 
 
 class _Page:
@@ -69,16 +61,6 @@ class _Page:
 
     def ep(self, v, buf):
         return 0 if v == 0 else self.u(8 + 4 * self.arr_c + 4 * ((v - 1) * self.arr_c + buf), 4)
-
-
-def _rows(pg, at, bw, W):
-    """Staged rows of a vector: bw rows of L = 1024/W words, plus the zero pad row the kernel never keeps."""
-    L = 1024 // W
-    dt = W2DT[W]
-    rows = np.zeros((bw + 1, L), dtype=np.uint64)
-    if bw:
-        rows[:bw] = np.frombuffer(pg.b[at: at + 128 * bw].tobytes(), dtype=dt).reshape(bw, L).astype(np.uint64)
-    return rows
 
 
 def _lane_value(rows, i, bit, bw, W):
@@ -139,52 +121,82 @@ def _int_vector(rows, bw, base, lb, W, delta):
     return out
 
 
+def stage_image(enc, t):
+    """The wave's LDS slot after the kernel stages tile t: each range's bytes (whole 16-byte chunks) at its LDS
+    offset; the pad row after range 0 and the rest of the slot hold garbage the decode must never use."""
+    rng = np.random.default_rng(int(t["page"]) + int(t["v0"]))
+    size = max(int(t["rlds"][r]) + (int(t["rlen"][r]) + 15) // 16 * 16 + 128 for r in range(int(t["n_rng"])))
+    img = rng.integers(0, 256, size, dtype=np.uint8)
+    page = enc[int(t["page"]):]
+    for r in range(int(t["n_rng"])):
+        n16 = (int(t["rlen"][r]) + 15) // 16 * 16
+        src, dst = int(t["rsrc"][r]), int(t["rlds"][r])
+        img[dst: dst + n16] = page[src: src + n16]
+    return img
+
+
+def _rows_at(img, at, bw, W):
+    """bw staged rows of L = 1024/W words starting at LDS byte `at`, plus the row after them (pad or next)."""
+    L = 1024 // W
+    dt = W2DT[W]
+    n = (bw + 1) * 128
+    return np.frombuffer(img[at: at + n].tobytes(), dtype=dt).reshape(bw + 1, L).astype(np.uint64)
+
+
 def emulate_tile(enc, t, out, dictionary=None):
-    """Decodes tile t of the encoded buffer `enc` into the byte buffer `out` as the kernel would. Returns the error
-    bits the kernel would set (1: RLE position clamped, 2: dictionary code clamped)."""
+    """Decodes tile t of the encoded buffer `enc` into the byte buffer `out` as the kernel does: the metadata comes
+    from the page (one vector per lane), every staged byte from the tile's LDS image at the offsets the kernel
+    computes (vector k's rows at ep_k - ep_0 in range 0, lane bases at range 1/2 + 128 k, run values at range 1 +
+    r0_k - r0_0). Returns the error bits the kernel would set (1: RLE position clamped, 2: dictionary code
+    clamped)."""
     fam, W = kind_parts(int(t["kind"]))
     pg = _Page(enc, t)
     dt = W2DT[W]
+    img = stage_image(enc, t)
+    rl = [int(x) for x in t["rlds"]]
+    nv = int(t["nv"])
+    v0 = int(t["v0"])
+    meta = []
+    for k in range(nv):  # the kernel's per-lane metadata loads
+        v = v0 + k
+        if fam == "unc":
+            meta.append((W, 0, v * 128 * W, 0, 0))
+        elif fam in ("ffor", "delta"):
+            O, P = (1, 3) if fam == "delta" else (0, 2)
+            meta.append((min(int(pg.b[pg.off[O] + v]), W), pg.u(pg.off[O + 1] + v * (W // 8), W // 8), pg.ep(v, P), 0, 0))
+        elif fam == "rle":
+            r1 = pg.ep(v + 1, 0) if v + 1 < int(t["vec_c"]) else pg.off[1] - pg.off[0]
+            meta.append((min(int(pg.b[pg.off[2] + v]), 16), pg.u(pg.off[3] + 2 * v, 2), pg.ep(v, 4), pg.ep(v, 0), r1))
+        elif fam == "dict1":
+            meta.append((32, 0, v * 4096, 0, 0))
+        else:
+            O, P = (1, 3) if fam == "dict7" else (0, 2)
+            meta.append((min(int(pg.b[pg.off[O] + v]), 32), pg.u(pg.off[O + 1] + 4 * v, 4), pg.ep(v, P), 0, 0))
+    ep0, r00 = meta[0][2], meta[0][3]
     err = 0
-    for k in range(int(t["nv"])):
-        v = int(t["v0"]) + k
+    for k in range(nv):
+        v = v0 + k
+        bw, base, ep, r0, r1 = meta[k]
+        at = rl[0] + ep - ep0
         if fam in ("unc", "ffor", "delta"):
-            O = 1 if fam == "delta" else 0
-            P = 0 if fam == "unc" else (2 if fam == "ffor" else 3)
-            if fam == "unc":
-                bw, base, at = W, 0, pg.off[0] + v * 128 * W
-            else:
-                bw = int(pg.b[pg.off[O] + v])
-                base = pg.u(pg.off[O + 1] + v * (W // 8), W // 8)
-                at = pg.off[P] + pg.ep(v, P)
             lb = None
             if fam == "delta":
-                lb = np.frombuffer(pg.b[pg.off[0] + 128 * v: pg.off[0] + 128 * v + 128].tobytes(), dtype=dt)
-            vals = _int_vector(_rows(pg, at, bw, W), bw, base, lb, W, fam == "delta").astype(dt)
+                lb = np.frombuffer(img[rl[1] + 128 * k: rl[1] + 128 * k + 128].tobytes(), dtype=dt)
+            vals = _int_vector(_rows_at(img, at, bw, W), bw, base, lb, W, fam == "delta").astype(dt)
         elif fam == "rle":
-            bw = int(pg.b[pg.off[2] + v])
-            base = pg.u(pg.off[3] + 2 * v, 2)
-            r0 = pg.ep(v, 0)
-            r1 = pg.ep(v + 1, 0) if v + 1 < int(t["vec_c"]) else pg.off[1] - pg.off[0]
-            runs = np.frombuffer(pg.b[pg.off[0] + r0: pg.off[0] + r1].tobytes(), dtype=dt)
-            lb = np.frombuffer(pg.b[pg.off[1] + 128 * v: pg.off[1] + 128 * v + 128].tobytes(), dtype=np.uint16)
-            idx = _int_vector(_rows(pg, pg.off[4] + pg.ep(v, 4), bw, 16), bw, base, lb, 16, True).astype(np.int64)
-            if runs.size == 0 or (idx > runs.size - 1).any():
+            n_runs = (r1 - r0) // (W // 8)
+            runs = np.frombuffer(img[rl[1] + r0 - r00: rl[1] + r1 - r00].tobytes(), dtype=dt)
+            lb = np.frombuffer(img[rl[2] + 128 * k: rl[2] + 128 * k + 128].tobytes(), dtype=np.uint16)
+            idx = _int_vector(_rows_at(img, at, bw, 16), bw, base, lb, 16, True).astype(np.int64)
+            if n_runs == 0 or (idx > n_runs - 1).any():
                 err |= 1
-            vals = runs[np.minimum(idx, max(runs.size - 1, 0))]
+            vals = runs[np.minimum(idx, max(n_runs - 1, 0))]
         else:
             var = int(fam[-1])
-            if var == 1:
-                bw, base, at, lb = 32, 0, pg.off[0] + 4096 * v, None
-            else:
-                O = 1 if var == 7 else 0
-                P = 3 if var == 7 else 2
-                bw = int(pg.b[pg.off[O] + v])
-                base = pg.u(pg.off[O + 1] + 4 * v, 4)
-                at = pg.off[P] + pg.ep(v, P)
-                lb = np.frombuffer(pg.b[pg.off[0] + 128 * v: pg.off[0] + 128 * v + 128].tobytes(),
-                                   dtype=np.uint32) if var == 7 else None
-            codes = _int_vector(_rows(pg, at, bw, 32), bw, base, lb, 32, var == 7).astype(np.int64)
+            lb = None
+            if var == 7:
+                lb = np.frombuffer(img[rl[1] + 128 * k: rl[1] + 128 * k + 128].tobytes(), dtype=np.uint32)
+            codes = _int_vector(_rows_at(img, at, bw, 32), bw, base, lb, 32, var == 7).astype(np.int64)
             last = int(t["dict_n"]) - 1
             if (codes > last).any():
                 err |= 2
@@ -194,11 +206,11 @@ def emulate_tile(enc, t, out, dictionary=None):
     return err
 
 
-def fa_plan(columns, offsets):
-    """Encoded columns (bytes, dtype) at the given offsets -> (host buffer, tiles, LDS bytes per wave)."""
-    tiles, wave_lds = fa.plan_work_list([c for c, _ in columns], [d for _, d in columns], offsets)
+def fa_plan(columns, offsets, waves=0):
+    """Encoded columns (bytes, dtype) at the given offsets -> (host buffer, tiles, LDS bytes per wave, wave runs)."""
+    tiles, wave_lds, runs = fa.plan_work_list([c for c, _ in columns], [d for _, d in columns], offsets, waves)
     size = max(o + c.size for (c, _), o in zip(columns, offsets)) + 16
     host = np.zeros(size, dtype=np.uint8)
     for (c, _), o in zip(columns, offsets):
         host[o: o + c.size] = c
-    return host, tiles, wave_lds
+    return host, tiles, wave_lds, runs

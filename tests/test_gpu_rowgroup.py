@@ -163,9 +163,9 @@ def test_rowgroup_persistent_kernel(torch_cuda, oracle):
     torch = torch_cuda
     cols = mixed_columns()
     rg, d_enc, want = build(torch, oracle, cols)
-    tiles, blocks, wave_lds = rg.work_list()
+    tiles, blocks, wave_lds, runs = rg.work_list()
     print(f"rowgroup persistent: {tiles.size} tiles, {blocks} blocks, {wave_lds} B LDS per wave", flush=True)
-    assert tiles.size > 0 and blocks > 0
+    assert tiles.size > 0 and blocks > 0 and runs[-1] == tiles.size
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
     d_out.fill_(0x3C)
     rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, fa.stream_ptr())
@@ -179,7 +179,7 @@ def test_rowgroup_persistent_kernel(torch_cuda, oracle):
 
 
 def test_rowgroup_persistent_grid_stride(torch_cuda, oracle):
-    """Tiles of one vector and a grid of one block: every wave claims many tiles one after another, through every
+    """Tiles of one vector and one block per compute unit: every wave decodes a long run of tiles, through every
     kind of the mixed row-group."""
     torch = torch_cuda
     lib = fa.lib()
@@ -188,8 +188,8 @@ def test_rowgroup_persistent_grid_stride(torch_cuda, oracle):
         rg, d_enc, want = build(torch, oracle, mixed_columns(seed=5, n_hi=1 << 14))
     finally:
         fa.check(lib.fls_gpu_tune_set(6, 0, 255, 64, 1, 0), "tune_set")
-    tiles, blocks, _ = rg.work_list()
-    assert (tiles["nv"] == 1).all()
+    tiles, blocks, _, runs = rg.work_list()
+    assert (tiles["nv"] == 1).all() and runs.size == 4 * blocks + 1
     d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
     print(f"rowgroup persistent, one-vector tiles: {tiles.size} tiles over {blocks} blocks", flush=True)
     rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, fa.stream_ptr())

@@ -39,7 +39,7 @@ def planned():
     for e, _ in encs:
         offs.append(pos)
         pos = (pos + e.size + 255) // 256 * 256
-    host, tiles, wave_lds = emu.fa_plan(encs, offs)
+    host, tiles, wave_lds, _ = emu.fa_plan(encs, offs)
     return cols, encs, offs, host, tiles, wave_lds
 
 
@@ -74,32 +74,48 @@ def test_fused_kinds_and_fallback_columns(planned):
 
 
 def test_tiles_fit_the_launch_lds(planned):
-    """Every vector's staged rows (128 * bw bytes plus one pad row) and RLE run values fit the LDS of one wave."""
+    """Every tile's staged ranges are the vectors' bytes, back to back: range 0 holds each vector's rows (128 * bw
+    bytes; UNCOMPRESSED values and codes at their fixed size), ranges 1 and 2 the lane bases (128 bytes per vector)
+    or the run values; and they fit the LDS slot of one wave with range 0's pad row."""
     cols, encs, offs, host, tiles, wave_lds = planned
     for t in tiles:
         fam, W = emu.kind_parts(int(t["kind"]))
         pg = emu._Page(host, t)
-        for v in range(int(t["v0"]), int(t["v0"]) + int(t["nv"])):
-            if fam in ("ffor", "delta"):
-                need = 128 * (int(pg.b[pg.off[1 if fam == "delta" else 0] + v]) + 1)
-            elif fam == "unc":
-                need = 128 * (W + 1)
-            elif fam == "rle":
-                need = 128 * 17 + 2048 + 16
-            elif fam == "dict1":
-                need = 128 * 33
-            else:
-                need = 128 * (int(pg.b[pg.off[1 if fam == "dict7" else 0] + v]) + 1)
-            assert need <= wave_lds, (fam, W, v, need, wave_lds)
+        nv, v0 = int(t["nv"]), int(t["v0"])
+        if fam in ("ffor", "delta", "dict2", "dict7"):
+            O = 1 if fam in ("delta", "dict7") else 0
+            rows = sum(128 * int(pg.b[pg.off[O] + v]) for v in range(v0, v0 + nv))
+        elif fam == "unc":
+            rows = 128 * W * nv
+        elif fam == "dict1":
+            rows = 4096 * nv
+        else:
+            rows = sum(128 * int(pg.b[pg.off[2] + v]) for v in range(v0, v0 + nv))
+        assert int(t["rlen"][0]) == rows, (fam, W, v0)
+        if fam in ("delta", "dict7"):
+            assert int(t["rlen"][1]) == 128 * nv and int(t["rsrc"][1]) == pg.off[0] + 128 * v0
+        if fam == "rle":
+            r1 = pg.ep(v0 + nv, 0) if v0 + nv < int(t["vec_c"]) else pg.off[1] - pg.off[0]
+            assert int(t["rlen"][1]) == r1 - pg.ep(v0, 0) and int(t["rlen"][2]) == 128 * nv
+        end = max(int(t["rlds"][r]) + (int(t["rlen"][r]) + 15) // 16 * 16 + (128 if r == 0 else 0)
+                  for r in range(int(t["n_rng"])))
+        assert end + 16 <= wave_lds, (fam, W, end, wave_lds)
+        assert int(t["chunks"]) == sum((int(t["rlen"][r]) + 15) // 16 for r in range(3))
 
 
-@pytest.mark.parametrize("n_waves", [1, 7, 4096])
-def test_claim_loop_takes_every_tile_once_and_ends(planned, n_waves):
-    tiles = planned[4]
-    claims, counter = emu.claim_loop(int(tiles.size), n_waves, np.random.default_rng(n_waves))
-    taken = sorted(i for c in claims for i in c)
-    assert taken == list(range(tiles.size))
-    assert counter == tiles.size + n_waves  # each wave's last claim is past the list, then the wave leaves
+@pytest.mark.parametrize("n_waves", [1, 7, 512])
+def test_wave_runs_take_every_tile_once_and_balance(planned, n_waves):
+    """The launch-order list split into one run per wave holds every tile exactly once, and the runs' estimated
+    work differs by at most one tile's (longest tile first to the least-loaded wave)."""
+    cols, encs, offs = planned[:3]
+    host, tiles, wave_lds, runs = emu.fa_plan(encs, offs, waves=n_waves)
+    _, ref, _, _ = emu.fa_plan(encs, offs)
+    assert runs[0] == 0 and runs[-1] == tiles.size and (np.diff(runs.astype(np.int64)) >= 0).all()
+    key = lambda t: (int(t["page"]), int(t["v0"]))
+    assert sorted(key(t) for r in emu.wave_runs(runs) for t in tiles[r]) == sorted(key(t) for t in ref)
+    loads = [sum(emu.tile_cost(t) for t in tiles[r]) for r in emu.wave_runs(runs)]
+    if tiles.size >= n_waves:
+        assert max(loads) - min(loads) <= max(emu.tile_cost(t) for t in tiles)
 
 
 def test_emulated_kernel_decodes_every_tile_like_the_oracle(planned, oracle):
