@@ -62,7 +62,10 @@ def lib() -> C.CDLL:
     L.fls_gpu_rowgroup_decode_graph.argtypes = [_vp, _vp, _vp, _vp]
     L.fls_gpu_rowgroup_check.argtypes = [_vp, _vp]
     L.fls_gpu_tune_set.argtypes = [_int, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint]
-    L.fls_gpu_rowgroup_work_list.argtypes = [_vp, C.POINTER(_u32), _vp, _u32, C.POINTER(_u32), C.POINTER(_u32), _vp]
+    L.fls_gpu_tune_get.argtypes = [_int, C.c_uint, C.c_uint, C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                   C.POINTER(C.c_uint)]
+    L.fls_gpu_rowgroup_work_list.argtypes = [_vp, C.POINTER(_u32), _vp, _u32, C.POINTER(_u32), C.POINTER(_u32),
+                                             C.POINTER(_u32), _vp]
     L.fls_rowgroup_plan_work_list.argtypes = [_u32, C.POINTER(_vp), C.POINTER(_u64), C.POINTER(_u64),
                                               C.POINTER(C.c_uint8), _u32, C.POINTER(_u32), _vp, _u32, _vp,
                                               C.POINTER(_u32)]
@@ -239,13 +242,14 @@ class RowGroup:
     def work_list(self):
         """(tiles as a RG_TILE structured array in launch order, blocks, LDS bytes per wave, wave runs) of the
         persistent launch: wave w decodes tiles[runs[w]:runs[w + 1]]."""
-        n, blocks, wlds = _u32(0), _u32(0), _u32(0)
-        check(lib().fls_gpu_rowgroup_work_list(self.h, C.byref(n), None, 0, C.byref(blocks), None, None),
-              "rowgroup_work_list")
+        n, blocks, waves, wlds = _u32(0), _u32(0), _u32(0), _u32(0)
+        check(lib().fls_gpu_rowgroup_work_list(self.h, C.byref(n), None, 0, C.byref(blocks), C.byref(waves), None,
+                                               None), "rowgroup_work_list")
         tiles = np.zeros(n.value, dtype=RG_TILE)
-        runs = np.zeros(4 * blocks.value + 1, dtype=np.uint32)
+        runs = np.zeros(waves.value * blocks.value + 1, dtype=np.uint32)
         check(lib().fls_gpu_rowgroup_work_list(self.h, C.byref(n), tiles.ctypes.data, n.value, C.byref(blocks),
-                                               C.byref(wlds), runs.ctypes.data), "rowgroup_work_list")
+                                               C.byref(waves), C.byref(wlds), runs.ctypes.data),
+              "rowgroup_work_list")
         return tiles, int(blocks.value), int(wlds.value), runs
 
     def check(self, stream: int = 0) -> None:

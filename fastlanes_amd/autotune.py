@@ -3,9 +3,11 @@
 For every kernel family (raw streams per element width and bit width; UNCOMPRESSED / FFOR / DELTA / RLE /
 dictionary pages per element width) it times candidate launch shapes -- threads per block, non-temporal stores,
 and a cap on the grid (persistent blocks) -- on the GPU and writes the fastest into a tuning table
-(`fastlanes_amd/tuning.txt` by default) that the library loads on first use.
+(`fastlanes_amd/tuning.txt` by default) that the library loads on first use. The row-group launch (family 6) is
+tuned on the mixed 64-column row-group: KiB of decoded values per tile and blocks per compute unit. Entries of
+families that a run does not tune are kept from the existing table.
 
-    python -m fastlanes_amd.autotune [--out PATH] [--log2n 26] [--families stream,pages]
+    python -m fastlanes_amd.autotune [--out PATH] [--log2n 26] [--families stream,pages,rowgroup]
 """
 from __future__ import annotations
 
@@ -19,7 +21,9 @@ import numpy as np
 import fastlanes_amd as fa
 from fastlanes_amd import workloads as wl
 
-FAM_STREAM, FAM_UNC, FAM_FFOR, FAM_DELTA, FAM_RLE, FAM_DICT = range(6)
+FAM_STREAM, FAM_UNC, FAM_FFOR, FAM_DELTA, FAM_RLE, FAM_DICT, FAM_ROWGROUP = range(7)
+RG_TILE_KIB = (16, 32, 64, 128)
+RG_CU_BLOCKS = (0, 8, 16, 32, 64, 128)  # 0 = resident blocks only (one long run per wave)
 BLOCKS = (128, 256, 512)
 NTS = (0, 1)
 CU_BLOCKS = (0, 1, 2, 4, 8)  # grid cap in blocks per compute unit (0 = one block per group of vectors)
@@ -128,6 +132,39 @@ def main():
                 lines.append(f"{fam} {W} 255 {blk} {nt} {mb}   # {gbs:.0f}")
                 print(lines[-1], flush=True)
                 del plan, d_in, d_out
+    if "rowgroup" in fams:
+        from fastlanes_amd import rowgroup_bench as rb
+
+        vals, encs, offs, _, enc_total = rb.build()
+        host = np.zeros(enc_total + 16, dtype=np.uint8)
+        for o, e in zip(offs, encs):
+            host[o: o + e.size] = e
+        d_enc = torch.from_numpy(host).to(dev)
+        del host
+        moved = sum(e.size for e in encs) + sum(x.nbytes for x in vals)
+        best, d_out = None, None
+        for kib in RG_TILE_KIB:
+            for per_cu in RG_CU_BLOCKS:
+                fa.check(lib.fls_gpu_tune_set(FAM_ROWGROUP, 0, 255, kib, 1, per_cu), "tune_set")
+                rg = fa.RowGroup(encs, [x.dtype for x in vals], offs)
+                if d_out is None:
+                    d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device=dev)
+                t = _time(lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, sp), stream)
+                print(f"# row-group tile {kib} KiB, {per_cu} blocks per CU: {t * 1e3:.3f} ms", flush=True)
+                if best is None or t < best[0]:
+                    best = (t, kib, per_cu)
+                del rg
+        fa.check(lib.fls_gpu_tune_set(FAM_ROWGROUP, 0, 255, best[1], 1, best[2]), "tune_set")
+        lines.append(f"{FAM_ROWGROUP} 0 255 {best[1]} 1 {best[2]}   # {moved / best[0] / 1e9:.0f}")
+        print(lines[-1], flush=True)
+        del d_enc, d_out
+    # keep the existing entries of families this run did not tune
+    tuned = {tuple(int(v) for v in ln.split()[:3]) for ln in lines if ln and not ln.startswith("#")}
+    if os.path.exists(a.out):
+        with open(a.out) as f:
+            for ln in f.read().splitlines():
+                if ln.strip() and not ln.startswith("#") and tuple(int(v) for v in ln.split()[:3]) not in tuned:
+                    lines.append(ln)
     with open(a.out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print(f"wrote {a.out}")

@@ -348,6 +348,16 @@ int fls_gpu_tune_set(int family, unsigned width, unsigned bw, unsigned block, un
     return FLS_OK;
 }
 
+int fls_gpu_tune_get(int family, unsigned width, unsigned bw, unsigned* block, unsigned* nontemporal,
+                     unsigned* max_blocks) {
+    if (family < 0 || family >= FAM_COUNT) return fail(FLS_E_ARG, "unknown kernel family");
+    const Tune t = tune_for(family, width, bw);
+    if (block) *block = t.block;
+    if (nontemporal) *nontemporal = t.nt;
+    if (max_blocks) *max_blocks = t.max_blocks;
+    return FLS_OK;
+}
+
 int fls_gpu_tune_load(const char* path) {
     if (!path) return fail(FLS_E_ARG, "null path");
     (void)tune_for(0, 8, 0);

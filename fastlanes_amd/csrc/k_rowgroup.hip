@@ -156,11 +156,13 @@ __device__ __forceinline__ void load_meta(const uint8_t* page, const RgTile& t, 
     }
 }
 
-// Accumulates values of T into 16-byte words and stores each word as soon as it is full, so a thread holds one
-// word of output at a time. Values are put in order: put(k, x) for k = 0, 1, ... with k * sizeof(T) < bytes.
+// Accumulates values of T into 16-byte words and writes each word, as soon as it is full, to the wave's LDS output
+// buffer (byte `off` of the vector, padded: lds_pad), so a thread holds one word of output at a time. Values are put
+// in order: put(k, x) for k = 0, 1, ... with k * sizeof(T) < bytes.
 template <typename T>
 struct WordOut {
-    uint8_t* dst;
+    uint8_t* ob;
+    uint32_t off;
     v4u      w;
     template <unsigned K>
     __device__ __forceinline__ void put(T x) {
@@ -173,9 +175,22 @@ struct WordOut {
             constexpr unsigned d = e * sizeof(T) / 4, sh = 8 * (e * sizeof(T) % 4);
             w[d] = sh ? (w[d] | ((uint32_t)x << sh)) : (uint32_t)x;
         }
-        if constexpr (e == per - 1) *reinterpret_cast<v4u*>(dst + 16 * (K / per)) = w;
+        if constexpr (e == per - 1) *reinterpret_cast<v4u*>(ob + lds_pad(off + 16 * (K / per))) = w;
     }
 };
+
+// Stores the decoded vector held in the wave's output buffer: lane-consecutive 16-byte words, so every store
+// instruction writes 1 KiB of contiguous output (whole cache lines), bypassing the caches (the output is not
+// read again by this launch).
+template <unsigned VB>
+__device__ __forceinline__ void vector_out(const uint8_t* ob, uint8_t* dst, unsigned lane) {
+    wave_sync();
+    static_for<VB / 1024>([&](auto k) {
+        const uint32_t c = lane + 64u * decltype(k)::value;
+        st16<true>(dst + 16 * c, *reinterpret_cast<const v4u*>(ob + lds_pad(16 * c)));
+    });
+    wave_sync();
+}
 
 // Ends a scheduling region: the compiler issues the staged-row reads of a batch of values together (one LDS round
 // trip per batch instead of one per value) and does not move them into the neighbouring batch.
@@ -232,7 +247,7 @@ __device__ __forceinline__ uint64_t bw_mask(uint32_t bw) {
 // each thread's prefix before the stores.
 template <int W, bool DELTA>
 __device__ __forceinline__ void int_vector(const uint8_t* lds, uint32_t bw, uint64_t base, uint64_t lb0, uint64_t lb1,
-                                           unsigned t, uint8_t* dst) {
+                                           unsigned t, uint8_t* ob) {
     using T             = typename uint_of<W>::type;
     const uint64_t mask = bw_mask<W>(bw);
     if constexpr (W == 64 || W == 32) {
@@ -244,8 +259,7 @@ __device__ __forceinline__ void int_vector(const uint8_t* lds, uint32_t bw, uint
             const unsigned c = W == 64 ? 8 * (R % 8) + 4 * (R / 8) : 4 * (R % 8) + 2 * (R / 8);
             return c * bw + gb;
         };
-        uint8_t* const d = dst + (W / 8) * (W == 64 ? 64 * i + 16 * g : seg_start<32>(i) + 16 * g);
-        WordOut<T>     o {d, {}};
+        WordOut<T> o {ob, (W / 8) * (W == 64 ? 64 * i + 16 * g : seg_start<32>(i) + 16 * g), {}};
         T              x[DELTA ? 16 : 1];
         T              acc = 0;
         static_for<16 / B>([&](auto bb) {
@@ -273,7 +287,7 @@ __device__ __forceinline__ void int_vector(const uint8_t* lds, uint32_t bw, uint
             static_for<16>([&](auto r) { o.template put<decltype(r)::value>((T)(x[r] + add)); });
         }
     } else if constexpr (W == 16) {
-        WordOut<T>  o {dst + 2 * seg_start<16>(t), {}};
+        WordOut<T>  o {ob, 2 * seg_start<16>(t), {}};
         T           acc = (T)lb0;
         RowPair<16> pr[16];
         static_for<16>([&](auto r) { pr[r] = lane_fetch<16>(lds, t, j_of_p<16>(decltype(r)::value) * bw); });
@@ -285,7 +299,7 @@ __device__ __forceinline__ void int_vector(const uint8_t* lds, uint32_t bw, uint
             o.template put<R>(v);
         });
     } else {
-        WordOut<T> o {dst + seg_start<8>(t), {}};
+        WordOut<T> o {ob, seg_start<8>(t), {}};
         T          a0 = (T)lb0, a1 = (T)lb1;
         RowPair<8> pa[8], pb[8];
         static_for<8>([&](auto r) {
@@ -325,7 +339,7 @@ __device__ __forceinline__ void load_lane_bases(const uint8_t* lbuf, unsigned t,
 // --------------------------------------------------------------------------------------------------------------
 template <int W, int MODE>
 __device__ __forceinline__ void rg_int(const RgTile& t, const TileMeta& m, uint8_t* obase, const uint8_t* lds,
-                                       unsigned lane_in) {
+                                       uint8_t* ob, unsigned lane_in) {
     using T               = typename uint_of<W>::type;
     constexpr unsigned SZ = W / 8;
     const unsigned     lane = opaque_lane(lane_in);
@@ -335,8 +349,8 @@ __device__ __forceinline__ void rg_int(const RgTile& t, const TileMeta& m, uint8
         const T        base = readlane_t((T)m.base, k);
         uint64_t       lb0 = 0, lb1 = 0;
         if constexpr (MODE == 2) load_lane_bases<W>(lds + t.rlds[1] + 128 * k, lane, lb0, lb1);
-        int_vector<W, MODE == 2>(lds + (readlane_t(m.ep, k) - ep0), bw, base, lb0, lb1, lane,
-                                 obase + (uint64_t)(t.v0 + k) * (1024 * SZ));
+        int_vector<W, MODE == 2>(lds + (readlane_t(m.ep, k) - ep0), bw, base, lb0, lb1, lane, ob);
+        vector_out<1024 * SZ>(ob, obase + (uint64_t)(t.v0 + k) * (1024 * SZ), lane);
     }
 }
 
@@ -348,7 +362,7 @@ __device__ __forceinline__ void rg_int(const RgTile& t, const TileMeta& m, uint8
 // --------------------------------------------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void rg_rle(const RgTile& t, const TileMeta& m, uint8_t* obase, const uint8_t* lds,
-                                       unsigned lane_in, uint32_t* err) {
+                                       uint8_t* ob, unsigned lane_in, uint32_t* err) {
     constexpr unsigned SZ   = sizeof(T);
     const unsigned     lane = opaque_lane(lane_in);
     const uint32_t     ep0 = readlane_t(m.ep, 0), r00 = readlane_t(m.r0, 0);
@@ -360,7 +374,7 @@ __device__ __forceinline__ void rg_rle(const RgTile& t, const TileMeta& m, uint8
         const uint8_t* rows   = lds + (readlane_t(m.ep, k) - ep0);
         const T*       runs   = reinterpret_cast<const T*>(lds + t.rlds[1] + (r0 - r00));
         const uint32_t mask   = (uint32_t)bw_mask<16>(bw);
-        WordOut<T>     o {obase + (uint64_t)(t.v0 + k) * (1024 * SZ) + SZ * seg_start<16>(lane), {}};
+        WordOut<T>     o {ob, SZ * seg_start<16>(lane), {}};
         RowPair<16>    pr[16];
         static_for<16>([&](auto r) { pr[r] = lane_fetch<16>(rows, lane, j_of_p<16>(decltype(r)::value) * bw); });
         batch_fence();
@@ -379,6 +393,7 @@ __device__ __forceinline__ void rg_rle(const RgTile& t, const TileMeta& m, uint8
             batch_fence();
             static_for<8>([&](auto r) { o.template put<H + decltype(r)::value>(x[r]); });
         });
+        vector_out<1024 * SZ>(ob, obase + (uint64_t)(t.v0 + k) * (1024 * SZ), lane);
         bad |= n_runs == 0;  // the host rejects such pages
     }
     if (bad) atomicOr(err, 1u);
@@ -392,7 +407,7 @@ __device__ __forceinline__ void rg_rle(const RgTile& t, const TileMeta& m, uint8
 // --------------------------------------------------------------------------------------------------------------
 template <typename T, int VAR>
 __device__ __forceinline__ void rg_dict(const RgTile& t, const TileMeta& m, uint8_t* obase, const uint8_t* lds,
-                                        unsigned lane_in, uint32_t* err) {
+                                        uint8_t* ob, unsigned lane_in, uint32_t* err) {
     using D               = typename uint_of<(sizeof(T) < 4 ? 32 : 8 * sizeof(T))>::type;
     constexpr unsigned SZ = sizeof(T);
     const unsigned     lane = opaque_lane(lane_in);
@@ -430,8 +445,9 @@ __device__ __forceinline__ void rg_dict(const RgTile& t, const TileMeta& m, uint
             x[r] = (T)dict[c > last ? last : c];
         });
         batch_fence();
-        WordOut<T> o {obase + (uint64_t)(t.v0 + k) * (1024 * SZ) + SZ * (seg_start<32>(i) + 16 * h), {}};
+        WordOut<T> o {ob, SZ * (seg_start<32>(i) + 16 * h), {}};
         static_for<16>([&](auto r) { o.template put<decltype(r)::value>(x[r]); });
+        vector_out<1024 * SZ>(ob, obase + (uint64_t)(t.v0 + k) * (1024 * SZ), lane);
     }
     if (bad) atomicOr(err, 2u);
 }
@@ -440,14 +456,15 @@ __device__ __forceinline__ void rg_dict(const RgTile& t, const TileMeta& m, uint
 
 __global__ __launch_bounds__(64 * kRgWaves) void k_rowgroup(
     const uint8_t* __restrict__ enc, uint8_t* __restrict__ out, const RgTile* __restrict__ tiles,
-    const uint32_t* __restrict__ wave_begin, uint32_t* __restrict__ err, uint32_t wave_lds) {
+    const uint32_t* __restrict__ wave_begin, uint32_t* __restrict__ err, uint32_t wave_lds, uint32_t out_lds) {
     extern __shared__ __attribute__((aligned(16))) uint8_t rg_lds[];
     // the wave index is uniform over the wave: readfirstlane tells the compiler so, and everything derived from it
     // (the run bounds, the tile structs, the switch over the tile kind) lives in scalar registers
     const unsigned lane = threadIdx.x % 64, wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-    uint8_t* const lds  = rg_lds + wave * wave_lds;
+    uint8_t* const lds  = rg_lds + wave * (wave_lds + out_lds);  // staged ranges
+    uint8_t* const ob   = lds + wave_lds;                        // output buffer
     // this wave's run of the work list (the host balanced the runs by bytes; no tile is shared between waves)
-    const uint32_t gw = blockIdx.x * kRgWaves + wave;
+    const uint32_t gw = blockIdx.x * (blockDim.x / 64) + wave;
     const uint32_t end = wave_begin[gw + 1];
     uint32_t       id  = wave_begin[gw];
     // software pipeline over the run: tile id + 1's struct, metadata and staged ranges are in flight while tile id
@@ -470,36 +487,36 @@ __global__ __launch_bounds__(64 * kRgWaves) void k_rowgroup(
             load_chunks(enc + tn.page, tn, lane, pf);
         }
         wave_sync();
-        uint8_t* ob = out + tc.out;
+        uint8_t* obase = out + tc.out;
         switch (tc.kind) {
-        case RG_UNC8: rg_int<8, 0>(tc, mc, ob, lds, lane); break;
-        case RG_UNC16: rg_int<16, 0>(tc, mc, ob, lds, lane); break;
-        case RG_UNC32: rg_int<32, 0>(tc, mc, ob, lds, lane); break;
-        case RG_UNC64: rg_int<64, 0>(tc, mc, ob, lds, lane); break;
-        case RG_FFOR8: rg_int<8, 1>(tc, mc, ob, lds, lane); break;
-        case RG_FFOR16: rg_int<16, 1>(tc, mc, ob, lds, lane); break;
-        case RG_FFOR32: rg_int<32, 1>(tc, mc, ob, lds, lane); break;
-        case RG_FFOR64: rg_int<64, 1>(tc, mc, ob, lds, lane); break;
-        case RG_DELTA8: rg_int<8, 2>(tc, mc, ob, lds, lane); break;
-        case RG_DELTA16: rg_int<16, 2>(tc, mc, ob, lds, lane); break;
-        case RG_DELTA32: rg_int<32, 2>(tc, mc, ob, lds, lane); break;
-        case RG_DELTA64: rg_int<64, 2>(tc, mc, ob, lds, lane); break;
-        case RG_RLE8: rg_rle<uint8_t>(tc, mc, ob, lds, lane, err); break;
-        case RG_RLE16: rg_rle<uint16_t>(tc, mc, ob, lds, lane, err); break;
-        case RG_RLE32: rg_rle<uint32_t>(tc, mc, ob, lds, lane, err); break;
-        case RG_RLE64: rg_rle<uint64_t>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT1_8: rg_dict<uint8_t, 1>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT1_16: rg_dict<uint16_t, 1>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT1_32: rg_dict<uint32_t, 1>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT1_64: rg_dict<uint64_t, 1>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT2_8: rg_dict<uint8_t, 2>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT2_16: rg_dict<uint16_t, 2>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT2_32: rg_dict<uint32_t, 2>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT2_64: rg_dict<uint64_t, 2>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT7_8: rg_dict<uint8_t, 7>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT7_16: rg_dict<uint16_t, 7>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT7_32: rg_dict<uint32_t, 7>(tc, mc, ob, lds, lane, err); break;
-        case RG_DICT7_64: rg_dict<uint64_t, 7>(tc, mc, ob, lds, lane, err); break;
+        case RG_UNC8: rg_int<8, 0>(tc, mc, obase, lds, ob, lane); break;
+        case RG_UNC16: rg_int<16, 0>(tc, mc, obase, lds, ob, lane); break;
+        case RG_UNC32: rg_int<32, 0>(tc, mc, obase, lds, ob, lane); break;
+        case RG_UNC64: rg_int<64, 0>(tc, mc, obase, lds, ob, lane); break;
+        case RG_FFOR8: rg_int<8, 1>(tc, mc, obase, lds, ob, lane); break;
+        case RG_FFOR16: rg_int<16, 1>(tc, mc, obase, lds, ob, lane); break;
+        case RG_FFOR32: rg_int<32, 1>(tc, mc, obase, lds, ob, lane); break;
+        case RG_FFOR64: rg_int<64, 1>(tc, mc, obase, lds, ob, lane); break;
+        case RG_DELTA8: rg_int<8, 2>(tc, mc, obase, lds, ob, lane); break;
+        case RG_DELTA16: rg_int<16, 2>(tc, mc, obase, lds, ob, lane); break;
+        case RG_DELTA32: rg_int<32, 2>(tc, mc, obase, lds, ob, lane); break;
+        case RG_DELTA64: rg_int<64, 2>(tc, mc, obase, lds, ob, lane); break;
+        case RG_RLE8: rg_rle<uint8_t>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_RLE16: rg_rle<uint16_t>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_RLE32: rg_rle<uint32_t>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_RLE64: rg_rle<uint64_t>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT1_8: rg_dict<uint8_t, 1>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT1_16: rg_dict<uint16_t, 1>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT1_32: rg_dict<uint32_t, 1>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT1_64: rg_dict<uint64_t, 1>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT2_8: rg_dict<uint8_t, 2>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT2_16: rg_dict<uint16_t, 2>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT2_32: rg_dict<uint32_t, 2>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT2_64: rg_dict<uint64_t, 2>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT7_8: rg_dict<uint8_t, 7>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT7_16: rg_dict<uint16_t, 7>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT7_32: rg_dict<uint32_t, 7>(tc, mc, obase, lds, ob, lane, err); break;
+        case RG_DICT7_64: rg_dict<uint64_t, 7>(tc, mc, obase, lds, ob, lane, err); break;
         default:
             if (lane == 0) atomicOr(err, 8u);
             break;
@@ -513,17 +530,18 @@ unsigned rg_kind_bits(int kind) {
     return 8u << (kind % 4);
 }
 
-unsigned rg_blocks_per_cu(unsigned wave_lds) {
+unsigned rg_blocks_per_cu(unsigned slot_lds, unsigned waves) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_rowgroup, 64 * kRgWaves, wave_lds * kRgWaves) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_rowgroup, 64 * waves, slot_lds * waves) != hipSuccess)
         return 1;
     return n > 0 ? (unsigned)n : 1u;
 }
 
 hipError_t launch_rowgroup(const uint8_t* enc, uint8_t* out, const RgTile* tiles, const uint32_t* wave_begin,
-                           uint32_t* err, unsigned wave_lds, unsigned blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_rowgroup, dim3(blocks), dim3(64 * kRgWaves), wave_lds * kRgWaves, s, enc, out, tiles,
-                       wave_begin, err, wave_lds);
+                           uint32_t* err, unsigned wave_lds, unsigned out_lds, unsigned blocks, unsigned waves,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_rowgroup, dim3(blocks), dim3(64 * waves), (wave_lds + out_lds) * waves, s, enc, out, tiles,
+                       wave_begin, err, wave_lds, out_lds);
     return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ struct fls_gpu_rowgroup {
     RgTile*                    d_tiles      = nullptr;
     uint32_t*                  d_wave_begin = nullptr;
     uint32_t*                  d_err        = nullptr;
-    unsigned                   wave_lds = 0, blocks = 0;
+    unsigned                   wave_lds = 0, out_lds = 0, blocks = 0, waves = kRgWaves;  // waves per block
     // captured graph
     hipGraph_t     graph = nullptr;
     hipGraphExec_t exec  = nullptr;
@@ -372,7 +372,8 @@ int decode_fused(fls_gpu_rowgroup* rg, const uint8_t* enc, uint8_t* out, hipStre
         }
     }
     if (rg->tiles.empty()) return FLS_OK;
-    hipError_t e = launch_rowgroup(enc, out, rg->d_tiles, rg->d_wave_begin, rg->d_err, rg->wave_lds, rg->blocks, s);
+    hipError_t e = launch_rowgroup(enc, out, rg->d_tiles, rg->d_wave_begin, rg->d_err, rg->wave_lds, rg->out_lds,
+                                   rg->blocks, rg->waves, s);
     return e == hipSuccess ? FLS_OK : hip_fail(e, "row-group launch");
 }
 
@@ -417,12 +418,18 @@ int fls_gpu_rowgroup_create(uint32_t n_columns, const uint8_t* const* host_colum
     // blocks of kRgWaves waves, each wave owning a run of the list: by default as many per compute unit as the
     // kernel's registers and LDS keep resident at once; the tuning table's max_blocks sets another count per compute
     // unit (more blocks than fit: shorter runs, the later blocks start as earlier ones end)
-    {
+    // (a block holds as many of the kRgWaves waves as its LDS limit allows: each wave's slot is its staged ranges
+    // plus an output buffer for one vector of the widest fused column)
+    if (!rg->tiles.empty()) {
+        for (const RgTile& t : rg->tiles) rg->out_lds = std::max(rg->out_lds, rg_out_lds(rg_kind_bits(t.kind)));
+        const size_t slot = (size_t)rg->wave_lds + rg->out_lds;
+        rg->waves         = (unsigned)std::min<size_t>(kRgWaves, prop.sharedMemPerBlock / slot);
+        if (!rg->waves) return free_rg(rg), fail(FLS_E_UNSUPPORTED, "row-group tile exceeds the block's LDS");
         const Tune     tn     = tune_for(FAM_ROWGROUP, 0, 255);
-        const unsigned fit    = rg_blocks_per_cu(rg->wave_lds);
+        const unsigned fit    = rg_blocks_per_cu((unsigned)slot, rg->waves);
         const unsigned per_cu = std::max(1u, std::min(tn.max_blocks ? tn.max_blocks : fit, 4096u));
         rg->blocks            = (unsigned)prop.multiProcessorCount * per_cu;
-        rg->wave_begin        = rg_assign_waves(rg->tiles, rg->blocks * kRgWaves);
+        rg->wave_begin        = rg_assign_waves(rg->tiles, rg->blocks * rg->waves);
     }
     if ((e = hipMalloc(&rg->d_err, 4)) != hipSuccess) return free_rg(rg), hip_fail(e, "hipMalloc");
     if ((e = hipMemset(rg->d_err, 0, 4)) != hipSuccess) return free_rg(rg), hip_fail(e, "hipMemset");
@@ -534,13 +541,14 @@ int fls_rowgroup_plan_work_list(uint32_t n_columns, const uint8_t* const* host_c
 }
 
 int fls_gpu_rowgroup_work_list(const fls_gpu_rowgroup* rg, uint32_t* n_tiles, void* tiles, uint32_t capacity,
-                               uint32_t* blocks, uint32_t* wave_lds, uint32_t* wave_begin) {
+                               uint32_t* blocks, uint32_t* waves, uint32_t* wave_lds, uint32_t* wave_begin) {
     if (!rg) return fail(FLS_E_ARG, "null argument");
     if (wave_begin && !rg->wave_begin.empty())
         std::memcpy(wave_begin, rg->wave_begin.data(), rg->wave_begin.size() * sizeof(uint32_t));
     if (n_tiles) *n_tiles = (uint32_t)rg->tiles.size();
     if (tiles) std::memcpy(tiles, rg->tiles.data(), std::min<size_t>(capacity, rg->tiles.size()) * sizeof(RgTile));
     if (blocks) *blocks = rg->blocks;
+    if (waves) *waves = rg->waves;
     if (wave_lds) *wave_lds = rg->wave_lds;
     return FLS_OK;
 }

@@ -60,6 +60,10 @@ constexpr unsigned kRgStage      = 1024 * kRgPrefetch;  // staged bytes the regi
                                                         // rest when they are staged)
 constexpr unsigned kRgDump       = 16;    // LDS bytes at the end of a wave slot for the lanes without a chunk
 
+// LDS bytes of a wave's output buffer for W-bit values: one decoded vector (128 W bytes) with 16 pad bytes after
+// every 256 (the threads' 16-byte words land on different banks), read back lane-consecutively for the stores.
+constexpr unsigned rg_out_lds(unsigned W) { return 128 * W + 8 * W; }
+
 // Element width (bits) of a kind's decoded values.
 unsigned rg_kind_bits(int kind);
 
@@ -79,11 +83,14 @@ std::string rg_validate_tiles(const std::vector<RgTile>& tiles, const std::vecto
 // staged bytes, plus a fixed cost per vector); within a run, tiles keep their list order.
 std::vector<uint32_t> rg_assign_waves(std::vector<RgTile>& tiles, uint32_t waves);
 
-// Blocks of the persistent kernel one compute unit holds at once with `wave_lds` LDS bytes per wave.
-unsigned rg_blocks_per_cu(unsigned wave_lds);
+// Blocks of `waves` waves one compute unit holds at once with `slot_lds` LDS bytes per wave.
+unsigned rg_blocks_per_cu(unsigned slot_lds, unsigned waves);
 
-// One persistent launch: `blocks` blocks of kRgWaves waves, wave w decoding tiles [wave_begin[w], wave_begin[w+1]).
+// One launch: `blocks` blocks of `waves` waves (1..kRgWaves), wave w decoding tiles [wave_begin[w],
+// wave_begin[w+1]). Each wave's LDS slot holds its staged ranges (`wave_lds` bytes) and then its output buffer
+// (`out_lds` bytes).
 hipError_t launch_rowgroup(const uint8_t* enc, uint8_t* out, const RgTile* tiles, const uint32_t* wave_begin,
-                           uint32_t* err, unsigned wave_lds, unsigned blocks, hipStream_t s);
+                           uint32_t* err, unsigned wave_lds, unsigned out_lds, unsigned blocks, unsigned waves,
+                           hipStream_t s);
 
 }  // namespace fls

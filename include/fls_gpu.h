@@ -107,9 +107,14 @@ int fls_gpu_unffor_stream(unsigned width, unsigned bw, const void* dev_packed, c
  * next to the library (or the file named by FLS_TUNING) on first use; fastlanes_amd/autotune.py measures and writes
  * it. family: 0 raw streams, 1 UNCOMPRESSED, 2 FFOR, 3 DELTA, 4 RLE, 5 dictionary pages; bw 255 = all bit widths.
  * nontemporal: bit 0 = non-temporal stores; bit 1 = direct stores of each lane's segment instead of staging the
- * output through LDS (families 1-3 with 32/64-bit values, families 4-5 with 64-bit values). */
+ * output through LDS (families 1-3 with 32/64-bit values, families 4-5 with 64-bit values).
+ * family 6 is the row-group launch: block = KiB of decoded values per tile, max_blocks = blocks per compute unit
+ * (0: as many as stay resident at once, each wave then owning one long run of the work list).
+ * fls_gpu_tune_get reads the shape in effect for (family, width, bw); any output pointer may be NULL. */
 int fls_gpu_tune_set(int family, unsigned width, unsigned bw, unsigned block, unsigned nontemporal,
                      unsigned max_blocks);
+int fls_gpu_tune_get(int family, unsigned width, unsigned bw, unsigned* block, unsigned* nontemporal,
+                     unsigned* max_blocks);
 int fls_gpu_tune_load(const char* path);
 
 /* ---- row groups --------------------------------------------------------------------------------------------------
@@ -137,11 +142,11 @@ int  fls_gpu_rowgroup_decode(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, u
  * change). */
 int  fls_gpu_rowgroup_decode_graph(fls_gpu_rowgroup* rg, const uint8_t* dev_encoded, uint8_t* dev_out, void* stream);
 /* The persistent launch's work list: tile count, then up to `capacity` tiles of 128 bytes (the layout of RgTile in
- * fastlanes_amd/csrc/rowgroup.hpp) in launch order, the launch shape (blocks of 4 waves, LDS bytes per wave) and
- * each wave's run of the list (blocks * 4 + 1 bounds: wave w decodes tiles [wave_begin[w], wave_begin[w + 1])).
- * Any pointer may be NULL. */
+ * fastlanes_amd/csrc/rowgroup.hpp) in launch order, the launch shape (blocks, waves per block, LDS bytes of staged
+ * ranges per wave) and each wave's run of the list (blocks * waves + 1 bounds: wave w decodes tiles
+ * [wave_begin[w], wave_begin[w + 1])). Any pointer may be NULL. */
 int  fls_gpu_rowgroup_work_list(const fls_gpu_rowgroup* rg, uint32_t* n_tiles, void* tiles, uint32_t capacity,
-                                uint32_t* blocks, uint32_t* wave_lds, uint32_t* wave_begin);
+                                uint32_t* blocks, uint32_t* waves, uint32_t* wave_lds, uint32_t* wave_begin);
 /* The same work list planned on the host alone (no device memory is touched; dictionary tiles carry a nonzero
  * placeholder instead of a dictionary address), with the same validation, for inspecting a row-group's tiles.
  * waves > 0 also assigns the tiles to that many waves (tiles in launch order, waves + 1 bounds in wave_begin). */
