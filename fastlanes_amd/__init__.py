@@ -50,6 +50,7 @@ def lib() -> C.CDLL:
     L.fls_gpu_selection_indices.argtypes = [_vp, _u64, _vp, _vp, _vp]
     L.fls_gpu_selection_and.argtypes = [_vp, _vp, _u64, _vp, _vp, _vp]
     L.fls_gpu_encode.argtypes = [_vp, _u64, C.c_uint8, C.c_uint8, _vp, _u64, C.POINTER(_u64), _vp]
+    L.fls_gpu_encode_noalloc.argtypes = [_vp, _u64, C.c_uint8, C.c_uint8, _vp, _u64, C.POINTER(_u64), _vp]
     L.fls_gpu_plan_string_offsets.argtypes = [_vp, _vp, _vp, _vp]
     L.fls_gpu_plan_decode_strings.argtypes = [_vp, _vp, _vp, _vp, _u64, _vp]
     L.fls_gpu_unffor_stream.argtypes = [C.c_uint, C.c_uint, _vp, _vp, _vp, _u64, _vp]

@@ -15,6 +15,8 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fls_gpu.h"
@@ -280,8 +282,17 @@ hipError_t launch_enc_w(unsigned W, int mode, const void* x, uint64_t n_vec, uin
 
 using namespace fls;
 
-extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t type, uint8_t expr_id, uint8_t* dev_out,
-                              uint64_t capacity, uint64_t* out_size, void* stream) {
+// The encoder's device scratch: per vector its bit width, FFOR base (RLE: run count), packed-buffer offset, page
+// index and RLE run-value offset, plus the page table. `fixed` != nullptr: carved from that buffer (at most
+// kEncFixedVec vectors and pages), else allocated on the stream and freed before returning.
+namespace fls {
+constexpr uint64_t kEncFixedVec = 16384;  // 2^24 values
+constexpr size_t   kEncFixedBytes = kEncFixedVec * (1 + 8 + 8 + 4 + 8 + sizeof(EncPage)) + 6 * 256;
+}  // namespace fls
+
+namespace {
+int encode_impl(const void* dev_values, uint64_t n, enum data_t type, uint8_t expr_id, uint8_t* dev_out,
+                uint64_t capacity, uint64_t* out_size, void* stream, uint8_t* fixed) {
     if (!out_size || !dev_out || (n && !dev_values)) return fail(FLS_E_ARG, "null argument");
     Expr e;
     if (type == STR || !expr_from_id((uint8_t)type, expr_id, e)) return fail(FLS_E_UNSUPPORTED, "unknown expression");
@@ -297,6 +308,8 @@ extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t ty
     const uint64_t    n_vec = n / 1024;
     hipStream_t       s     = (hipStream_t)stream;
     hipError_t        err;
+    if (fixed && n_vec > kEncFixedVec)
+        return fail(FLS_E_CAPACITY, "the allocation-free encoder takes at most 2^24 values per call");
     std::vector<uint8_t>  bw(n_vec, 0);
     std::vector<uint64_t> runs(mode >= 3 ? n_vec : 0);
     uint8_t*          d_bw   = nullptr;
@@ -306,14 +319,24 @@ extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t ty
     uint64_t*         d_rv   = nullptr;
     EncPage*          d_pg   = nullptr;
     auto              release = [&](int rc) {
-        for (void* q : {(void*)d_bw, (void*)d_base, (void*)d_pk, (void*)d_vp, (void*)d_pg, (void*)d_rv})
-            if (q) (void)hipFreeAsync(q, s);
+        if (!fixed)
+            for (void* q : {(void*)d_bw, (void*)d_base, (void*)d_pk, (void*)d_vp, (void*)d_pg, (void*)d_rv})
+                if (q) (void)hipFreeAsync(q, s);
         (void)hipStreamSynchronize(s);
         return rc;
     };
+    // scratch region k of `bytes` bytes: carved from the fixed buffer (regions of kEncFixedVec units, 256-aligned)
+    // or allocated on the stream
+    auto get = [&](auto** q, unsigned k, uint64_t unit, uint64_t count) -> hipError_t {
+        if (!fixed) return hipMallocAsync((void**)q, unit * count, s);
+        static const uint64_t kUnit[6] = {1, 8, 8, 4, 8, sizeof(EncPage)};
+        uint64_t at = 0;
+        for (unsigned r = 0; r < k; ++r) at += (kUnit[r] * kEncFixedVec + 255) / 256 * 256;
+        *q = reinterpret_cast<std::remove_pointer_t<decltype(q)>>(fixed + at);
+        return hipSuccess;
+    };
     if (n_vec) {
-        if ((err = hipMallocAsync((void**)&d_bw, n_vec, s)) != hipSuccess ||
-            (err = hipMallocAsync((void**)&d_base, 8 * n_vec, s)) != hipSuccess)
+        if ((err = get(&d_bw, 0, 1, n_vec)) != hipSuccess || (err = get(&d_base, 1, 8, n_vec)) != hipSuccess)
             return release(hip_fail(err, "hipMallocAsync"));
         if (mode) {
             if ((err = launch_enc_w(W, mode, dev_values, n_vec, d_bw, d_base, s, true)) != hipSuccess ||
@@ -376,21 +399,21 @@ extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t ty
     }
     *out_size = pos;
     if (pos > capacity) return release(fail(FLS_E_CAPACITY, "encoded column exceeds capacity"));
+    if (fixed && pages.size() > kEncFixedVec) return release(fail(FLS_E_CAPACITY, "too many pages"));
     // headers and buffer offsets (host -> device), then every vector's buffers
     const uint64_t hb = 8 + 4ull * arr_c;
     for (size_t q = 0; q < pages.size(); ++q)
         if ((err = hipMemcpyAsync(dev_out + pages[q].start, heads.data() + q * hb, hb, hipMemcpyHostToDevice, s)) != hipSuccess)
             return release(hip_fail(err, "hipMemcpyAsync"));
     if (n_vec) {
-        if ((err = hipMallocAsync((void**)&d_pk, 8 * n_vec, s)) != hipSuccess ||
-            (err = hipMallocAsync((void**)&d_vp, 4 * n_vec, s)) != hipSuccess ||
-            (err = hipMallocAsync((void**)&d_pg, sizeof(EncPage) * pages.size(), s)) != hipSuccess)
+        if ((err = get(&d_pk, 2, 8, n_vec)) != hipSuccess || (err = get(&d_vp, 3, 4, n_vec)) != hipSuccess ||
+            (err = get(&d_pg, 5, sizeof(EncPage), pages.size())) != hipSuccess)
             return release(hip_fail(err, "hipMallocAsync"));
         if ((err = hipMemcpyAsync(d_pk, pk_rel.data(), 8 * n_vec, hipMemcpyHostToDevice, s)) != hipSuccess ||
             (err = hipMemcpyAsync(d_vp, vpage.data(), 4 * n_vec, hipMemcpyHostToDevice, s)) != hipSuccess ||
             (err = hipMemcpyAsync(d_pg, pages.data(), sizeof(EncPage) * pages.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
             return release(hip_fail(err, "hipMemcpyAsync"));
-        if (mode >= 3 && ((err = hipMallocAsync((void**)&d_rv, 8 * n_vec, s)) != hipSuccess ||
+        if (mode >= 3 && ((err = get(&d_rv, 4, 8, n_vec)) != hipSuccess ||
                           (err = hipMemcpyAsync(d_rv, rv_rel.data(), 8 * n_vec, hipMemcpyHostToDevice, s)) != hipSuccess))
             return release(hip_fail(err, "run-value offsets"));
         if ((err = launch_enc_w(W, mode, dev_values, n_vec, d_bw, mode >= 3 ? d_rv : d_base, s, false, d_pk, d_vp, d_pg,
@@ -398,4 +421,42 @@ extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t ty
             return release(hip_fail(err, "kernel launch"));
     }
     return release(FLS_OK);
+}
+}  // namespace
+
+// Device scratch of fls_gpu_encode_noalloc: one copy per GPU, held until the process ends. Host code never uses
+// this name as an address: each device's address comes from hipGetSymbolAddress (scratch_on_device).
+static __device__ __align__(256) uint8_t scratch[fls::kEncFixedBytes];
+
+namespace {
+// The address of this device's copy of `scratch`, looked up once per device with that device current.
+int scratch_on_device(uint8_t** out) {
+    static std::mutex            mu;
+    static std::vector<uint8_t*> by_dev;
+    int                          dev = 0;
+    hipError_t                   e   = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev >= by_dev.size()) by_dev.resize(dev + 1, nullptr);
+    if (!by_dev[dev]) {
+        void* p = nullptr;
+        if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return hip_fail(e, "hipGetSymbolAddress");
+        by_dev[dev] = static_cast<uint8_t*>(p);
+    }
+    *out = by_dev[dev];
+    return FLS_OK;
+}
+}  // namespace
+
+extern "C" int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t type, uint8_t expr_id, uint8_t* dev_out,
+                              uint64_t capacity, uint64_t* out_size, void* stream) {
+    return encode_impl(dev_values, n, type, expr_id, dev_out, capacity, out_size, stream, nullptr);
+}
+
+extern "C" int fls_gpu_encode_noalloc(const void* dev_values, uint64_t n, enum data_t type, uint8_t expr_id,
+                                      uint8_t* dev_out, uint64_t capacity, uint64_t* out_size, void* stream) {
+    if (n / 1024 > kEncFixedVec) return fail(FLS_E_CAPACITY, "the allocation-free encoder takes at most 2^24 values per call");
+    uint8_t* base = nullptr;
+    if (int rc = scratch_on_device(&base)) return rc;
+    return encode_impl(dev_values, n, type, expr_id, dev_out, capacity, out_size, stream, base);
 }

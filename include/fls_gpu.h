@@ -86,6 +86,14 @@ int fls_gpu_selection_and(const uint64_t* dev_a, const uint64_t* dev_b, uint64_t
  * after `stream` has completed. */
 int fls_gpu_encode(const void* dev_values, uint64_t n, enum data_t type, uint8_t expr_id, uint8_t* dev_out,
                    uint64_t capacity, uint64_t* out_size, void* stream);
+/* fls_gpu_encode without device allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The encoder's scratch (per-vector bit widths, bases, buffer offsets and the page table) is a fixed device array
+ * of about 1.1 MiB per GPU, kept until the process ends, instead of stream allocations; n is at most 2^24 values
+ * (FLS_E_CAPACITY beyond). Every call on one GPU shares that GPU's array: calls that overlap on one GPU race.
+ * Same bytes, errors and synchronization as fls_gpu_encode. */
+int fls_gpu_encode_noalloc(const void* dev_values, uint64_t n, enum data_t type, uint8_t expr_id, uint8_t* dev_out,
+                           uint64_t capacity, uint64_t* out_size, void* stream);
 
 /* Encode a string column with FSST (1024-string vectors; n must be a multiple of 1024): strings i are
  * bytes[offsets[i], offsets[i+1]). Writes *out_size bytes. No counterpart in the reference's C API, which accepts
