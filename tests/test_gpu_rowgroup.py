@@ -198,3 +198,22 @@ def test_rowgroup_persistent_grid_stride(torch_cuda, oracle):
     print(f"rowgroup persistent, one-vector tiles: {tiles.size} tiles over {blocks} blocks", flush=True)
     rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, fa.stream_ptr())
     verify(torch, rg, d_out, want, "one-vector tiles")
+
+
+def test_rowgroup_large_single_page_columns(torch_cuda, oracle):
+    """Long single-page columns (u64 DELTA of 2 * 10^7 values, u32 DELTA, u64 FFOR): many two-vector tiles per
+    page, every vector checked; an earlier build of the kernel got one staged row wrong in about 0.3% of such
+    vectors."""
+    torch = torch_cuda
+    cols = [(wl.rowgroup_column("delta", 64, 20000 * 1024, 17004), 11),
+            (wl.rowgroup_column("delta", 32, 20000 * 1024, 17005), 11),
+            (wl.rowgroup_column("ffor", 64, 12000 * 1024, 17006), 1)]
+    rg, d_enc, want = build(torch, oracle, cols)
+    d_out = torch.empty(rg.total_bytes + 256, dtype=torch.uint8, device="cuda")
+    tiles, blocks, _, _ = rg.work_list()
+    print(f"rowgroup large single-page columns: {tiles.size} tiles over {blocks} blocks", flush=True)
+    for rep in range(2):
+        d_out.fill_(0x77)
+        rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 0, fa.stream_ptr())
+        verify(torch, rg, d_out, want, f"large columns, run {rep}")
+
