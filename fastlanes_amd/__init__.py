@@ -271,7 +271,8 @@ def unffor_stream(width: int, bw: int, dev_packed: int, dev_bases: int | None, d
 def encode_from_memory(values: np.ndarray, expr_id: int, capacity: int | None = None) -> np.ndarray:
     """include/fastlanes.h encode_from_memory on a host array; returns the encoded bytes."""
     values = np.ascontiguousarray(values)
-    cap = capacity if capacity is not None else values.nbytes * 2 + (1 << 16) + 64 * (values.size // 1024)
+    # default room: dictionary pages hold 4-byte codes whatever the value width, plus the dictionary itself
+    cap = capacity if capacity is not None else values.size * 4 + values.nbytes * 2 + (1 << 16) + 64 * (values.size // 1024)
     out = np.empty(cap, dtype=np.uint8)
     size, meta = _u64(0), _u64(0)
     ok = lib().encode_from_memory(values.ctypes.data, values.size, cap, out.ctypes.data, C.byref(size), C.byref(meta),

@@ -19,7 +19,7 @@ enum Family : int {
     FAM_PAGE_DELTA,   // DELTA pages
     FAM_PAGE_RLE,     // RLE pages (both variants)
     FAM_PAGE_DICT,    // dictionary pages (all variants)
-    FAM_ROWGROUP,     // persistent row-group launch: block = vectors per tile, max_blocks = blocks per compute unit
+    FAM_ROWGROUP,     // row-group launch: block = KiB of decoded values per tile, max_blocks = blocks per compute unit
     FAM_COUNT
 };
 

@@ -4,7 +4,8 @@ device-resident row-group is decoded four ways -- the persistent launch over the
 captured once in a HIP graph and replayed, one launch per page in column order (the per-column form), and those
 launches spread over 4 parallel stream branches -- and every column is compared with its input values.
 
-    python -m fastlanes_amd.rowgroup_bench [--min-log2 20] [--max-log2 26] [--columns 64]
+    python -m fastlanes_amd.rowgroup_bench [--min-log2 20] [--max-log2 26] [--columns 64] [--forms persistent,...]
+                                           [--no-per-kind]
 """
 from __future__ import annotations
 
@@ -50,7 +51,10 @@ def _time(fn, stream, reps, warm=2):
     return float(np.median(ts))
 
 
-def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
+ALL_FORMS = ("persistent", "graph", "branched", "per_page")
+
+
+def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10, forms_run=ALL_FORMS, per_kind=True):
     import torch
 
     t0 = time.time()
@@ -86,6 +90,8 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
              "branched": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 2, sp),
              "per_page": lambda: rg.decode(d_enc.data_ptr(), d_out.data_ptr(), 1, sp)}
     for name, fn in forms.items():
+        if name not in forms_run:
+            continue
         d_out.fill_(0)
         fn()
         check(name)
@@ -93,9 +99,11 @@ def run(dev, stream, n_columns=64, seed=17, min_log2=20, max_log2=26, reps=10):
         gbs = algo / t / 1e9
         res[name] = {"ms": round(t * 1e3, 3), "gvalues_s": round(n_values / t / 1e9, 2), "gb_s": round(gbs, 1),
                      "hbm_frac": round(gbs / HBM, 4)}
-    res["graph_vs_per_page"] = round(res["per_page"]["ms"] / res["graph"]["ms"], 3)
-    res["persistent_vs_per_page"] = round(res["per_page"]["ms"] / res["persistent"]["ms"], 3)
-    res["branched_vs_per_page"] = round(res["per_page"]["ms"] / res["branched"]["ms"], 3)
+    for name in ("graph", "persistent", "branched"):
+        if name in res and "per_page" in res:
+            res[f"{name}_vs_per_page"] = round(res["per_page"]["ms"] / res[name]["ms"], 3)
+    if not per_kind:
+        return res
     # where the per-column time goes: each column decoded alone, grouped by (encoding, width)
     per_kind = {}
     for c, (W, kind, n, _) in enumerate(spec):
@@ -130,12 +138,15 @@ def main():
     ap.add_argument("--max-log2", type=int, default=26)
     ap.add_argument("--columns", type=int, default=64)
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--forms", default=",".join(ALL_FORMS), help="comma list of " + ", ".join(ALL_FORMS))
+    ap.add_argument("--no-per-kind", action="store_true", help="skip the per-kind breakdown")
     a = ap.parse_args()
     import torch
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
-    print(json.dumps(run(dev, stream, a.columns, 17, a.min_log2, a.max_log2, a.reps)), flush=True)
+    print(json.dumps(run(dev, stream, a.columns, 17, a.min_log2, a.max_log2, a.reps, tuple(a.forms.split(",")),
+                         not a.no_per_kind)), flush=True)
 
 
 if __name__ == "__main__":

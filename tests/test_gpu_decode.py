@@ -473,6 +473,59 @@ def test_integer_pages_direct_stores(torch_cuda, oracle, W):
         assert lib.fls_gpu_tune_load(path.encode()) > 0
 
 
+@pytest.mark.parametrize("W", [8, 16, 32, 64])
+def test_page_families_grid_capped(torch_cuda, oracle, W):
+    """Every page family at width W with the grid capped to one or two blocks, so each block strides over many
+    vectors and takes the next-stride metadata prefetch (plain, RLE, dictionary and patched pages); against the
+    oracle, bit-exact. The default table is restored afterwards."""
+    import ctypes as C
+    import os
+
+    torch = torch_cuda
+    lib = fa.lib()
+    dt = W2DT[W]
+    eids = (0, 1, 11, 6, 7, 16, 17, 22, 28, 29) + ((2, 4, 5) if W >= 32 else ())
+    fam_of = {0: 1, 1: 2, 2: 2, 4: 2, 5: 2, 11: 3, 6: 4, 7: 4, 16: 5, 17: 5, 22: 5, 28: 5, 29: 5}
+    try:
+        for eid in eids:
+            x = _column(dt, eid, 41 * 1024, seed=eid * 7 + W)
+            enc = oracle.encode_column(x, eid)
+            want = oracle.decode_column(enc, dt, x.size)
+            np.testing.assert_array_equal(want, x)
+            plan = fa.Plan(enc, dt)
+            d_in = dev(torch, enc)
+            for block, maxb in ((64, 1), (256, 2)):
+                print(f"grid-capped W={W} expr={eid} block={block} max_blocks={maxb}", flush=True)
+                fa.check(lib.fls_gpu_tune_set(fam_of[eid], W, 255, block, 1, maxb), "tune_set")
+                d_out = empty(torch, x.nbytes)
+                plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+                plan.check()
+                np.testing.assert_array_equal(to_np(d_out, dt, x.size), want, err_msg=f"expr {eid} block {block}")
+    finally:
+        path = os.path.join(os.path.dirname(os.path.abspath(fa.__file__)), "tuning.txt")
+        lib.fls_gpu_tune_load.argtypes = [C.c_char_p]
+        assert lib.fls_gpu_tune_load(path.encode()) > 0
+
+
+@pytest.mark.parametrize("eid", [16, 17, 22])
+def test_zipf_int32_dictionary(torch_cuda, oracle, eid):
+    """The categorical column of the cascade benchmark: int32 codes over 4096 keys drawn with Zipf s = 1.1 over key
+    rank, as DICT with plain, FFOR and DELTA codes; 2^20 values, against the oracle."""
+    torch = torch_cuda
+    x = wl.zipf_codes(1 << 20, 4096, 1.1, seed=5, dtype=np.int32)
+    enc = oracle.encode_column(x, eid)
+    want = oracle.decode_column(enc, np.int32, x.size)
+    np.testing.assert_array_equal(want, x)
+    print(f"zipf int32 dictionary expr={eid}: {enc.size} B", flush=True)
+    plan = fa.Plan(enc, np.int32)
+    d_in = dev(torch, enc)
+    d_out = empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    plan.check()
+    np.testing.assert_array_equal(to_np(d_out, np.int32, x.size), want)
+    np.testing.assert_array_equal(fa.encode_from_memory(x, eid), enc)
+
+
 def _bitmap_words(mask: np.ndarray) -> np.ndarray:
     """SelectionDS bitmap words of a boolean mask: per 1024 values, 16 uint64 words, bit i of word j = value 64j+i."""
     bits = np.packbits(mask.reshape(-1, 64), axis=1, bitorder="little")
