@@ -55,10 +55,22 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def cpu_baseline(widths, threads, samples):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(widths, threads, samples, log2n=24, passes=10):
     """The CPU leg: (1) checks the sampled GPU outputs against the oracle's unffor; (2) times the reference's
-    compiled generated::unffor (oracle/_ref) on the host cores over a bounded sample: every (type, width) FFOR case
-    at 2^20 values (the same per-case shape as the GPU sweep). Returns (baseline, mismatching cases)."""
+    compiled generated::unffor (oracle/_ref) on the host cores: every (type, width) FFOR case at 2^log2n values, on
+    one thread and on `threads` threads (started before the timed passes), median of `passes` passes per case.
+    Returns (baseline, mismatching cases)."""
     from oracle.py_oracle import Oracle, Reference, reference_available
 
     orc = Oracle()
@@ -71,22 +83,49 @@ def cpu_baseline(widths, threads, samples):
         return None, bad
     ref = Reference()
     rng = np.random.default_rng(0)
-    total_vals, total_t = 0, 0.0
-    n_vec = 1024
-    for W in widths:
-        dt = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[W]
-        out = np.empty(n_vec * 1024, dtype=dt)
-        for w in range(W + 1):
-            packed = rng.integers(0, 256, n_vec * 128 * w + 64, dtype=np.uint8)
-            bases = rng.integers(0, 2 ** min(W - w, 62) + 1, n_vec).astype(dt)
-            t = ref.lib.ref_time_unffor(W, packed.ctypes.data, out.ctypes.data, n_vec, w, bases.ctypes.data, 0, 3,
-                                        threads)
-            total_vals += n_vec * 1024
-            total_t += t
-    return {"value": round(total_vals / total_t / 1e9, 3), "unit": "Gvalues/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"reference generated::unffor, every (type,width) FFOR case of {widths} at 2^20 values, "
-                      f"median of 3 passes, {threads} threads"}, bad
+    n_vec = (1 << log2n) // 1024
+    packed = rng.integers(0, 256, n_vec * 128 * 64 + 64, dtype=np.uint8)  # sliced per case: the widest needs all
+    rates = {}
+    for nt in (1, threads):
+        total_vals, total_t = 0, 0.0
+        for W in widths:
+            dt = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}[W]
+            out = np.empty(n_vec * 1024, dtype=dt)
+            for w in range(W + 1):
+                bases = rng.integers(0, 2 ** min(W - w, 62) + 1, n_vec).astype(dt)
+                t = ref.lib.ref_time_unffor(W, packed.ctypes.data, out.ctypes.data, n_vec, w, bases.ctypes.data, 0,
+                                            passes, nt)
+                total_vals += n_vec * 1024
+                total_t += t
+            del out
+        rates[nt] = total_vals / total_t / 1e9
+    return {"value": round(rates[threads], 3), "unit": "Gvalues/s", "cores": threads, "kind": "reference",
+            "single_core": round(rates[1], 3), "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "sample": f"reference generated::unffor, every (type,width) FFOR case of {widths} at 2^{log2n} values, "
+                      f"median of {passes} passes per case after one untimed pass, on 1 thread (single_core) and "
+                      f"on {threads} threads started before the timing"}, bad
+
+
+def copy_bandwidth(dev, nbytes=1 << 30, reps=5):
+    """The device's copy rate in GB/s (bytes read + written per second) for a contiguous nbytes copy: the box's
+    attainable stream rate, reported beside the 8 TB/s peak."""
+    import torch
+
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes / float(np.median(ts)) / 1e9
 
 
 def other_configs(dev):
@@ -251,6 +290,7 @@ def main():
                 row["unpack" if plain else "ffor"].append(round(alg_bytes(W, w, plain) / t / 1e9 / HBM_PEAK_GBS, 3))
         by_width[f"u{W}"] = row
     agg_bytes = sum(alg_bytes(W, w, p) for (W, w, p) in per_case)
+    copy_gbs = copy_bandwidth(dev)
     u32w7 = per_case.get((32, 7, False))
     line = {
         "metric": METRIC,
@@ -271,7 +311,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kname + (" (plain unpack)" if dplain else " (FFOR)"),
-                     "bytes_per_launch": d_bytes, "avg_launch_s": d_avg},
+                     "bytes_per_launch": d_bytes, "avg_launch_s": d_avg,
+                     "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
         "sweep_hbm_frac": sweep,
         "sweep_aggregate_hbm_frac": round(agg_bytes / kernel_time / 1e9 / HBM_PEAK_GBS, 4),
         "sweep_hbm_frac_by_bit_width": by_width,

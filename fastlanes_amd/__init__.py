@@ -34,6 +34,13 @@ def lib() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise FlsError(f"{LIB_PATH} is not built: run `make -C fastlanes_amd` (hipcc --offload-arch=gfx950)")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7 and loads it by path. Loaded first, it is
+    # the runtime this library binds to (same soname); loaded after this library, it would be a second runtime
+    # beside the system one, and the one initialised later finds no device. So PyTorch, when installed, goes first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.fls_gpu_error_string.restype = C.c_char_p
     L.fls_gpu_plan_create.argtypes = [_vp, _u64, C.c_uint8, C.POINTER(_vp)]

@@ -25,12 +25,13 @@
 namespace fls {
 namespace {
 
-constexpr unsigned kRG = 65536, kVPR = 64, kK = 8;
+constexpr unsigned kRG = 65536, kVPR = 64, kK = 13;
 constexpr uint64_t kEmpty = ~0ull;
 constexpr unsigned kHtBits = 17, kHt = 1u << kHtBits;      // exhaustive table: 131072 slots (load <= 0.5)
 constexpr unsigned kSampleV = 4, kSample = kSampleV * 1024;  // learned selector's sample
 constexpr unsigned kSampleVec[kSampleV] = {0, 21, 42, 63};
-constexpr uint8_t kCandExpr[kK] = {0, 1, 11, 6, 7, 16, 17, 22};
+constexpr uint8_t kCandExpr[kK] = {0, 1, 11, 6, 7, 16, 17, 22, 2, 3, 4, 5, 27};
+constexpr unsigned kExcLimit = 100;  // CFG::CMPR::EXC_LIMIT_C (include/fls/cfg/cfg.hpp:67)
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint32_t bitlen64(uint64_t d) { return d ? 64u - (uint32_t)__clzll((long long)d) : 0u; }
@@ -61,6 +62,97 @@ __device__ __forceinline__ T wave_max(T v) {
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
     return v;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sorts the 1024 values of buf (the wave's LDS buffer) ascending: a bitonic network, 8 pairs per lane per stage.
+__device__ void wave_sort1024(uint64_t* buf) {
+    const unsigned lane = threadIdx.x % 64;
+    for (unsigned k = 2; k <= 1024; k <<= 1)
+        for (unsigned j = k >> 1; j > 0; j >>= 1) {
+            wave_lds_sync();
+#pragma unroll
+            for (unsigned m = 0; m < 8; ++m) {
+                const unsigned t = lane + 64 * m;           // pair 0..511
+                const unsigned i = 2 * t - (t & (j - 1));   // bit j of i is clear; its partner is i + j
+                const uint64_t a = buf[i], b = buf[i + j];
+                if ((a > b) == ((i & k) == 0)) buf[i] = b, buf[i + j] = a;
+            }
+        }
+    wave_lds_sync();
+}
+
+// Original position of transposed index k (the 1024-value transpose of every FastLanes type: k = 128r + 16b + c
+// holds value r + 8 ORDER[b] + 64c).
+__device__ __forceinline__ unsigned orig_of(unsigned k) {
+    const unsigned b = (k >> 4) & 7;
+    return (k >> 7) + 8 * (((b & 1) << 2) | (b & 2) | (b >> 2)) + 64 * (k & 15);
+}
+
+// Patch plan of one vector (find_best_params, src/cor/prm/ffor/ffor_analyze.cpp:41-75, as the host encoder's
+// plan_patch): over bit widths bw < 64 and window bases s[i] (first index of each distinct value among the 101
+// smallest), the window [s[i], s[i] + 2^bw - 1] (bw = 0: only base 0, window {0}) leaving at most 100 values
+// outside; the plan minimises 128 bw + 8 (values outside), ties to the smaller i, then the larger bw; it exists only
+// below 128 * 64 bytes (else bw = 64, base 0, no exceptions). s: the vector sorted ascending (LDS). A window wider
+// than the vector's FFOR width bwf is never better than bw = bwf at i = 0, so bw runs up to min(bwf, 63) -- and up
+// to 1 for a constant nonzero vector, which no zero-width window holds.
+struct PatchPlan {
+    uint32_t bw, exc;
+    uint64_t base;
+};
+__device__ PatchPlan patch_plan(const uint64_t* s, uint32_t bwf) {
+    const unsigned lane = threadIdx.x % 64, bw_hi = bwf < 1 ? 1 : bwf < 63 ? bwf : 63;
+    uint64_t       key  = ~0ull;  // (size << 32) | (i << 8) | (63 - bw): the smallest key is the plan
+    for (unsigned h = 0; h < 2; ++h) {
+        const unsigned i = lane + 64 * h;
+        if (i > kExcLimit) continue;
+        const uint64_t si = s[i];
+        if (i && si == s[i - 1]) continue;
+        // the window must hold s[i .. 923 + i]: its width is at least bitlen(s[923 + i] - s[i])
+        const uint32_t bw_lo = bitlen64(s[1023 - kExcLimit + i] - si);
+        for (unsigned bw = bw_lo; bw <= bw_hi; ++bw) {
+            uint64_t upper;
+            if (bw == 0) {
+                if (si != 0) continue;  // a zero-width window holds only zeros
+                upper = 0;
+            } else {
+                const uint64_t range = (1ull << bw) - 1;
+                upper = si > ~0ull - range ? ~0ull : si + range;
+            }
+            // first index past the window, searched above 923 + i (every smaller index is inside it)
+            unsigned lo = 1023 - kExcLimit + i, hi = 1024;
+            while (hi - lo > 1) {
+                const unsigned mid = (lo + hi) / 2;
+                if (s[mid] <= upper) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t exc  = 1024 - (hi - i);
+            const uint64_t size = 128ull * bw + 8ull * exc;
+            const uint64_t kk   = (size << 32) | ((uint64_t)i << 8) | (63 - bw);
+            key = kk < key ? kk : key;
+        }
+    }
+    key = wave_min(key);
+    PatchPlan pp {64, 0, 0};
+    if ((key >> 32) < 128ull * 64) {
+        const unsigned i = (unsigned)(key >> 8) & 0xffffff;
+        pp.bw   = 63 - (uint32_t)(key & 0xff);
+        pp.base = s[i];
+        pp.exc  = (uint32_t)(((key >> 32) - 128ull * pp.bw) / 8);
+    }
+    return pp;
+}
+__device__ __forceinline__ bool is_exception(uint64_t v, const PatchPlan& pp) {
+    if (pp.bw >= 64) return false;
+    if (pp.bw == 0) return v != 0;
+    const uint64_t range = (1ull << pp.bw) - 1;
+    const uint64_t upper = pp.base > ~0ull - range ? ~0ull : pp.base + range;
+    return v < pp.base || v > upper;
 }
 
 // Statistics of one 1024-value vector, computed by one wave (lane l owns values 16l .. 16l+15).
@@ -116,12 +208,13 @@ __device__ VecStats vector_stats(const uint64_t* __restrict__ x) {
 // Insert v (first seen at position o) into an open-addressing table whose empty keys are kEmpty; the value kEmpty
 // itself lives in the extra slot `mask + 1`. Returns the slot (never fails: callers size tables above their input).
 template <typename KeyPtr, typename FirstPtr>
-__device__ __forceinline__ uint32_t ht_insert(KeyPtr keys, FirstPtr first, uint32_t mask, uint64_t v, uint32_t o) {
+__device__ __forceinline__ uint32_t ht_insert(KeyPtr keys, FirstPtr first, uint32_t mask, uint64_t v, uint32_t o,
+                                              uint32_t start = ~0u) {
     if (v == kEmpty) {
         atomicMin(&first[mask + 1], o);
         return mask + 1;
     }
-    uint32_t h = mix64(v) & mask;
+    uint32_t h = start == ~0u ? mix64(v) & mask : start;
     for (uint32_t p = 0; p <= mask; ++p) {
         const uint64_t old = atomicCAS((unsigned long long*)&keys[h], (unsigned long long)kEmpty, (unsigned long long)v);
         if (old == kEmpty || old == v) {
@@ -176,6 +269,8 @@ __device__ uint32_t block_popcount_scan(const uint32_t* bitmap, uint32_t* base, 
 
 // ---------------------------------------------------------------------------------------------------------------
 // Exhaustive sizing: one 1024-thread block per row-group (grid-stride); per block a global hash table region.
+// Only the block's own threads touch its table, slots and first positions, so the barriers' workgroup-scope
+// ordering is all the phases need (a device-scope fence would write back and invalidate the XCD's whole L2).
 // ---------------------------------------------------------------------------------------------------------------
 struct SizerLds {
     uint32_t bitmap[kRG / 32];
@@ -184,12 +279,21 @@ struct SizerLds {
     uint32_t cmin[kVPR], cmax[kVPR], cdmin[kVPR], cdmax[kVPR];
     unsigned long long dmin[kVPR], dmax[kVPR];  // dictionary chunk min / max (chunks of 1024 codes)
     uint32_t tmp[32];
+    // patched FFOR: per vector bit width, base, exceptions and LL chain length; the page's exception values in
+    // chunks of 1024 (S_PATCH_REC's HZL chain)
+    uint32_t           pbw[kVPR], pexc[kVPR], pchain[kVPR], excpref[kVPR + 1];
+    unsigned long long pbase[kVPR], excmin[8], excmax[8];
+    // DICT_RLE: the page's run codes in chunks of 1024 (its HZL chain)
+    uint32_t rcmin[kVPR], rcmax[kVPR], runpref[kVPR + 1];
+    uint64_t sortbuf[16][1024];  // one vector per wave, sorted for the patch plan
 };
 
 __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restrict__ values, uint64_t n_rg,
                                                        uint64_t* __restrict__ tables, uint32_t* __restrict__ firsts_all,
                                                        uint32_t* __restrict__ slots_all, uint64_t* __restrict__ sizes) {
-    __shared__ SizerLds L;
+    // dynamic LDS (the launch passes sizeof(SizerLds), about 150 KB)
+    extern __shared__ __attribute__((aligned(16))) uint8_t wiz_lds[];
+    SizerLds&      L = *reinterpret_cast<SizerLds*>(wiz_lds);
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
     uint64_t* keys   = tables + (uint64_t)blockIdx.x * (kHt + 64);  // kHt keys + padding
     uint32_t* first  = firsts_all + (uint64_t)blockIdx.x * (2 * kHt + 64);
@@ -201,29 +305,102 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
         if (t < kVPR) {
             L.cmin[t] = ~0u, L.cmax[t] = 0, L.cdmin[t] = ~0u, L.cdmax[t] = 0;
             L.dmin[t] = kEmpty, L.dmax[t] = 0;
+            L.rcmin[t] = ~0u, L.rcmax[t] = 0;
         }
-        // per-vector statistics
+        if (t < 8) L.excmin[t] = kEmpty, L.excmax[t] = 0;
+        // per-vector statistics and patch plans (wave w takes vectors w, w + 16, ...)
         for (unsigned v = wave; v < kVPR; v += blockDim.x / 64) {
             const VecStats s = vector_stats(x + v * 1024);
-            if (lane == 0) L.bwf[v] = s.bwf, L.bwd[v] = s.bwd, L.runs[v] = s.runs, L.bwp[v] = s.bwp;
+            uint64_t*      sb = L.sortbuf[wave];
+#pragma unroll
+            for (unsigned i = 0; i < 16; ++i) sb[64 * i + lane] = x[v * 1024 + 64 * i + lane];
+            wave_sort1024(sb);
+            const PatchPlan pp = patch_plan(sb, s.bwf);
+            // exceptions in transposed order (the order the encoder stores them) and the LL chain's hop entries:
+            // a gap g between consecutive exceptions adds floor((g - 1) / hop) links, hop = 2^min(bw, 11)
+            const unsigned hop = 1u << (pp.bw < 11 ? pp.bw : 11);
+            uint32_t       n = 0, extra = 0;
+            int            last = -1;  // last exception index seen by the wave so far
+            for (unsigned q = 0; q < 16; ++q) {
+                const unsigned k  = 64 * q + lane;
+                const bool     ex = is_exception(x[v * 1024 + orig_of(k)], pp);
+                const uint64_t m  = __ballot(ex);
+                if (ex) {
+                    const uint64_t below = m & ((1ull << lane) - 1);
+                    const int prev = below ? (int)(64 * q + 63 - __clzll((long long)below)) : last;
+                    if (prev >= 0) extra += (k - (unsigned)prev - 1) / hop;
+                }
+                if (m) last = (int)(64 * q + 63 - __clzll((long long)m));
+                n += (uint32_t)__popcll(m);
+            }
+            extra = wave_sum(extra);
+            if (lane == 0) {
+                L.bwf[v] = s.bwf, L.bwd[v] = s.bwd, L.runs[v] = s.runs, L.bwp[v] = s.bwp;
+                L.pbw[v] = pp.bw, L.pbase[v] = pp.base, L.pexc[v] = n, L.pchain[v] = n + extra;
+            }
         }
-        // dictionary: insert every value, remember its slot
-        for (unsigned o = t; o < kRG; o += blockDim.x) slot[o] = ht_insert(keys, first, kHt - 1, x[o], o);
-        __threadfence();
+        // dictionary: insert every value, remember its slot. A thread takes kBatch positions at a time: their loads
+        // and first probes are issued together, so kBatch atomics are in flight per thread instead of one.
+        constexpr unsigned kBatch = 4;
+        const unsigned     stride = blockDim.x;
+        for (unsigned o0 = t; o0 < kRG; o0 += kBatch * stride) {
+            uint64_t           v[kBatch];
+            uint32_t           h[kBatch];
+            unsigned long long old[kBatch];
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) v[k] = x[o0 + k * stride];
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) h[k] = v[k] == kEmpty ? kHt : mix64(v[k]) & (kHt - 1);
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k)
+                old[k] = v[k] == kEmpty ? 0ull
+                                        : atomicCAS((unsigned long long*)&keys[h[k]], (unsigned long long)kEmpty,
+                                                    (unsigned long long)v[k]);
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) {
+                const unsigned o = o0 + k * stride;
+                uint32_t       hs;
+                if (v[k] == kEmpty || old[k] == kEmpty || old[k] == v[k]) {
+                    hs = h[k];
+                    atomicMin(&first[hs], o);
+                } else {  // the first probe met another value: continue probing from the next slot
+                    hs = ht_insert(keys, first, kHt - 1, v[k], o, (h[k] + 1) & (kHt - 1));
+                }
+                slot[o] = hs;
+            }
+        }
         __syncthreads();
-        for (unsigned o = t; o < kRG; o += blockDim.x) {
-            const uint32_t f = __hip_atomic_load(&first[slot[o]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            fpos[o]          = f;
-            if (f == o) atomicOr(&L.bitmap[o >> 5], 1u << (o & 31));
+        for (unsigned o0 = t; o0 < kRG; o0 += kBatch * stride) {
+            uint32_t sl[kBatch], f[kBatch];
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) sl[k] = slot[o0 + k * stride];
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k)
+                f[k] = __hip_atomic_load(&first[sl[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) {
+                const unsigned o = o0 + k * stride;
+                fpos[o]          = f[k];
+                if (f[k] == o) atomicOr(&L.bitmap[o >> 5], 1u << (o & 31));
+            }
         }
-        __threadfence();
         __syncthreads();
         const uint32_t D = block_popcount_scan(L.bitmap, L.base, kRG / 32, L.tmp);
         // codes: per vector min/max and 32-segment delta min/max; dictionary chunk min/max
-        for (unsigned o = t; o < kRG; o += blockDim.x) {
-            const uint32_t f  = fpos[o];
+        for (unsigned o0 = t; o0 < kRG; o0 += kBatch * stride) {
+          uint32_t fb[kBatch], fpb[kBatch];
+#pragma unroll
+          for (unsigned k = 0; k < kBatch; ++k) {
+              const unsigned o = o0 + k * stride;
+              fb[k]            = fpos[o];
+              fpb[k]           = (o % 32) ? fpos[o - 1] : 0u;
+          }
+#pragma unroll
+          for (unsigned k = 0; k < kBatch; ++k) {
+            const unsigned o  = o0 + k * stride;
+            const uint32_t f  = fb[k];
             const uint32_t c  = rank_of(L.bitmap, L.base, f);
-            const uint32_t cp = (o % 32) ? rank_of(L.bitmap, L.base, fpos[o - 1]) : c;
+            const uint32_t cp = (o % 32) ? rank_of(L.bitmap, L.base, fpb[k]) : c;
             const uint32_t d  = c - cp;  // 0 at a segment start
             const uint32_t cmn = wave_min(c), cmx = wave_max(c), dmn = wave_min(d), dmx = wave_max(d);
             if (lane == 0) {
@@ -234,6 +411,40 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
             if (f == o) {
                 atomicMin(&L.dmin[c >> 10], (unsigned long long)x[o]);
                 atomicMax(&L.dmax[c >> 10], (unsigned long long)x[o]);
+            }
+          }
+        }
+        __syncthreads();
+        if (t == 0) {
+            uint32_t a = 0, b = 0;
+            for (unsigned v = 0; v < kVPR; ++v) L.runpref[v] = a, a += L.runs[v], L.excpref[v] = b, b += L.pexc[v];
+            L.runpref[kVPR] = a, L.excpref[kVPR] = b;
+        }
+        __syncthreads();
+        // DICT_RLE run codes and S_PATCH_REC exception values, each as the page's chain in chunks of 1024
+        for (unsigned v = wave; v < kVPR; v += blockDim.x / 64) {
+            const uint64_t* xv = x + v * 1024;
+            PatchPlan       pp {L.pbw[v], 0, L.pbase[v]};
+            uint32_t        nr = L.runpref[v], ne = L.excpref[v];
+            for (unsigned q = 0; q < 16; ++q) {
+                const unsigned o   = 64 * q + lane;
+                const uint64_t val = xv[o];
+                const bool     chg = o == 0 || val != xv[o - 1];
+                const uint64_t mr  = __ballot(chg);
+                if (chg) {
+                    const uint32_t c     = rank_of(L.bitmap, L.base, fpos[v * 1024 + o]);
+                    const uint32_t chunk = (nr + (uint32_t)__popcll(mr & ((1ull << lane) - 1))) >> 10;
+                    atomicMin(&L.rcmin[chunk], c), atomicMax(&L.rcmax[chunk], c);
+                }
+                nr += (uint32_t)__popcll(mr);
+                const uint64_t ev = xv[orig_of(o)];
+                const bool     ex = is_exception(ev, pp);
+                const uint64_t me = __ballot(ex);
+                if (ex) {
+                    const uint32_t chunk = (ne + (uint32_t)__popcll(me & ((1ull << lane) - 1))) >> 10;
+                    atomicMin(&L.excmin[chunk], (unsigned long long)ev), atomicMax(&L.excmax[chunk], (unsigned long long)ev);
+                }
+                ne += (uint32_t)__popcll(me);
             }
         }
         __syncthreads();
@@ -247,8 +458,30 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
             }
             hzl += 12 + 8ull * (D - pos);
             const uint64_t dict = 8 + hzl;
-            uint64_t       s[kK] = {};
+            // HZL chains of the page's run codes (u32) and exception values (u64): 12 + 128 bw per full chunk of 1024
+            // while bw is below the type width, then 12 + the raw rest
+            uint64_t       rchain = 0, echain = 0;
+            {
+                const uint32_t R = L.runpref[kVPR];
+                uint32_t       p = 0;
+                for (; R - p >= 1024; p += 1024) {
+                    const uint32_t bw = bitlen64(L.rcmax[p >> 10] - L.rcmin[p >> 10]);
+                    if (bw >= 32) break;
+                    rchain += 12 + 128ull * bw;
+                }
+                rchain += 12 + 4ull * (R - p);
+                const uint32_t E = L.excpref[kVPR];
+                p = 0;
+                for (; E - p >= 1024; p += 1024) {
+                    const uint32_t bw = bitlen64(L.excmax[p >> 10] - L.excmin[p >> 10]);
+                    if (bw >= 64) break;
+                    echain += 12 + 128ull * bw;
+                }
+                echain += 12 + 8ull * (E - p);
+            }
+            uint64_t s[kK] = {};
             for (unsigned v = 0; v < kVPR; ++v) {
+                const uint64_t pk = 9 + 128ull * L.pbw[v], n = L.pexc[v];
                 s[0] += 8192;
                 s[1] += 9 + 128ull * L.bwf[v];
                 s[2] += 137 + 128ull * L.bwd[v];
@@ -257,20 +490,33 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
                 s[5] += 4096;
                 s[6] += 5 + 128ull * bitlen64(L.cmax[v] - L.cmin[v]);
                 s[7] += 133 + 128ull * bitlen64(L.cdmax[v] - L.cdmin[v]);
+                s[8] += 8 * n + 2 + 2 * n + pk;             // S_PATCH [exceptions][count, positions][bw][base][packed]
+                s[9] += 2 + 2 * n + pk;                     // S_PATCH_REC: exceptions in the page's chain
+                s[10] += 8 * n + 128 + pk;                  // B_PATCH [exceptions][bitmap]...
+                s[11] += 8ull * L.pchain[v] + 4 + pk;       // LL_PATCH [chain values][count, first]...
+                s[12] += 131 + 128ull * L.bwp[v];           // DICT_RLE [run codes: chain][lane bases][bw][base][packed]
             }
-            constexpr unsigned arr[kK] = {1, 3, 4, 5, 2, 1, 3, 4};
+            s[9] += echain;
+            s[12] += rchain;
+            constexpr unsigned arr[kK]  = {1, 3, 4, 5, 2, 1, 3, 4, 5, 5, 5, 5, 5};
+            constexpr bool     dic[kK]  = {false, false, false, false, false, true, true, true,
+                                           false, false, false, false, true};
             for (unsigned k = 0; k < kK; ++k) {
-                s[k] += 8 + 4ull * arr[k] * kVPR + (k >= 5 ? dict : 0);
+                s[k] += 8 + 4ull * arr[k] * kVPR + (dic[k] ? dict : 0);
                 sizes[r * kK + k] = s[k];
             }
         }
         // reset the slots this row-group used (the next row-group of this block reuses the table)
-        for (unsigned o = t; o < kRG; o += blockDim.x) {
-            const uint32_t h = slot[o];
-            keys[h]          = kEmpty;  // slot kHt (the kEmpty value's own slot) is padding here
-            first[h]         = ~0u;
+        for (unsigned o0 = t; o0 < kRG; o0 += kBatch * stride) {
+            uint32_t sl[kBatch];
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) sl[k] = slot[o0 + k * stride];
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) {
+                keys[sl[k]]  = kEmpty;  // slot kHt (the kEmpty value's own slot) is padding here
+                first[sl[k]] = ~0u;
+            }
         }
-        __threadfence();
         __syncthreads();
     }
 }
@@ -288,10 +534,13 @@ __global__ void k_wizard_best(const uint64_t* __restrict__ sizes, uint64_t n_rg,
 // ---------------------------------------------------------------------------------------------------------------
 // Learned selector's statistics: one 256-thread block per row-group, reading the 4 sample vectors only.
 // ---------------------------------------------------------------------------------------------------------------
-constexpr unsigned kSHt = kSample;  // 4096 slots (+1 for the kEmpty value): never full for 4096 inputs
+constexpr unsigned kFHt = 2 * kSample;  // feature table: 8192 slots of a first position (load <= 0.5)
 struct FeatLds {
-    unsigned long long keys[kSHt];
-    uint32_t           first[kSHt + 1];
+    union {  // the sample vectors' sort buffers, then the sample values in position order
+        uint64_t sortbuf[kSampleV][1024];
+        uint64_t xs[kSample];
+    } u;
+    uint32_t           tab[kFHt];  // first position of each distinct value (then its multiplicity)
     uint16_t           slot[kSample];
     uint32_t           bitmap[kSample / 32], base[kSample / 32];
     uint32_t           bwf[kSampleV], bwd[kSampleV], runs[kSampleV], bwp[kSampleV], eq[kSampleV], nondec[kSampleV],
@@ -301,41 +550,73 @@ struct FeatLds {
     uint32_t           cmin[kSampleV], cmax[kSampleV], cdmin[kSampleV], cdmax[kSampleV];
     uint32_t           top;
     uint32_t           tmp[8];
+    uint32_t           pbw[kSampleV], pexc[kSampleV];  // patch plan of each sample vector
 };
+
+// Inserts sample position p (value xs[p]) into the position table: a slot holds the smallest position of one value
+// (values compared through xs). Returns the slot.
+__device__ __forceinline__ uint32_t pos_insert(uint32_t* tab, const uint64_t* xs, uint32_t p) {
+    const uint64_t v = xs[p];
+    uint32_t       h = mix64(v) & (kFHt - 1);
+    for (;;) {
+        uint32_t cur = tab[h];
+        if (cur == ~0u) {
+            cur = atomicCAS(&tab[h], ~0u, p);
+            if (cur == ~0u) return h;
+        }
+        if (xs[cur] == v) {
+            atomicMin(&tab[h], p);
+            return h;
+        }
+        h = (h + 1) & (kFHt - 1);  // never full: 4096 values, 8192 slots
+    }
+}
 
 __device__ __forceinline__ float log2f_(float x) { return __log2f(x); }
 
 __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restrict__ values, uint64_t n_rg,
                                                          float* __restrict__ feat) {
-    __shared__ FeatLds L;
+    extern __shared__ __attribute__((aligned(16))) uint8_t wiz_lds[];  // sizeof(FeatLds), about 74 KB
+    FeatLds&       L = *reinterpret_cast<FeatLds*>(wiz_lds);
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
     for (uint64_t r = blockIdx.x; r < n_rg; r += gridDim.x) {
         const uint64_t* x = values + r * kRG;
-        for (unsigned i = t; i < kSHt; i += blockDim.x) L.keys[i] = kEmpty, L.first[i] = ~0u;
-        if (t == 0) L.first[kSHt] = ~0u, L.top = 0;
+        if (t == 0) L.top = 0;
         for (unsigned i = t; i < kSample / 32; i += blockDim.x) L.bitmap[i] = 0;
         if (t < kSampleV) L.cmin[t] = ~0u, L.cmax[t] = 0, L.cdmin[t] = ~0u, L.cdmax[t] = 0;
         __syncthreads();
         {  // wave w: sample vector w
-            const VecStats s = vector_stats(x + kSampleVec[wave] * 1024);
+            const VecStats s  = vector_stats(x + kSampleVec[wave] * 1024);
+            uint64_t*      sb = L.u.sortbuf[wave];
+#pragma unroll
+            for (unsigned i = 0; i < 16; ++i) sb[64 * i + lane] = x[kSampleVec[wave] * 1024 + 64 * i + lane];
+            wave_sort1024(sb);
+            const PatchPlan pp = patch_plan(sb, s.bwf);
             if (lane == 0) {
+                L.pbw[wave] = pp.bw, L.pexc[wave] = pp.exc;
                 L.bwf[wave] = s.bwf, L.bwd[wave] = s.bwd, L.runs[wave] = s.runs, L.bwp[wave] = s.bwp;
                 L.eq[wave] = s.eq, L.nondec[wave] = s.nondec, L.inc[wave] = s.inc;
                 L.umin[wave] = s.umin, L.umax[wave] = s.umax, L.smin[wave] = s.smin, L.smax[wave] = s.smax;
             }
         }
-        uint64_t xv[kSample / 256];
+        __syncthreads();  // the sort buffers become the sample values in position order
+        for (unsigned i = t; i < kFHt; i += blockDim.x) L.tab[i] = ~0u;
 #pragma unroll
         for (unsigned j = 0; j < kSample / 256; ++j) {
             const unsigned p = t + 256 * j;
-            xv[j]            = x[kSampleVec[p >> 10] * 1024 + (p & 1023)];
-            L.slot[p]        = (uint16_t)ht_insert(L.keys, L.first, kSHt - 1, xv[j], p);
+            L.u.xs[p]        = x[kSampleVec[p >> 10] * 1024 + (p & 1023)];
         }
         __syncthreads();
 #pragma unroll
         for (unsigned j = 0; j < kSample / 256; ++j) {
             const unsigned p = t + 256 * j;
-            if (L.first[L.slot[p]] == p) atomicOr(&L.bitmap[p >> 5], 1u << (p & 31));
+            L.slot[p]        = (uint16_t)pos_insert(L.tab, L.u.xs, p);
+        }
+        __syncthreads();
+#pragma unroll
+        for (unsigned j = 0; j < kSample / 256; ++j) {
+            const unsigned p = t + 256 * j;
+            if (L.tab[L.slot[p]] == p) atomicOr(&L.bitmap[p >> 5], 1u << (p & 31));
         }
         __syncthreads();
         const uint32_t D = block_popcount_scan(L.bitmap, L.base, kSample / 32, L.tmp);
@@ -343,9 +624,9 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
 #pragma unroll
         for (unsigned j = 0; j < kSample / 256; ++j) {
             const unsigned p  = t + 256 * j;
-            fp[j]             = L.first[L.slot[p]];
+            fp[j]             = L.tab[L.slot[p]];
             const uint32_t c  = rank_of(L.bitmap, L.base, fp[j]);
-            const uint32_t cp = (p % 32) ? rank_of(L.bitmap, L.base, L.first[L.slot[p - 1]]) : c;
+            const uint32_t cp = (p % 32) ? rank_of(L.bitmap, L.base, L.tab[L.slot[p - 1]]) : c;
             const uint32_t d  = c - cp;
             const uint32_t cmn = wave_min(c), cmx = wave_max(c), dmn = wave_min(d), dmx = wave_max(d);
             if (lane == 0) {
@@ -355,15 +636,15 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
             }
         }
         __syncthreads();
-        // multiplicities: reuse first[] as counters
-        for (unsigned i = t; i <= kSHt; i += blockDim.x) L.first[i] = 0;
+        // multiplicities: the table's slots become counters
+        for (unsigned i = t; i < kFHt; i += blockDim.x) L.tab[i] = 0;
         __syncthreads();
 #pragma unroll
-        for (unsigned j = 0; j < kSample / 256; ++j) atomicAdd(&L.first[L.slot[t + 256 * j]], 1u);
+        for (unsigned j = 0; j < kSample / 256; ++j) atomicAdd(&L.tab[L.slot[t + 256 * j]], 1u);
         __syncthreads();
         {
             uint32_t m = 0;
-            for (unsigned i = t; i <= kSHt; i += blockDim.x) m = L.first[i] > m ? L.first[i] : m;
+            for (unsigned i = t; i < kFHt; i += blockDim.x) m = L.tab[i] > m ? L.tab[i] : m;
             m = wave_max(m);
             if (lane == 0) atomicMax(&L.top, m);
         }
@@ -372,7 +653,7 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
             float*         F  = feat + r * 64;
             const float    S  = (float)kSampleV, vv = (float)kVPR;
             uint32_t       sbwf = 0, sbwd = 0, sruns = 0, sbwp = 0, sbwc = 0, sbwcd = 0, eq = 0, nondec = 0, inc = 0;
-            uint32_t       mxbwf = 0, mnbwf = 64, mxbwd = 0, mnbwd = 64, mxr = 0, mnr = 1024;
+            uint32_t       mxbwf = 0, mnbwf = 64, mxbwd = 0, mnbwd = 64, mxr = 0, mnr = 1024, spbw = 0, sexc = 0;
             uint64_t       umax = 0, umin = kEmpty;
             int64_t        smax = INT64_MIN, smin = INT64_MAX;
             uint32_t       bwc[kSampleV], bwcd[kSampleV];
@@ -381,6 +662,7 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
                 bwcd[i] = bitlen64(L.cdmax[i] - L.cdmin[i]);
                 sbwf += L.bwf[i], sbwd += L.bwd[i], sruns += L.runs[i], sbwp += L.bwp[i];
                 sbwc += bwc[i], sbwcd += bwcd[i];
+                spbw += L.pbw[i], sexc += L.pexc[i];
                 eq += L.eq[i], nondec += L.nondec[i], inc += L.inc[i];
                 mxbwf = max(mxbwf, L.bwf[i]), mnbwf = min(mnbwf, L.bwf[i]);
                 mxbwd = max(mxbwd, L.bwd[i]), mnbwd = min(mnbwd, L.bwd[i]);
@@ -392,7 +674,7 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
             }
             const float mbwf = (float)sbwf / S, mbwd = (float)sbwd / S, mruns = (float)sruns / S,
                         mbwp = (float)sbwp / S, mbwc = (float)sbwc / S, mbwcd = (float)sbwcd / S;
-            const float fD = (float)D;
+            const float fD = (float)D, mpbw = (float)spbw / S, mexc = (float)sexc / S;
             float       est[kK];
             est[0] = 8.f + 4.f * vv + 8192.f * vv;
             est[1] = 8.f + 12.f * vv + vv * (9.f + 128.f * mbwf);
@@ -402,29 +684,36 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
             est[5] = 28.f + 8.f * fD + 4.f * vv + 4096.f * vv;
             est[6] = 28.f + 8.f * fD + 12.f * vv + vv * (5.f + 128.f * mbwc);
             est[7] = 28.f + 8.f * fD + 16.f * vv + vv * (133.f + 128.f * mbwcd);
-            for (unsigned k = 0; k < kK; ++k) F[k] = est[k] / (float)(kRG * 8), F[8 + k] = log2f_(est[k]) / 24.f;
-            F[16] = mbwf / 64.f, F[17] = (float)mxbwf / 64.f, F[18] = (float)mnbwf / 64.f;
-            F[19] = mbwd / 64.f, F[20] = (float)mxbwd / 64.f, F[21] = (float)mnbwd / 64.f;
-            F[22] = mruns / 1024.f, F[23] = (float)mxr / 1024.f, F[24] = (float)mnr / 1024.f;
-            F[25] = log2f_(mruns) / 10.f;
-            F[26] = mbwp;
-            F[27] = fD / (S * 1024.f);
-            F[28] = log2f_(fD) / 12.f;
-            F[29] = (float)bitlen64(D - 1) / 16.f;
-            F[30] = (float)L.top / (S * 1024.f);
-            F[31] = (float)eq / (S * 1023.f);
-            F[32] = (float)nondec / (S * 1023.f);
-            F[33] = (float)inc / (S * 1023.f);
-            F[34] = (float)bitlen64(umax) / 64.f;
-            F[35] = (float)bitlen64(umin) / 64.f;
-            F[36] = (float)bitlen64((uint64_t)smax - (uint64_t)smin) / 64.f;
-            F[37] = mbwc / 32.f;
-            F[38] = mbwcd / 32.f;
-            for (unsigned i = 0; i < kSampleV; ++i) {
-                F[39 + 6 * i] = (float)L.bwf[i] / 64.f, F[40 + 6 * i] = (float)L.bwd[i] / 64.f;
-                F[41 + 6 * i] = (float)L.runs[i] / 1024.f, F[42 + 6 * i] = (float)L.bwp[i];
-                F[43 + 6 * i] = (float)bwc[i] / 32.f, F[44 + 6 * i] = (float)bwcd[i] / 32.f;
-            }
+            const float pk = 9.f + 128.f * mpbw;  // patched FFOR: bw, base and packed bytes per vector
+            est[8]  = 8.f + 20.f * vv + vv * (10.f * mexc + 2.f + pk);
+            est[9]  = 8.f + 20.f * vv + vv * (2.f * mexc + 2.f + pk) + 12.f + 8.f * mexc * vv;
+            est[10] = 8.f + 20.f * vv + vv * (8.f * mexc + 128.f + pk);
+            est[11] = 8.f + 20.f * vv + vv * (8.f * mexc + 4.f + pk);
+            est[12] = 28.f + 8.f * fD + 20.f * vv + vv * (131.f + 128.f * mbwp) + 12.f + 4.f * mruns * vv;
+            for (unsigned k = 0; k < kK; ++k) F[k] = est[k] / (float)(kRG * 8), F[kK + k] = log2f_(est[k]) / 24.f;
+            float* G = F + 2 * kK;  // 26
+            G[0] = mbwf / 64.f, G[1] = (float)mxbwf / 64.f, G[2] = (float)mnbwf / 64.f;
+            G[3] = mbwd / 64.f, G[4] = (float)mxbwd / 64.f, G[5] = (float)mnbwd / 64.f;
+            G[6] = mruns / 1024.f, G[7] = (float)mxr / 1024.f, G[8] = (float)mnr / 1024.f;
+            G[9]  = log2f_(mruns) / 10.f;
+            G[10] = mbwp;
+            G[11] = fD / (S * 1024.f);
+            G[12] = log2f_(fD) / 12.f;
+            G[13] = (float)bitlen64(D - 1) / 16.f;
+            G[14] = (float)L.top / (S * 1024.f);
+            G[15] = (float)eq / (S * 1023.f);
+            G[16] = (float)nondec / (S * 1023.f);
+            G[17] = (float)inc / (S * 1023.f);
+            G[18] = (float)bitlen64(umax) / 64.f;
+            G[19] = (float)bitlen64(umin) / 64.f;
+            G[20] = (float)bitlen64((uint64_t)smax - (uint64_t)smin) / 64.f;
+            G[21] = mbwc / 32.f;
+            G[22] = mbwcd / 32.f;
+            for (unsigned i = 0; i < kSampleV; ++i)  // 49 .. 60
+                G[23 + 3 * i] = (float)L.bwf[i] / 64.f, G[24 + 3 * i] = (float)L.runs[i] / 1024.f,
+                           G[25 + 3 * i] = (float)bwc[i] / 32.f;
+            F[61] = mpbw / 64.f;
+            F[62] = mexc / 100.f;
             F[63] = 1.f;
         }
         __syncthreads();
@@ -484,7 +773,7 @@ int fls_wizard_sizes(const int64_t* dev_values, uint64_t n_rowgroups, uint64_t* 
     int      rc = sizer_ws(w);
     if (rc) return rc;
     const unsigned blocks = (unsigned)std::min<uint64_t>(n_rowgroups, w->blocks);
-    hipLaunchKernelGGL(k_wizard_sizes, dim3(blocks), dim3(1024), 0, (hipStream_t)stream, (const uint64_t*)dev_values,
+    hipLaunchKernelGGL(k_wizard_sizes, dim3(blocks), dim3(1024), sizeof(SizerLds), (hipStream_t)stream, (const uint64_t*)dev_values,
                        n_rowgroups, w->tables, w->firsts, w->slots, dev_sizes);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? FLS_OK : hip_fail(e, "wizard sizes launch");
@@ -504,7 +793,7 @@ int fls_wizard_features(const int64_t* dev_values, uint64_t n_rowgroups, float* 
     if (!dev_values || !dev_features) return fail(FLS_E_ARG, "null argument");
     if ((uintptr_t)dev_values & 15) return fail(FLS_E_ARG, "values must be 16-byte aligned");
     const unsigned blocks = (unsigned)std::min<uint64_t>(n_rowgroups, 65535u);
-    hipLaunchKernelGGL(k_wizard_features, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_wizard_features, dim3(blocks), dim3(256), sizeof(FeatLds), (hipStream_t)stream,
                        (const uint64_t*)dev_values, n_rowgroups, dev_features);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? FLS_OK : hip_fail(e, "wizard features launch");

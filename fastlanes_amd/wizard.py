@@ -7,7 +7,7 @@ smallest (ExpColEncoder<PT>::encode_all + AllExpEncodedCol::GetBest, src/encoder
 * exhaustive -- `fls_wizard_sizes` computes the exact encoded size under every candidate in closed form on the GPU
   (what `encode_from_memory` would return), `fls_wizard_best` takes the argmin;
 * learned    -- `fls_wizard_features` reads 4 of the 64 vectors and computes 64 statistics, the MLP
-  (64-256-256-8, BF16 MFMA, FP32 accumulation) proposes the expression.
+  (64-256-256-K over the K candidates, BF16 MFMA, FP32 accumulation) proposes the expression.
 
 The MLP is trained on the GPU (fls_mlp_forward_backward / fls_mlp_adam_step) on labels from the exhaustive sizer.
 Either choice is then encoded by the same bit-exact codec: the selector changes sizes, never decoded values.
@@ -31,7 +31,7 @@ RG = 65536
 N_FEATURES = 64
 _vp, _u64, _u32 = C.c_void_p, C.c_uint64, C.c_uint32
 WEIGHTS_MAGIC = b"FLSWIZ01"
-FEATURE_SCHEMA = "v1:est8,log8,bw,runs,dict,order,mag,sample4x6,bias"
+FEATURE_SCHEMA = "v2:est13,log13,bw,runs,dict,order,mag,sample4x3,patch2,bias"
 
 
 def _lib():
@@ -144,8 +144,8 @@ class MLP:
 
 
 def sample_estimate_choice(feat: np.ndarray) -> np.ndarray:
-    """The selector without the network: argmin of the sample-based size estimates (features 0..7)."""
-    return np.argmin(feat[:, :8], axis=1).astype(np.int32)
+    """The selector without the network: argmin of the sample-based size estimates (features 0 .. K-1)."""
+    return np.argmin(feat[:, :len(candidates())], axis=1).astype(np.int32)
 
 
 def _events(torch, stream):
@@ -184,8 +184,9 @@ def train(torch, mlp: MLP, d_feat, d_lab, idx: np.ndarray, steps: int, batch: in
     return curve
 
 
-def run(dev, stream, n_columns=512, n=1 << 20, steps=3000, batch=512, lr=2e-3, seed=99, reps=5):
-    """Config-5 study: exhaustive labels, training, held-out regret and selection times. Returns a dict."""
+def run(dev, stream, n_columns=512, n=1 << 20, steps=3000, batch=512, lr=2e-3, seed=99, reps=5, return_model=False):
+    """Config-5 study: exhaustive labels, training, held-out regret and selection times. Returns a dict (and the
+    trained MLP with return_model)."""
     import torch
 
     sp = int(stream.cuda_stream)
@@ -257,7 +258,7 @@ def run(dev, stream, n_columns=512, n=1 << 20, steps=3000, batch=512, lr=2e-3, s
         "per_family_test_regret": {f: regret(pred, test & (np.array(fams)[col] == f)) for f in wl.WIZARD_FAMILIES},
     }
     res["meets_targets"] = bool(res["speedup"] >= 10 and res["test_regret"] <= 0.02)
-    return res
+    return (res, mlp) if return_model else res
 
 
 def bench(dev):

@@ -16,6 +16,7 @@
 #include "fls_gen/untranspose/untranspose.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstring>
@@ -150,33 +151,58 @@ uint64_t ref_fsst_decode(const uint8_t* table, const uint8_t* comp, const uint32
 
 // CPU baseline: the reference's per-vector decode of an FFOR stream (unffor + Materializer untranspose_i,
 // src/encoder/exp_col_decoder.cpp:13-26, src/encoder/materializer.cpp:13-21) or of the raw primitive
-// (untranspose = 0), over n_vec vectors on n_threads threads. Returns seconds per pass (median of n_iter).
+// (untranspose = 0), over n_vec vectors on n_threads threads. The threads are started, and each decodes its slice
+// once, before the timed passes; a pass releases every thread through a generation counter and ends when the last
+// one has finished its slice. Returns seconds per pass (median of n_iter).
 double ref_time_unffor(unsigned w, const void* in, void* out, uint64_t n_vec, unsigned bw, const void* bases,
                        int untranspose, int n_iter, int n_threads) {
-    const unsigned sz = w / 8;
+    const unsigned    sz = w / 8;
+    std::atomic<int>  gen {0}, done {0};
+    std::atomic<bool> quit {false};
+    auto slice = [&](int k, std::vector<uint8_t>& tmp) {
+        const uint64_t v0 = n_vec * k / n_threads, v1 = n_vec * (k + 1) / n_threads;
+        for (uint64_t v = v0; v < v1; ++v) {
+            const uint8_t* src = static_cast<const uint8_t*>(in) + v * 128 * bw;
+            uint8_t*       dst = static_cast<uint8_t*>(out) + v * 1024 * sz;
+            const uint8_t* b   = static_cast<const uint8_t*>(bases) + v * sz;
+            if (untranspose) {
+                ref_unffor(w, src, tmp.data(), bw, b);
+                ref_untranspose_i(w, tmp.data(), dst);
+            } else {
+                ref_unffor(w, src, dst, bw, b);
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 0; k < n_threads; ++k)
+        th.emplace_back([&, k] {
+            std::vector<uint8_t> tmp(1024 * 8);
+            int                  seen = 0;
+            for (;;) {
+                gen.wait(seen, std::memory_order_acquire);  // sleeps until the next pass (or the end)
+                if (quit.load(std::memory_order_acquire)) return;
+                seen = gen.load(std::memory_order_acquire);
+                slice(k, tmp);
+                if (done.fetch_add(1, std::memory_order_acq_rel) + 1 == n_threads) done.notify_one();
+            }
+        });
+    auto pass = [&] {
+        done.store(0, std::memory_order_release);
+        gen.fetch_add(1, std::memory_order_acq_rel);
+        gen.notify_all();
+        for (int d; (d = done.load(std::memory_order_acquire)) < n_threads;) done.wait(d, std::memory_order_acquire);
+    };
+    pass();  // untimed: every thread has run and touched its slice
     std::vector<double> t(n_iter);
     for (int it = 0; it < n_iter; ++it) {
-        auto                     t0 = std::chrono::steady_clock::now();
-        std::vector<std::thread> th;
-        for (int k = 0; k < n_threads; ++k)
-            th.emplace_back([&, k] {
-                std::vector<uint8_t> tmp(1024 * 8);
-                uint64_t             v0 = n_vec * k / n_threads, v1 = n_vec * (k + 1) / n_threads;
-                for (uint64_t v = v0; v < v1; ++v) {
-                    const uint8_t* src = static_cast<const uint8_t*>(in) + v * 128 * bw;
-                    uint8_t*       dst = static_cast<uint8_t*>(out) + v * 1024 * sz;
-                    const uint8_t* b   = static_cast<const uint8_t*>(bases) + v * sz;
-                    if (untranspose) {
-                        ref_unffor(w, src, tmp.data(), bw, b);
-                        ref_untranspose_i(w, tmp.data(), dst);
-                    } else {
-                        ref_unffor(w, src, dst, bw, b);
-                    }
-                }
-            });
-        for (auto& x : th) x.join();
+        const auto t0 = std::chrono::steady_clock::now();
+        pass();
         t[it] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+    quit.store(true, std::memory_order_release);
+    gen.fetch_add(1, std::memory_order_acq_rel);
+    gen.notify_all();
+    for (auto& x : th) x.join();
     std::sort(t.begin(), t.end());
     return t[n_iter / 2];
 }

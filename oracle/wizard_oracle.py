@@ -29,7 +29,11 @@ from __future__ import annotations
 
 import numpy as np
 
-CANDIDATES = (0, 1, 11, 6, 7, 16, 17, 22)  # expression ids (i64 pool)
+CANDIDATES = (0, 1, 11, 6, 7, 16, 17, 22, 2, 3, 4, 5, 27)  # expression ids (i64 pool)
+EXC_LIMIT = 100  # CFG::CMPR::EXC_LIMIT_C (include/fls/cfg/cfg.hpp:67)
+_ORDER8 = np.array([0, 4, 2, 6, 1, 5, 3, 7])
+_K = np.arange(1024)
+ORIG_OF = (_K >> 7) + 8 * _ORDER8[(_K >> 4) & 7] + 64 * (_K & 15)  # transposed index k -> original position
 K = len(CANDIDATES)
 RG = 65536
 V = RG // 1024
@@ -68,18 +72,60 @@ def _vec_bw(x2: np.ndarray) -> np.ndarray:
     return bitlen((mx - mn).astype(np.uint64))
 
 
-def hzl_size(dvals: np.ndarray) -> int:
-    """Bytes of the HZL-compressed dictionary payload (no exceptions; a full-width chunk ends the FFOR part)."""
+def hzl_size(dvals: np.ndarray, width: int = 64) -> int:
+    """Bytes of an HZL chain of `width`-bit values (no exceptions; a full-width chunk ends the FFOR part):
+    12 + 128 bw per full chunk of 1024, then 12 + the raw rest (src/cor/eng/hzl_cmp.cpp:43-176)."""
     n = dvals.size
     pos, size = 0, 0
     while n - pos >= 1024:
         c = dvals[pos: pos + 1024].astype(np.uint64)
         bw = int(bitlen(np.array([c.max() - c.min()], dtype=np.uint64))[0])
-        if bw >= 64:
+        if bw >= width:
             break
         size += 12 + 128 * bw
         pos += 1024
-    return size + 12 + 8 * (n - pos)
+    return size + 12 + (width // 8) * (n - pos)
+
+
+def patch_plan(v: np.ndarray):
+    """find_best_params (src/cor/prm/ffor/ffor_analyze.cpp:41-75) on one vector of uint64 values: the (bw, base)
+    minimising 128 bw + 8 (values outside [base, base + 2^bw - 1]) with at most EXC_LIMIT values outside, bw < 64,
+    base = the first occurrence of one of the 101 smallest values (bw = 0: only base 0, window {0}); ties to the
+    smaller start index, then the larger bw; none below 128 * 64 bytes -> (64, 0). Every (bw, start) pair is
+    evaluated (one search per pair)."""
+    sv = np.sort(v.astype(np.uint64))
+    M = np.uint64((1 << 64) - 1)
+    starts = np.array([i for i in range(EXC_LIMIT + 1) if i == 0 or sv[i] != sv[i - 1]])
+    si = sv[starts]                                               # (S,)
+    bws = np.arange(1, 64, dtype=np.uint64)
+    rng_ = (np.uint64(1) << bws) - np.uint64(1)                   # (63,)
+    upper = np.where(si[None, :] > M - rng_[:, None], M, si[None, :] + rng_[:, None])  # (63, S), saturating
+    inside = np.searchsorted(sv, upper.ravel(), side="right").reshape(upper.shape) - starts[None, :]
+    exc = 1024 - inside
+    best = None
+    for b in range(63):
+        for j in range(starts.size):
+            if exc[b, j] <= EXC_LIMIT:
+                key = (128 * (b + 1) + 8 * int(exc[b, j]), int(starts[j]), -(b + 1))
+                best = key if best is None or key < best else best
+    zeros = int(np.count_nonzero(sv == 0))  # bw = 0: the window {0}, from base 0 only
+    if sv[0] == 0 and 1024 - zeros <= EXC_LIMIT:
+        key = (8 * (1024 - zeros), 0, 0)
+        best = key if best is None or key < best else best
+    if best is None or best[0] >= 128 * 64:
+        return 64, 0
+    return -best[2], int(sv[best[1]])
+
+
+def patch_exceptions(v: np.ndarray, bw: int, base: int) -> np.ndarray:
+    """Transposed indices of a vector's exceptions under plan (bw, base), ascending (the encoder's order)."""
+    t = v.astype(np.uint64)[ORIG_OF]
+    if bw >= 64:
+        return np.zeros(0, dtype=np.int64)
+    if bw == 0:
+        return np.nonzero(t != 0)[0]
+    upper = min(base + (1 << bw) - 1, (1 << 64) - 1)
+    return np.nonzero((t < np.uint64(base)) | (t > np.uint64(upper)))[0]
 
 
 def dictionary(x: np.ndarray):
@@ -125,6 +171,25 @@ def candidate_sizes(x: np.ndarray) -> np.ndarray:
         dict_page + page(3, 5 + 128 * bwc),                      # 17 DICT, FFOR codes
         dict_page + page(4, 133 + 128 * bwcd),                   # 22 DICT, delta codes
     ]
+    # patched FFOR (2 S_PATCH, 3 S_PATCH_REC, 4 B_PATCH, 5 LL_PATCH): per vector [exceptions][positions][bw][base][packed]
+    s_b, srec_b, b_b, ll_b, chain = [], [], [], [], []
+    for vv in range(V):
+        bw, base = patch_plan(u2[vv])
+        pos = patch_exceptions(u2[vv], bw, base)
+        n = int(pos.size)
+        hop = 1 << min(bw, 11)
+        m = n + int(sum((int(g) - 1) // hop for g in np.diff(pos)))  # the LL chain's hop links
+        pk = 9 + 128 * bw
+        s_b.append(8 * n + 2 + 2 * n + pk)
+        srec_b.append(2 + 2 * n + pk)
+        b_b.append(8 * n + 128 + pk)
+        ll_b.append(8 * m + 4 + pk)
+        chain.append(u2[vv][ORIG_OF][pos])
+    srec_chain = hzl_size(np.concatenate(chain) if chain else np.zeros(0, np.uint64))
+    # DICT_RLE (27): per vector [run codes: the page's HZL chain of u32][pos lane bases][bw][base u16][packed]
+    run_codes = codes[chg]
+    sizes += [page(5, s_b), page(5, srec_b) + srec_chain, page(5, b_b), page(5, ll_b),
+              dict_page + page(5, 131 + 128 * bwp) + hzl_size(run_codes.astype(np.uint64), 32)]
     return np.array(sizes, dtype=np.uint64)
 
 
@@ -166,6 +231,9 @@ def features(x: np.ndarray) -> np.ndarray:
     umax, umin = int(flat.max()), int(flat.min())
     smax, smin = int(si.max()), int(si.min())
     bws = int(bitlen(np.array([(smax - smin) % (1 << 64)], dtype=np.uint64))[0])
+    plans = [patch_plan(s[i]) for i in range(S)]
+    pbw = np.array([bw for bw, _ in plans])
+    pexc = np.array([patch_exceptions(s[i], bw, base).size for i, (bw, base) in enumerate(plans)])
 
     f32 = np.float32
     mbwf, mbwd, mruns, mbwp = (f32(np.sum(a)) / f32(S) for a in (bwf, bwd, runs, bwp))
@@ -181,29 +249,42 @@ def features(x: np.ndarray) -> np.ndarray:
         f32(28) + f32(8) * f32(D) + f32(12) * vv + vv * (f32(5) + f32(128) * mbwc),
         f32(28) + f32(8) * f32(D) + f32(16) * vv + vv * (f32(133) + f32(128) * mbwcd),
     ], dtype=np.float32)
+    mpbw, mexc = f32(np.sum(pbw)) / f32(S), f32(np.sum(pexc)) / f32(S)
+    pk = f32(9) + f32(128) * mpbw
+    base20 = f32(8) + f32(20) * vv
+    est = np.concatenate([est, np.array([
+        base20 + vv * (f32(10) * mexc + f32(2) + pk),
+        base20 + vv * (f32(2) * mexc + f32(2) + pk) + f32(12) + f32(8) * mexc * vv,
+        base20 + vv * (f32(8) * mexc + f32(128) + pk),
+        base20 + vv * (f32(8) * mexc + f32(4) + pk),
+        f32(28) + f32(8) * f32(D) + f32(20) * vv + vv * (f32(131) + f32(128) * mbwp) + f32(12) + f32(4) * mruns * vv,
+    ], dtype=np.float32)])
+    K = len(CANDIDATES)
     F = np.zeros(N_FEATURES, dtype=np.float32)
-    F[0:8] = est / f32(RG * 8)
-    F[8:16] = _log2f(est) / f32(24)
-    F[16], F[17], F[18] = mbwf / f32(64), f32(bwf.max()) / f32(64), f32(bwf.min()) / f32(64)
-    F[19], F[20], F[21] = mbwd / f32(64), f32(bwd.max()) / f32(64), f32(bwd.min()) / f32(64)
-    F[22], F[23], F[24] = mruns / f32(1024), f32(runs.max()) / f32(1024), f32(runs.min()) / f32(1024)
-    F[25] = _log2f(mruns) / f32(10)
-    F[26] = mbwp
-    F[27] = f32(D) / f32(S * 1024)
-    F[28] = _log2f(D) / f32(12)
-    F[29] = f32(int(bitlen(np.array([D - 1], dtype=np.uint64))[0])) / f32(16)
-    F[30] = f32(top) / f32(S * 1024)
-    F[31] = f32(eq) / f32(S * 1023)
-    F[32] = f32(nondec) / f32(S * 1023)
-    F[33] = f32(inc) / f32(S * 1023)
-    F[34] = f32(int(bitlen(np.array([umax], dtype=np.uint64))[0])) / f32(64)
-    F[35] = f32(int(bitlen(np.array([umin], dtype=np.uint64))[0])) / f32(64)
-    F[36] = f32(bws) / f32(64)
-    F[37] = mbwc / f32(32)
-    F[38] = mbwcd / f32(32)
+    F[0:K] = est / f32(RG * 8)
+    F[K:2 * K] = _log2f(est) / f32(24)
+    G = 2 * K
+    F[G + 0], F[G + 1], F[G + 2] = mbwf / f32(64), f32(bwf.max()) / f32(64), f32(bwf.min()) / f32(64)
+    F[G + 3], F[G + 4], F[G + 5] = mbwd / f32(64), f32(bwd.max()) / f32(64), f32(bwd.min()) / f32(64)
+    F[G + 6], F[G + 7], F[G + 8] = mruns / f32(1024), f32(runs.max()) / f32(1024), f32(runs.min()) / f32(1024)
+    F[G + 9] = _log2f(mruns) / f32(10)
+    F[G + 10] = mbwp
+    F[G + 11] = f32(D) / f32(S * 1024)
+    F[G + 12] = _log2f(D) / f32(12)
+    F[G + 13] = f32(int(bitlen(np.array([D - 1], dtype=np.uint64))[0])) / f32(16)
+    F[G + 14] = f32(top) / f32(S * 1024)
+    F[G + 15] = f32(eq) / f32(S * 1023)
+    F[G + 16] = f32(nondec) / f32(S * 1023)
+    F[G + 17] = f32(inc) / f32(S * 1023)
+    F[G + 18] = f32(int(bitlen(np.array([umax], dtype=np.uint64))[0])) / f32(64)
+    F[G + 19] = f32(int(bitlen(np.array([umin], dtype=np.uint64))[0])) / f32(64)
+    F[G + 20] = f32(bws) / f32(64)
+    F[G + 21] = mbwc / f32(32)
+    F[G + 22] = mbwcd / f32(32)
     for i in range(S):
-        F[39 + 6 * i: 45 + 6 * i] = [f32(bwf[i]) / f32(64), f32(bwd[i]) / f32(64), f32(runs[i]) / f32(1024),
-                                      f32(bwp[i]), f32(bwc[i]) / f32(32), f32(bwcd[i]) / f32(32)]
+        F[G + 23 + 3 * i: G + 26 + 3 * i] = [f32(bwf[i]) / f32(64), f32(runs[i]) / f32(1024), f32(bwc[i]) / f32(32)]
+    F[61] = mpbw / f32(64)
+    F[62] = mexc / f32(100)
     F[63] = f32(1)
     return F
 

@@ -128,23 +128,31 @@ def test_training_gains_over_rule_without_network(torch_cuda, wiz):
     size regret than argmin of the sample estimates alone, and its choice decodes bit-exact."""
     torch = torch_cuda
     print("wizard training study", flush=True)
-    res = wiz.run(torch.device("cuda:0"), torch.cuda.current_stream(), n_columns=60, n=1 << 18, steps=600,
-                  batch=256, reps=2)
+    res, mlp = wiz.run(torch.device("cuda:0"), torch.cuda.current_stream(), n_columns=60, n=1 << 18, steps=600,
+                       batch=256, reps=2, return_model=True)
     print(res, flush=True)
     assert res["loss_curve"][-1][1] < res["loss_curve"][0][1]
     assert res["test_regret"] < res["no_network_test_regret"]
     assert res["test_regret"] < 0.05
-    # the selector only picks which bit-exact codec runs: encode one held-out row-group its way and decode on GPU
-    x = wl.wizard_column("zipf", wo.RG, seed=77)
-    d_x = torch.from_numpy(x).cuda()
-    d_f = torch.zeros((1, 64), dtype=torch.float32, device="cuda")
-    wiz.features(d_x.data_ptr(), 1, d_f.data_ptr())
-    torch.cuda.synchronize()
-    eid = wo.CANDIDATES[int(wo.best(wo.candidate_sizes(x)))]
-    enc = fa.encode_from_memory(x, eid)
-    plan = fa.Plan(enc, np.int64)
-    d_enc = torch.from_numpy(np.concatenate([enc, np.zeros(16, np.uint8)])).cuda()
-    d_out = torch.empty(x.nbytes, dtype=torch.uint8, device="cuda")
-    plan.decode(d_enc.data_ptr(), d_out.data_ptr(), x.nbytes)
-    torch.cuda.synchronize()
-    np.testing.assert_array_equal(d_out.cpu().numpy().view(np.int64), x)
+    # the selector only picks which bit-exact codec runs: held-out row-groups of every family are encoded with the
+    # expression the trained network predicts from their features, decoded on the GPU and compared
+    for fam in wl.WIZARD_FAMILIES:
+        x = wl.wizard_column(fam, wo.RG, seed=77)
+        d_x = torch.from_numpy(x).cuda()
+        d_f = torch.zeros((1, 64), dtype=torch.float32, device="cuda")
+        d_c = torch.full((1,), -1, dtype=torch.int32, device="cuda")
+        wiz.features(d_x.data_ptr(), 1, d_f.data_ptr())
+        mlp.predict(d_f.data_ptr(), 1, d_c.data_ptr())
+        torch.cuda.synchronize()
+        k = int(d_c.cpu().numpy()[0])
+        assert 0 <= k < len(wo.CANDIDATES)
+        eid = wo.CANDIDATES[k]
+        print(f"learned choice for {fam}: expression {eid}", flush=True)
+        enc = fa.encode_from_memory(x, eid)
+        assert enc.size == int(wo.candidate_sizes(x)[k])
+        plan = fa.Plan(enc, np.int64)
+        d_enc = torch.from_numpy(np.concatenate([enc, np.zeros(16, np.uint8)])).cuda()
+        d_out = torch.empty(x.nbytes, dtype=torch.uint8, device="cuda")
+        plan.decode(d_enc.data_ptr(), d_out.data_ptr(), x.nbytes)
+        plan.check()
+        np.testing.assert_array_equal(d_out.cpu().numpy().view(np.int64), x, err_msg=f"{fam}, expression {eid}")

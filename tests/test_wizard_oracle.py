@@ -84,15 +84,16 @@ def test_features_against_loops(family):
     F = wo.features(x)
     b = _brute_features(x)
     assert F.dtype == np.float32 and F.shape == (64,)
+    G = 2 * len(wo.CANDIDATES)
     for i in range(4):
-        assert F[39 + 6 * i] * 64 == b["bwf"][i]
-        assert F[40 + 6 * i] * 64 == b["bwd"][i]
-        assert F[41 + 6 * i] * 1024 == b["runs"][i]
-        assert F[42 + 6 * i] == b["bwp"][i]
-        assert F[43 + 6 * i] * 32 == b["bwc"][i]
-        assert F[44 + 6 * i] * 32 == b["bwcd"][i]
-    assert F[27] * 4096 == b["D"]
-    assert F[30] * 4096 == b["top"]
+        assert F[G + 23 + 3 * i] * 64 == b["bwf"][i]
+        assert F[G + 24 + 3 * i] * 1024 == b["runs"][i]
+        assert F[G + 25 + 3 * i] * 32 == b["bwc"][i]
+    assert F[G + 3] * 64 == np.float32(sum(b["bwd"])) / np.float32(4)
+    assert F[G + 10] == np.float32(sum(b["bwp"])) / np.float32(4)
+    assert F[G + 22] * 32 == np.float32(sum(b["bwcd"])) / np.float32(4)
+    assert F[G + 11] * 4096 == b["D"]
+    assert F[G + 14] * 4096 == b["top"]
     assert F[63] == 1.0
     # the sample-size estimates of the non-dictionary candidates use the closed forms of candidate_sizes
     assert abs(F[1] * wo.RG * 8 - (8 + 12 * 64 + 64 * (9 + 128 * np.mean(b["bwf"])))) < 1e-2 * wo.RG
@@ -159,3 +160,56 @@ def test_spell_operator_lists():
     cols, kinds, refs = wl.spell_example_1()
     ops = spell.operators(kinds, refs)
     assert ops[:3] == [["CONSTANT"], ["CONSTANT"], ["ZERO_COPY"]] and ops[5] == ["EQUAL"]
+
+
+def _patch_plan_loops(v):
+    """plan_patch as literal loops over the sorted vector (the host encoder's search): (bw, base)."""
+    sv = sorted(int(a) for a in v.view(np.uint64))
+    M = (1 << 64) - 1
+    best_size, best_i, best_bw, found = 128 * 64, 0, 64, False
+    for bw in range(64):
+        rng_ = (1 << bw) - 1
+        j = 0
+        for i in range(1024):
+            if i and sv[i] == sv[i - 1]:
+                continue
+            if bw == 0:
+                inw = 0
+                if sv[i] == 0:
+                    while inw < 1024 and sv[inw] == 0:
+                        inw += 1
+            else:
+                upper = M if sv[i] > M - rng_ else sv[i] + rng_
+                j = max(j, i)
+                while j < 1024 and sv[j] <= upper:
+                    j += 1
+                inw = j - i
+            exc = 1024 - inw
+            size = 128 * bw + 8 * exc
+            if exc > wo.EXC_LIMIT:
+                continue
+            if size < best_size or (size == best_size and found and (i < best_i or (i == best_i and bw > best_bw))):
+                best_size, best_i, best_bw, found = size, i, bw, True
+    return (best_bw, sv[best_i]) if found else (64, 0)
+
+
+@pytest.mark.parametrize("case", ["outliers", "uniform", "zeros", "wide", "ties"])
+def test_patch_plan_against_loops(case):
+    """The oracle's vectorized patch plan equals the literal search on vectors built to exercise its branches: a
+    narrow body with outliers, uniform values, mostly zeros (bw = 0 windows), full 64-bit values, equal-size ties."""
+    g = np.random.default_rng(hash(case) % 1000)
+    if case == "outliers":
+        v = g.integers(1000, 1100, 1024).astype(np.uint64)
+        v[g.choice(1024, 37, replace=False)] = g.integers(0, 1 << 62, 37).astype(np.uint64)
+    elif case == "uniform":
+        v = g.integers(0, 1 << 20, 1024).astype(np.uint64)
+    elif case == "zeros":
+        v = np.zeros(1024, np.uint64)
+        v[g.choice(1024, 60, replace=False)] = g.integers(1, 1 << 40, 60).astype(np.uint64)
+    elif case == "wide":
+        v = g.integers(0, (1 << 64) - 1, 1024, dtype=np.uint64, endpoint=True)
+    else:
+        v = np.repeat(np.arange(16, dtype=np.uint64) * 7, 64)
+        v[:3] = np.uint64((1 << 64) - 1)
+    assert wo.patch_plan(v) == _patch_plan_loops(v)
+
