@@ -156,8 +156,9 @@ void init_random_access(uint8_t* in, uint64_t vec_id, uint64_t* n_buf, uint64_t*
     const uint8_t* pg  = in + pos;
     Expr           e;
     if (!expr_from_page(pg[0], pg[1], e)) return;
-    // DICT_RLE and S_PATCH_REC compress buffer 0 page-wide (one HZL chain): no vector owns a slice of it
-    if (e.exp_t == EXP_DICT_RLE || (e.exp_t == EXP_FFOR && e.var == kFforSRec)) return;
+    // buffers stored as one HZL chain per page (DICT_RLE, S_PATCH_REC, the RLE and B_PATCH_REC chain forms): no
+    // vector owns a slice of them
+    if (expr_rec(e)) return;
     const unsigned vec_c = (unsigned)pg[4] | ((unsigned)pg[5] << 8);
     if (vec_id >= vec_c) return;
     const unsigned v = (unsigned)vec_id;
@@ -171,7 +172,7 @@ void init_random_access(uint8_t* in, uint64_t vec_id, uint64_t* n_buf, uint64_t*
             switch (e.exp_t) {
             case EXP_UNCOMPRESSED: tail = 8192; break;
             case EXP_FSST: tail = 4096; break;
-            case EXP_RLE: tail = e.var == 1 ? 2048 : 128ull * pg[off_of(pg, 2) + ep_of(pg, e.arr_c, v, 2)]; break;
+            case EXP_RLE: tail = e.var == 1 || e.var == kRleRec ? 2048 : 128ull * pg[off_of(pg, 2) + ep_of(pg, e.arr_c, v, 2)]; break;
             default: tail = e.var == 1 && is_dict_exp(e.exp_t) ? 4096 : 128ull * pg[off_of(pg, b - 2) + ep_of(pg, e.arr_c, v, b - 2)];
             }
             end = s + tail;

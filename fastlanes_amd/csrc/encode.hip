@@ -58,18 +58,21 @@ __device__ __forceinline__ T wave_max(T x) {
     return x;
 }
 
-// One wave per vector: base = min, bw = bit length of max - min over the values FFOR will pack.
-template <typename T, int MODE>
+// One wave per vector: base = min, bw = bit length of max - min over the values FFOR will pack. SGN: min and max
+// of int64 columns compare as signed values (ffor_prm::ResolveAnalyzeFunc, src/cor/prm/ffor/ffor_analyze.cpp:145-160),
+// done here on the values with their sign bit flipped.
+template <typename T, int MODE, bool SGN>
 __global__ __launch_bounds__(256) void k_enc_analyze(const T* __restrict__ x, uint64_t n_vec, uint8_t* __restrict__ bw,
                                                      uint64_t* __restrict__ base) {
     using C              = std::conditional_t<sizeof(T) == 8, unsigned long long, unsigned>;  // shuffle type
+    constexpr C    bias  = SGN ? (C)((T)1 << (8 * sizeof(T) - 1)) : (C)0;
     const unsigned lane  = threadIdx.x % 64;
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
     for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; v < n_vec; v += waves) {
         const T* xv = x + v * 1024;
         C        mn = (C)(T)~(T)0, mx = 0;
         for (unsigned o = lane; o < 1024; o += 64) {
-            const C val = (C)enc_value<T, MODE>(xv, o);
+            const C val = (C)enc_value<T, MODE>(xv, o) ^ bias;
             mn          = val < mn ? val : mn;
             mx          = val > mx ? val : mx;
         }
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(256) void k_enc_analyze(const T* __restrict__ x, ui
         if (lane == 0) {
             const uint64_t d = (uint64_t)(T)(mx - mn);
             bw[v]            = (uint8_t)(d ? 64 - __clzll((long long)d) : 0);
-            base[v]          = (uint64_t)mn;
+            base[v]          = (uint64_t)(T)(mn ^ bias);
         }
     }
 }
@@ -244,12 +247,16 @@ __global__ __launch_bounds__(128) void k_enc_rle_pack(const T* __restrict__ x, c
 template <typename T>
 hipError_t launch_enc(int mode, const void* x, uint64_t n_vec, uint8_t* bw, uint64_t* base, hipStream_t s, bool analyze,
                       const uint64_t* pk_rel, const uint32_t* vpage, const EncPage* pages, uint32_t arr_c,
-                      uint8_t* out) {
+                      uint8_t* out, bool sgn) {
     const T* xv = static_cast<const T*>(x);
     if (analyze) {
         const unsigned blocks = (unsigned)std::min<uint64_t>((n_vec + 3) / 4, 65536);
-        if (mode == 1) hipLaunchKernelGGL((k_enc_analyze<T, 1>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base);
-        else if (mode == 2) hipLaunchKernelGGL((k_enc_analyze<T, 2>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base);
+#define A(M, S) hipLaunchKernelGGL((k_enc_analyze<T, M, S>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base)
+        if (mode == 1 && sgn) A(1, true);
+        else if (mode == 1) A(1, false);
+        else if (mode == 2 && sgn) A(2, true);
+        else if (mode == 2) A(2, false);
+#undef A
         else hipLaunchKernelGGL((k_enc_rle_analyze<T>), dim3(blocks), dim3(256), 0, s, xv, n_vec, bw, base);
     } else if (mode >= 3) {  // RLE: `base` carries the run-value offsets
 #define R(V) hipLaunchKernelGGL((k_enc_rle_pack<T, V>), dim3((unsigned)n_vec), dim3(128), 0, s, xv, bw, base, pk_rel, vpage, pages, arr_c, out)
@@ -268,12 +275,12 @@ hipError_t launch_enc(int mode, const void* x, uint64_t n_vec, uint8_t* bw, uint
 
 hipError_t launch_enc_w(unsigned W, int mode, const void* x, uint64_t n_vec, uint8_t* bw, uint64_t* base, hipStream_t s,
                         bool analyze, const uint64_t* pk_rel = nullptr, const uint32_t* vpage = nullptr,
-                        const EncPage* pages = nullptr, uint32_t arr_c = 0, uint8_t* out = nullptr) {
+                        const EncPage* pages = nullptr, uint32_t arr_c = 0, uint8_t* out = nullptr, bool sgn = false) {
     switch (W) {
-    case 8: return launch_enc<uint8_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out);
-    case 16: return launch_enc<uint16_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out);
-    case 32: return launch_enc<uint32_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out);
-    default: return launch_enc<uint64_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out);
+    case 8: return launch_enc<uint8_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out, sgn);
+    case 16: return launch_enc<uint16_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out, sgn);
+    case 32: return launch_enc<uint32_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out, sgn);
+    default: return launch_enc<uint64_t>(mode, x, n_vec, bw, base, s, analyze, pk_rel, vpage, pages, arr_c, out, sgn);
     }
 }
 
@@ -296,12 +303,13 @@ int encode_impl(const void* dev_values, uint64_t n, enum data_t type, uint8_t ex
     if (!out_size || !dev_out || (n && !dev_values)) return fail(FLS_E_ARG, "null argument");
     Expr e;
     if (type == STR || !expr_from_id((uint8_t)type, expr_id, e)) return fail(FLS_E_UNSUPPORTED, "unknown expression");
-    const int mode = e.exp_t == EXP_UNCOMPRESSED           ? 0
-                     : (e.exp_t == EXP_FFOR && e.var == 1) ? 1
-                     : e.exp_t == EXP_DELTA                ? 2
-                     : e.exp_t == EXP_RLE                  ? (e.var == 1 ? 3 : 4)
-                                                           : -1;
-    if (mode < 0) return fail(FLS_E_UNSUPPORTED, "the GPU encoder writes UNCOMPRESSED, FFOR, DELTA and RLE pages");
+    const int mode = e.exp_t == EXP_UNCOMPRESSED            ? 0
+                     : (e.exp_t == EXP_FFOR && e.var == 1)  ? 1
+                     : (e.exp_t == EXP_DELTA && e.var == 1) ? 2
+                     : (e.exp_t == EXP_RLE && e.var == 1)   ? 3
+                     : (e.exp_t == EXP_RLE && e.var == 2)   ? 4
+                                                            : -1;
+    if (mode < 0) return fail(FLS_E_UNSUPPORTED, "the GPU encoder writes UNCOMPRESSED, FFOR, DELTA and RLE pages (no patches, no chains)");
     if (type == DOUBLE && mode) return fail(FLS_E_UNSUPPORTED, "double columns support UNCOMPRESSED only");
     if (n % 1024) return fail(FLS_E_UNSUPPORTED, "value count must be a multiple of 1024");
     const unsigned    W = dtype_bits((uint8_t)type), SZ = W / 8, arr_c = e.arr_c;
@@ -339,7 +347,8 @@ int encode_impl(const void* dev_values, uint64_t n, enum data_t type, uint8_t ex
         if ((err = get(&d_bw, 0, 1, n_vec)) != hipSuccess || (err = get(&d_base, 1, 8, n_vec)) != hipSuccess)
             return release(hip_fail(err, "hipMallocAsync"));
         if (mode) {
-            if ((err = launch_enc_w(W, mode, dev_values, n_vec, d_bw, d_base, s, true)) != hipSuccess ||
+            if ((err = launch_enc_w(W, mode, dev_values, n_vec, d_bw, d_base, s, true, nullptr, nullptr, nullptr, 0,
+                                    nullptr, type == INT64)) != hipSuccess ||
                 (err = hipMemcpyAsync(bw.data(), d_bw, n_vec, hipMemcpyDeviceToHost, s)) != hipSuccess ||
                 (mode >= 3 && (err = hipMemcpyAsync(runs.data(), d_base, 8 * n_vec, hipMemcpyDeviceToHost, s)) != hipSuccess) ||
                 (err = hipStreamSynchronize(s)) != hipSuccess)

@@ -22,54 +22,108 @@ unsigned dtype_bits(uint8_t dt) {
     }
 }
 
+bool expr_from_page(uint8_t exp_t, uint8_t var, Expr& e) {
+    auto set = [&](uint8_t arr_c, bool dict) { return e = {exp_t, var, arr_c, dict}, true; };
+    switch (exp_t) {
+    case EXP_UNCOMPRESSED: return var == 1 && set(1, false);
+    case EXP_FFOR:
+        if (var == 1) return set(3, false);
+        return (var == kFforLL || var == kFforS || var == kFforB || var == kFforSRec) && set(5, false);
+    case EXP_DELTA:
+        if (var == 1) return set(4, false);
+        return var >= kDeltaLL && var <= kDeltaBRec && set(6, false);
+    case EXP_RLE:
+        if (var == 1 || var == kRleRec) return set(2, false);
+        return (var == 2 || var == kRleDeltaRec || var == kRleDeltaRec2) && set(5, false);
+    case EXP_DICT:
+    case EXP_DICT_FRQ:
+    case EXP_DICT_VAL:
+        if (var == 1) return set(1, true);
+        if (var == 2) return set(3, true);
+        if (var >= kDictFforLL && var <= kDictFforSRec) return set(5, true);
+        if (var == kDictDelta) return set(4, true);
+        return var >= kDictDeltaLL && var <= kDictDeltaBRec && set(6, true);
+    case EXP_DICT_RLE: return var == kDictRleRec && set(5, true);
+    case EXP_FSST: return var == 1 && set(2, true);
+    default: return false;
+    }
+}
+
+uint8_t expr_patch(const Expr& e) {
+    switch (e.exp_t) {
+    case EXP_FFOR: return e.var == kFforSRec ? kFforS : e.var == 1 ? 0 : e.var;
+    case EXP_DELTA: return e.var == kDeltaLL ? kFforLL : e.var == kDeltaS ? kFforS : e.var >= kDeltaB ? kFforB : 0;
+    case EXP_DICT:
+    case EXP_DICT_FRQ:
+    case EXP_DICT_VAL:
+        switch (e.var) {
+        case kDictFforLL:
+        case kDictDeltaLL: return kFforLL;
+        case kDictFforS:
+        case kDictFforSRec:
+        case kDictDeltaS: return kFforS;
+        case kDictFforB:
+        case kDictDeltaB:
+        case kDictDeltaBRec: return kFforB;
+        default: return 0;
+        }
+    default: return 0;
+    }
+}
+bool expr_rsum(const Expr& e) {
+    return e.exp_t == EXP_DELTA || ((e.exp_t == EXP_DICT || e.exp_t == EXP_DICT_FRQ || e.exp_t == EXP_DICT_VAL) &&
+                                    e.var >= kDictDelta);
+}
+unsigned expr_rec(const Expr& e) {
+    switch (e.exp_t) {
+    case EXP_FFOR: return e.var == kFforSRec ? 1u : 0u;
+    case EXP_DELTA: return e.var == kDeltaBRec ? 2u : 0u;
+    case EXP_RLE: return e.var == kRleRec || e.var == kRleDeltaRec ? 1u : e.var == kRleDeltaRec2 ? 3u : 0u;
+    case EXP_DICT:
+    case EXP_DICT_FRQ:
+    case EXP_DICT_VAL: return e.var == kDictFforSRec ? 1u : e.var == kDictDeltaBRec ? 2u : 0u;
+    case EXP_DICT_RLE: return 1u;
+    default: return 0u;
+    }
+}
+unsigned expr_rec_width(const Expr& e) { return is_dict_exp(e.exp_t) ? 32u : 64u; }
+bool     expr_ffor_shaped(const Expr& e) {
+    return e.exp_t == EXP_FFOR || e.exp_t == EXP_DELTA ||
+           ((e.exp_t == EXP_DICT || e.exp_t == EXP_DICT_FRQ || e.exp_t == EXP_DICT_VAL) && e.var != 1);
+}
+bool expr_fits_type(const Expr& e, uint8_t dtype) {
+    const unsigned W = dtype_bits(dtype);
+    if (dtype == DT_STR) return e.exp_t == EXP_FSST;
+    if (!W || e.exp_t == EXP_FSST) return false;
+    if (dtype == DT_DOUBLE) return e.exp_t == EXP_UNCOMPRESSED;
+    if (!e.dict && expr_patch(e) && W < 32) return false;            // patched values
+    if (!e.dict && expr_rec(e) && W != 64) return false;             // THzlCmp<uint64_t> chains
+    return true;
+}
+
 bool expr_from_id(uint8_t dtype, uint8_t id, Expr& e) {
     if (dtype == DT_STR) {
         if (id == 13) return e = {EXP_FSST, 1, 2, true}, true;
         return false;
     }
-    switch (id) {
-    case 0: e = {EXP_UNCOMPRESSED, 1, 1, false}; return true;
-    case 1: e = {EXP_FFOR, 1, 3, false}; return true;
-    case 2: e = {EXP_FFOR, kFforS, 5, false}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE;   // S_PATCH
-    case 3: e = {EXP_FFOR, kFforSRec, 5, false}; return dtype_bits(dtype) == 64 && dtype != DT_DOUBLE;  // S_PATCH_REC
-    case 4: e = {EXP_FFOR, kFforB, 5, false}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE;   // B_PATCH
-    case 5: e = {EXP_FFOR, kFforLL, 5, false}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE;  // LL_PATCH
-    case 6: e = {EXP_RLE, 2, 5, false}; return true;
-    case 7: e = {EXP_RLE, 1, 2, false}; return true;
-    case 11: e = {EXP_DELTA, 1, 4, false}; return true;
-    case 16: e = {EXP_DICT, 1, 1, true}; return true;
-    case 17: e = {EXP_DICT, 2, 3, true}; return true;
-    case 22: e = {EXP_DICT, 7, 4, true}; return true;
-    case 27: e = {EXP_DICT_RLE, kDictRleRec, 5, true}; return dtype != DT_DOUBLE;
-    case 28: e = {EXP_DICT_FRQ, 1, 1, true}; return true;
-    case 29: e = {EXP_DICT_FRQ, 2, 3, true}; return true;
-    default: return false;
-    }
-}
-
-bool expr_from_page(uint8_t exp_t, uint8_t var, Expr& e) {
-    switch (exp_t) {
-    case EXP_UNCOMPRESSED: e = {exp_t, var, 1, false}; return var == 1;
-    case EXP_FFOR:
-        if (var == 1) return e = {exp_t, var, 3, false}, true;
-        if (var == kFforLL || var == kFforS || var == kFforB || var == kFforSRec) return e = {exp_t, var, 5, false}, true;
-        return false;
-    case EXP_DELTA: e = {exp_t, var, 4, false}; return var == 1;
-    case EXP_RLE:
-        if (var == 1) return e = {exp_t, var, 2, false}, true;
-        if (var == 2) return e = {exp_t, var, 5, false}, true;
-        return false;
-    case EXP_DICT:
-    case EXP_DICT_FRQ:
-    case EXP_DICT_VAL:
-        if (var == 1) return e = {exp_t, var, 1, true}, true;
-        if (var == 2) return e = {exp_t, var, 3, true}, true;
-        if (var == 7) return e = {exp_t, var, 4, true}, true;
-        return false;
-    case EXP_DICT_RLE: e = {exp_t, var, 5, true}; return var == kDictRleRec;
-    case EXP_FSST: e = {exp_t, var, 2, true}; return var == 1;
-    default: return false;
-    }
+    // (ExpT, var) of the integer pool, src/cor/exp_pool.cpp:26-64
+    static const uint8_t pool[39][2] = {
+        {EXP_UNCOMPRESSED, 1},        {EXP_FFOR, 1},                 {EXP_FFOR, kFforS},
+        {EXP_FFOR, kFforSRec},        {EXP_FFOR, kFforB},            {EXP_FFOR, kFforLL},
+        {EXP_RLE, 2},                 {EXP_RLE, 1},                  {EXP_RLE, kRleRec},
+        {EXP_RLE, kRleDeltaRec},      {EXP_RLE, kRleDeltaRec2},      {EXP_DELTA, 1},
+        {EXP_DELTA, kDeltaS},         {EXP_DELTA, kDeltaB},          {EXP_DELTA, kDeltaLL},
+        {EXP_DELTA, kDeltaBRec},      {EXP_DICT, 1},                 {EXP_DICT, 2},
+        {EXP_DICT, kDictFforB},       {EXP_DICT, kDictFforS},        {EXP_DICT, kDictFforLL},
+        {EXP_DICT, kDictFforSRec},    {EXP_DICT, kDictDelta},        {EXP_DICT, kDictDeltaS},
+        {EXP_DICT, kDictDeltaLL},     {EXP_DICT, kDictDeltaB},       {EXP_DICT, kDictDeltaBRec},
+        {EXP_DICT_RLE, kDictRleRec},  {EXP_DICT_FRQ, 1},             {EXP_DICT_FRQ, 2},
+        {EXP_DICT_FRQ, kDictFforB},   {EXP_DICT_FRQ, kDictFforS},    {EXP_DICT_FRQ, kDictFforLL},
+        {EXP_DICT_FRQ, kDictFforSRec}, {EXP_DICT_FRQ, kDictDelta},   {EXP_DICT_FRQ, kDictDeltaS},
+        {EXP_DICT_FRQ, kDictDeltaLL}, {EXP_DICT_FRQ, kDictDeltaB},   {EXP_DICT_FRQ, kDictDeltaBRec},
+    };
+    if (id >= 39 || !expr_from_page(pool[id][0], pool[id][1], e)) return false;
+    return expr_fits_type(e, dtype);
 }
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -158,6 +212,7 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
     if (dtype == DT_STR && e.exp_t != EXP_FSST) return "string column with integer expression";
     if (dtype != DT_STR && e.exp_t == EXP_FSST) return "integer column with string expression";
     if (dtype == DT_DOUBLE && e.exp_t != EXP_UNCOMPRESSED) return "double columns support UNCOMPRESSED only";
+    if (!expr_fits_type(e, dtype)) return "expression not defined for this data type";
     const unsigned arr_c = e.arr_c;
     const uint64_t meta  = 8 + 4ull * arr_c + 4ull * arr_c * (pe.vec_c ? pe.vec_c - 1 : 0);
     if (!r.has(off, meta)) return "page metadata truncated";
@@ -171,22 +226,25 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
     // per-vector extents
     const unsigned sz = W / 8;
     auto           ep = [&](unsigned v, unsigned b) { return ep_of(pg, arr_c, v, b); };
-    // DICT_RLE / S_PATCH_REC: buffer 0 is an HZL chain (u32 run codes / u64 exception values) whose entry points
-    // address the decoded values
-    const bool     srec = e.exp_t == EXP_FFOR && e.var == kFforSRec;
-    const bool     drle = e.exp_t == EXP_DICT_RLE || srec;
-    const unsigned rdw  = srec ? 64 : 32;
-    if (srec && W != 64) return "S_PATCH_REC needs 64-bit values";
-    if (drle) {
+    // buffers stored as one HZL chain per page: their entry points address the decoded bytes
+    const unsigned rec = expr_rec(e), rdw = expr_rec_width(e);
+    pe.rec = (uint8_t)rec, pe.rec_width = (uint8_t)rdw;
+    uint64_t rec_bytes[2] = {0, 0};
+    for (unsigned b = 0; b < 2; ++b) {
+        if (!(rec >> b & 1)) continue;
+        auto&    chunks = b ? pe.rec1_hzl : pe.run_hzl;
         uint64_t used = 0;
-        auto     err  = parse_hzl(r, off + boff[0], rdw, ~0u, pe.run_hzl, used);
-        if (!err.empty()) return "run codes: " + err;
-        if (used != boff[1] - boff[0]) return "run codes do not fill their buffer";
-        for (const auto& c : pe.run_hzl) pe.run_codes += c.n;
-        if (rdw / 8 * pe.run_codes > 0xffffffffull) return "recursive buffer exceeds 32-bit entry points";
+        auto     err  = parse_hzl(r, off + boff[b], rdw, ~0u, chunks, used);
+        if (!err.empty()) return "recursive buffer " + std::to_string(b) + ": " + err;
+        if (used != boff[b + 1] - boff[b]) return "recursive buffer " + std::to_string(b) + " does not fill its slot";
+        uint64_t n = 0;
+        for (const auto& c : chunks) n += c.n;
+        (b ? pe.rec1_n : pe.run_codes) = n;
+        rec_bytes[b]                   = rdw / 8 * n;
+        if (rec_bytes[b] > 0xffffffffull) return "recursive buffer exceeds 32-bit entry points";
     }
     auto cap = [&](unsigned b) {
-        return drle && b == 0 ? rdw / 8 * pe.run_codes : (b + 1 < arr_c ? boff[b + 1] : avail) - boff[b];
+        return b < 2 && (rec >> b & 1) ? rec_bytes[b] : (b + 1 < arr_c ? boff[b + 1] : avail) - boff[b];
     };
     auto           need = [&](unsigned v, unsigned b, uint64_t n) -> bool {
         const uint64_t s = ep(v, b);
@@ -194,104 +252,83 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
         return s <= lim && lim <= cap(b) && n <= lim - s;
     };
     auto bwc = [&](unsigned v, unsigned b) { return (unsigned)pg[boff[b] + ep(v, b)]; };
+    // FFOR-shaped pages: [lane bases][exceptions][positions][bw][base][packed] over CW-bit values
+    const bool     ffs   = expr_ffor_shaped(e);
+    const uint8_t  patch = expr_patch(e);
+    const unsigned CW    = e.dict ? 32 : W, csz = CW / 8;
+    const unsigned o     = expr_rsum(e) ? 1 : 0;  // first buffer after the lane bases
+    const unsigned q     = o + (patch ? 2 : 0);   // bw buffer
     uint64_t end = 0;  // bytes of the last buffer used by the last vector
     for (unsigned v = 0; v < pe.vec_c; ++v) {
         bool ok = true;
-        switch (e.exp_t) {
-        case EXP_UNCOMPRESSED: ok = need(v, 0, 1024ull * sz); break;
-        case EXP_FFOR:
-            if (e.var != 1) {  // patched: [exc values][exc positions | bitmap | count+first][bw][base][packed]
-                if (W < 32) return "patched FFOR needs 32- or 64-bit values";
-                ok = need(v, 2, 1) && need(v, 3, sz);
-                if (!ok) break;
-                const unsigned bw = bwc(v, 2);
-                ok = bw <= W && need(v, 4, 128ull * bw);
-                if (!ok) break;
-                const uint8_t* b1  = pg + boff[1] + ep(v, 1);
+        if (ffs) {
+            if (o) ok = need(v, 0, 128);
+            ok = ok && need(v, q, 1) && need(v, q + 1, csz);
+            if (ok) {
+                const unsigned bw = bwc(v, q);
+                ok = bw <= CW && need(v, q + 2, 128ull * bw);
+            }
+            if (ok && patch) {  // [exception values][positions | bitmap | count + first position]
+                const uint8_t* b1  = pg + boff[o + 1] + ep(v, o + 1);
                 uint32_t       exc = 0;
-                const uint8_t  pv  = srec ? kFforS : e.var;
-                if (pv == kFforB) {
-                    ok = need(v, 1, 128);
-                    if (!ok) break;
-                    for (unsigned k = 0; k < 128; ++k) exc += (uint32_t)__builtin_popcount(b1[k]);
+                if (patch == kFforB) {
+                    ok = need(v, o + 1, 128);
+                    for (unsigned k = 0; ok && k < 128; ++k) exc += (uint32_t)__builtin_popcount(b1[k]);
                 } else {
-                    ok = need(v, 1, pv == kFforS ? 2 : 4);
-                    if (!ok) break;
-                    exc = (uint32_t)b1[0] | ((uint32_t)b1[1] << 8);
-                    if (pv == kFforS) {
-                        ok = exc <= 1024 && need(v, 1, 2 + 2ull * exc);
-                        for (uint32_t k = 0; ok && k < exc; ++k) ok = ((uint32_t)b1[2 + 2 * k] | ((uint32_t)b1[3 + 2 * k] << 8)) < 1024;
-                    } else {
-                        ok = exc <= 1024 && (exc == 0 || ((uint32_t)b1[2] | ((uint32_t)b1[3] << 8)) < 1024);
+                    ok = need(v, o + 1, patch == kFforS ? 2 : 4);
+                    if (ok) {
+                        exc = (uint32_t)b1[0] | ((uint32_t)b1[1] << 8);
+                        if (patch == kFforS) {
+                            ok = exc <= 1024 && need(v, o + 1, 2 + 2ull * exc);
+                            for (uint32_t k = 0; ok && k < exc; ++k)
+                                ok = ((uint32_t)b1[2 + 2 * k] | ((uint32_t)b1[3 + 2 * k] << 8)) < 1024;
+                        } else {
+                            ok = exc <= 1024 && (exc == 0 || ((uint32_t)b1[2] | ((uint32_t)b1[3] << 8)) < 1024);
+                        }
                     }
-                    if (!ok) break;
                 }
-                ok = need(v, 0, (uint64_t)exc * sz);
+                ok = ok && need(v, o, (uint64_t)exc * csz);
+            }
+        } else {
+            switch (e.exp_t) {
+            case EXP_UNCOMPRESSED: ok = need(v, 0, 1024ull * sz); break;
+            case EXP_RLE:
+            case EXP_DICT_RLE: {  // [run values | run codes][positions: u16 plain, or RSUM + FFOR]
+                const unsigned rsz = e.exp_t == EXP_DICT_RLE ? 4 : sz;
+                ok = ep(v, 0) % rsz == 0 && need(v, 0, rsz);  // at least one run
+                if (e.var == 1 || e.var == kRleRec) ok = ok && need(v, 1, 2048);
+                else {
+                    ok = ok && need(v, 1, 128) && need(v, 2, 1) && need(v, 3, 2);
+                    if (!ok) break;
+                    const unsigned bw = bwc(v, 2);
+                    ok = bw <= 16 && need(v, 4, 128ull * bw);
+                }
                 break;
             }
-            [[fallthrough]];
-        case EXP_DELTA: {
-            const unsigned o = e.exp_t == EXP_DELTA;
-            if (o) ok &= need(v, 0, 128);
-            ok = ok && need(v, o, 1) && need(v, o + 1, sz);
-            if (!ok) break;
-            const unsigned bw = bwc(v, o);
-            ok = bw <= W && need(v, o + 2, 128ull * bw);
-            break;
-        }
-        case EXP_RLE: {
-            ok = need(v, 0, sz);  // at least one run value
-            if (e.var == 1) ok = ok && need(v, 1, 2048);
-            else {
-                ok = ok && need(v, 1, 128) && need(v, 2, 1) && need(v, 3, 2);
-                if (!ok) break;
-                const unsigned bw = bwc(v, 2);
-                ok = bw <= 16 && need(v, 4, 128ull * bw);
+            case EXP_FSST: ok = need(v, 1, 4096) && need(v, 0, 0); break;
+            default: ok = need(v, 0, 4096); break;  // dictionary codes, UNCOMPRESSED
             }
-            break;
-        }
-        case EXP_FSST: ok = need(v, 1, 4096) && need(v, 0, 0); break;
-        case EXP_DICT_RLE: {  // RLE DELTA_NO_PATCH positions over u32 run codes
-            ok = ep(v, 0) % 4 == 0 && need(v, 0, 4) && need(v, 1, 128) && need(v, 2, 1) && need(v, 3, 2);
-            if (!ok) break;
-            const unsigned bw = bwc(v, 2);
-            ok = bw <= 16 && need(v, 4, 128ull * bw);
-            break;
-        }
-        default: {  // dictionary codes
-            if (e.var == 1) ok = need(v, 0, 4096);
-            else {
-                const unsigned o = e.var == 7;
-                if (o) ok &= need(v, 0, 128);
-                ok = ok && need(v, o, 1) && need(v, o + 1, 4);
-                if (!ok) break;
-                const unsigned bw = bwc(v, o);
-                ok = bw <= 32 && need(v, o + 2, 128ull * bw);
-            }
-        }
         }
         if (!ok) return "vector " + std::to_string(v) + " exceeds its page buffers";
     }
     for (unsigned b = 0; b < arr_c; ++b) pe.boff[b] = (uint32_t)boff[b];
     {  // fixed per-vector sizes of the metadata buffers (0 = variable size)
         unsigned fsz[8] = {};
-        switch (e.exp_t) {
-        case EXP_UNCOMPRESSED: fsz[0] = 1024 * sz; break;
-        case EXP_FFOR:
-            if (e.var == 1) fsz[0] = 1, fsz[1] = sz;
-            else fsz[2] = 1, fsz[3] = sz, fsz[1] = e.var == kFforB ? 128 : e.var == kFforLL ? 4 : 0;  // S, S_REC: 0
-            break;
-        case EXP_DELTA: fsz[0] = 128, fsz[1] = 1, fsz[2] = sz; break;
-        case EXP_RLE:
-            if (e.var == 1) fsz[1] = 2048;
-            else fsz[1] = 128, fsz[2] = 1, fsz[3] = 2;
-            break;
-        case EXP_FSST: fsz[1] = 4096; break;
-        case EXP_DICT_RLE: fsz[1] = 128, fsz[2] = 1, fsz[3] = 2; break;
-        default:
-            if (e.var == 1) fsz[0] = 4096;
-            else if (e.var == 2) fsz[0] = 1, fsz[1] = 4;
-            else fsz[0] = 128, fsz[1] = 1, fsz[2] = 4;
+        if (ffs) {
+            if (o) fsz[0] = 128;
+            fsz[q] = 1, fsz[q + 1] = csz;
+            if (patch) fsz[o + 1] = patch == kFforB ? 128 : patch == kFforLL ? 4 : 0;  // S: variable
+        } else {
+            switch (e.exp_t) {
+            case EXP_UNCOMPRESSED: fsz[0] = 1024 * sz; break;
+            case EXP_RLE:
+            case EXP_DICT_RLE:
+                if (e.var == 1 || e.var == kRleRec) fsz[1] = 2048;
+                else fsz[1] = 128, fsz[2] = 1, fsz[3] = 2;
+                break;
+            case EXP_FSST: fsz[1] = 4096; break;
+            default: fsz[0] = 4096;
+            }
         }
         bool fixed = true;
         for (unsigned v = 0; v < pe.vec_c && fixed; ++v)
@@ -303,14 +340,13 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
     {
         const unsigned v = pe.vec_c - 1, b = arr_c - 1;
         uint64_t       tail = 0;
-        switch (e.exp_t) {
-        case EXP_UNCOMPRESSED: tail = 1024ull * sz; break;
-        case EXP_FFOR:
-        case EXP_DELTA: tail = 128ull * bwc(v, b - 2); break;  // patched FFOR: b - 2 is its bw buffer too
-        case EXP_RLE: tail = e.var == 1 ? 2048 : 128ull * bwc(v, b - 2); break;
-        case EXP_FSST: tail = 4096; break;
-        default: tail = e.var == 1 ? 4096 : 128ull * bwc(v, b - 2); break;
-        }
+        if (ffs) tail = 128ull * bwc(v, b - 2);
+        else switch (e.exp_t) {
+            case EXP_UNCOMPRESSED: tail = 1024ull * sz; break;
+            case EXP_RLE:
+            case EXP_DICT_RLE: tail = e.var == 1 || e.var == kRleRec ? 2048 : 128ull * bwc(v, b - 2); break;
+            default: tail = 4096; break;  // FSST offsets, dictionary codes
+            }
         end = boff[b] + ep(v, b) + tail;
     }
     // FSST: compressed byte extents must cover the offsets' last value (checked in parse_column via host walk).
@@ -424,11 +460,18 @@ void unrsum(const T* in, T* out) {  // generated unrsum: first step 0, then lane
         for (unsigned s = 1; s < W; ++s) out[lane_pos<T>(i, s)] = (T)(in[lane_pos<T>(i, s)] - in[lane_pos<T>(i, s - 1)]);
     }
 }
+// ANALYZE (src/cor/prm/ffor/ffor_analyze.cpp:78-102): base = min, bw = exact bit length of max - min. `sgn`: min and
+// max as signed values, as ffor_prm::ResolveAnalyzeFunc resolves it for int64 columns (ffor_analyze.cpp:145-160);
+// every other type compares unsigned.
 template <typename T>
-uint8_t analyze(const T* in, T& base) {  // base = min, bw = exact bit length of max - min
-    T mn = in[0], mx = in[0];
-    for (unsigned i = 1; i < 1024; ++i) mn = in[i] < mn ? in[i] : mn, mx = in[i] > mx ? in[i] : mx;
-    base             = mn;
+uint8_t analyze(const T* in, T& base, bool sgn = false) {
+    const T bias = sgn ? (T)((T)1 << (8 * sizeof(T) - 1)) : (T)0;
+    T       mn = (T)(in[0] ^ bias), mx = mn;
+    for (unsigned i = 1; i < 1024; ++i) {
+        const T x = (T)(in[i] ^ bias);
+        mn = x < mn ? x : mn, mx = x > mx ? x : mx;
+    }
+    base             = (T)(mn ^ bias);
     const uint64_t d = (uint64_t)(T)(mx - mn);
     return (uint8_t)(d ? 64 - __builtin_clzll(d) : 0);
 }
@@ -522,11 +565,13 @@ struct Buf {
 };
 
 struct PageW {
-    unsigned              arr_c = 0, vec_c = 0;
+    unsigned              arr_c = 0, vec_c = 0, rec_c = 0;  // rec_c: buffers stored as HZL chains
     Buf                   bufs[6], eps;
     uint32_t              ep[6] = {0};
+    // Page::m_ttl_sz: header, offsets, entry points, the chains' record headers and the uncompressed buffers
+    // (src/cor/lyt/page/page.cpp:44-56, :87-104)
     uint64_t              size() const {
-        uint64_t s = 8 + 4ull * arr_c + eps.b.size();
+        uint64_t s = 8 + 4ull * arr_c + eps.b.size() + 12ull * rec_c;
         for (unsigned i = 0; i < arr_c; ++i) s += bufs[i].b.size();
         return s;
     }
@@ -568,8 +613,78 @@ void hzl_encode(const std::vector<D>& vals, Buf& out) {  // no exceptions; a ful
     out.put(vals.data() + pos, rest);
 }
 
+using Parts = std::vector<std::pair<const uint8_t*, uint64_t>>;
+
+// Byte buffers of one vector's parts: exceptions, positions, packed values, lane bases, RLE packed positions.
+struct VecBufs {
+    std::vector<uint8_t> exc = std::vector<uint8_t>(8192), pos = std::vector<uint8_t>(8192),
+                         packed = std::vector<uint8_t>(8192), lbase = std::vector<uint8_t>(8192),
+                         rle = std::vector<uint8_t>(8192);
+    uint8_t  bw   = 0;
+    uint64_t base = 0;
+};
+
+// One vector of an FFOR-shaped page from its transposed values t (modified): [lane bases] when the expression has
+// RSUM (src/cor/prm/rsum/rsum_compress.cpp:10-22: the first 128 bytes, then unrsum in place), [exceptions]
+// [positions] for a patch (PA_ANALYZE + S/B/LL_PATCH, s_patch_compress.cpp, b_patch_compress.cpp,
+// ll_patch_compress.cpp), then [bw][base][packed] (ANALYZE / the patch plan + FFOR).
+template <typename S>
+void shaped_vector(const Expr& e, S* t, Parts& parts, VecBufs& vb, bool sgn) {
+    unsigned o = 0;
+    if (expr_rsum(e)) {
+        S d[1024];
+        std::memcpy(vb.lbase.data(), t, 128);
+        parts[0] = {vb.lbase.data(), 128};
+        unrsum<S>(t, d);
+        std::copy(d, d + 1024, t);
+        o = 1;
+    }
+    const uint8_t pv = expr_patch(e);
+    if (!pv) {
+        S base;
+        vb.bw   = analyze<S>(t, base, sgn);
+        vb.base = 0;
+        std::memcpy(&vb.base, &base, sizeof(S));
+        ffor<S>(t, vb.packed.data(), vb.bw, base);
+    } else if constexpr (sizeof(S) >= 4) {
+        const PatchPlan<S> pp = plan_patch<S>(t, pv);
+        const uint16_t     n  = (uint16_t)pp.pos.size();
+        for (uint16_t k = 0; k < n; ++k) std::memcpy(vb.exc.data() + k * sizeof(S), &t[pp.pos[k]], sizeof(S));
+        parts[o] = {vb.exc.data(), (uint64_t)n * sizeof(S)};
+        if (pv == kFforB) {
+            std::memset(vb.pos.data(), 0, 128);
+            for (uint16_t p : pp.pos) vb.pos[p >> 3] |= (uint8_t)(1u << (p & 7));
+            parts[o + 1] = {vb.pos.data(), 128};
+        } else if (pv == kFforS) {
+            std::memcpy(vb.pos.data(), &n, 2);
+            for (uint16_t k = 0; k < n; ++k) std::memcpy(vb.pos.data() + 2 + 2 * k, &pp.pos[k], 2);
+            parts[o + 1] = {vb.pos.data(), 2ull + 2ull * n};
+        } else {
+            const uint16_t first = n ? pp.pos[0] : 0;
+            std::memcpy(vb.pos.data(), &n, 2), std::memcpy(vb.pos.data() + 2, &first, 2);
+            parts[o + 1] = {vb.pos.data(), 4};
+        }
+        // S and B leave the exceptions in place (FFOR keeps their low bw bits of value - base); LL stores in each
+        // exception's slot the distance to the next one, minus one, plus base (the last slot's link is never
+        // followed: it gets distance 1)
+        if (pv == kFforLL)
+            for (uint16_t k = 0; k < n; ++k) {
+                const uint64_t gap = k + 1 < n ? (uint64_t)(pp.pos[k + 1] - pp.pos[k]) : 1;
+                t[pp.pos[k]]       = (S)(pp.base + (S)(gap - 1));
+            }
+        vb.bw   = (uint8_t)pp.bw;
+        vb.base = 0;
+        std::memcpy(&vb.base, &pp.base, sizeof(S));
+        ffor<S>(t, vb.packed.data(), pp.bw, pp.base);
+        o += 2;
+    }
+    parts[o]     = {&vb.bw, 1};
+    parts[o + 1] = {reinterpret_cast<const uint8_t*>(&vb.base), sizeof(S)};
+    parts[o + 2] = {vb.packed.data(), 128ull * vb.bw};
+}
+
 template <typename T>
-int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* meta) {
+int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* meta, bool sgn) {
     using U = T;
     const uint64_t        n_vec = n / 1024;
     std::vector<uint32_t> codes;
@@ -607,134 +722,63 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
     };
     std::vector<VecOut> vo(n_vec);
     auto encode_range = [&](uint64_t v0, uint64_t v1) {
-        std::vector<uint8_t> b0(8192), b1(8192), b2(8192), b3(8192), b4(8192);
-        T                    t[1024], d[1024];
-        uint32_t             ct[1024], cd[1024];
-        uint8_t              bwb;
-        uint64_t             baseb;
+        VecBufs  vb;
+        T        t[1024];
+        uint32_t ct[1024];
         for (uint64_t v = v0; v < v1; ++v) {
             const T* x = in + v * 1024;
-            std::vector<std::pair<const uint8_t*, uint64_t>> parts(e.arr_c);
-            auto ffor_parts = [&](auto* src, unsigned o, uint8_t* packed) {
-                using S = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
-                S             base;
-                const uint8_t bw = analyze<S>(src, base);
-                bwb              = bw;
-                baseb            = 0;
-                std::memcpy(&baseb, &base, sizeof(S));
-                ffor<S>(src, packed, bw, base);
-                parts[o]     = {&bwb, 1};
-                parts[o + 1] = {reinterpret_cast<const uint8_t*>(&baseb), sizeof(S)};
-                parts[o + 2] = {packed, 128ull * bw};
-            };
+            Parts    parts(e.arr_c);
             switch (e.exp_t) {
             case EXP_UNCOMPRESSED:
-                transpose<T>(x, reinterpret_cast<T*>(b0.data()));
-                parts[0] = {b0.data(), 1024ull * sizeof(T)};
+                transpose<T>(x, reinterpret_cast<T*>(vb.exc.data()));
+                parts[0] = {vb.exc.data(), 1024ull * sizeof(T)};
                 break;
             case EXP_FFOR:
-                transpose<T>(x, t);
-                if (e.var == 1) {
-                    ffor_parts(t, 0, b2.data());
-                } else if constexpr (sizeof(T) >= 4) {
-                    const uint8_t      pv = e.var == kFforSRec ? kFforS : e.var;  // S_PATCH_REC vectors are S_PATCH
-                    const PatchPlan<T> pp = plan_patch<T>(t, pv);
-                    const uint16_t     n  = (uint16_t)pp.pos.size();
-                    for (uint16_t k = 0; k < n; ++k) std::memcpy(b0.data() + k * sizeof(T), &t[pp.pos[k]], sizeof(T));
-                    parts[0] = {b0.data(), (uint64_t)n * sizeof(T)};
-                    if (pv == kFforB) {
-                        std::memset(b1.data(), 0, 128);
-                        for (uint16_t p : pp.pos) b1[p >> 3] |= (uint8_t)(1u << (p & 7));
-                        parts[1] = {b1.data(), 128};
-                    } else if (pv == kFforS) {
-                        std::memcpy(b1.data(), &n, 2);
-                        for (uint16_t k = 0; k < n; ++k) std::memcpy(b1.data() + 2 + 2 * k, &pp.pos[k], 2);
-                        parts[1] = {b1.data(), 2ull + 2ull * n};
-                    } else {
-                        const uint16_t first = n ? pp.pos[0] : 0;
-                        std::memcpy(b1.data(), &n, 2), std::memcpy(b1.data() + 2, &first, 2);
-                        parts[1] = {b1.data(), 4};
-                    }
-                    // S and B leave the exceptions in place (FFOR keeps their low bw bits of value - base); LL
-                    // stores in each exception's slot the distance to the next one, minus one, plus base (the last
-                    // slot's link is never followed: it gets distance 1)
-                    if (pv == kFforLL)
-                        for (uint16_t k = 0; k < n; ++k) {
-                            const uint64_t gap = k + 1 < n ? (uint64_t)(pp.pos[k + 1] - pp.pos[k]) : 1;
-                            t[pp.pos[k]]       = (T)(pp.base + (T)(gap - 1));
-                        }
-                    bwb   = (uint8_t)pp.bw;
-                    baseb = 0;
-                    std::memcpy(&baseb, &pp.base, sizeof(T));
-                    ffor<T>(t, b2.data(), pp.bw, pp.base);
-                    parts[2] = {&bwb, 1};
-                    parts[3] = {reinterpret_cast<const uint8_t*>(&baseb), sizeof(T)};
-                    parts[4] = {b2.data(), 128ull * pp.bw};
-                }
-                break;
             case EXP_DELTA:
                 transpose<T>(x, t);
-                std::memcpy(b0.data(), t, 128);
-                parts[0] = {b0.data(), 128};
-                unrsum<T>(t, d);
-                ffor_parts(d, 1, b3.data());
+                shaped_vector<T>(e, t, parts, vb, sgn);
                 break;
-            case EXP_RLE: {
-                uint16_t pos[1024], posT[1024], dl[1024];
-                uint64_t r = 0;
-                for (unsigned i = 0; i < 1024; ++i) {
-                    if (i == 0 || x[i] != x[i - 1]) {
-                        if (i) r++;
-                        std::memcpy(b0.data() + r * sizeof(T), &x[i], sizeof(T));
-                    }
-                    pos[i] = (uint16_t)r;
-                }
-                parts[0] = {b0.data(), (r + 1) * sizeof(T)};
-                transpose<uint16_t>(pos, posT);
-                if (e.var == 1) {
-                    std::memcpy(b1.data(), posT, 2048);
-                    parts[1] = {b1.data(), 2048};
-                } else {
-                    std::memcpy(b1.data(), posT, 128);
-                    parts[1] = {b1.data(), 128};
-                    unrsum<uint16_t>(posT, dl);
-                    ffor_parts(dl, 2, b4.data());
-                }
-                break;
-            }
-            case EXP_DICT_RLE: {  // Vec::Init split of the codes (src/cor/lyt/vec.cpp:72-96), then RLE DELTA_NO_PATCH
-                const uint32_t* c = codes.data() + v * 1024;
+            case EXP_RLE:
+            case EXP_DICT_RLE: {  // DICT_RLE: Vec::Init's split of the codes (src/cor/lyt/vec.cpp:72-96) first
+                const uint32_t* c   = e.exp_t == EXP_DICT_RLE ? codes.data() + v * 1024 : nullptr;
+                const unsigned  rsz = c ? 4 : sizeof(T);
                 uint16_t        pos[1024], posT[1024], dl[1024];
                 uint64_t        r = 0;
                 for (unsigned i = 0; i < 1024; ++i) {
-                    if (i == 0 || c[i] != c[i - 1]) {
+                    const bool brk = i && (c ? c[i] != c[i - 1] : x[i] != x[i - 1]);
+                    if (i == 0 || brk) {
                         if (i) r++;
-                        std::memcpy(b0.data() + r * 4, &c[i], 4);
+                        std::memcpy(vb.exc.data() + r * rsz, c ? (const void*)&c[i] : (const void*)&x[i], rsz);
                     }
                     pos[i] = (uint16_t)r;
                 }
-                parts[0] = {b0.data(), (r + 1) * 4};
+                parts[0] = {vb.exc.data(), (r + 1) * rsz};
                 transpose<uint16_t>(pos, posT);
-                std::memcpy(b1.data(), posT, 128);
-                parts[1] = {b1.data(), 128};
-                unrsum<uint16_t>(posT, dl);
-                ffor_parts(dl, 2, b4.data());
+                if (e.var == 1 || e.var == kRleRec) {
+                    std::memcpy(vb.pos.data(), posT, 2048);
+                    parts[1] = {vb.pos.data(), 2048};
+                } else {
+                    std::memcpy(vb.pos.data(), posT, 128);
+                    parts[1] = {vb.pos.data(), 128};
+                    unrsum<uint16_t>(posT, dl);
+                    uint16_t base;
+                    vb.bw   = analyze<uint16_t>(dl, base);
+                    vb.base = base;
+                    ffor<uint16_t>(dl, vb.rle.data(), vb.bw, base);
+                    parts[2] = {&vb.bw, 1};
+                    parts[3] = {reinterpret_cast<const uint8_t*>(&vb.base), 2};
+                    parts[4] = {vb.rle.data(), 128ull * vb.bw};
+                }
                 break;
             }
-            default: {
+            default:  // dictionary codes (u32)
                 transpose<uint32_t>(codes.data() + v * 1024, ct);
                 if (e.var == 1) {
-                    std::memcpy(b0.data(), ct, 4096);
-                    parts[0] = {b0.data(), 4096};
-                } else if (e.var == 2) {
-                    ffor_parts(ct, 0, b2.data());
+                    std::memcpy(vb.exc.data(), ct, 4096);
+                    parts[0] = {vb.exc.data(), 4096};
                 } else {
-                    std::memcpy(b0.data(), ct, 128);
-                    parts[0] = {b0.data(), 128};
-                    unrsum<uint32_t>(ct, cd);
-                    ffor_parts(cd, 1, b3.data());
+                    shaped_vector<uint32_t>(e, ct, parts, vb, false);
                 }
-            }
             }
             VecOut& o = vo[v];
             for (auto& pp : parts) o.bytes.insert(o.bytes.end(), pp.first, pp.first + pp.second), o.sizes.push_back(pp.second);
@@ -756,6 +800,7 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
     for (;;) {
         PageW pg;
         pg.arr_c = e.arr_c;
+        pg.rec_c = (unsigned)__builtin_popcount(expr_rec(e));
         const uint64_t stop = std::min(n_vec, v + per_pg);
         for (; v < stop; ++v) {
             std::vector<std::pair<const uint8_t*, uint64_t>> parts(e.arr_c);
@@ -778,19 +823,21 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
                 hzl_encode<uint64_t>(dvals, out);
             }
         }
-        if (e.exp_t == EXP_DICT_RLE && pg.vec_c) {  // the page's run codes as one HZL chain (THzlCmp<idx_t>, hzl_cmp.cpp:43-176)
-            std::vector<uint32_t> rc(pg.bufs[0].b.size() / 4);
-            if (!rc.empty()) std::memcpy(rc.data(), pg.bufs[0].b.data(), rc.size() * 4);
+        // buffers stored as one HZL chain per page (THzlCmp::compress_buf + finalize_buf, hzl_cmp.cpp:43-176)
+        const unsigned rec = expr_rec(e);
+        for (unsigned b = 0; b < 2 && pg.vec_c; ++b) {
+            if (!(rec >> b & 1)) continue;
             Buf h;
-            hzl_encode<uint32_t>(rc, h);
-            pg.bufs[0].b.swap(h.b);
-        }
-        if (e.exp_t == EXP_FFOR && e.var == kFforSRec && pg.vec_c) {  // exception values, THzlCmp<uint64_t>
-            std::vector<uint64_t> xv(pg.bufs[0].b.size() / 8);
-            if (!xv.empty()) std::memcpy(xv.data(), pg.bufs[0].b.data(), xv.size() * 8);
-            Buf h;
-            hzl_encode<uint64_t>(xv, h);
-            pg.bufs[0].b.swap(h.b);
+            if (expr_rec_width(e) == 32) {
+                std::vector<uint32_t> xv(pg.bufs[b].b.size() / 4);
+                if (!xv.empty()) std::memcpy(xv.data(), pg.bufs[b].b.data(), xv.size() * 4);
+                hzl_encode<uint32_t>(xv, h);
+            } else {
+                std::vector<uint64_t> xv(pg.bufs[b].b.size() / 8);
+                if (!xv.empty()) std::memcpy(xv.data(), pg.bufs[b].b.data(), xv.size() * 8);
+                hzl_encode<uint64_t>(xv, h);
+            }
+            pg.bufs[b].b.swap(h.b);
         }
         if (first_page && meta) *meta = out.b.size() + 8 + 4ull * e.arr_c * std::max(1u, pg.vec_c);
         first_page = false;
@@ -812,10 +859,10 @@ int64_t encode_column(uint8_t dtype, uint8_t expr_id, const void* in, uint64_t n
     Buf     b;
     int64_t r = -1;
     switch (dtype_bits(dtype)) {
-    case 8: r = encode_t<uint8_t>(e, (const uint8_t*)in, n, b, metadata_size); break;
-    case 16: r = encode_t<uint16_t>(e, (const uint16_t*)in, n, b, metadata_size); break;
-    case 32: r = encode_t<uint32_t>(e, (const uint32_t*)in, n, b, metadata_size); break;
-    case 64: r = encode_t<uint64_t>(e, (const uint64_t*)in, n, b, metadata_size); break;
+    case 8: r = encode_t<uint8_t>(e, (const uint8_t*)in, n, b, metadata_size, false); break;
+    case 16: r = encode_t<uint16_t>(e, (const uint16_t*)in, n, b, metadata_size, false); break;
+    case 32: r = encode_t<uint32_t>(e, (const uint32_t*)in, n, b, metadata_size, false); break;
+    case 64: r = encode_t<uint64_t>(e, (const uint64_t*)in, n, b, metadata_size, dtype == DT_I64); break;
     default: return -1;
     }
     if (r < 0) return r;

@@ -67,6 +67,32 @@ struct Expr {
 // Pool ids (src/cor/exp_pool.cpp:23-100) -> expression; false if not supported.
 bool expr_from_id(uint8_t dtype, uint8_t id, Expr& e);
 bool expr_from_page(uint8_t exp_t, uint8_t var, Expr& e);
+// Whether a column of this type may hold pages of the expression: patches resolve for 32/64-bit values only
+// (src/cor/prm/patch/*/..._compress.cpp ResolveCompressFunc), chains of THzlCmp<uint64_t> hold 64-bit values here,
+// doubles are UNCOMPRESSED only.
+bool expr_fits_type(const Expr& e, uint8_t dtype);
+
+// Variant numbering of DELTA (include/fls/cor/exp/delta/delta_exp.hpp:24-30), RLE
+// (include/fls/cor/exp/rle/rle_exp.hpp:24-30) and DICT / DICT_FRQ / DICT_VAL
+// (include/fls/cor/exp/dict/dict_exp.hpp:30-42).
+constexpr uint8_t kDeltaLL = 2, kDeltaS = 3, kDeltaB = 4, kDeltaBRec = 5;
+constexpr uint8_t kRleRec = 3, kRleDeltaRec = 4, kRleDeltaRec2 = 5;
+constexpr uint8_t kDictFforLL = 3, kDictFforS = 4, kDictFforB = 5, kDictFforSRec = 6, kDictDelta = 7, kDictDeltaLL = 8,
+                  kDictDeltaS = 9, kDictDeltaB = 10, kDictDeltaBRec = 11;
+
+// The FFOR-shaped pages -- FFOR, DELTA and the dictionary code pages other than UNCOMPRESSED -- lay a vector out as
+// [lane bases (RSUM)] [exceptions][positions | bitmap | count+first (patch)] [bw][base][packed]; these describe it.
+// Patch kind in FFOR numbering (kFforLL / kFforS / kFforB), 0 without a patch step.
+uint8_t expr_patch(const Expr& e);
+// Values pass RSUM after FFOR (and after the patch): DELTA, dictionary DELTA_* variants.
+bool expr_rsum(const Expr& e);
+// Buffers that a page stores as one HZL chain each (RPN is_compressed flags): bit b = buffer b.
+unsigned expr_rec(const Expr& e);
+// Width of those chains' values: THzlCmp<idx_t> for dictionary expressions, THzlCmp<uint64_t> otherwise
+// (src/cor/eng/hzl_cmp_fac.cpp:15-35).
+unsigned expr_rec_width(const Expr& e);
+// FFOR-shaped page (see above).
+bool expr_ffor_shaped(const Expr& e);
 
 struct HzlChunkH {
     uint64_t src, dst, base;
@@ -90,10 +116,14 @@ struct PageEntry {
     // dictionary (integer columns)
     uint32_t               dict_n = 0;
     std::vector<HzlChunkH> hzl;
-    // DICT_RLE / S_PATCH_REC: buffer 0 (the page's run codes, u32 / exception values, u64) is one HZL chain; its
-    // chunks and decoded value count
+    // Buffers stored as one HZL chain per page (expr_rec): buffer 0 (DICT_RLE run codes, S_PATCH_REC exception
+    // values, RLE run values) and buffer 1 (exceptions of the DELTA B_PATCH_REC forms, RLE position lane bases);
+    // their chunks and decoded value counts (in values of expr_rec_width bits)
     std::vector<HzlChunkH> run_hzl;
     uint64_t               run_codes = 0;
+    std::vector<HzlChunkH> rec1_hzl;
+    uint64_t               rec1_n = 0;
+    uint8_t                rec = 0, rec_width = 0;
     // FSST (string columns)
     FsstTableH fsst;
 };

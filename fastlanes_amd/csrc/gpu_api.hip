@@ -37,7 +37,14 @@ struct DevDict {
     uint32_t  n_run_chunks = 0;
     bool      run_exc      = false;
     uint32_t* codes        = nullptr;
-    uint8_t*  run_values   = nullptr;
+    // DICT_RLE: run values; other forms with a chain in buffer 0 (S_PATCH_REC exceptions, RLE run values): its
+    // decoded bytes
+    uint8_t* run_values = nullptr;
+    // chain in buffer 1 (B_PATCH_REC exceptions, RLE position lane bases): chunks and decoded bytes
+    HzlChunk* rec1_chunks   = nullptr;
+    uint32_t  n_rec1_chunks = 0;
+    bool      rec1_exc      = false;
+    uint8_t*  rec1_values   = nullptr;
 };
 
 namespace {
@@ -111,15 +118,61 @@ hipError_t drle_page(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded, c
     return e;
 }
 
-// S_PATCH_REC page: the exception values from the page's HZL chain, then the S_PATCH kernel reads them there.
-hipError_t srec_page(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded, const uint8_t* page, const PageMeta& pm,
-                     uint8_t* out, const SelArgs* sel, hipStream_t s) {
+// Decoded bytes of the page's chained buffers (buffer 0, buffer 1), written by one HZL launch each.
+hipError_t decode_chains(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded, hipStream_t s) {
     const PageEntry& pe = plan->col.pages[i];
     DevDict&         d  = plan->dicts[i];
-    if (!pe.vec_c) return hipSuccess;
-    hipError_t e = launch_hzl(64, dev_encoded, d.run_chunks, d.n_run_chunks, d.run_values, d.run_exc, s);
-    if (e == hipSuccess) e = launch_page_patch(plan->W, kFforS, page, pm, pe.vec_c, out, plan->d_err, s, sel, d.run_values);
+    hipError_t       e  = hipSuccess;
+    if (d.n_run_chunks && pe.exp_t != EXP_DICT_RLE)
+        e = launch_hzl(pe.rec_width, dev_encoded, d.run_chunks, d.n_run_chunks, d.run_values, d.run_exc, s);
+    if (e == hipSuccess && d.n_rec1_chunks)
+        e = launch_hzl(pe.rec_width, dev_encoded, d.rec1_chunks, d.n_rec1_chunks, d.rec1_values, d.rec1_exc, s);
     return e;
+}
+
+// One page: decode into `out`, or (sel) write its selection bitmaps.
+hipError_t page_launch(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded, uint8_t* out, const SelArgs* sel,
+                       hipStream_t s) {
+    const PageEntry& pe   = plan->col.pages[i];
+    const unsigned   W    = plan->W;
+    const uint8_t*   page = dev_encoded + pe.page_off;
+    DevDict&         d    = plan->dicts[i];
+    PageMeta         pm {};
+    for (unsigned b = 0; b < pe.arr_c && b < 6; ++b) pm.off[b] = pe.boff[b];
+    pm.arr_c = pe.arr_c, pm.fixed = pe.fixed_meta;
+    if (!pe.vec_c) return hipSuccess;
+    Expr e;
+    expr_from_page(pe.exp_t, pe.var, e);
+    hipError_t     r     = pe.rec ? decode_chains(plan, i, dev_encoded, s) : hipSuccess;
+    const uint8_t  patch = expr_patch(e);
+    const bool     rsum  = expr_rsum(e);
+    // exceptions of a *_REC form: the decoded chain of their buffer (0 without lane bases, 1 with them)
+    const uint8_t* exc = (pe.rec & 1) ? d.run_values : (pe.rec & 2) ? d.rec1_values : nullptr;
+    if (r != hipSuccess) return r;
+    switch (pe.exp_t) {
+    case EXP_UNCOMPRESSED: return launch_page_int(W, 0, page, pm, pe.vec_c, out, s, sel);
+    case EXP_FFOR:
+    case EXP_DELTA:
+        if (!patch) return launch_page_int(W, rsum ? 2 : 1, page, pm, pe.vec_c, out, s, sel);
+        return launch_page_patch(W, patch, rsum, nullptr, 0, page, pm, pe.vec_c, out, plan->d_err, s, sel, exc);
+    case EXP_RLE: {
+        const int kv = pe.var == 1 || pe.var == kRleRec ? 1 : 2;  // positions: plain u16, or RSUM + FFOR
+        if (!pe.rec) return launch_page_rle(W, kv, page, pm, pe.vec_c, out, plan->d_err, s, sel);
+        const RunSrc rs {d.run_values, (uint32_t)(8 * pe.run_codes), 3u, (pe.rec & 2) ? d.rec1_values : nullptr};
+        return launch_page_rle(W, kv, page, pm, pe.vec_c, out, plan->d_err, s, sel, &rs);
+    }
+    case EXP_DICT_RLE: return drle_page(plan, i, dev_encoded, page, pm, out, sel, s);
+    default: {
+        const HzlSrc hz {dev_encoded, d.chunks, d.n_chunks, d.exc ? 1u : 0u};
+        if (pe.var == 1 || pe.var == 2 || pe.var == kDictDelta)
+            return launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, out, plan->d_err, s, sel, &hz);
+        // patched code pages: the dictionary is decoded first, then the patch kernel gathers from it
+        r = launch_hzl(plan->DW, dev_encoded, d.chunks, d.n_chunks, d.values, d.exc, s);
+        if (r != hipSuccess) return r;
+        return launch_page_patch(W, patch, rsum, d.values, pe.dict_n, page, pm, pe.vec_c, out, plan->d_err, s, sel,
+                                 exc);
+    }
+    }
 }
 }  // namespace
 
@@ -152,8 +205,8 @@ int fls_gpu_plan_create(const uint8_t* host_encoded, uint64_t encoded_size, enum
             if ((e = hipMemcpy(d.chunks, ch.data(), ch.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
                 return hip_fail(e, "hipMemcpy");
         }
-        if (!pe.run_hzl.empty()) {  // DICT_RLE run codes / S_PATCH_REC exception values
-            std::vector<HzlChunk> rc = device_chunks(pe.run_hzl, d.run_exc, pe.exp_t == EXP_DICT_RLE ? 32u : 64u);
+        if (!pe.run_hzl.empty()) {  // chain in buffer 0: DICT_RLE run codes, S_PATCH_REC exceptions, RLE run values
+            std::vector<HzlChunk> rc = device_chunks(pe.run_hzl, d.run_exc, pe.rec_width);
             d.n_run_chunks           = (uint32_t)rc.size();
             if ((e = hipMalloc(&d.run_chunks, rc.size() * sizeof(HzlChunk))) != hipSuccess) return hip_fail(e, "hipMalloc");
             if ((e = hipMemcpy(d.run_chunks, rc.data(), rc.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
@@ -163,9 +216,18 @@ int fls_gpu_plan_create(const uint8_t* host_encoded, uint64_t encoded_size, enum
                 if ((e = hipMalloc(&d.codes, (pe.run_codes + 1024) * 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
                 if ((e = hipMalloc(&d.run_values, pe.run_codes * (plan->W / 8) + 16)) != hipSuccess)
                     return hip_fail(e, "hipMalloc");
-            } else if ((e = hipMalloc(&d.run_values, (pe.run_codes + 1024) * 8)) != hipSuccess) {
-                return hip_fail(e, "hipMalloc");  // S_PATCH_REC: decoded u64 exception values
+            } else if ((e = hipMalloc(&d.run_values, (pe.run_codes + 1024) * (pe.rec_width / 8))) != hipSuccess) {
+                return hip_fail(e, "hipMalloc");  // decoded chain bytes
             }
+        }
+        if (!pe.rec1_hzl.empty()) {  // chain in buffer 1: B_PATCH_REC exceptions, RLE position lane bases
+            std::vector<HzlChunk> rc = device_chunks(pe.rec1_hzl, d.rec1_exc, pe.rec_width);
+            d.n_rec1_chunks          = (uint32_t)rc.size();
+            if ((e = hipMalloc(&d.rec1_chunks, rc.size() * sizeof(HzlChunk))) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipMemcpy(d.rec1_chunks, rc.data(), rc.size() * sizeof(HzlChunk), hipMemcpyHostToDevice)) != hipSuccess)
+                return hip_fail(e, "hipMemcpy");
+            if ((e = hipMalloc(&d.rec1_values, (pe.rec1_n + 1024) * (pe.rec_width / 8))) != hipSuccess)
+                return hip_fail(e, "hipMalloc");
         }
     }
     if (type == STR) {
@@ -184,6 +246,8 @@ void fls_gpu_plan_destroy(fls_gpu_plan* plan) {
         if (d.run_chunks) (void)hipFree(d.run_chunks);
         if (d.codes) (void)hipFree(d.codes);
         if (d.run_values) (void)hipFree(d.run_values);
+        if (d.rec1_chunks) (void)hipFree(d.rec1_chunks);
+        if (d.rec1_values) (void)hipFree(d.rec1_values);
     }
     strings_release(plan->strings);
     if (plan->d_err) (void)hipFree(plan->d_err);
@@ -203,29 +267,9 @@ int fls_gpu_plan_decode(fls_gpu_plan* plan, const uint8_t* dev_encoded, void* de
     if (out_capacity < plan->col.n_values * (W / 8)) return fail(FLS_E_CAPACITY, "output capacity too small");
     hipStream_t s = (hipStream_t)stream;
     for (size_t i = 0; i < plan->col.pages.size(); ++i) {
-        const PageEntry& pe   = plan->col.pages[i];
-        const uint8_t*   page = dev_encoded + pe.page_off;
-        uint8_t*         out  = (uint8_t*)dev_out + pe.out_index * (W / 8);
-        hipError_t       e    = hipSuccess;
-        PageMeta pm {};
-        for (unsigned b = 0; b < pe.arr_c && b < 6; ++b) pm.off[b] = pe.boff[b];
-        pm.arr_c = pe.arr_c, pm.fixed = pe.fixed_meta;
-        switch (pe.exp_t) {
-        case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pm, pe.vec_c, out, s); break;
-        case EXP_FFOR:
-            e = pe.var == 1           ? launch_page_int(W, 1, page, pm, pe.vec_c, out, s)
-                : pe.var == kFforSRec ? srec_page(plan, i, dev_encoded, page, pm, out, nullptr, s)
-                                      : launch_page_patch(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s);
-            break;
-        case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, out, s); break;
-        case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, out, plan->d_err, s); break;
-        case EXP_DICT_RLE: e = drle_page(plan, i, dev_encoded, page, pm, out, nullptr, s); break;
-        default: {
-            DevDict&     d = plan->dicts[i];
-            const HzlSrc hz {dev_encoded, d.chunks, d.n_chunks, d.exc ? 1u : 0u};
-            e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, out, plan->d_err, s, nullptr, &hz);
-        }
-        }
+        const PageEntry& pe  = plan->col.pages[i];
+        uint8_t*         out = (uint8_t*)dev_out + pe.out_index * (W / 8);
+        const hipError_t e   = page_launch(plan, i, dev_encoded, out, nullptr, s);
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
     }
     return FLS_OK;
@@ -271,31 +315,11 @@ int fls_gpu_plan_select(fls_gpu_plan* plan, const uint8_t* dev_encoded, int op, 
     }
     hipStream_t s = (hipStream_t)stream;
     for (size_t i = 0; i < plan->col.pages.size(); ++i) {
-        const PageEntry& pe   = plan->col.pages[i];
-        const uint8_t*   page = dev_encoded + pe.page_off;
-        const uint64_t   v0   = pe.out_index / 1024;
-        SelArgs          ps   = sa;
+        const PageEntry& pe = plan->col.pages[i];
+        const uint64_t   v0 = pe.out_index / 1024;
+        SelArgs          ps = sa;
         ps.bitmap = dev_bitmap + v0 * 16, ps.counts = dev_counts + v0;
-        hipError_t e = hipSuccess;
-        PageMeta   pm {};
-        for (unsigned b = 0; b < pe.arr_c && b < 6; ++b) pm.off[b] = pe.boff[b];
-        pm.arr_c = pe.arr_c, pm.fixed = pe.fixed_meta;
-        switch (pe.exp_t) {
-        case EXP_UNCOMPRESSED: e = launch_page_int(W, 0, page, pm, pe.vec_c, nullptr, s, &ps); break;
-        case EXP_FFOR:
-            e = pe.var == 1           ? launch_page_int(W, 1, page, pm, pe.vec_c, nullptr, s, &ps)
-                : pe.var == kFforSRec ? srec_page(plan, i, dev_encoded, page, pm, nullptr, &ps, s)
-                                      : launch_page_patch(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps);
-            break;
-        case EXP_DELTA: e = launch_page_int(W, 2, page, pm, pe.vec_c, nullptr, s, &ps); break;
-        case EXP_RLE: e = launch_page_rle(W, pe.var, page, pm, pe.vec_c, nullptr, plan->d_err, s, &ps); break;
-        case EXP_DICT_RLE: e = drle_page(plan, i, dev_encoded, page, pm, nullptr, &ps, s); break;
-        default: {
-            DevDict&     d = plan->dicts[i];
-            const HzlSrc hz {dev_encoded, d.chunks, d.n_chunks, d.exc ? 1u : 0u};
-            e = launch_page_dict(W, pe.var, page, pm, pe.vec_c, d.values, pe.dict_n, nullptr, plan->d_err, s, &ps, &hz);
-        }
-        }
+        const hipError_t e = page_launch(plan, i, dev_encoded, nullptr, &ps, s);
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
     }
     return FLS_OK;

@@ -79,12 +79,15 @@ hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const Page
     }
 }
 
-hipError_t launch_page_patch(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                             uint32_t* err, hipStream_t s, const SelArgs* sel, const uint8_t* exc) {
+hipError_t launch_page_patch(unsigned W, int var, bool rsum, const void* dict, uint32_t dict_n, const uint8_t* page,
+                             const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err, hipStream_t s,
+                             const SelArgs* sel, const uint8_t* exc) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 32: return page_patch_w<32>(var, page, pm, vec_c, out, err, sel, exc, s);
-    case 64: return page_patch_w<64>(var, page, pm, vec_c, out, err, sel, exc, s);
+    case 8: return page_patch_w<8>(var, rsum, dict, dict_n, page, pm, vec_c, out, err, sel, exc, s);
+    case 16: return page_patch_w<16>(var, rsum, dict, dict_n, page, pm, vec_c, out, err, sel, exc, s);
+    case 32: return page_patch_w<32>(var, rsum, dict, dict_n, page, pm, vec_c, out, err, sel, exc, s);
+    case 64: return page_patch_w<64>(var, rsum, dict, dict_n, page, pm, vec_c, out, err, sel, exc, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -1,4 +1,4 @@
-// page kernels for 32-bit elements: RLE and dictionary pages
+// page kernels for 32-bit elements: RLE and dictionary pages (patched dictionary pages included)
 #define FLS_W 32
 #define FLS_PART 2
 #include "k_page_w.inc"

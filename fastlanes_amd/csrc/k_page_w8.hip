@@ -1,4 +1,4 @@
-// page kernels for 8-bit elements: integer and patched-FFOR pages
+// page kernels for 8-bit elements: UNCOMPRESSED, FFOR and DELTA pages
 #define FLS_W 8
 #define FLS_PART 1
 #include "k_page_w.inc"

@@ -61,9 +61,13 @@ hipError_t launch_drle_fused(unsigned W, const uint8_t* enc, const HzlChunk* run
 hipError_t launch_page_dict(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c,
                             const void* dict, uint32_t dict_n, uint8_t* out, uint32_t* err, hipStream_t s,
                             const SelArgs* sel = nullptr, const HzlSrc* hz = nullptr);
-// exc: the exception values (S_PATCH_REC, decoded from the page's chain); nullptr: the page's buffer 0
-hipError_t launch_page_patch(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                             uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr, const uint8_t* exc = nullptr);
+// Patched FFOR-shaped pages. var: patch kind in FFOR numbering (2 LL_PATCH, 3 S_PATCH, 4 B_PATCH); rsum: lane bases
+// and RSUM (DELTA forms); dict: the decoded dictionary of a dictionary page (u32 codes, values dict[code]), nullptr
+// for FFOR / DELTA pages of 32/64-bit values; exc: the exception values (decoded chain of a *_REC form), nullptr:
+// the page's exception buffer.
+hipError_t launch_page_patch(unsigned W, int var, bool rsum, const void* dict, uint32_t dict_n, const uint8_t* page,
+                             const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err, hipStream_t s,
+                             const SelArgs* sel = nullptr, const uint8_t* exc = nullptr);
 hipError_t launch_hzl(unsigned DW, const uint8_t* enc, const HzlChunk* chunks, uint32_t n_chunks, uint8_t* dict,
                       bool exceptions, hipStream_t s);  // exceptions: some chunk has them (else that pass is skipped)
 
@@ -72,8 +76,13 @@ template <int W>
 hipError_t page_int_w(int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                       const SelArgs* sel, hipStream_t s);
 template <int W>
-hipError_t page_patch_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err,
-                        const SelArgs* sel, const uint8_t* exc, hipStream_t s);
+hipError_t page_patch_w(int var, bool rsum, const void* dict, uint32_t dict_n, const uint8_t* page, const PageMeta& pm,
+                        uint32_t vec_c, uint8_t* out, uint32_t* err, const SelArgs* sel, const uint8_t* exc,
+                        hipStream_t s);
+template <int W>
+hipError_t page_patch_dict_w(int var, bool rsum, const void* dict, uint32_t dict_n, const uint8_t* page,
+                             const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err, const SelArgs* sel,
+                             const uint8_t* exc, hipStream_t s);
 template <int W>
 hipError_t page_rle_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err,
                       const SelArgs* sel, const RunSrc* runs, hipStream_t s);

@@ -385,26 +385,57 @@ __device__ __forceinline__ unsigned orig_of(unsigned k) {
 }
 
 struct PatchMeta {
-    uint64_t base;
+    uint64_t base, lb;                    // FFOR base; the thread's lane base (RSUM forms)
     uint32_t bw, ep_exc, ep_pos, ep_pk;
 };
 
-template <int W>
-__device__ __forceinline__ PatchMeta patch_meta(const uint8_t* page, const PageMeta& pm, uint64_t v) {
-    using T            = typename uint_of<W>::type;
+// Vector v's metadata of a patched FFOR-shaped page over CW-bit values: [lane bases (RSUM)][exceptions][positions]
+// [bw][base][packed]; o = 1 with lane bases. Thread t of the vector's slot reads lane base t.
+template <int CW, bool RSUM>
+__device__ __forceinline__ PatchMeta patch_meta(const uint8_t* page, const PageMeta& pm, uint64_t v, unsigned t) {
+    using T            = typename uint_of<CW>::type;
+    constexpr unsigned o = RSUM ? 1 : 0;
     const u32_ua* eps  = reinterpret_cast<const u32_ua*>(page + 8 + 4 * pm.arr_c);
-    const u32_ua* row  = eps + (v ? (v - 1) * 5 : 0);
+    const u32_ua* row  = eps + (v ? (v - 1) * pm.arr_c : 0);
     PatchMeta     m {};
-    if (v) m.ep_exc = row[0], m.ep_pos = row[1], m.ep_pk = row[4];
+    if (v) m.ep_exc = row[o], m.ep_pos = row[o + 1], m.ep_pk = row[o + 4];
     if (pm.fixed) {
-        m.bw   = page[pm.off[2] + v];
-        m.base = load_t<T>(page + pm.off[3] + v * (W / 8));
+        m.bw   = page[pm.off[o + 2] + v];
+        m.base = load_t<T>(page + pm.off[o + 3] + v * (CW / 8));
+        if constexpr (RSUM) m.lb = load_t<T>(page + pm.off[0] + 128 * v + t * (CW / 8));
     } else {
-        const uint32_t e2 = v ? (uint32_t)row[2] : 0u, e3 = v ? (uint32_t)row[3] : 0u;
-        m.bw   = page[pm.off[2] + e2];
-        m.base = load_t<T>(page + pm.off[3] + e3);
+        const uint32_t e2 = v ? (uint32_t)row[o + 2] : 0u, e3 = v ? (uint32_t)row[o + 3] : 0u;
+        m.bw   = page[pm.off[o + 2] + e2];
+        m.base = load_t<T>(page + pm.off[o + 3] + e3);
+        if constexpr (RSUM) m.lb = load_t<T>(page + pm.off[0] + (v ? (uint32_t)row[0] : 0u) + t * (CW / 8));
     }
     return m;
+}
+
+// RSUM over a vector staged untransposed in LDS (delta_exp.hpp:56-115 decompress: RSUM after the patch): lane t's
+// values are the CW consecutive positions from seg_start<CW>(t); each becomes the lane base plus the running sum of
+// the lane's differences up to it (gen/rsum_src.cpp). 16 bytes at a time.
+template <int CW>
+__device__ __forceinline__ void lds_rsum(uint8_t* vl, uint64_t lb, unsigned t) {
+    using T              = typename uint_of<CW>::type;
+    constexpr unsigned K = 16 / (CW / 8);  // values per 16-byte chunk
+    const uint32_t     o = seg_start<CW>(t) * (CW / 8);
+    T                  acc = (T)lb;
+    static_for<CW / K>([&](auto q) {
+        v4u* c = reinterpret_cast<v4u*>(vl + lds_pad(o + 16 * q));
+        v4u  x = *c;
+        if constexpr (CW == 64) {
+            uint64_t a = (uint64_t)x[0] | ((uint64_t)x[1] << 32), b = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
+            acc += a, a = acc, acc += b, b = acc;
+            x = v4u {(uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32)};
+        } else {
+            static_for<4>([&](auto j) {
+                constexpr int k = decltype(j)::value;
+                acc += x[k], x[k] = acc;
+            });
+        }
+        *c = x;
+    });
 }
 
 // Exceptions of one vector into its LDS staging `vl`, by the TPV threads of its slot (t = 0 .. TPV-1).
@@ -465,41 +496,97 @@ __device__ __forceinline__ void apply_patches(const uint8_t* exc, const uint8_t*
     }
 }
 
-// Same pipeline as k_page_int (MODE 1) with a patch step between the LDS decode and the copy-out.
-template <int W, int VAR, bool NT, bool SEL = false>
+// Dictionary codes of a staged vector (u32, untransposed, slot `cl`) into their values (slot `ol`): codes >= dict_n
+// are clamped (err bit 2). TPV threads per vector, four codes at a time.
+template <typename T, typename D, unsigned TPV>
+__device__ __forceinline__ void lds_gather(const uint8_t* cl, uint8_t* ol, const D* __restrict__ dict, uint32_t dict_n,
+                                           unsigned t, uint32_t* err) {
+    const uint32_t last = dict_n - 1;
+    bool           bad  = false;
+    for (unsigned c4 = t; c4 < 256; c4 += TPV) {
+        const v4u c = *reinterpret_cast<const v4u*>(cl + lds_pad(16 * c4));
+        T         x[4];
+        static_for<4>([&](auto j) {
+            constexpr int k = decltype(j)::value;
+            bad |= c[k] > last;
+            x[k] = (T)dict[c[k] > last ? last : c[k]];
+        });
+        if constexpr (sizeof(T) == 8) {
+            *reinterpret_cast<v4u*>(ol + lds_pad(32 * c4)) =
+                v4u {(uint32_t)x[0], (uint32_t)(x[0] >> 32), (uint32_t)x[1], (uint32_t)(x[1] >> 32)};
+            *reinterpret_cast<v4u*>(ol + lds_pad(32 * c4 + 16)) =
+                v4u {(uint32_t)x[2], (uint32_t)(x[2] >> 32), (uint32_t)x[3], (uint32_t)(x[3] >> 32)};
+        } else if constexpr (sizeof(T) == 4) {
+            *reinterpret_cast<v4u*>(ol + lds_pad(16 * c4)) = v4u {x[0], x[1], x[2], x[3]};
+        } else if constexpr (sizeof(T) == 2) {
+            *reinterpret_cast<v2u*>(ol + lds_pad(8 * c4)) =
+                v2u {(uint32_t)x[0] | ((uint32_t)x[1] << 16), (uint32_t)x[2] | ((uint32_t)x[3] << 16)};
+        } else {
+            *reinterpret_cast<uint32_t*>(ol + lds_pad(4 * c4)) =
+                (uint32_t)x[0] | ((uint32_t)x[1] << 8) | ((uint32_t)x[2] << 16) | ((uint32_t)x[3] << 24);
+        }
+    }
+    if (bad) atomicOr(err, 2u);
+}
+
+// Patched FFOR-shaped pages, pipelined like k_page_int: the CW-bit values of a vector are FFOR-decoded into the
+// wave's LDS staging, the exceptions overwrite their positions, RSUM adds the lane bases (DELTA and dictionary
+// DELTA_* forms), and dictionary pages (DICT, CW = 32) gather their T-typed values into a second staging area
+// before the copy-out.
+//   FFOR  [exceptions][positions][bw][base][packed]                 (include/fls/cor/exp/ffor/ffor_exp.hpp:56-130)
+//   DELTA [lane bases][exceptions][positions][bw][base][packed]     (include/fls/cor/exp/delta/delta_exp.hpp:56-131)
+//   DICT  either form over u32 codes                                (include/fls/cor/exp/dict/dict_exp.hpp:62-209)
+// Exception values come from exc_base + the vector's entry point: the page's exception buffer, or the decoded HZL
+// chain of the *_REC forms.
+template <typename T, int CW, int VAR, bool RSUM, bool DICT, bool NT, bool SEL = false>
 __global__ __launch_bounds__(kMaxBlock) void k_page_patch(const uint8_t* __restrict__ page, PageMeta pm,
                                                           uint32_t vec_c, uint8_t* __restrict__ out,
                                                           uint32_t* __restrict__ err, SelArgs sa,
-                                                          const uint8_t* __restrict__ exc_base) {
+                                                          const uint8_t* __restrict__ exc_base,
+                                                          const typename uint_of<(8 * sizeof(T) < 32 ? 32 : 8 * sizeof(T))>::type* __restrict__ dict,
+                                                          uint32_t dict_n) {
     extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
-    constexpr unsigned TPV = lane_tpv<W>(), VB = 128 * W;
+    static_assert(DICT || 8 * sizeof(T) == CW, "value pages decode T-typed values");
+    constexpr unsigned TPV = lane_tpv<CW>(), VB = 128 * CW, OB = 1024 * sizeof(T);
     using WS               = WaveStage<TPV, VB>;
+    using OS               = WaveStage<TPV, OB>;
+    constexpr unsigned WB  = WS::BYTES + (DICT ? OS::BYTES : 0u);  // LDS bytes per wave
     const unsigned lane = threadIdx.x % 64, wave = threadIdx.x / 64, waves = blockDim.x / 64;
     const unsigned t = lane % TPV, slot = lane / TPV;
-    uint8_t*       wl = fls_lds + wave * WS::BYTES;
+    uint8_t*       wl = fls_lds + wave * WB;
+    uint8_t*       ow = DICT ? wl + WS::BYTES : wl;  // output staging
     uint8_t*       vl = wl + slot * WS::PVB;
     const LdsSink  sink {vl};
     const uint64_t N      = vec_c;
     const uint64_t stride = (uint64_t)gridDim.x * waves * WS::VPW;
     uint64_t       v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
+    constexpr unsigned o  = RSUM ? 1 : 0;
     PatchMeta      m {};
-    if (v0 + slot < N) m = patch_meta<W>(page, pm, v0 + slot);
+    if (v0 + slot < N) m = patch_meta<CW, RSUM>(page, pm, v0 + slot, t);
     for (; v0 < N; v0 += stride) {
         const uint64_t  v   = v0 + slot;
         const PatchMeta cur = m;
         if (v < N) {
-            const uint8_t* pk = page + pm.off[4] + cur.ep_pk;
-            dispatch_bw<W>(cur.bw, [&](auto BW) { ffor_untransposed<W, BW, false>(pk, cur.base, 0, t, sink); });
+            const uint8_t* pk = page + pm.off[o + 4] + cur.ep_pk;
+            dispatch_bw<CW>(cur.bw, [&](auto BW) { ffor_untransposed<CW, BW, false>(pk, cur.base, 0, t, sink); });
         }
-        if (v + stride < N) m = patch_meta<W>(page, pm, v + stride);
+        if (v + stride < N) m = patch_meta<CW, RSUM>(page, pm, v + stride, t);
         wave_sync();
         if (v < N)
-            apply_patches<W, VAR, TPV>(exc_base + cur.ep_exc, page + pm.off[1] + cur.ep_pos, cur.base, t, vl,
-                                       err);
+            apply_patches<CW, VAR, TPV>(exc_base + cur.ep_exc, page + pm.off[o + 1] + cur.ep_pos, cur.base, t, vl,
+                                        err);
         wave_sync();
+        if constexpr (RSUM) {
+            if (v < N) lds_rsum<CW>(vl, cur.lb, t);
+            wave_sync();
+        }
+        if constexpr (DICT) {
+            if (v < N && dict_n) lds_gather<T, typename uint_of<(8 * sizeof(T) < 32 ? 32 : 8 * sizeof(T))>::type, TPV>(
+                vl, ow + slot * OS::PVB, dict, dict_n, t, err);
+            wave_sync();
+        }
         const uint64_t left = N - v0;
-        wave_out<typename uint_of<W>::type, TPV, VB, NT, SEL>(wl, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW,
-                                                              lane, sa);
+        wave_out<T, TPV, OB, NT, SEL>(ow, out, v0, left < WS::VPW ? (uint32_t)left : WS::VPW, lane, sa);
         wave_sync();
     }
 }
@@ -575,16 +662,19 @@ struct RleMeta {
 };
 
 // Where a page's run values are: `base` holds them (T-typed), entry points of buffer 0 count 2^esh bytes per run and
-// `bytes` is buffer 0's extent in those units. RLE pages: the page's own buffer 0 (esh = log2 sizeof(T)). DICT_RLE
-// pages: run values looked up from the page's decoded run codes beforehand (esh = 2, u32 codes).
+// `bytes` is buffer 0's extent in those units. RLE pages: the page's own buffer 0 (esh = log2 sizeof(T)), or its
+// decoded chain (the *_REC forms). DICT_RLE pages: run values looked up from the page's decoded run codes beforehand
+// (esh = 2, u32 codes). `lbase`: the position lane bases' decoded chain (RLE DELTA_NO_PATCH_REC2), nullptr: the
+// page's buffer 1.
 struct RunSrc {
     const uint8_t* base;
     uint32_t       bytes, esh;
+    const uint8_t* lbase = nullptr;
 };
 
 template <int VAR>
 __device__ __forceinline__ RleMeta rle_meta(const uint8_t* page, const PageMeta& pm, uint64_t v, uint64_t N, unsigned t,
-                                            uint32_t run_bytes) {
+                                            uint32_t run_bytes, const uint8_t* lbase) {
     const u32_ua* eps = reinterpret_cast<const u32_ua*>(page + 8 + 4 * pm.arr_c);
     RleMeta       m {};
     m.r0 = v ? eps[(v - 1) * pm.arr_c] : 0u;
@@ -592,12 +682,13 @@ __device__ __forceinline__ RleMeta rle_meta(const uint8_t* page, const PageMeta&
     if constexpr (VAR == 1) {
         m.ep = pm.fixed ? (uint32_t)(v * 2048) : (v ? eps[(v - 1) * pm.arr_c + 1] : 0u);
     } else {
+        const uint8_t* lb = lbase ? lbase : page + pm.off[1];
         if (pm.fixed) {
-            m.lb   = reinterpret_cast<const u32_ua*>(page + pm.off[1] + 128 * v)[t];
+            m.lb   = reinterpret_cast<const u32_ua*>(lb + 128 * v)[t];
             m.bw   = page[pm.off[2] + v];
             m.base = *reinterpret_cast<const u16_ua*>(page + pm.off[3] + 2 * v);
         } else {
-            m.lb   = reinterpret_cast<const u32_ua*>(vbuf(page, pm.arr_c, v, 1))[t];
+            m.lb   = reinterpret_cast<const u32_ua*>(lb + (v ? eps[(v - 1) * pm.arr_c + 1] : 0u))[t];
             m.bw   = *vbuf(page, pm.arr_c, v, 2);
             m.base = *reinterpret_cast<const u16_ua*>(vbuf(page, pm.arr_c, v, 3));
         }
@@ -629,7 +720,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
     const uint64_t     stride = (uint64_t)gridDim.x * waves * WS::VPW;
     uint64_t           v0     = ((uint64_t)blockIdx.x * waves + wave) * WS::VPW;
     RleMeta            m {};
-    if (v0 + slot < N) m = rle_meta<VAR>(page, pm, v0 + slot, N, t, rs.bytes);
+    if (v0 + slot < N) m = rle_meta<VAR>(page, pm, v0 + slot, N, t, rs.bytes, rs.lbase);
     for (; v0 < N; v0 += stride) {
         const uint64_t v      = v0 + slot;
         const uint8_t* runs   = rs.base + (uint64_t)(m.r0 >> rs.esh) * sizeof(T);
@@ -656,7 +747,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
             else if (n_runs <= kRleLds) positions(static_cast<const T*>(lruns));
             else positions(runs);
         }
-        if (v + stride < N) m = rle_meta<VAR>(page, pm, v + stride, N, t, rs.bytes);
+        if (v + stride < N) m = rle_meta<VAR>(page, pm, v + stride, N, t, rs.bytes, rs.lbase);
         wave_sync();
         if constexpr (!DIRECT) {
             const uint64_t left = N - v0;

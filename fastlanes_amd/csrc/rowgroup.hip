@@ -141,7 +141,7 @@ int kind_of(const PageEntry& pe, unsigned W) {
     switch (pe.exp_t) {
     case EXP_UNCOMPRESSED: return RG_UNC8 + wc;
     case EXP_FFOR: return pe.var == 1 ? RG_FFOR8 + wc : -1;
-    case EXP_DELTA: return RG_DELTA8 + wc;
+    case EXP_DELTA: return pe.var == 1 ? RG_DELTA8 + wc : -1;
     case EXP_RLE: return pe.var == 2 ? RG_RLE8 + wc : -1;
     case EXP_DICT:
     case EXP_DICT_FRQ:

@@ -99,14 +99,14 @@ __device__ __forceinline__ unsigned orig_of(unsigned k) {
 // smallest), the window [s[i], s[i] + 2^bw - 1] (bw = 0: only base 0, window {0}) leaving at most 100 values
 // outside; the plan minimises 128 bw + 8 (values outside), ties to the smaller i, then the larger bw; it exists only
 // below 128 * 64 bytes (else bw = 64, base 0, no exceptions). s: the vector sorted ascending (LDS). A window wider
-// than the vector's FFOR width bwf is never better than bw = bwf at i = 0, so bw runs up to min(bwf, 63) -- and up
-// to 1 for a constant nonzero vector, which no zero-width window holds.
+// than the vector's unsigned FFOR width bwu is never better than bw = bwu at i = 0, so bw runs up to min(bwu, 63) --
+// and up to 1 for a constant nonzero vector, which no zero-width window holds.
 struct PatchPlan {
     uint32_t bw, exc;
     uint64_t base;
 };
-__device__ PatchPlan patch_plan(const uint64_t* s, uint32_t bwf) {
-    const unsigned lane = threadIdx.x % 64, bw_hi = bwf < 1 ? 1 : bwf < 63 ? bwf : 63;
+__device__ PatchPlan patch_plan(const uint64_t* s, uint32_t bwu) {
+    const unsigned lane = threadIdx.x % 64, bw_hi = bwu < 1 ? 1 : bwu < 63 ? bwu : 63;
     uint64_t       key  = ~0ull;  // (size << 32) | (i << 8) | (63 - bw): the smallest key is the plan
     for (unsigned h = 0; h < 2; ++h) {
         const unsigned i = lane + 64 * h;
@@ -156,8 +156,10 @@ __device__ __forceinline__ bool is_exception(uint64_t v, const PatchPlan& pp) {
 }
 
 // Statistics of one 1024-value vector, computed by one wave (lane l owns values 16l .. 16l+15).
+// bwf / bwd: FFOR widths of the values and of their lane differences as ANALYZE computes them for int64 columns
+// (min and max signed, ffor_analyze<i64_pt>); bwu: the unsigned width of the values (bounds the patch search).
 struct VecStats {
-    uint32_t bwf, bwd, runs, bwp, eq, nondec, inc;
+    uint32_t bwf, bwd, bwu, runs, bwp, eq, nondec, inc;
     uint64_t umin, umax;
     int64_t  smin, smax;
 };
@@ -173,8 +175,8 @@ __device__ VecStats vector_stats(const uint64_t* __restrict__ x) {
         v[2 * i + 1] = (uint64_t)q[2] | ((uint64_t)q[3] << 32);
     }
     uint64_t prev = __shfl_up(v[15], 1, 64);
-    uint64_t mn = kEmpty, mx = 0, dmn = kEmpty, dmx = 0;
-    int64_t  smn = INT64_MAX, smx = INT64_MIN;
+    uint64_t mn = kEmpty, mx = 0;
+    int64_t  smn = INT64_MAX, smx = INT64_MIN, dmn = INT64_MAX, dmx = INT64_MIN;
     uint32_t runs = 0, inner = 0, eq = 0, nondec = 0, inc = 0;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -184,7 +186,7 @@ __device__ VecStats vector_stats(const uint64_t* __restrict__ x) {
         mx = v[i] > mx ? v[i] : mx;
         smn = (int64_t)v[i] < smn ? (int64_t)v[i] : smn;
         smx = (int64_t)v[i] > smx ? (int64_t)v[i] : smx;
-        const uint64_t d = (o % 64) ? v[i] - pv : 0ull;
+        const int64_t  d = (o % 64) ? (int64_t)(v[i] - pv) : 0;
         dmn = d < dmn ? d : dmn;
         dmx = d > dmx ? d : dmx;
         const bool has_prev = o != 0;
@@ -198,8 +200,9 @@ __device__ VecStats vector_stats(const uint64_t* __restrict__ x) {
     VecStats s;
     s.umin = wave_min(mn), s.umax = wave_max(mx);
     s.smin = wave_min(smn), s.smax = wave_max(smx);
-    s.bwf  = bitlen64(s.umax - s.umin);
-    s.bwd  = bitlen64(wave_max(dmx) - wave_min(dmn));
+    s.bwf  = bitlen64((uint64_t)s.smax - (uint64_t)s.smin);
+    s.bwu  = bitlen64(s.umax - s.umin);
+    s.bwd  = bitlen64((uint64_t)wave_max(dmx) - (uint64_t)wave_min(dmn));
     s.runs = wave_sum(runs), s.bwp = wave_sum(inner) != 0;
     s.eq = wave_sum(eq), s.nondec = wave_sum(nondec), s.inc = wave_sum(inc);
     return s;
@@ -315,7 +318,7 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
 #pragma unroll
             for (unsigned i = 0; i < 16; ++i) sb[64 * i + lane] = x[v * 1024 + 64 * i + lane];
             wave_sort1024(sb);
-            const PatchPlan pp = patch_plan(sb, s.bwf);
+            const PatchPlan pp = patch_plan(sb, s.bwu);
             // exceptions in transposed order (the order the encoder stores them) and the LL chain's hop entries:
             // a gap g between consecutive exceptions adds floor((g - 1) / hop) links, hop = 2^min(bw, 11)
             const unsigned hop = 1u << (pp.bw < 11 ? pp.bw : 11);
@@ -591,7 +594,7 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
 #pragma unroll
             for (unsigned i = 0; i < 16; ++i) sb[64 * i + lane] = x[kSampleVec[wave] * 1024 + 64 * i + lane];
             wave_sort1024(sb);
-            const PatchPlan pp = patch_plan(sb, s.bwf);
+            const PatchPlan pp = patch_plan(sb, s.bwu);
             if (lane == 0) {
                 L.pbw[wave] = pp.bw, L.pexc[wave] = pp.exc;
                 L.bwf[wave] = s.bwf, L.bwd[wave] = s.bwd, L.runs[wave] = s.runs, L.bwp[wave] = s.bwp;

@@ -332,6 +332,55 @@ def spell_edge_tables():
     yield ("empty", [np.zeros(0, np.int64), np.zeros(0, np.uint8)], ["CONSTANT", "CONSTANT"], [-1, -1])
 
 
+def delta_patch_column(dtype, n: int, seed: int = 9) -> np.ndarray:
+    """Data for the patched DELTA variants (32/64-bit): per vector a running sum of small steps (0 to 2^k - 1 for a
+    random k <= 12) with up to 120 jumps anywhere in the type's range, so the lane differences are a narrow body with
+    outliers; every eighth vector a corner case in turn: constant, strictly increasing by one, a running sum with
+    exactly 100 and exactly 101 jumps, and differences over the full range."""
+    dt = np.dtype(dtype)
+    bits = dt.itemsize * 8
+    udt = np.uint64 if bits == 64 else np.uint32
+    g = rng(seed)
+    nv = n // 1024
+    out = np.empty((nv, 1024), dtype=np.uint64)
+    mask = np.uint64((1 << bits) - 1)
+    for v in range(nv):
+        corner = (v // 8) % 5 if v % 8 == 7 else -1
+        start = np.uint64(int(g.integers(0, 1 << (bits - 1))))
+        if corner == 0:
+            steps = np.zeros(1024, dtype=np.uint64)
+        elif corner == 1:
+            steps = np.ones(1024, dtype=np.uint64)
+        elif corner == 4:
+            steps = g.integers(0, 1 << 63, 1024, dtype=np.uint64) * np.uint64(2) + g.integers(0, 2, 1024, dtype=np.uint64)
+        else:
+            k = int(g.integers(0, 13))
+            steps = g.integers(0, 1 << k, 1024, dtype=np.uint64)
+            nj = 100 if corner == 2 else 101 if corner == 3 else int(g.integers(0, 121))
+            pos = g.choice(1024, nj, replace=False)
+            steps[pos] = g.integers(0, 1 << 62, nj, dtype=np.uint64) * np.uint64(3)
+        with np.errstate(over="ignore"):
+            out[v] = (start + np.cumsum(steps, dtype=np.uint64)) & mask
+    return out.astype(udt).reshape(-1).view(dt)
+
+
+def dict_rare_keys(n: int, dtype=np.int64, n_common: int = 200, n_rare: int = 40, rare_frac: float = 0.01,
+                   seed: int = 12) -> np.ndarray:
+    """Dictionary data whose codes suit the patched dictionary variants: n_common keys drawn uniformly (codes below
+    n_common, in first-occurrence or frequency order alike) and, at a rate of rare_frac, n_rare other keys that first
+    appear late in the column (high codes: the exceptions of a narrow code width); runs of one key give the DELTA
+    code variants narrow differences."""
+    info = np.iinfo(dtype)
+    g = rng(seed)
+    keys = g.choice(np.arange(info.min, info.max, max(1, (int(info.max) - int(info.min)) // (4 * (n_common + n_rare))),
+                              dtype=np.int64)[: 4 * (n_common + n_rare)], n_common + n_rare, replace=False)
+    idx = geometric_runs(n, 6.0, seed=seed + 1, value_bits=16, dtype=np.int64) % n_common
+    rare = g.random(n) < rare_frac
+    rare[: n // 2] = False
+    idx[rare] = n_common + g.integers(0, n_rare, int(rare.sum()))
+    return keys[idx].astype(dtype)
+
+
 def dict_runs(n: int, dtype=np.int64, n_keys: int = 300, mean: float = 12.0, seed: int = 6) -> np.ndarray:
     """Runs (Geometric(1/mean) lengths) over n_keys random keys of the type: the shape DICT_RLE compresses."""
     info = np.iinfo(dtype)

@@ -153,6 +153,21 @@ uint8_t orc_ffor_analyze(unsigned W, const void* in, void* base_out) {
     st(W, base_out, 0, mn);
     return (uint8_t)(delta ? 64 - __builtin_clzll(delta) : 0);
 }
+/* ANALYZE as resolved for the column type (ffor_prm::ResolveAnalyzeFunc, ffor_analyze.cpp:145-160): int64 columns
+ * take min and max as signed values (ffor_analyze<i64_pt>), every other type as unsigned. The base is the minimum's
+ * bit pattern; max - min is the same unsigned difference either way. */
+static uint8_t ffor_analyze_typed(unsigned W, const void* in, void* base_out, int sgn) {
+    const uint64_t bias = sgn ? 1ULL << (W - 1) : 0;
+    uint64_t       mn = wmask(W), mx = 0;
+    for (unsigned i = 0; i < 1024; ++i) {
+        uint64_t v = ld(W, in, i) ^ bias;
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+    }
+    uint64_t delta = mx - mn;
+    st(W, base_out, 0, mn ^ bias);
+    return (uint8_t)(delta ? 64 - __builtin_clzll(delta) : 0);
+}
 /* The reference's floating-point formula, kept for tests that show where it and the exact form agree. */
 uint8_t orc_ffor_bw_reference_formula(uint64_t delta) { return (uint8_t)ceil(log2((double)(delta + 1))); }
 
@@ -242,58 +257,103 @@ typedef struct {
 } expr_t;
 /* FFOR var_t (include/fls/cor/exp/ffor/ffor_exp.hpp:24-30) */
 enum { FFOR_LL = 2, FFOR_S = 3, FFOR_B = 4, FFOR_S_REC = 5 };
-/* S_PATCH_REC: S_PATCH with the page's exception values as one HZL chain of u64 values                          */
-/* (include/fls/cor/exp/ffor/ffor_exp.hpp:87-94, src/cor/eng/hzl_cmp_fac.cpp:24-31); 64-bit integer columns     */
+/* DELTA var_t (include/fls/cor/exp/delta/delta_exp.hpp:24-30) */
+enum { DELTA_LL = 2, DELTA_S = 3, DELTA_B = 4, DELTA_B_REC = 5 };
+/* RLE var_t (include/fls/cor/exp/rle/rle_exp.hpp:24-30): 3 = UNCOMPRESSED with the run values as a chain, 4 = the */
+/* same for DELTA_NO_PATCH, 5 = DELTA_NO_PATCH with run values and position lane bases as chains (RPN :76-111) */
+enum { RLE_REC = 3, RLE_DELTA_REC = 4, RLE_DELTA_REC2 = 5 };
+/* DICT var_t (include/fls/cor/exp/dict/dict_exp.hpp:30-42), shared by DICT_FRQ and DICT_VAL */
+enum { DICT_FFOR_LL = 3, DICT_FFOR_S = 4, DICT_FFOR_B = 5, DICT_FFOR_S_REC = 6, DICT_DELTA = 7, DICT_DELTA_LL = 8,
+       DICT_DELTA_S = 9, DICT_DELTA_B = 10, DICT_DELTA_B_REC = 11 };
 #define EXC_LIMIT 100 /* CFG::CMPR::EXC_LIMIT_C, include/fls/cfg/cfg.hpp */
-/* Pool ids and (ExpT, var) per src/cor/exp_pool.cpp:23-65 (i64) and :66-100 (str); arr_c from the RPN tables. */
-static int expr_lookup(uint8_t dtype, uint8_t id, expr_t* e) {
-    if (dtype == DT_STR) {
-        if (id == 13) { *e = (expr_t){EXP_FSST, 1, 2, 1}; return 0; }
-        return -1;
-    }
-    switch (id) {
-    case 0: *e = (expr_t){EXP_UNCOMPRESSED, 1, 1, 0}; return 0;
-    case 1: *e = (expr_t){EXP_FFOR, 1, 3, 0}; return 0;
-    /* patched FFOR, 32/64-bit integers only (ffor_patch_analyze resolves u32/u64, ffor_analyze.cpp:145-160) */
-    case 2: *e = (expr_t){EXP_FFOR, FFOR_S, 5, 0}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE ? 0 : -1;
-    case 3: *e = (expr_t){EXP_FFOR, FFOR_S_REC, 5, 0}; return dtype_bits(dtype) == 64 && dtype != DT_DOUBLE ? 0 : -1;
-    case 4: *e = (expr_t){EXP_FFOR, FFOR_B, 5, 0}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE ? 0 : -1;
-    case 5: *e = (expr_t){EXP_FFOR, FFOR_LL, 5, 0}; return dtype_bits(dtype) >= 32 && dtype != DT_DOUBLE ? 0 : -1;
-    case 6: *e = (expr_t){EXP_RLE, 2, 5, 0}; return 0;
-    case 7: *e = (expr_t){EXP_RLE, 1, 2, 0}; return 0;
-    case 11: *e = (expr_t){EXP_DELTA, 1, 4, 0}; return 0;
-    case 16: *e = (expr_t){EXP_DICT, 1, 1, 1}; return 0;
-    case 17: *e = (expr_t){EXP_DICT, 2, 3, 1}; return 0;
-    case 22: *e = (expr_t){EXP_DICT, 7, 4, 1}; return 0;
-    case 27: *e = (expr_t){EXP_DICT_RLE, DICT_RLE_REC, 5, 1}; return dtype != DT_DOUBLE ? 0 : -1; /* exp_pool.cpp:53 */
-    case 28: *e = (expr_t){EXP_DICT_FRQ, 1, 1, 1}; return 0;
-    case 29: *e = (expr_t){EXP_DICT_FRQ, 2, 3, 1}; return 0;
-    default: return -1;
-    }
-}
+
+static int is_dict_family(uint8_t t) { return t == EXP_DICT || t == EXP_DICT_FRQ || t == EXP_DICT_VAL; }
+
+/* Buffer count per (ExpT, var) from the compress RPNs' des_buf_c. */
 static int expr_from_page(uint8_t exp_t, uint8_t var, expr_t* e) {
+    unsigned arr = 0, dict = 0;
     switch (exp_t) {
-    case EXP_UNCOMPRESSED: *e = (expr_t){exp_t, var, 1, 0}; return var == 1 ? 0 : -1;
-    case EXP_FFOR:
-        if (var == 1) { *e = (expr_t){exp_t, var, 3, 0}; return 0; }
-        if (var == FFOR_LL || var == FFOR_S || var == FFOR_B || var == FFOR_S_REC) { *e = (expr_t){exp_t, var, 5, 0}; return 0; }
-        return -1;
-    case EXP_DELTA: *e = (expr_t){exp_t, var, 4, 0}; return var == 1 ? 0 : -1;
-    case EXP_RLE:
-        if (var == 1) { *e = (expr_t){exp_t, var, 2, 0}; return 0; }
-        if (var == 2) { *e = (expr_t){exp_t, var, 5, 0}; return 0; }
-        return -1;
+    case EXP_UNCOMPRESSED: arr = var == 1 ? 1 : 0; break;
+    case EXP_FFOR: arr = var == 1 ? 3 : (var >= FFOR_LL && var <= FFOR_S_REC ? 5 : 0); break;
+    case EXP_DELTA: arr = var == 1 ? 4 : (var >= DELTA_LL && var <= DELTA_B_REC ? 6 : 0); break;
+    case EXP_RLE: arr = var == 1 || var == RLE_REC ? 2 : (var == 2 || var == RLE_DELTA_REC || var == RLE_DELTA_REC2 ? 5 : 0); break;
     case EXP_DICT:
     case EXP_DICT_FRQ:
     case EXP_DICT_VAL:
-        if (var == 1) { *e = (expr_t){exp_t, var, 1, 1}; return 0; }
-        if (var == 2) { *e = (expr_t){exp_t, var, 3, 1}; return 0; }
-        if (var == 7) { *e = (expr_t){exp_t, var, 4, 1}; return 0; }
-        return -1;
-    case EXP_DICT_RLE: *e = (expr_t){exp_t, var, 5, 1}; return var == DICT_RLE_REC ? 0 : -1;
-    case EXP_FSST: *e = (expr_t){exp_t, var, 2, 1}; return var == 1 ? 0 : -1;
-    default: return -1;
+        dict = 1;
+        arr  = var == 1 ? 1 : var == 2 ? 3 : var <= DICT_FFOR_S_REC ? 5 : var == DICT_DELTA ? 4 : var <= DICT_DELTA_B_REC ? 6 : 0;
+        if (var == 0) arr = 0;
+        break;
+    case EXP_DICT_RLE: arr = var == DICT_RLE_REC ? 5 : 0, dict = 1; break;
+    case EXP_FSST: arr = var == 1 ? 2 : 0, dict = 1; break;
+    default: break;
     }
+    if (!arr) return -1;
+    *e = (expr_t){exp_t, var, (uint8_t)arr, (uint8_t)dict};
+    return 0;
+}
+/* Patch step in FFOR numbering (FFOR_LL / FFOR_S / FFOR_B), 0 = none. */
+static uint8_t expr_patch(const expr_t* e) {
+    switch (e->exp_t) {
+    case EXP_FFOR: return e->var == 1 ? 0 : e->var == FFOR_S_REC ? FFOR_S : e->var;
+    case EXP_DELTA: return e->var == DELTA_LL ? FFOR_LL : e->var == DELTA_S ? FFOR_S : e->var >= DELTA_B ? FFOR_B : 0;
+    case EXP_DICT:
+    case EXP_DICT_FRQ:
+    case EXP_DICT_VAL:
+        if (e->var == DICT_FFOR_LL || e->var == DICT_DELTA_LL) return FFOR_LL;
+        if (e->var == DICT_FFOR_S || e->var == DICT_FFOR_S_REC || e->var == DICT_DELTA_S) return FFOR_S;
+        if (e->var == DICT_FFOR_B || e->var == DICT_DELTA_B || e->var == DICT_DELTA_B_REC) return FFOR_B;
+        return 0;
+    default: return 0;
+    }
+}
+/* Values pass RSUM: DELTA and the dictionary DELTA_* variants (their compress RPNs hold prm_t::RSUM). */
+static int expr_rsum(const expr_t* e) { return e->exp_t == EXP_DELTA || (is_dict_family(e->exp_t) && e->var >= DICT_DELTA); }
+/* Buffers whose RPN is_compressed flag is set (bit b = buffer b): one HZL chain per page. */
+static unsigned expr_rec(const expr_t* e) {
+    switch (e->exp_t) {
+    case EXP_FFOR: return e->var == FFOR_S_REC;
+    case EXP_DELTA: return e->var == DELTA_B_REC ? 2 : 0;
+    case EXP_RLE: return e->var == RLE_REC || e->var == RLE_DELTA_REC ? 1 : e->var == RLE_DELTA_REC2 ? 3 : 0;
+    case EXP_DICT:
+    case EXP_DICT_FRQ:
+    case EXP_DICT_VAL: return e->var == DICT_FFOR_S_REC ? 1 : e->var == DICT_DELTA_B_REC ? 2 : 0;
+    case EXP_DICT_RLE: return 1;
+    default: return 0;
+    }
+}
+/* Chain value width: THzlCmp<idx_t> for dictionary expressions, THzlCmp<uint64_t> otherwise (hzl_cmp_fac.cpp:15-35). */
+static unsigned expr_rec_width(const expr_t* e) { return e->dict ? 32 : 64; }
+/* FFOR-shaped vectors: [lane bases][exceptions][positions][bw][base][packed] (FFOR, DELTA, dictionary codes). */
+static int expr_ffor_shaped(const expr_t* e) {
+    return e->exp_t == EXP_FFOR || e->exp_t == EXP_DELTA || (is_dict_family(e->exp_t) && e->var != 1);
+}
+/* Patches resolve for 32/64-bit values only (patch ResolveCompressFunc); chains of u64 need 64-bit values here. */
+static int expr_fits(uint8_t dtype, const expr_t* e) {
+    unsigned W = dtype_bits(dtype);
+    if (dtype == DT_STR) return e->exp_t == EXP_FSST;
+    if (!W || e->exp_t == EXP_FSST) return 0;
+    if (dtype == DT_DOUBLE) return e->exp_t == EXP_UNCOMPRESSED;
+    if (!e->dict && expr_patch(e) && W < 32) return 0;
+    if (!e->dict && expr_rec(e) && W != 64) return 0;
+    return 1;
+}
+/* Pool ids and (ExpT, var) per src/cor/exp_pool.cpp:26-64 (i64) and :66-100 (str). */
+static int expr_lookup(uint8_t dtype, uint8_t id, expr_t* e) {
+    static const uint8_t pool[39][2] = {
+        {EXP_UNCOMPRESSED, 1}, {EXP_FFOR, 1}, {EXP_FFOR, FFOR_S}, {EXP_FFOR, FFOR_S_REC}, {EXP_FFOR, FFOR_B},
+        {EXP_FFOR, FFOR_LL}, {EXP_RLE, 2}, {EXP_RLE, 1}, {EXP_RLE, RLE_REC}, {EXP_RLE, RLE_DELTA_REC},
+        {EXP_RLE, RLE_DELTA_REC2}, {EXP_DELTA, 1}, {EXP_DELTA, DELTA_S}, {EXP_DELTA, DELTA_B}, {EXP_DELTA, DELTA_LL},
+        {EXP_DELTA, DELTA_B_REC}, {EXP_DICT, 1}, {EXP_DICT, 2}, {EXP_DICT, DICT_FFOR_B}, {EXP_DICT, DICT_FFOR_S},
+        {EXP_DICT, DICT_FFOR_LL}, {EXP_DICT, DICT_FFOR_S_REC}, {EXP_DICT, DICT_DELTA}, {EXP_DICT, DICT_DELTA_S},
+        {EXP_DICT, DICT_DELTA_LL}, {EXP_DICT, DICT_DELTA_B}, {EXP_DICT, DICT_DELTA_B_REC},
+        {EXP_DICT_RLE, DICT_RLE_REC}, {EXP_DICT_FRQ, 1}, {EXP_DICT_FRQ, 2}, {EXP_DICT_FRQ, DICT_FFOR_B},
+        {EXP_DICT_FRQ, DICT_FFOR_S}, {EXP_DICT_FRQ, DICT_FFOR_LL}, {EXP_DICT_FRQ, DICT_FFOR_S_REC},
+        {EXP_DICT_FRQ, DICT_DELTA}, {EXP_DICT_FRQ, DICT_DELTA_S}, {EXP_DICT_FRQ, DICT_DELTA_LL},
+        {EXP_DICT_FRQ, DICT_DELTA_B}, {EXP_DICT_FRQ, DICT_DELTA_B_REC}};
+    if (dtype == DT_STR) return id == 13 && !expr_from_page(EXP_FSST, 1, e) ? 0 : -1;
+    if (id >= 39 || expr_from_page(pool[id][0], pool[id][1], e)) return -1;
+    return expr_fits(dtype, e) ? 0 : -1;
 }
 
 /* ------------------------------------------------------------------------------------------------------------- */
@@ -308,12 +368,13 @@ typedef struct {
     buf_t    bufs[MAX_ARR];
     buf_t    eps;
     uint32_t ep[MAX_ARR];
-    unsigned arr_c, vec_c;
+    unsigned arr_c, vec_c, rec_c; /* rec_c: buffers written as HZL chains */
 } page_w;
 
-static void page_begin(page_w* pg, unsigned arr_c) {
+static void page_begin(page_w* pg, unsigned arr_c, unsigned rec_c) {
     memset(pg, 0, sizeof(*pg));
     pg->arr_c = arr_c;
+    pg->rec_c = rec_c;
 }
 static int page_absorb(page_w* pg, const uint8_t* const* parts, const uint64_t* sizes) {
     if (pg->vec_c >= 1 && buf_put(&pg->eps, pg->ep, 4ULL * pg->arr_c)) return -1;
@@ -324,8 +385,10 @@ static int page_absorb(page_w* pg, const uint8_t* const* parts, const uint64_t* 
     pg->vec_c++;
     return 0;
 }
+/* Page::m_ttl_sz (src/cor/lyt/page/page.cpp:44-56, :87-104): the chains count their 12-byte record header and    */
+/* their uncompressed bytes.                                                                                       */
 static uint64_t page_size(const page_w* pg) {
-    uint64_t s = 8 + 4ULL * pg->arr_c + pg->eps.n;
+    uint64_t s = 8 + 4ULL * pg->arr_c + pg->eps.n + 12ULL * pg->rec_c;
     for (unsigned i = 0; i < pg->arr_c; ++i) s += pg->bufs[i].n;
     return s;
 }
@@ -534,8 +597,8 @@ typedef struct {
 
 /* FFOR chain on a (already transposed) vector of width W into parts[o..o+2]. */
 static void enc_ffor_parts(unsigned W, const void* t, vec_parts* vp, unsigned o, uint8_t* bwbuf, uint8_t* basebuf,
-                           uint8_t* packed) {
-    uint8_t bw = orc_ffor_analyze(W, t, basebuf);
+                           uint8_t* packed, int sgn) {
+    uint8_t bw = ffor_analyze_typed(W, t, basebuf, sgn);
     bwbuf[0]   = bw;
     orc_ffor(W, t, packed, bw, basebuf);
     vp->parts[o] = bwbuf, vp->s[o] = 1;
@@ -564,7 +627,7 @@ static int cmp_u64(const void* a, const void* b) {
     uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
     return x < y ? -1 : x > y;
 }
-static void enc_patch(unsigned W, uint8_t var, uint8_t* t, vec_parts* vp, uint8_t* bwb, uint8_t* baseb) {
+static void enc_patch(unsigned W, uint8_t var, uint8_t* t, vec_parts* vp, unsigned o, uint8_t* bwb, uint8_t* baseb) {
     const unsigned sz = W / 8;
     uint64_t       srt[1024], val[1024], rep[1024];
     unsigned       nd = 0;
@@ -606,31 +669,49 @@ static void enc_patch(unsigned W, uint8_t var, uint8_t* t, vec_parts* vp, uint8_
         memcpy(pos, chain, 2 * m), n = m;
     }
     for (unsigned k = 0; k < n; ++k) st(W, vp->b0, k, ld(W, t, pos[k]));
-    vp->parts[0] = vp->b0, vp->s[0] = (uint64_t)n * sz;
+    vp->parts[o] = vp->b0, vp->s[o] = (uint64_t)n * sz;
     uint16_t n16 = (uint16_t)n;
     if (var == FFOR_B) {
         memset(vp->b1, 0, 128);
         for (unsigned k = 0; k < n; ++k) vp->b1[pos[k] / 8] |= (uint8_t)(1u << (pos[k] % 8));
-        vp->parts[1] = vp->b1, vp->s[1] = 128;
+        vp->parts[o + 1] = vp->b1, vp->s[o + 1] = 128;
     } else if (var == FFOR_S) {
         memcpy(vp->b1, &n16, 2), memcpy(vp->b1 + 2, pos, 2ULL * n);
-        vp->parts[1] = vp->b1, vp->s[1] = 2 + 2ULL * n;
+        vp->parts[o + 1] = vp->b1, vp->s[o + 1] = 2 + 2ULL * n;
     } else {
         uint16_t first = n ? pos[0] : 0;
         memcpy(vp->b1, &n16, 2), memcpy(vp->b1 + 2, &first, 2);
-        vp->parts[1] = vp->b1, vp->s[1] = 4;
+        vp->parts[o + 1] = vp->b1, vp->s[o + 1] = 4;
         for (unsigned k = 0; k < n; ++k) st(W, t, pos[k], best_base + (k + 1 < n ? (uint64_t)(pos[k + 1] - pos[k]) : 1) - 1);
     }
     bwb[0] = (uint8_t)best_bw;
     memset(baseb, 0, 8), memcpy(baseb, &best_base, sz);
     orc_ffor(W, t, vp->b2, best_bw, baseb);
-    vp->parts[2] = bwb, vp->s[2] = 1;
-    vp->parts[3] = baseb, vp->s[3] = sz;
-    vp->parts[4] = vp->b2, vp->s[4] = 128ULL * best_bw;
+    vp->parts[o + 2] = bwb, vp->s[o + 2] = 1;
+    vp->parts[o + 3] = baseb, vp->s[o + 3] = sz;
+    vp->parts[o + 4] = vp->b2, vp->s[o + 4] = 128ULL * best_bw;
 }
 
-static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint32_t* codes, vec_parts* vp) {
-    static __thread uint8_t t[8 * 1024], d[8 * 1024], bwb[8], baseb[8];
+/* FFOR-shaped vector from transposed W-bit values t (clobbered): [lane bases] + unrsum when the expression has  */
+/* RSUM (rsum_compress.cpp:10-22), then the patch step or plain ANALYZE + FFOR.                                 */
+static void enc_shaped(unsigned W, const expr_t* e, uint8_t* t, vec_parts* vp, uint8_t* bwb, uint8_t* baseb, int sgn) {
+    static __thread uint8_t d[8 * 1024];
+    unsigned                o = 0;
+    if (expr_rsum(e)) {
+        memcpy(vp->b3, t, 128); /* lane bases: the first row */
+        vp->parts[0] = vp->b3, vp->s[0] = 128;
+        orc_unrsum(W, t, d);
+        memcpy(t, d, 128ULL * W);
+        o = 1;
+    }
+    uint8_t pk = expr_patch(e);
+    if (pk) enc_patch(W, pk, t, vp, o, bwb, baseb);
+    else enc_ffor_parts(W, t, vp, o, bwb, baseb, vp->b2, sgn);
+}
+
+/* sgn: signed ANALYZE of the column values (int64 columns) */
+static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint32_t* codes, vec_parts* vp, int sgn) {
+    static __thread uint8_t t[8 * 1024], bwb[8], baseb[8];
     const unsigned          sz = W / 8;
     switch (e->exp_t) {
     case EXP_UNCOMPRESSED:
@@ -638,74 +719,46 @@ static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint
         vp->parts[0] = vp->b0, vp->s[0] = 1024ULL * sz;
         break;
     case EXP_FFOR:
-        orc_transpose(W, vals, t);
-        if (e->var == 1) enc_ffor_parts(W, t, vp, 0, bwb, baseb, vp->b2);
-        else enc_patch(W, e->var == FFOR_S_REC ? FFOR_S : e->var, t, vp, bwb, baseb);
-        break;
     case EXP_DELTA:
         orc_transpose(W, vals, t);
-        memcpy(vp->b0, t, 128); /* lane bases (src/cor/prm/rsum/rsum_compress.cpp:15) */
-        vp->parts[0] = vp->b0, vp->s[0] = 128;
-        orc_unrsum(W, t, d);
-        enc_ffor_parts(W, d, vp, 1, bwb, baseb, vp->b3);
+        enc_shaped(W, e, t, vp, bwb, baseb, sgn);
         break;
-    case EXP_RLE: {
-        uint16_t pos[1024], posT[1024], dl[1024];
-        uint64_t r = 0;
+    case EXP_RLE:
+    case EXP_DICT_RLE: { /* DICT_RLE: Vec::Init's run split of the codes (src/cor/lyt/vec.cpp:72-96) */
+        uint16_t       pos[1024], posT[1024], dl[1024];
+        uint64_t       r  = 0;
+        const unsigned rw = e->exp_t == EXP_DICT_RLE ? 32 : W;
         for (unsigned i = 0; i < 1024; ++i) {
-            uint64_t v = ld(W, vals, i);
-            if (i == 0 || v != ld(W, vals, i - 1)) {
+            uint64_t x = rw == 32 && codes ? codes[i] : ld(W, vals, i);
+            uint64_t p = i ? (rw == 32 && codes ? codes[i - 1] : ld(W, vals, i - 1)) : 0;
+            if (i == 0 || x != p) {
                 if (i) r++;
-                st(W, vp->b0, r, v);
+                st(rw, vp->b0, r, x);
             }
             pos[i] = (uint16_t)r;
         }
-        vp->parts[0] = vp->b0, vp->s[0] = (r + 1) * sz;
+        vp->parts[0] = vp->b0, vp->s[0] = (r + 1) * (rw / 8);
         orc_transpose(16, pos, posT);
-        if (e->var == 1) {
+        if (e->var == 1 || e->var == RLE_REC) {
             memcpy(vp->b1, posT, 2048);
             vp->parts[1] = vp->b1, vp->s[1] = 2048;
         } else {
             memcpy(vp->b1, posT, 128);
             vp->parts[1] = vp->b1, vp->s[1] = 128;
             orc_unrsum(16, posT, dl);
-            enc_ffor_parts(16, dl, vp, 2, bwb, baseb, vp->b4);
+            enc_ffor_parts(16, dl, vp, 2, bwb, baseb, vp->b4, 0);
         }
         break;
     }
-    case EXP_DICT_RLE: { /* Vec::Init's run split of the codes (src/cor/lyt/vec.cpp:72-96), then RLE positions */
-        uint16_t pos[1024], posT[1024], dl[1024];
-        uint64_t r = 0;
-        for (unsigned i = 0; i < 1024; ++i) {
-            if (i == 0 || codes[i] != codes[i - 1]) {
-                if (i) r++;
-                st(32, vp->b0, r, codes[i]);
-            }
-            pos[i] = (uint16_t)r;
-        }
-        vp->parts[0] = vp->b0, vp->s[0] = (r + 1) * 4;
-        orc_transpose(16, pos, posT);
-        memcpy(vp->b1, posT, 128);
-        vp->parts[1] = vp->b1, vp->s[1] = 128;
-        orc_unrsum(16, posT, dl);
-        enc_ffor_parts(16, dl, vp, 2, bwb, baseb, vp->b4);
-        break;
-    }
-    default: { /* dictionary: codes are u32 */
+    default: /* dictionary: codes are u32 */
         orc_transpose(32, codes, t);
         if (e->var == 1) {
             memcpy(vp->b0, t, 4096);
             vp->parts[0] = vp->b0, vp->s[0] = 4096;
-        } else if (e->var == 2) {
-            enc_ffor_parts(32, t, vp, 0, bwb, baseb, vp->b2);
         } else {
-            memcpy(vp->b0, t, 128);
-            vp->parts[0] = vp->b0, vp->s[0] = 128;
-            orc_unrsum(32, t, d);
-            enc_ffor_parts(32, d, vp, 1, bwb, baseb, vp->b3);
+            enc_shaped(32, e, t, vp, bwb, baseb, 0);
         }
         break;
-    }
     }
 }
 
@@ -733,12 +786,14 @@ static int64_t encode_impl(uint8_t dtype, uint8_t expr_id, const void* in, uint6
     uint64_t   v      = 0;
     int        rc     = 0;
     while (!rc) {
-        page_w pg;
-        page_begin(&pg, e.arr_c);
+        page_w   pg;
+        unsigned rec = expr_rec(&e);
+        page_begin(&pg, e.arr_c, (unsigned)__builtin_popcount(rec));
         uint64_t stop = v + per_pg;
         if (stop > n_vec) stop = n_vec;
         while (v < stop) {
-            enc_vector(W, &e, (const uint8_t*)in + v * 1024 * (W / 8), codes ? codes + v * 1024 : NULL, vp);
+            enc_vector(W, &e, (const uint8_t*)in + v * 1024 * (W / 8), codes ? codes + v * 1024 : NULL, vp,
+                       dtype == DT_I64);
             uint64_t add = 4ULL * e.arr_c;
             for (unsigned i = 0; i < e.arr_c; ++i) add += vp->s[i];
             if (pg.vec_c && page_size(&pg) + add >= DATA_PG_SZ) break; /* TryFlushVec limit, compressor.cpp:47 */
@@ -747,17 +802,18 @@ static int64_t encode_impl(uint8_t dtype, uint8_t expr_id, const void* in, uint6
         }
         int more = v < n_vec;
         if (!rc && e.dict) rc = dict_page_write(W, dvals, (uint64_t)ne, out);
-        if (!rc && (e.exp_t == EXP_DICT_RLE || (e.exp_t == EXP_FFOR && e.var == FFOR_S_REC)) && pg.vec_c) {
-            /* THzlCmp<idx_t> (DICT_RLE run codes) / THzlCmp<uint64_t> (exception values): compress_buf +
-             * finalize_buf of buffer 0 */
-            unsigned rw  = e.exp_t == EXP_DICT_RLE ? 32 : 64;
-            uint64_t nc  = pg.bufs[0].n / (rw / 8);
-            int64_t  hsz = orc_hzl_encode(rw, pg.bufs[0].p, nc, NULL);
+        /* buffers whose RPN is_compressed flag is set become one HZL chain each: THzlCmp<idx_t> for dictionary */
+        /* expressions, THzlCmp<uint64_t> otherwise; compress_buf + finalize_buf (hzl_cmp.cpp:43-176)             */
+        for (unsigned b = 0; b < 2 && !rc && pg.vec_c; ++b) {
+            if (!(rec >> b & 1)) continue;
+            unsigned rw  = expr_rec_width(&e);
+            uint64_t nc  = pg.bufs[b].n / (rw / 8);
+            int64_t  hsz = orc_hzl_encode(rw, pg.bufs[b].p, nc, NULL);
             buf_t    h   = {0};
             h.p = (uint8_t*)malloc((size_t)hsz + 8), h.n = (uint64_t)hsz, h.cap = (uint64_t)hsz + 8;
-            orc_hzl_encode(rw, pg.bufs[0].p, nc, h.p);
-            buf_free(&pg.bufs[0]);
-            pg.bufs[0] = h;
+            orc_hzl_encode(rw, pg.bufs[b].p, nc, h.p);
+            buf_free(&pg.bufs[b]);
+            pg.bufs[b] = h;
         }
         if (!rc) rc = page_flush(&pg, e.exp_t, e.var, more, out);
         page_free(&pg);
@@ -794,14 +850,14 @@ typedef struct {
     unsigned       vec_c, arr_c;
     uint32_t       off[MAX_ARR];
     const uint8_t* eps;
-    uint8_t*       runs;   /* DICT_RLE: decoded run codes of the page (u32) */
-    uint64_t       runs_n; /* their count */
+    uint8_t*       dec[2];   /* buffers stored as HZL chains (expr_rec): their decoded bytes */
+    uint64_t       dec_n[2]; /* and byte counts */
 } page_r;
 
 static int page_open(const uint8_t* p, uint64_t avail, page_r* pg, expr_t* e) {
     if (avail < 8) return -1;
     pg->p     = p;
-    pg->runs  = NULL, pg->runs_n = 0;
+    pg->dec[0] = pg->dec[1] = NULL, pg->dec_n[0] = pg->dec_n[1] = 0;
     pg->exp_t = p[0], pg->var = p[1];
     pg->vec_c = (unsigned)p[4] | ((unsigned)p[5] << 8);
     pg->more  = p[6];
@@ -814,10 +870,19 @@ static int page_open(const uint8_t* p, uint64_t avail, page_r* pg, expr_t* e) {
         if (pg->off[i] > avail) return -1;
     return 0;
 }
-static inline const uint8_t* vbuf(const page_r* pg, unsigned v, unsigned i) {
+static inline uint32_t ep_at(const page_r* pg, unsigned v, unsigned i) {
     uint32_t ep = 0;
     if (v) memcpy(&ep, pg->eps + 4ULL * ((uint64_t)(v - 1) * pg->arr_c + i), 4);
-    return pg->p + pg->off[i] + ep;
+    return ep;
+}
+/* Vector v's slice of buffer i; a chained buffer's entry points address its decoded bytes. */
+static inline const uint8_t* vbuf(const page_r* pg, unsigned v, unsigned i) {
+    if (i < 2 && pg->dec[i]) return pg->dec[i] + ep_at(pg, v, i);
+    return pg->p + pg->off[i] + ep_at(pg, v, i);
+}
+/* Whether n bytes at vector v's slice of chained buffer i lie inside its decoded bytes (page buffers: always). */
+static inline int vbuf_fits(const page_r* pg, unsigned v, unsigned i, uint64_t n) {
+    return !(i < 2 && pg->dec[i]) || (uint64_t)ep_at(pg, v, i) + n <= pg->dec_n[i];
 }
 
 static int unffor_parts(unsigned W, const page_r* pg, unsigned v, unsigned o, void* out) {
@@ -834,123 +899,127 @@ static uint64_t page_end(unsigned W, const page_r* pg, const expr_t* e) {
     unsigned       v    = pg->vec_c - 1, last = pg->arr_c - 1;
     const uint8_t* q    = vbuf(pg, v, last);
     uint64_t       tail = 0;
-    switch (e->exp_t) {
-    case EXP_UNCOMPRESSED: tail = 1024ULL * (W / 8); break;
-    case EXP_FFOR:
-    case EXP_DELTA: tail = 128ULL * *vbuf(pg, v, last - 2); break;
-    case EXP_RLE: tail = e->var == 1 ? 2048 : 128ULL * *vbuf(pg, v, last - 2); break;
-    case EXP_FSST: tail = 4096; break;
-    default: tail = e->var == 1 ? 4096 : 128ULL * *vbuf(pg, v, last - 2); break;
-    }
+    if (expr_ffor_shaped(e)) tail = 128ULL * *vbuf(pg, v, last - 2);
+    else switch (e->exp_t) {
+        case EXP_UNCOMPRESSED: tail = 1024ULL * (W / 8); break;
+        case EXP_RLE:
+        case EXP_DICT_RLE: tail = e->var == 1 || e->var == RLE_REC ? 2048 : 128ULL * *vbuf(pg, v, last - 2); break;
+        default: tail = 4096; break; /* FSST offsets, dictionary codes */
+        }
     return (uint64_t)(q - pg->p) + tail;
 }
 
-/* Patched FFOR vector into transposed t: unffor of [bw][base][packed] (buffers 2..4), then the patch step
- * (S: s_patch_decompress.cpp:13-30; B: patch.cpp:71-83; LL: ll_patch_decompress.cpp:13-42, where each followed
- * slot's decoded value minus base plus one is the distance to the next exception). Positions outside the vector
- * are a format error here (the reference would write out of bounds). */
-static int decode_patched(unsigned W, const page_r* pg, const expr_t* e, unsigned v, uint8_t* t) {
-    if (unffor_parts(W, pg, v, 2, t)) return -1;
-    const uint8_t* exc = vbuf(pg, v, 0);
-    uint8_t        var = e->var;
-    if (var == FFOR_S_REC) { /* exception values from the page's decoded chain, at the vector's entry point */
-        uint32_t x0 = 0;
-        if (v) memcpy(&x0, pg->eps + 4ULL * ((uint64_t)(v - 1) * pg->arr_c), 4);
-        if (x0 > pg->runs_n * 8) return -1;
-        exc = pg->runs + x0;
-        var = FFOR_S;
+/* Patched FFOR of CW-bit values into transposed t: unffor of [bw][base][packed] (buffers o+2..o+4), then the patch
+ * step over [exceptions] (o) and [positions] (o+1) (S: s_patch_decompress.cpp:13-30; B: patch.cpp:71-83;
+ * LL: ll_patch_decompress.cpp:13-42, where each followed slot's decoded value minus base plus one is the distance
+ * to the next exception). Positions outside the vector are a format error here (the reference would write out
+ * of bounds). */
+static int decode_patched(unsigned CW, const page_r* pg, uint8_t pk, unsigned v, unsigned o, uint8_t* t) {
+    if (unffor_parts(CW, pg, v, o + 2, t)) return -1;
+    const uint8_t* exc = vbuf(pg, v, o);
+    const uint8_t* p1  = vbuf(pg, v, o + 1);
+    const unsigned sz  = CW / 8;
+    uint64_t       base = 0, x = 0, c = 0;
+    memcpy(&base, vbuf(pg, v, o + 3), sz);
+    if (pk == FFOR_S) {
+        uint16_t c16;
+        memcpy(&c16, p1, 2), c = c16;
+    } else if (pk == FFOR_B) {
+        for (unsigned w = 0; w < 16; ++w) {
+            uint64_t unit;
+            memcpy(&unit, p1 + 8 * w, 8), c += (uint64_t)__builtin_popcountll(unit);
+        }
+    } else {
+        uint16_t c16;
+        memcpy(&c16, p1, 2), c = c16;
     }
-    const uint8_t* p1  = vbuf(pg, v, 1);
-    const unsigned sz  = W / 8;
-    uint64_t       base = 0, x = 0;
-    memcpy(&base, vbuf(pg, v, 3), sz);
-    if (var == FFOR_S) {
-        uint16_t c, q;
-        memcpy(&c, p1, 2);
+    if (!vbuf_fits(pg, v, o, c * sz)) return -1;
+    if (pk == FFOR_S) {
+        uint16_t q;
         for (unsigned i = 0; i < c; ++i) {
             memcpy(&q, p1 + 2 + 2 * i, 2);
             if (q >= 1024) return -1;
-            memcpy(&x, exc + (size_t)i * sz, sz), st(W, t, q, x);
+            memcpy(&x, exc + (size_t)i * sz, sz), st(CW, t, q, x);
         }
-    } else if (var == FFOR_B) {
-        unsigned c = 0;
+    } else if (pk == FFOR_B) {
+        unsigned k = 0;
         for (unsigned w = 0; w < 16; ++w) {
             uint64_t unit;
             memcpy(&unit, p1 + 8 * w, 8);
             while (unit) {
                 unsigned r = (unsigned)__builtin_ctzll(unit);
-                memcpy(&x, exc + (size_t)c++ * sz, sz), st(W, t, 64 * w + r, x);
+                memcpy(&x, exc + (size_t)k++ * sz, sz), st(CW, t, 64 * w + r, x);
                 unit &= unit - 1;
             }
         }
     } else {
-        uint16_t c, first;
-        memcpy(&c, p1, 2), memcpy(&first, p1 + 2, 2);
+        uint16_t first;
+        memcpy(&first, p1 + 2, 2);
         uint64_t cur = first;
         for (unsigned i = 0; i < c; ++i) {
             if (cur >= 1024) return -1;
-            uint64_t off = (ld(W, t, cur) - base + 1) & wmask(W);
-            memcpy(&x, exc + (size_t)i * sz, sz), st(W, t, cur, x);
-            cur = (cur + off) & wmask(W);
+            uint64_t off = (ld(CW, t, cur) - base + 1) & wmask(CW);
+            memcpy(&x, exc + (size_t)i * sz, sz), st(CW, t, cur, x);
+            cur = (cur + off) & wmask(CW);
         }
     }
     return 0;
 }
 
+/* FFOR-shaped vector of CW-bit values into transposed t: FFOR (+ patch), then RSUM with the lane bases. */
+static int decode_shaped(unsigned CW, const page_r* pg, const expr_t* e, unsigned v, uint8_t* t) {
+    uint8_t  d[8 * 1024];
+    unsigned o   = expr_rsum(e) ? 1 : 0;
+    uint8_t  pk  = expr_patch(e);
+    uint8_t* dst = o ? d : t;
+    if (pk ? decode_patched(CW, pg, pk, v, o, dst) : unffor_parts(CW, pg, v, o, dst)) return -1;
+    if (o) orc_rsum(CW, d, t, vbuf(pg, v, 0));
+    return 0;
+}
+
 static int decode_vector(unsigned W, const page_r* pg, const expr_t* e, unsigned v, const uint64_t* dict,
                          uint64_t dict_n, void* out) {
-    uint8_t        t[8 * 1024], d[8 * 1024];
+    uint8_t        t[8 * 1024];
     const unsigned sz = W / 8;
     switch (e->exp_t) {
     case EXP_UNCOMPRESSED: memcpy(t, vbuf(pg, v, 0), 1024ULL * sz); break;
     case EXP_FFOR:
-        if (e->var == 1) {
-            if (unffor_parts(W, pg, v, 0, t)) return -1;
-        } else if (decode_patched(W, pg, e, v, t)) return -1;
-        break;
     case EXP_DELTA:
-        if (unffor_parts(W, pg, v, 1, d)) return -1;
-        orc_rsum(W, d, t, vbuf(pg, v, 0));
+        if (decode_shaped(W, pg, e, v, t)) return -1;
         break;
-    case EXP_RLE: {
-        uint16_t posT[1024], dl[1024];
-        if (e->var == 1) memcpy(posT, vbuf(pg, v, 1), 2048);
+    case EXP_RLE:
+    case EXP_DICT_RLE: { /* RLE positions, then the run value (vec.cpp:318-332); DICT_RLE: dict[run code]     */
+                         /* (include/fls/encoder/assert_eq.hpp:329-400)                                         */
+        uint16_t       posT[1024], dl[1024];
+        const unsigned rsz = e->exp_t == EXP_DICT_RLE ? 4 : sz;
+        uint32_t       r0  = ep_at(pg, v, 0), r1;
+        uint64_t       rb  = pg->dec[0] ? pg->dec_n[0] : pg->off[1] - pg->off[0];
+        r1                 = v + 1 < pg->vec_c ? ep_at(pg, v + 1, 0) : (uint32_t)rb;
+        if (r0 % rsz || r1 < r0 + rsz || r1 > rb) return -1;
+        if (e->var == 1 || e->var == RLE_REC) memcpy(posT, vbuf(pg, v, 1), 2048);
         else {
-            if (unffor_parts(16, pg, v, 2, dl)) return -1;
+            if (!vbuf_fits(pg, v, 1, 128) || unffor_parts(16, pg, v, 2, dl)) return -1;
             orc_rsum(16, dl, posT, vbuf(pg, v, 1));
         }
-        const uint8_t* runs = vbuf(pg, v, 0);
-        for (unsigned k = 0; k < 1024; ++k) memcpy(t + (size_t)k * sz, runs + (size_t)posT[k] * sz, sz); /* vec.cpp:318-332 */
-        break;
-    }
-    case EXP_DICT_RLE: { /* RLE positions, then dict[run code] (include/fls/encoder/assert_eq.hpp:329-400) */
-        uint16_t posT[1024], dl[1024];
-        uint32_t r0 = 0, r1;
-        if (v) memcpy(&r0, pg->eps + 4ULL * ((uint64_t)(v - 1) * pg->arr_c), 4);
-        if (v + 1 < pg->vec_c) memcpy(&r1, pg->eps + 4ULL * ((uint64_t)v * pg->arr_c), 4);
-        else r1 = (uint32_t)(pg->runs_n * 4);
-        if (r0 % 4 || r1 < r0 + 4 || r1 > pg->runs_n * 4) return -1;
-        if (unffor_parts(16, pg, v, 2, dl)) return -1;
-        orc_rsum(16, dl, posT, vbuf(pg, v, 1));
-        const uint32_t n_runs = (r1 - r0) / 4;
+        const uint8_t* runs   = vbuf(pg, v, 0);
+        const uint32_t n_runs = (r1 - r0) / rsz;
         for (unsigned k = 0; k < 1024; ++k) {
             if (posT[k] >= n_runs) return -1;
-            uint32_t code;
-            memcpy(&code, pg->runs + r0 + 4ULL * posT[k], 4);
-            if (code >= dict_n) return -1;
-            st(W, t, k, dict[code]);
+            if (e->exp_t == EXP_RLE) {
+                memcpy(t + (size_t)k * sz, runs + (size_t)posT[k] * sz, sz);
+            } else {
+                uint32_t code;
+                memcpy(&code, runs + 4ULL * posT[k], 4);
+                if (code >= dict_n) return -1;
+                st(W, t, k, dict[code]);
+            }
         }
         break;
     }
     default: { /* dictionary codes */
-        uint32_t c[1024], cd[1024];
+        uint32_t c[1024];
         if (e->var == 1) memcpy(c, vbuf(pg, v, 0), 4096);
-        else if (e->var == 2) {
-            if (unffor_parts(32, pg, v, 0, c)) return -1;
-        } else {
-            if (unffor_parts(32, pg, v, 1, cd)) return -1;
-            orc_rsum(32, cd, c, vbuf(pg, v, 0));
-        }
+        else if (decode_shaped(32, pg, e, v, (uint8_t*)c)) return -1;
         for (unsigned k = 0; k < 1024; ++k) {
             if (c[k] >= dict_n) return -1;
             st(W, t, k, dict[c[k]]);
@@ -997,21 +1066,24 @@ int64_t orc_decode_column(uint8_t dtype, const uint8_t* in, uint64_t in_size, vo
         expr_t e;
         if (page_open(in + pos, in_size - pos, &pg, &e)) { free(dict); return -1; }
         if ((n + 1024ULL * pg.vec_c) * (W / 8) > cap) { free(dict); return -2; }
-        if ((e.exp_t == EXP_DICT_RLE || (e.exp_t == EXP_FFOR && e.var == FFOR_S_REC)) && pg.vec_c) {
-            /* THzlDeCmp::decompress_buf of buffer 0 (hzl_de_cmp.cpp:28-77) */
-            unsigned rw    = e.exp_t == EXP_DICT_RLE ? 32 : 64;
-            uint64_t bytes = pg.off[1] - pg.off[0], consumed = 0;
-            pg.runs        = (uint8_t*)malloc((bytes / 12 + 1) * 1024 * (rw / 8) + bytes + 8);
-            int64_t nr     = orc_hzl_decode(rw, pg.p + pg.off[0], pg.runs, &consumed);
-            if (nr < 0 || consumed != bytes) { free(pg.runs), free(dict); return -1; }
-            pg.runs_n = (uint64_t)nr;
+        if (!expr_fits(dtype, &e)) { free(dict); return -1; }
+        unsigned rec = expr_rec(&e);
+        for (unsigned b = 0; b < 2 && pg.vec_c; ++b) {
+            if (!(rec >> b & 1)) continue;
+            /* THzlDeCmp::decompress_buf (hzl_de_cmp.cpp:28-77) */
+            unsigned rw    = expr_rec_width(&e);
+            uint64_t bytes = pg.off[b + 1] - pg.off[b], consumed = 0;
+            pg.dec[b]      = (uint8_t*)malloc((bytes / 12 + 1) * 1024 * (rw / 8) + bytes + 8);
+            int64_t nr     = pg.off[b + 1] < pg.off[b] ? -1 : orc_hzl_decode(rw, pg.p + pg.off[b], pg.dec[b], &consumed);
+            if (nr < 0 || consumed != bytes) { free(pg.dec[0]), free(pg.dec[1]), free(dict); return -1; }
+            pg.dec_n[b] = (uint64_t)nr * (rw / 8);
         }
         for (unsigned v = 0; v < pg.vec_c; ++v)
             if (decode_vector(W, &pg, &e, v, dict, dn, (uint8_t*)out + (n + 1024ULL * v) * (W / 8))) {
-                free(pg.runs), free(dict);
+                free(pg.dec[0]), free(pg.dec[1]), free(dict);
                 return -1;
             }
-        free(pg.runs);
+        free(pg.dec[0]), free(pg.dec[1]);
         n += 1024ULL * pg.vec_c;
         free(dict);
         if (!pg.more) break;
@@ -1246,7 +1318,7 @@ int64_t orc_encode_str_column(uint8_t expr_id, const uint32_t* offsets, const ui
     int      rc     = fenc_init(fe, table);
     for (;;) {
         page_w pg;
-        page_begin(&pg, 2);
+        page_begin(&pg, 2, 0);
         uint64_t stop = v + per_pg;
         if (stop > n_vec) stop = n_vec;
         for (; v < stop && !rc; ++v) {

@@ -66,9 +66,11 @@ def _seg_deltas(x: np.ndarray, seg: int, mask=None) -> np.ndarray:
 
 
 def _vec_bw(x2: np.ndarray) -> np.ndarray:
-    """Per-row FFOR bit width of a (n_vec, 1024) unsigned array: bitlen(max - min)."""
-    mn = x2.min(axis=1)
-    mx = x2.max(axis=1)
+    """Per-row FFOR bit width of a (n_vec, 1024) uint64 array holding int64 values: bitlen(max - min) with min and max
+    taken as signed values, as ANALYZE does for int64 columns (ffor_analyze.cpp:145-160, ffor_analyze<i64_pt>)."""
+    b = np.asarray(x2, dtype=np.uint64) ^ _U64(1 << 63)
+    mn = b.min(axis=1)
+    mx = b.max(axis=1)
     return bitlen((mx - mn).astype(np.uint64))
 
 

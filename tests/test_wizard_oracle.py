@@ -61,9 +61,11 @@ def _brute_features(x):
             if v not in seen:
                 seen[v] = len(seen)
             codes.append(seen[v])
+    sgn = lambda u: u - (1 << 64) if u >> 63 else u  # noqa: E731  (int64 columns: ANALYZE compares signed values)
     for i, vec in enumerate(s):
-        out["bwf"].append((max(vec) - min(vec)).bit_length())
-        d = [0 if o % 64 == 0 else (vec[o] - vec[o - 1]) & M for o in range(1024)]
+        sv = [sgn(v) for v in vec]
+        out["bwf"].append((max(sv) - min(sv)).bit_length())
+        d = [0 if o % 64 == 0 else sgn((vec[o] - vec[o - 1]) & M) for o in range(1024)]
         out["bwd"].append((max(d) - min(d)).bit_length())
         out["runs"].append(1 + sum(vec[o] != vec[o - 1] for o in range(1, 1024)))
         out["bwp"].append(int(any(vec[o] != vec[o - 1] for o in range(1, 1024) if o % 16)))
