@@ -302,6 +302,7 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
                     if (!ok) break;
                     const unsigned bw = bwc(v, 2);
                     ok = bw <= 16 && need(v, 4, 128ull * bw);
+                    if (bw > pe.pos_bw_max) pe.pos_bw_max = (uint8_t)bw;
                 }
                 break;
             }

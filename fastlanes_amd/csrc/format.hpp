@@ -124,6 +124,9 @@ struct PageEntry {
     std::vector<HzlChunkH> rec1_hzl;
     uint64_t               rec1_n = 0;
     uint8_t                rec = 0, rec_width = 0;
+    // RLE / DICT_RLE with DELTA positions: the largest position bit width over the page's vectors (0 or 1 for
+    // every page the encoders write)
+    uint8_t pos_bw_max = 0;
     // FSST (string columns)
     FsstTableH fsst;
 };

@@ -114,7 +114,7 @@ hipError_t drle_page(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded, c
             e = launch_drle_values(plan->W, d.codes, pe.run_codes, d.values, pe.dict_n, d.run_values, plan->d_err, s);
     }
     const RunSrc rs {d.run_values, (uint32_t)(4 * pe.run_codes), 2u};
-    if (e == hipSuccess) e = launch_page_rle(plan->W, 2, page, pm, pe.vec_c, out, plan->d_err, s, sel, &rs);
+    if (e == hipSuccess) e = launch_page_rle(plan->W, 2, page, pm, pe.vec_c, out, plan->d_err, s, sel, &rs, pe.pos_bw_max <= 1);
     return e;
 }
 
@@ -157,9 +157,10 @@ hipError_t page_launch(fls_gpu_plan* plan, size_t i, const uint8_t* dev_encoded,
         return launch_page_patch(W, patch, rsum, nullptr, 0, page, pm, pe.vec_c, out, plan->d_err, s, sel, exc);
     case EXP_RLE: {
         const int kv = pe.var == 1 || pe.var == kRleRec ? 1 : 2;  // positions: plain u16, or RSUM + FFOR
-        if (!pe.rec) return launch_page_rle(W, kv, page, pm, pe.vec_c, out, plan->d_err, s, sel);
+        const bool bw1 = pe.pos_bw_max <= 1;
+        if (!pe.rec) return launch_page_rle(W, kv, page, pm, pe.vec_c, out, plan->d_err, s, sel, nullptr, bw1);
         const RunSrc rs {d.run_values, (uint32_t)(8 * pe.run_codes), 3u, (pe.rec & 2) ? d.rec1_values : nullptr};
-        return launch_page_rle(W, kv, page, pm, pe.vec_c, out, plan->d_err, s, sel, &rs);
+        return launch_page_rle(W, kv, page, pm, pe.vec_c, out, plan->d_err, s, sel, &rs, bw1);
     }
     case EXP_DICT_RLE: return drle_page(plan, i, dev_encoded, page, pm, out, sel, s);
     default: {
@@ -368,7 +369,7 @@ int fls_gpu_tune_set(int family, unsigned width, unsigned bw, unsigned block, un
         return fail(FLS_E_ARG, "block must be a multiple of 64, <= 512");
     if (family == FAM_ROWGROUP && block == 0) return fail(FLS_E_ARG, "tile size must be positive");
     (void)tune_for(family, width, bw);  // make sure the default table is loaded first
-    tune_set(family, width, bw, Tune {(uint16_t)block, (uint16_t)(nontemporal & 3), max_blocks});
+    tune_set(family, width, bw, Tune {(uint16_t)block, (uint16_t)(nontemporal & 7), max_blocks});
     return FLS_OK;
 }
 

@@ -60,7 +60,7 @@ int tune_load(const char* path) {
         if (std::sscanf(line, "%d %u %u %u %u %u", &fam, &W, &bw, &block, &nt, &maxb) != 6) continue;
         if (fam < 0 || fam >= FAM_COUNT || block == 0) continue;
         if (fam != FAM_ROWGROUP && (block % 64 || block > kMaxBlock)) continue;
-        tune_set(fam, W, bw, Tune {(uint16_t)block, (uint16_t)(nt & 3), maxb});  // bit 0 non-temporal, bit 1 direct
+        tune_set(fam, W, bw, Tune {(uint16_t)block, (uint16_t)(nt & 7), maxb});  // bits: 0 non-temporal, 1 direct, 2 kernel choice
         ++n;
     }
     std::fclose(f);
@@ -93,13 +93,13 @@ hipError_t launch_page_patch(unsigned W, int var, bool rsum, const void* dict, u
 }
 
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                           uint32_t* err, hipStream_t s, const SelArgs* sel, const RunSrc* runs) {
+                           uint32_t* err, hipStream_t s, const SelArgs* sel, const RunSrc* runs, bool pos_bw1) {
     if (!vec_c) return hipSuccess;
     switch (W) {
-    case 8: return page_rle_w<8>(var, page, pm, vec_c, out, err, sel, runs, s);
-    case 16: return page_rle_w<16>(var, page, pm, vec_c, out, err, sel, runs, s);
-    case 32: return page_rle_w<32>(var, page, pm, vec_c, out, err, sel, runs, s);
-    case 64: return page_rle_w<64>(var, page, pm, vec_c, out, err, sel, runs, s);
+    case 8: return page_rle_w<8>(var, page, pm, vec_c, out, err, sel, runs, pos_bw1, s);
+    case 16: return page_rle_w<16>(var, page, pm, vec_c, out, err, sel, runs, pos_bw1, s);
+    case 32: return page_rle_w<32>(var, page, pm, vec_c, out, err, sel, runs, pos_bw1, s);
+    case 64: return page_rle_w<64>(var, page, pm, vec_c, out, err, sel, runs, pos_bw1, s);
     default: return hipErrorInvalidValue;
     }
 }

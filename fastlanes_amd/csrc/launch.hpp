@@ -46,9 +46,12 @@ hipError_t launch_unffor_stream(unsigned W, unsigned bw, const uint8_t* packed, 
                                 uint64_t n_vec, hipStream_t s);
 hipError_t launch_page_int(unsigned W, int mode, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
                            hipStream_t s, const SelArgs* sel = nullptr);
-// runs: where the run values are (DICT_RLE pages); nullptr: the page's buffer 0
+// runs: where the run values are (DICT_RLE pages, chained run values); nullptr: the page's buffer 0.
+// pos_bw1: every vector's DELTA positions have bit width 0 or 1 (8/16-bit values then take the one-lane-per-thread
+// kernel)
 hipError_t launch_page_rle(unsigned W, int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out,
-                           uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr, const RunSrc* runs = nullptr);
+                           uint32_t* err, hipStream_t s, const SelArgs* sel = nullptr, const RunSrc* runs = nullptr,
+                           bool pos_bw1 = false);
 // DICT_RLE: rv[i] = dict[codes[i]] (W-bit values from a max(32, W)-bit dictionary)
 hipError_t launch_drle_values(unsigned W, const uint32_t* codes, uint64_t n, const void* dict, uint32_t dict_n,
                               uint8_t* rv, uint32_t* err, hipStream_t s);
@@ -85,7 +88,7 @@ hipError_t page_patch_dict_w(int var, bool rsum, const void* dict, uint32_t dict
                              const uint8_t* exc, hipStream_t s);
 template <int W>
 hipError_t page_rle_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, uint8_t* out, uint32_t* err,
-                      const SelArgs* sel, const RunSrc* runs, hipStream_t s);
+                      const SelArgs* sel, const RunSrc* runs, bool pos_bw1, hipStream_t s);
 template <int W>
 hipError_t page_dict_w(int var, const uint8_t* page, const PageMeta& pm, uint32_t vec_c, const void* dict,
                        uint32_t dict_n, uint8_t* out, uint32_t* err, const SelArgs* sel, const HzlSrc* hz,

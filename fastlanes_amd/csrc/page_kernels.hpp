@@ -758,6 +758,134 @@ __global__ __launch_bounds__(kMaxBlock) void k_page_rle(const uint8_t* __restric
     }
 }
 
+// Bits j_of_p<16>(0) .. j_of_p<16>(p) of a u16 position lane word: the differences up to value p in orig order.
+__host__ __device__ constexpr uint32_t prefix_mask16(unsigned p) {
+    uint32_t m = 0;
+    for (unsigned q = 0; q <= p; ++q) m |= 1u << j_of_p<16>(q);
+    return m;
+}
+
+// RLE pages with DELTA positions whose bit width is at most 1 -- every page the encoders write: a run index grows by
+// 0 or 1 per value, and each lane's first difference is 0, so FFOR's base is 0 -- for 8/16-bit values. A wave
+// decodes VPS vectors per step, one u16 position lane of each per thread: the lane's 16 differences are one packed u16 word, its positions are
+// the lane base plus running bit counts (value p in orig order is bit j_of_p<16>(p)), and its 16 values -- a
+// contiguous segment of the output -- are gathered from the vector's run values in LDS and stored straight to HBM
+// (the wave's 64 segments tile the vector: whole lines). The last position bounds all 16 (positions grow along the
+// lane): only a lane past the run count clamps (err bit 1).
+template <typename T, bool NT>
+__global__ __launch_bounds__(kMaxBlock) void k_page_rle_lanes(const uint8_t* __restrict__ page, PageMeta pm,
+                                                              uint32_t vec_c, uint8_t* __restrict__ out, uint32_t* err,
+                                                              RunSrc rs) {
+    static_assert(sizeof(T) <= 2, "narrow values only");
+    // vectors per wave step (two measured no faster for 8-bit values: the step's latency is not what limits them)
+    constexpr int VPS = 1;
+    extern __shared__ __attribute__((aligned(16))) uint8_t fls_lds[];
+    // the wave index made provably uniform: the vector index and every per-vector address below stay in scalar
+    // registers, and the per-vector metadata is read once per wave
+    const unsigned lane = threadIdx.x % 64, wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64),
+                   waves = blockDim.x / 64;
+    T*             lruns = reinterpret_cast<T*>(fls_lds) + wave * 1024 * VPS;  // at most 1024 runs per vector
+    const uint64_t N      = vec_c;
+    const uint64_t stride = (uint64_t)gridDim.x * waves * VPS;
+    uint64_t       v      = ((uint64_t)blockIdx.x * waves + wave) * VPS;  // the step's vectors: v .. v + VPS - 1
+    // Pipelined two steps deep: while a step's VPS vectors are decoded, the run values of the next step's are in
+    // flight into registers (the first 256 of each; the rare rest is loaded in its own step) and the metadata of the
+    // step after.
+    constexpr unsigned K = 4;  // run values per lane and vector held in registers
+    auto meta = [&](uint64_t u) {
+        RleMeta r = rle_meta<2>(page, pm, u, N, lane >> 1, rs.bytes, rs.lbase);
+        r.lb      = (r.lb >> (16 * (lane & 1))) & 0xffffu;  // rle_meta reads the u32 of lanes 2t, 2t + 1
+        r.r0      = __builtin_amdgcn_readfirstlane(r.r0);
+        r.r1      = __builtin_amdgcn_readfirstlane(r.r1);
+        r.bw      = __builtin_amdgcn_readfirstlane(r.bw);
+        r.base    = __builtin_amdgcn_readfirstlane(r.base);
+        r.ep      = __builtin_amdgcn_readfirstlane(r.ep);
+        return r;
+    };
+    // the vector's packed word for this lane (bit width 0 or 1: row 0 holds lane l's u16 at 2 l)
+    auto load_word = [&](const RleMeta& r) {
+        return r.bw ? (uint32_t)*reinterpret_cast<const u16_ua*>(page + pm.off[4] + r.ep + 2 * lane) : 0u;
+    };
+    auto n_of    = [&](const RleMeta& r) { return (r.r1 - r.r0) >> rs.esh; };
+    auto runs_of = [&](const RleMeta& r) { return rs.base + (uint64_t)(r.r0 >> rs.esh) * sizeof(T); };
+    auto load_runs = [&](const RleMeta& r, T (&dst)[K]) {
+        const uint8_t* rp = runs_of(r);
+        const uint32_t n  = n_of(r);
+        static_for<K>([&](auto k) {
+            const uint32_t i = lane + 64 * decltype(k)::value;
+            dst[k]           = i < n ? load_t<T>(rp + (uint64_t)i * sizeof(T)) : (T)0;
+        });
+    };
+    RleMeta  m0[VPS] = {}, m1[VPS] = {};
+    uint32_t w0[VPS] = {}, w1[VPS] = {};
+    T        r0v[VPS][K] = {};
+    static_for<VPS>([&](auto k) {
+        if (v + k < N) m0[k] = meta(v + k), w0[k] = load_word(m0[k]), load_runs(m0[k], r0v[k]);
+        if (v + stride + k < N) m1[k] = meta(v + stride + k), w1[k] = load_word(m1[k]);
+    });
+    for (; v < N; v += stride) {
+        RleMeta  cur[VPS];
+        uint32_t w[VPS];
+        static_for<VPS>([&](auto k) {
+            cur[k]             = m0[k];
+            w[k]               = w0[k];
+            const uint32_t n   = v + k < N ? n_of(cur[k]) : 0u;
+            T*             lr  = lruns + 1024 * k;
+            static_for<K>([&](auto q) {
+                const uint32_t i = lane + 64 * decltype(q)::value;
+                if (i < n) lr[i] = r0v[k][q];
+            });
+            if (n > 64 * K) {  // the runs beyond the registers (uniform: the vector's run count)
+                const uint8_t* rp = runs_of(cur[k]);
+                for (uint32_t i = lane + 64 * K; i < n && i < 1024; i += 64) lr[i] = load_t<T>(rp + (uint64_t)i * sizeof(T));
+            }
+        });
+        // the next step's runs and words, the metadata of the step after
+        static_for<VPS>([&](auto k) {
+            m0[k] = m1[k], w0[k] = w1[k];
+            if (v + stride + k < N) load_runs(m0[k], r0v[k]);
+            if (v + 2 * stride + k < N) m1[k] = meta(v + 2 * stride + k), w1[k] = load_word(m1[k]);
+        });
+        wave_sync();
+        static_for<VPS>([&](auto k) {
+            if (v + k >= N) return;
+            const uint32_t n_runs = n_of(cur[k]);
+            const T*       lr     = lruns + 1024 * k;
+            if (n_runs == 0) {  // the host rejects such pages; never read an empty run buffer
+                if (lane == 0) atomicOr(err, 1u);
+                return;
+            }
+            const uint32_t last  = (n_runs < 1024 ? n_runs : 1024) - 1;
+            const uint32_t base  = cur[k].base & 0xffffu;
+            // the largest position of the lane, without the u16 wrap (a lane that wraps clamps)
+            const uint32_t top   = cur[k].lb + (uint32_t)__builtin_popcount(w[k]) + 16 * base;
+            const bool     clamp = top > last;
+            T              vals[16];
+            if (base == 0 && !__any(clamp)) {
+                // position p = lane base + the count of set bits among the lane word's first p + 1 differences:
+                // one AND and one bit count (which adds the lane's first run address) per value
+                const T* r0 = lr + cur[k].lb;
+                static_for<16>([&](auto p) {
+                    constexpr uint32_t M = prefix_mask16(decltype(p)::value);
+                    vals[p] = r0[__builtin_popcount(w[k] & M)];
+                });
+            } else {
+                uint32_t pos = cur[k].lb;
+                static_for<16>([&](auto p) {
+                    constexpr unsigned j = j_of_p<16>(p);
+                    pos = (pos + ((w[k] >> j) & 1u) + base) & 0xffffu;
+                    vals[p] = lr[pos < last ? pos : last];
+                });
+            }
+            if (clamp) atomicOr(err, 1u);
+            // a segment starts at a multiple of 16 values: 16-byte stores
+            store_run<T, 16, 16>(vals, sizeof(T) * seg_start<16>(lane),
+                                 GlobalSink<NT> {out + (v + k) * 1024 * sizeof(T)});
+        });
+        wave_sync();
+    }
+}
+
 // DICT_RLE run values: rv[i] = dict[codes[i]] over a page's decoded run codes, so that the RLE kernel gathers
 // values (include/fls/encoder/assert_eq.hpp:329-400 flattens the codes first and looks them up after; the lookup
 // commutes with the gather). Codes >= dict_n are clamped (err bit 2).

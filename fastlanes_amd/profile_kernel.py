@@ -30,8 +30,32 @@ def main():
     ap.add_argument("--page-ffor", action="store_true",
                     help="instead: decode an FFOR column (pool id 1) of 2^log2n values of --width bits, values in "
                          "[0, 128) (bit width 7)")
+    ap.add_argument("--page-kind", choices=("ffor", "bitpack", "delta", "rle", "dict"),
+                    help="instead: decode the row-group workload's column of that kind (pool id 1, 1, 11, 6, 17) of "
+                         "2^log2n values of --width bits")
     a = ap.parse_args()
     import torch
+
+    if a.page_kind:
+        import numpy as np
+
+        from fastlanes_amd import workloads as wl
+
+        eid = {"ffor": 1, "bitpack": 1, "delta": 11, "rle": 6, "dict": 17}[a.page_kind]
+        x = wl.rowgroup_column(a.page_kind, a.width, 1 << a.log2n, seed=5,
+                               w=7 if a.page_kind in ("ffor", "bitpack") else None)
+        enc = fa.encode_from_memory(x, eid)
+        plan = fa.Plan(enc, x.dtype)
+        d_in = torch.from_numpy(enc).cuda()
+        d_out = torch.empty(x.nbytes + 16, dtype=torch.uint8, device="cuda")
+        for _ in range(a.reps):
+            plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+        torch.cuda.synchronize()
+        plan.check()
+        ok = bool(np.array_equal(d_out[: x.nbytes].cpu().numpy().view(x.dtype), x))
+        print(json.dumps({"kind": a.page_kind, "values": x.size, "width": a.width, "encoded_bytes": int(enc.size),
+                          "pages": plan.n_pages, "reps": a.reps, "bit_exact": ok}), flush=True)
+        return
 
     if a.page_ffor:
         import numpy as np

@@ -115,7 +115,8 @@ int fls_gpu_unffor_stream(unsigned width, unsigned bw, const void* dev_packed, c
  * next to the library (or the file named by FLS_TUNING) on first use; fastlanes_amd/autotune.py measures and writes
  * it. family: 0 raw streams, 1 UNCOMPRESSED, 2 FFOR, 3 DELTA, 4 RLE, 5 dictionary pages; bw 255 = all bit widths.
  * nontemporal: bit 0 = non-temporal stores; bit 1 = direct stores of each lane's segment instead of staging the
- * output through LDS (families 1-3 with 32/64-bit values, families 4-5 with 64-bit values).
+ * output through LDS (families 1-3 with 32/64-bit values, families 4-5 with 64-bit values); bit 2 (family 4, 8/16-bit
+ * values) = decode position lanes 2 per thread with staging instead of one per thread with direct stores.
  * family 6 is the row-group launch: block = KiB of decoded values per tile, max_blocks = blocks per compute unit
  * (0: as many as stay resident at once, each wave then owning one long run of the work list).
  * fls_gpu_tune_get reads the shape in effect for (family, width, bw); any output pointer may be NULL. */

@@ -214,6 +214,55 @@ def test_out_of_range_codes_are_clamped_and_reported(torch_cuda, oracle):
         plan.check()
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16])
+def test_rle_positions_past_the_runs_are_clamped_and_reported(torch_cuda, oracle, dtype):
+    """A narrow RLE page whose lane base points past its vector's runs: the one-lane-per-thread kernel takes its
+    clamping path for that vector (err bit 1, fls_gpu_plan_check fails) and the other vectors decode exactly."""
+    torch = torch_cuda
+    x = wl.geometric_runs(8 * 1024, 8.0, seed=4, value_bits=7).astype(dtype)
+    enc = oracle.encode_column(x, 6)
+    off = np.frombuffer(enc[8:28].tobytes(), np.uint32)
+    bad = enc.copy()
+    lb = off[1] + 3 * 128 + 2 * 5  # vector 3 (fixed 128-byte lane bases), lane 5
+    bad[lb: lb + 2] = np.frombuffer(np.uint16(2000).tobytes(), np.uint8)
+    plan = fa.Plan(bad, dtype)
+    d_in, d_out = dev(torch, bad), empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    with pytest.raises(fa.FlsError):
+        plan.check()
+    got = to_np(d_out, dtype, x.size).reshape(8, 1024)
+    keep = [v for v in range(8) if v != 3]
+    np.testing.assert_array_equal(got[keep], x.reshape(8, 1024)[keep])
+
+
+@pytest.mark.parametrize("W", [8, 16])
+def test_rle_two_lane_kernel(torch_cuda, oracle, W):
+    """Tuning bit 2 of the RLE family switches narrow pages back to the two-lanes-per-thread kernel (the one that
+    also serves predicates and position bit widths above 1): both kernels against the oracle."""
+    import ctypes as C
+
+    torch = torch_cuda
+    lib = fa.lib()
+    dt = W2DT[W]
+    x = _column(dt, 6, 41 * 1024, seed=W)
+    enc = oracle.encode_column(x, 6)
+    want = oracle.decode_column(enc, dt, x.size)
+    plan = fa.Plan(enc, dt)
+    d_in = dev(torch, enc)
+    try:
+        for nt in (1, 5):
+            print(f"rle W={W} nt={nt}", flush=True)
+            fa.check(lib.fls_gpu_tune_set(4, W, 255, 256, nt, 0), "tune_set")
+            d_out = empty(torch, x.nbytes)
+            plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+            plan.check()
+            np.testing.assert_array_equal(to_np(d_out, dt, x.size), want, err_msg=f"nt {nt}")
+    finally:
+        path = os.path.join(os.path.dirname(os.path.abspath(fa.__file__)), "tuning.txt")
+        lib.fls_gpu_tune_load.argtypes = [C.c_char_p]
+        assert lib.fls_gpu_tune_load(path.encode()) > 0
+
+
 def test_fsst_strings_match_oracle(torch_cuda, oracle):
     torch = torch_cuda
     offs, data = wl.vocab_strings(6 * 1024, seed=5)
