@@ -25,12 +25,12 @@
 namespace fls {
 namespace {
 
-constexpr unsigned kRG = 65536, kVPR = 64, kK = 13;
+constexpr unsigned kRG = 65536, kVPR = 64, kK = 16;
 constexpr uint64_t kEmpty = ~0ull;
 constexpr unsigned kHtBits = 17, kHt = 1u << kHtBits;      // exhaustive table: 131072 slots (load <= 0.5)
 constexpr unsigned kSampleV = 4, kSample = kSampleV * 1024;  // learned selector's sample
 constexpr unsigned kSampleVec[kSampleV] = {0, 21, 42, 63};
-constexpr uint8_t kCandExpr[kK] = {0, 1, 11, 6, 7, 16, 17, 22, 2, 3, 4, 5, 27};
+constexpr uint8_t kCandExpr[kK] = {0, 1, 11, 6, 7, 16, 17, 22, 2, 3, 4, 5, 27, 12, 13, 14};
 constexpr unsigned kExcLimit = 100;  // CFG::CMPR::EXC_LIMIT_C (include/fls/cfg/cfg.hpp:67)
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
@@ -157,9 +157,9 @@ __device__ __forceinline__ bool is_exception(uint64_t v, const PatchPlan& pp) {
 
 // Statistics of one 1024-value vector, computed by one wave (lane l owns values 16l .. 16l+15).
 // bwf / bwd: FFOR widths of the values and of their lane differences as ANALYZE computes them for int64 columns
-// (min and max signed, ffor_analyze<i64_pt>); bwu: the unsigned width of the values (bounds the patch search).
+// (min and max signed, ffor_analyze<i64_pt>); bwu / bwdu: their unsigned widths (bound the patch searches).
 struct VecStats {
-    uint32_t bwf, bwd, bwu, runs, bwp, eq, nondec, inc;
+    uint32_t bwf, bwd, bwu, bwdu, runs, bwp, eq, nondec, inc;
     uint64_t umin, umax;
     int64_t  smin, smax;
 };
@@ -175,7 +175,7 @@ __device__ VecStats vector_stats(const uint64_t* __restrict__ x) {
         v[2 * i + 1] = (uint64_t)q[2] | ((uint64_t)q[3] << 32);
     }
     uint64_t prev = __shfl_up(v[15], 1, 64);
-    uint64_t mn = kEmpty, mx = 0;
+    uint64_t mn = kEmpty, mx = 0, dun = kEmpty, dux = 0;
     int64_t  smn = INT64_MAX, smx = INT64_MIN, dmn = INT64_MAX, dmx = INT64_MIN;
     uint32_t runs = 0, inner = 0, eq = 0, nondec = 0, inc = 0;
 #pragma unroll
@@ -187,6 +187,8 @@ __device__ VecStats vector_stats(const uint64_t* __restrict__ x) {
         smn = (int64_t)v[i] < smn ? (int64_t)v[i] : smn;
         smx = (int64_t)v[i] > smx ? (int64_t)v[i] : smx;
         const int64_t  d = (o % 64) ? (int64_t)(v[i] - pv) : 0;
+        dun = (uint64_t)d < dun ? (uint64_t)d : dun;
+        dux = (uint64_t)d > dux ? (uint64_t)d : dux;
         dmn = d < dmn ? d : dmn;
         dmx = d > dmx ? d : dmx;
         const bool has_prev = o != 0;
@@ -203,6 +205,7 @@ __device__ VecStats vector_stats(const uint64_t* __restrict__ x) {
     s.bwf  = bitlen64((uint64_t)s.smax - (uint64_t)s.smin);
     s.bwu  = bitlen64(s.umax - s.umin);
     s.bwd  = bitlen64((uint64_t)wave_max(dmx) - (uint64_t)wave_min(dmn));
+    s.bwdu = bitlen64(wave_max(dux) - wave_min(dun));
     s.runs = wave_sum(runs), s.bwp = wave_sum(inner) != 0;
     s.eq = wave_sum(eq), s.nondec = wave_sum(nondec), s.inc = wave_sum(inc);
     return s;
@@ -275,6 +278,37 @@ __device__ uint32_t block_popcount_scan(const uint32_t* bitmap, uint32_t* base, 
 // Only the block's own threads touch its table, slots and first positions, so the barriers' workgroup-scope
 // ordering is all the phases need (a device-scope fence would write back and invalidate the XCD's whole L2).
 // ---------------------------------------------------------------------------------------------------------------
+// DELTA patches (delta_exp.hpp:56-131: RSUM, then PA_ANALYZE over the lane differences): the patch plan of one
+// vector's differences (the value minus its predecessor inside each 64-value lane, 0 at a lane start), sorted in the
+// wave's buffer sb, and its exceptions in transposed order with the LL chain's hop links, as for the values.
+__device__ PatchPlan delta_patch(const uint64_t* __restrict__ xv, uint64_t* sb, uint32_t bwdu, uint32_t& n,
+                                 uint32_t& extra) {
+    const unsigned lane = threadIdx.x % 64;
+    auto diff = [&](unsigned o) { return (o % 64) ? xv[o] - xv[o - 1] : 0ull; };
+    wave_lds_sync();  // the previous plan has read the sort buffer
+#pragma unroll
+    for (unsigned i = 0; i < 16; ++i) sb[64 * i + lane] = diff(64 * i + lane);
+    wave_sort1024(sb);
+    const PatchPlan pd  = patch_plan(sb, bwdu);
+    const unsigned  hop = 1u << (pd.bw < 11 ? pd.bw : 11);
+    int             last = -1;
+    n = 0, extra = 0;
+    for (unsigned q = 0; q < 16; ++q) {
+        const unsigned k  = 64 * q + lane;
+        const bool     ex = is_exception(diff(orig_of(k)), pd);
+        const uint64_t m  = __ballot(ex);
+        if (ex) {
+            const uint64_t below = m & ((1ull << lane) - 1);
+            const int      prev  = below ? (int)(64 * q + 63 - __clzll((long long)below)) : last;
+            if (prev >= 0) extra += (k - (unsigned)prev - 1) / hop;
+        }
+        if (m) last = (int)(64 * q + 63 - __clzll((long long)m));
+        n += (uint32_t)__popcll(m);
+    }
+    extra = wave_sum(extra);
+    return pd;
+}
+
 struct SizerLds {
     uint32_t bitmap[kRG / 32];
     uint32_t base[kRG / 32];
@@ -285,6 +319,8 @@ struct SizerLds {
     // patched FFOR: per vector bit width, base, exceptions and LL chain length; the page's exception values in
     // chunks of 1024 (S_PATCH_REC's HZL chain)
     uint32_t           pbw[kVPR], pexc[kVPR], pchain[kVPR], excpref[kVPR + 1];
+    // DELTA patches: the same plan over each vector's lane differences
+    uint32_t           dbw[kVPR], dexc[kVPR], dchain[kVPR];
     unsigned long long pbase[kVPR], excmin[8], excmax[8];
     // DICT_RLE: the page's run codes in chunks of 1024 (its HZL chain)
     uint32_t rcmin[kVPR], rcmax[kVPR], runpref[kVPR + 1];
@@ -341,6 +377,9 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
                 L.bwf[v] = s.bwf, L.bwd[v] = s.bwd, L.runs[v] = s.runs, L.bwp[v] = s.bwp;
                 L.pbw[v] = pp.bw, L.pbase[v] = pp.base, L.pexc[v] = n, L.pchain[v] = n + extra;
             }
+            uint32_t dn = 0, dextra = 0;
+            const PatchPlan pd = delta_patch(x + v * 1024, sb, s.bwdu, dn, dextra);
+            if (lane == 0) L.dbw[v] = pd.bw, L.dexc[v] = dn, L.dchain[v] = dn + dextra;
         }
         // dictionary: insert every value, remember its slot. A thread takes kBatch positions at a time: their loads
         // and first probes are issued together, so kBatch atomics are in flight per thread instead of one.
@@ -498,12 +537,16 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
                 s[10] += 8 * n + 128 + pk;                  // B_PATCH [exceptions][bitmap]...
                 s[11] += 8ull * L.pchain[v] + 4 + pk;       // LL_PATCH [chain values][count, first]...
                 s[12] += 131 + 128ull * L.bwp[v];           // DICT_RLE [run codes: chain][lane bases][bw][base][packed]
+                const uint64_t pkd = 9 + 128ull * L.dbw[v], dn = L.dexc[v];
+                s[13] += 128 + 8 * dn + 2 + 2 * dn + pkd;   // DELTA S_PATCH [lane bases][exceptions][count, positions]...
+                s[14] += 128 + 8 * dn + 128 + pkd;          // DELTA B_PATCH [lane bases][exceptions][bitmap]...
+                s[15] += 128 + 8ull * L.dchain[v] + 4 + pkd; // DELTA LL_PATCH [lane bases][chain values][count, first]...
             }
             s[9] += echain;
             s[12] += rchain;
-            constexpr unsigned arr[kK]  = {1, 3, 4, 5, 2, 1, 3, 4, 5, 5, 5, 5, 5};
+            constexpr unsigned arr[kK]  = {1, 3, 4, 5, 2, 1, 3, 4, 5, 5, 5, 5, 5, 6, 6, 6};
             constexpr bool     dic[kK]  = {false, false, false, false, false, true, true, true,
-                                           false, false, false, false, true};
+                                           false, false, false, false, true, false, false, false};
             for (unsigned k = 0; k < kK; ++k) {
                 s[k] += 8 + 4ull * arr[k] * kVPR + (dic[k] ? dict : 0);
                 sizes[r * kK + k] = s[k];
@@ -553,7 +596,8 @@ struct FeatLds {
     uint32_t           cmin[kSampleV], cmax[kSampleV], cdmin[kSampleV], cdmax[kSampleV];
     uint32_t           top;
     uint32_t           tmp[8];
-    uint32_t           pbw[kSampleV], pexc[kSampleV];  // patch plan of each sample vector
+    uint32_t           pbw[kSampleV], pexc[kSampleV];    // patch plan of each sample vector
+    uint32_t           dpbw[kSampleV], dpexc[kSampleV];  // and of its lane differences
 };
 
 // Inserts sample position p (value xs[p]) into the position table: a slot holds the smallest position of one value
@@ -595,8 +639,11 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
             for (unsigned i = 0; i < 16; ++i) sb[64 * i + lane] = x[kSampleVec[wave] * 1024 + 64 * i + lane];
             wave_sort1024(sb);
             const PatchPlan pp = patch_plan(sb, s.bwu);
+            uint32_t        dn = 0, dextra = 0;
+            const PatchPlan pd = delta_patch(x + kSampleVec[wave] * 1024, sb, s.bwdu, dn, dextra);
             if (lane == 0) {
                 L.pbw[wave] = pp.bw, L.pexc[wave] = pp.exc;
+                L.dpbw[wave] = pd.bw, L.dpexc[wave] = dn;
                 L.bwf[wave] = s.bwf, L.bwd[wave] = s.bwd, L.runs[wave] = s.runs, L.bwp[wave] = s.bwp;
                 L.eq[wave] = s.eq, L.nondec[wave] = s.nondec, L.inc[wave] = s.inc;
                 L.umin[wave] = s.umin, L.umax[wave] = s.umax, L.smin[wave] = s.smin, L.smax[wave] = s.smax;
@@ -657,6 +704,7 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
             const float    S  = (float)kSampleV, vv = (float)kVPR;
             uint32_t       sbwf = 0, sbwd = 0, sruns = 0, sbwp = 0, sbwc = 0, sbwcd = 0, eq = 0, nondec = 0, inc = 0;
             uint32_t       mxbwf = 0, mnbwf = 64, mxbwd = 0, mnbwd = 64, mxr = 0, mnr = 1024, spbw = 0, sexc = 0;
+            uint32_t       sdpbw = 0, sdexc = 0;
             uint64_t       umax = 0, umin = kEmpty;
             int64_t        smax = INT64_MIN, smin = INT64_MAX;
             uint32_t       bwc[kSampleV], bwcd[kSampleV];
@@ -666,6 +714,7 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
                 sbwf += L.bwf[i], sbwd += L.bwd[i], sruns += L.runs[i], sbwp += L.bwp[i];
                 sbwc += bwc[i], sbwcd += bwcd[i];
                 spbw += L.pbw[i], sexc += L.pexc[i];
+                sdpbw += L.dpbw[i], sdexc += L.dpexc[i];
                 eq += L.eq[i], nondec += L.nondec[i], inc += L.inc[i];
                 mxbwf = max(mxbwf, L.bwf[i]), mnbwf = min(mnbwf, L.bwf[i]);
                 mxbwd = max(mxbwd, L.bwd[i]), mnbwd = min(mnbwd, L.bwd[i]);
@@ -678,6 +727,7 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
             const float mbwf = (float)sbwf / S, mbwd = (float)sbwd / S, mruns = (float)sruns / S,
                         mbwp = (float)sbwp / S, mbwc = (float)sbwc / S, mbwcd = (float)sbwcd / S;
             const float fD = (float)D, mpbw = (float)spbw / S, mexc = (float)sexc / S;
+            const float mdpbw = (float)sdpbw / S, mdexc = (float)sdexc / S;
             float       est[kK];
             est[0] = 8.f + 4.f * vv + 8192.f * vv;
             est[1] = 8.f + 12.f * vv + vv * (9.f + 128.f * mbwf);
@@ -693,8 +743,12 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
             est[10] = 8.f + 20.f * vv + vv * (8.f * mexc + 128.f + pk);
             est[11] = 8.f + 20.f * vv + vv * (8.f * mexc + 4.f + pk);
             est[12] = 28.f + 8.f * fD + 20.f * vv + vv * (131.f + 128.f * mbwp) + 12.f + 4.f * mruns * vv;
+            const float pkd = 9.f + 128.f * mdpbw;  // DELTA patches: bw, base and packed bytes per vector
+            est[13] = 8.f + 24.f * vv + vv * (10.f * mdexc + 130.f + pkd);
+            est[14] = 8.f + 24.f * vv + vv * (8.f * mdexc + 256.f + pkd);
+            est[15] = 8.f + 24.f * vv + vv * (8.f * mdexc + 132.f + pkd);
             for (unsigned k = 0; k < kK; ++k) F[k] = est[k] / (float)(kRG * 8), F[kK + k] = log2f_(est[k]) / 24.f;
-            float* G = F + 2 * kK;  // 26
+            float* G = F + 2 * kK;  // 32
             G[0] = mbwf / 64.f, G[1] = (float)mxbwf / 64.f, G[2] = (float)mnbwf / 64.f;
             G[3] = mbwd / 64.f, G[4] = (float)mxbwd / 64.f, G[5] = (float)mnbwd / 64.f;
             G[6] = mruns / 1024.f, G[7] = (float)mxr / 1024.f, G[8] = (float)mnr / 1024.f;
@@ -712,11 +766,11 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
             G[20] = (float)bitlen64((uint64_t)smax - (uint64_t)smin) / 64.f;
             G[21] = mbwc / 32.f;
             G[22] = mbwcd / 32.f;
-            for (unsigned i = 0; i < kSampleV; ++i)  // 49 .. 60
-                G[23 + 3 * i] = (float)L.bwf[i] / 64.f, G[24 + 3 * i] = (float)L.runs[i] / 1024.f,
-                           G[25 + 3 * i] = (float)bwc[i] / 32.f;
-            F[61] = mpbw / 64.f;
-            F[62] = mexc / 100.f;
+            for (unsigned i = 0; i < kSampleV; ++i) G[23 + i] = (float)L.bwf[i] / 64.f;  // 55 .. 58
+            F[59] = mpbw / 64.f;
+            F[60] = mexc / 100.f;
+            F[61] = mdpbw / 64.f;
+            F[62] = mdexc / 100.f;
             F[63] = 1.f;
         }
         __syncthreads();

@@ -31,7 +31,7 @@ RG = 65536
 N_FEATURES = 64
 _vp, _u64, _u32 = C.c_void_p, C.c_uint64, C.c_uint32
 WEIGHTS_MAGIC = b"FLSWIZ01"
-FEATURE_SCHEMA = "v2:est13,log13,bw,runs,dict,order,mag,sample4x3,patch2,bias"
+FEATURE_SCHEMA = "v3:est16,log16,bw,runs,dict,order,mag,sample4,patch2,dpatch2,bias"
 
 
 def _lib():

@@ -29,7 +29,7 @@ from __future__ import annotations
 
 import numpy as np
 
-CANDIDATES = (0, 1, 11, 6, 7, 16, 17, 22, 2, 3, 4, 5, 27)  # expression ids (i64 pool)
+CANDIDATES = (0, 1, 11, 6, 7, 16, 17, 22, 2, 3, 4, 5, 27, 12, 13, 14)  # expression ids (i64 pool)
 EXC_LIMIT = 100  # CFG::CMPR::EXC_LIMIT_C (include/fls/cfg/cfg.hpp:67)
 _ORDER8 = np.array([0, 4, 2, 6, 1, 5, 3, 7])
 _K = np.arange(1024)
@@ -192,6 +192,21 @@ def candidate_sizes(x: np.ndarray) -> np.ndarray:
     run_codes = codes[chg]
     sizes += [page(5, s_b), page(5, srec_b) + srec_chain, page(5, b_b), page(5, ll_b),
               dict_page + page(5, 131 + 128 * bwp) + hzl_size(run_codes.astype(np.uint64), 32)]
+    # DELTA patches (12 S_PATCH, 13 B_PATCH, 14 LL_PATCH): the patch plan of the lane differences, per vector
+    # [lane bases][exceptions][positions | bitmap | count + first][bw][base][packed]
+    dl = _seg_deltas(u, 64).reshape(V, 1024)
+    ds_b, db_b, dll_b = [], [], []
+    for vv in range(V):
+        bw, base = patch_plan(dl[vv])
+        pos = patch_exceptions(dl[vv], bw, base)
+        n = int(pos.size)
+        hop = 1 << min(bw, 11)
+        m = n + int(sum((int(g) - 1) // hop for g in np.diff(pos)))
+        pk = 9 + 128 * bw
+        ds_b.append(128 + 8 * n + 2 + 2 * n + pk)
+        db_b.append(128 + 8 * n + 128 + pk)
+        dll_b.append(128 + 8 * m + 4 + pk)
+    sizes += [page(6, ds_b), page(6, db_b), page(6, dll_b)]
     return np.array(sizes, dtype=np.uint64)
 
 
@@ -236,6 +251,10 @@ def features(x: np.ndarray) -> np.ndarray:
     plans = [patch_plan(s[i]) for i in range(S)]
     pbw = np.array([bw for bw, _ in plans])
     pexc = np.array([patch_exceptions(s[i], bw, base).size for i, (bw, base) in enumerate(plans)])
+    sd = [_seg_deltas(s[i], 64) for i in range(S)]
+    dplans = [patch_plan(sd[i]) for i in range(S)]
+    dpbw = np.array([bw for bw, _ in dplans])
+    dpexc = np.array([patch_exceptions(sd[i], bw, base).size for i, (bw, base) in enumerate(dplans)])
 
     f32 = np.float32
     mbwf, mbwd, mruns, mbwp = (f32(np.sum(a)) / f32(S) for a in (bwf, bwd, runs, bwp))
@@ -261,6 +280,14 @@ def features(x: np.ndarray) -> np.ndarray:
         base20 + vv * (f32(8) * mexc + f32(4) + pk),
         f32(28) + f32(8) * f32(D) + f32(20) * vv + vv * (f32(131) + f32(128) * mbwp) + f32(12) + f32(4) * mruns * vv,
     ], dtype=np.float32)])
+    mdpbw, mdexc = f32(np.sum(dpbw)) / f32(S), f32(np.sum(dpexc)) / f32(S)
+    pkd = f32(9) + f32(128) * mdpbw
+    base24 = f32(8) + f32(24) * vv
+    est = np.concatenate([est, np.array([
+        base24 + vv * (f32(10) * mdexc + f32(130) + pkd),
+        base24 + vv * (f32(8) * mdexc + f32(256) + pkd),
+        base24 + vv * (f32(8) * mdexc + f32(132) + pkd),
+    ], dtype=np.float32)])
     K = len(CANDIDATES)
     F = np.zeros(N_FEATURES, dtype=np.float32)
     F[0:K] = est / f32(RG * 8)
@@ -284,9 +311,11 @@ def features(x: np.ndarray) -> np.ndarray:
     F[G + 21] = mbwc / f32(32)
     F[G + 22] = mbwcd / f32(32)
     for i in range(S):
-        F[G + 23 + 3 * i: G + 26 + 3 * i] = [f32(bwf[i]) / f32(64), f32(runs[i]) / f32(1024), f32(bwc[i]) / f32(32)]
-    F[61] = mpbw / f32(64)
-    F[62] = mexc / f32(100)
+        F[G + 23 + i] = f32(bwf[i]) / f32(64)
+    F[59] = mpbw / f32(64)
+    F[60] = mexc / f32(100)
+    F[61] = mdpbw / f32(64)
+    F[62] = mdexc / f32(100)
     F[63] = f32(1)
     return F
 

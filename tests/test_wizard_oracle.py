@@ -88,9 +88,9 @@ def test_features_against_loops(family):
     assert F.dtype == np.float32 and F.shape == (64,)
     G = 2 * len(wo.CANDIDATES)
     for i in range(4):
-        assert F[G + 23 + 3 * i] * 64 == b["bwf"][i]
-        assert F[G + 24 + 3 * i] * 1024 == b["runs"][i]
-        assert F[G + 25 + 3 * i] * 32 == b["bwc"][i]
+        assert F[G + 23 + i] * 64 == b["bwf"][i]
+    assert F[G + 6] * 1024 == np.float32(sum(b["runs"])) / np.float32(4)
+    assert F[G + 21] * 32 == np.float32(sum(b["bwc"])) / np.float32(4)
     assert F[G + 3] * 64 == np.float32(sum(b["bwd"])) / np.float32(4)
     assert F[G + 10] == np.float32(sum(b["bwp"])) / np.float32(4)
     assert F[G + 22] * 32 == np.float32(sum(b["bwcd"])) / np.float32(4)
