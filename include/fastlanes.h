@@ -49,10 +49,11 @@ enum data_t
 };
 
 /* i64 expression ids; reference include/fastlanes.h:37-41 (ids index src/cor/exp_pool.cpp:23-65). The same ids
- * select the same expressions for every integer type here; the library also accepts 6 (RLE+DELTA), 7 (RLE),
- * 16/17/22 (DICT, DICT+FFOR, DICT+DELTA), 27 (DICT_RLE: run codes compressed page-wide, RLE+DELTA positions),
- * 28/29 (DICT_FRQ, DICT_FRQ+FFOR), 2/4/5 (FFOR S_PATCH, B_PATCH, LL_PATCH) and 3 (S_PATCH_REC: exception values
- * compressed page-wide; 64-bit columns). init_random_access reports no buffers for DICT_RLE and S_PATCH_REC pages. */
+ * select the same expressions for every integer type here, and the library accepts every id 0-38 of that pool:
+ * FFOR patches 2-5, RLE 6-10, DELTA 11-15, DICT 16-26, DICT_RLE 27, DICT_FRQ 28-38. Patched FFOR and DELTA (2-5,
+ * 12-15) need 32/64-bit values; the forms that store a buffer as one HZL chain of u64 values per page (3, 8-10,
+ * 15) need 64-bit values; dictionary forms take every integer width. init_random_access reports no buffers for a
+ * page with such a chain (3, 8-10, 15, 21, 26, 27, 33, 38): no vector owns a slice of it. */
 enum i64_expr_encoding_t
 #ifdef __cplusplus
     : uint8_t
