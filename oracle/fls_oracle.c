@@ -30,6 +30,15 @@
  *                              vector that pins S_PATCH; identity with the pinned S_PATCH page (buffers 1-4, entry
  *                              points; buffer 0 HZL-decodes to its exception values), round trips, a page without
  *                              exceptions, byte equality with the library's encoder.
+ *   enc_shaped/decode_shaped   (RLE chains 8-10, DELTA patches 12-15, dictionary code patches 18-26, 30-34, and
+ *   + chained buffers          signed ANALYZE for int64) all 35 recorded sizes of i64_pt1vec.csv rows 0-34 at once on
+ *                              one vector of the recorded table's shape (i >> 3 plus 5 exceptions at run
+ *                              boundaries); byte-level layouts of DELTA S_PATCH (lane bases = first row, exceptions
+ *                              = the differences at the listed positions) and RLE REC2 (raw chain headers around the
+ *                              id-6 buffers); round trips of every id on every type it is defined for; byte
+ *                              equality with the library's encoder. PARITY UNPINNED for ids 35-38 (DICT_FRQ with
+ *                              DELTA patches): their recorded sizes depend on the reference's std::sort tie order
+ *                              over an unordered_map; their layouts are those of ids 23-26.
  *   FSST decode                byte-identical to the compiled reference fsst_decompress on tables and strings
  *                              produced by the compiled reference fsst_create/fsst_compress (tests/golden fixtures),
  *                              and round trip with this file's own table builder.
