@@ -2,8 +2,9 @@
 
 For every kernel family (raw streams per element width and bit width; UNCOMPRESSED / FFOR / DELTA / RLE /
 dictionary pages per element width) it times candidate launch shapes -- threads per block, non-temporal stores,
-and a cap on the grid (persistent blocks) -- on the GPU and writes the fastest into a tuning table
-(`fastlanes_amd/tuning.txt` by default) that the library loads on first use. The row-group launch (family 6) is
+a cap on the grid (persistent blocks), direct stores for wide values, and for narrow RLE pages the vectors per
+wavefront (one vector with one position lane per thread, or two with two) -- on the GPU and writes the fastest into a
+tuning table (`fastlanes_amd/tuning.txt` by default) that the library loads on first use. The row-group launch (family 6) is
 tuned on the mixed 64-column row-group: KiB of decoded values per tile and blocks per compute unit. Entries of
 families that a run does not tune are kept from the existing table.
 
@@ -57,6 +58,10 @@ def tune_family(lib, family, W, bw, fn, stream, n_cu, bytes_moved):
     # integer pages of 32/64-bit values may also write without LDS staging (bit 1 of the store flags)
     nts = NTS + (2, 3) if (family in (FAM_UNC, FAM_FFOR, FAM_DELTA) and W >= 32) or \
         (family in (FAM_RLE, FAM_DICT) and W == 64) else NTS
+    # RLE pages of 8/16-bit values: one vector per wave with one position lane per thread, or (bit 2) two vectors per
+    # wave with two lanes per thread and LDS staging -- the vectors-per-wavefront choice of this family
+    if family == FAM_RLE and W <= 16:
+        nts = NTS + (4, 5)
     for block, nt, maxb in _candidates(n_cu, nts):
         fa.check(lib.fls_gpu_tune_set(family, W, bw, block, nt, maxb), "tune_set")
         t = _time(fn, stream)

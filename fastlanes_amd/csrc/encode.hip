@@ -310,7 +310,8 @@ int encode_impl(const void* dev_values, uint64_t n, enum data_t type, uint8_t ex
                      : (e.exp_t == EXP_RLE && e.var == 2)   ? 4
                                                             : -1;
     if (mode < 0) return fail(FLS_E_UNSUPPORTED, "the GPU encoder writes UNCOMPRESSED, FFOR, DELTA and RLE pages (no patches, no chains)");
-    if (type == DOUBLE && mode) return fail(FLS_E_UNSUPPORTED, "double columns support UNCOMPRESSED only");
+    // double RLE splits runs by double equality (+0.0 == -0.0, NaN != NaN): the host encoder writes those pages
+    if (type == DOUBLE && mode) return fail(FLS_E_UNSUPPORTED, "the GPU encoder writes double columns UNCOMPRESSED only");
     if (n % 1024) return fail(FLS_E_UNSUPPORTED, "value count must be a multiple of 1024");
     const unsigned    W = dtype_bits((uint8_t)type), SZ = W / 8, arr_c = e.arr_c;
     const uint64_t    n_vec = n / 1024;

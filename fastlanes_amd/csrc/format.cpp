@@ -95,7 +95,9 @@ bool expr_fits_type(const Expr& e, uint8_t dtype) {
     const unsigned W = dtype_bits(dtype);
     if (dtype == DT_STR) return e.exp_t == EXP_FSST;
     if (!W || e.exp_t == EXP_FSST) return false;
-    if (dtype == DT_DOUBLE) return e.exp_t == EXP_UNCOMPRESSED;
+    // doubles: UNCOMPRESSED and the RLE forms of the double pool (src/cor/exp_pool.cpp:105-112); its dictionary
+    // forms store the dictionary through ALP (src/cor/lyt/dic/dic.cpp: TDic<dbl_pt>::Compress), out of scope here
+    if (dtype == DT_DOUBLE) return e.exp_t == EXP_UNCOMPRESSED || e.exp_t == EXP_RLE;
     if (!e.dict && expr_patch(e) && W < 32) return false;            // patched values
     if (!e.dict && expr_rec(e) && W != 64) return false;             // THzlCmp<uint64_t> chains
     return true;
@@ -211,7 +213,6 @@ std::string parse_page(const Reader& r, uint64_t off, uint8_t dtype, PageEntry& 
     if (!W) return "unsupported data type";
     if (dtype == DT_STR && e.exp_t != EXP_FSST) return "string column with integer expression";
     if (dtype != DT_STR && e.exp_t == EXP_FSST) return "integer column with string expression";
-    if (dtype == DT_DOUBLE && e.exp_t != EXP_UNCOMPRESSED) return "double columns support UNCOMPRESSED only";
     if (!expr_fits_type(e, dtype)) return "expression not defined for this data type";
     const unsigned arr_c = e.arr_c;
     const uint64_t meta  = 8 + 4ull * arr_c + 4ull * arr_c * (pe.vec_c ? pe.vec_c - 1 : 0);
@@ -684,8 +685,23 @@ void shaped_vector(const Expr& e, S* t, Parts& parts, VecBufs& vb, bool sgn) {
     parts[o + 2] = {vb.packed.data(), 128ull * vb.bw};
 }
 
+// Whether consecutive values start a new run: bit inequality, or for doubles the reference's `!=` on double values
+// (Vec::Init compares the column's own type, src/cor/lyt/vec.cpp:50-71): +0.0 and -0.0 stay in one run, and a NaN
+// differs from everything, itself included.
 template <typename T>
-int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* meta, bool sgn) {
+bool run_break(T a, T b, bool dbl) {
+    if constexpr (sizeof(T) == 8) {
+        if (dbl) {
+            double x, y;
+            std::memcpy(&x, &a, 8), std::memcpy(&y, &b, 8);
+            return !(x == y);
+        }
+    }
+    return a != b;
+}
+
+template <typename T>
+int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* meta, bool sgn, bool dbl = false) {
     using U = T;
     const uint64_t        n_vec = n / 1024;
     std::vector<uint32_t> codes;
@@ -746,7 +762,7 @@ int64_t encode_t(const Expr& e, const T* in, uint64_t n, Buf& out, uint64_t* met
                 uint16_t        pos[1024], posT[1024], dl[1024];
                 uint64_t        r = 0;
                 for (unsigned i = 0; i < 1024; ++i) {
-                    const bool brk = i && (c ? c[i] != c[i - 1] : x[i] != x[i - 1]);
+                    const bool brk = i && (c ? c[i] != c[i - 1] : run_break<T>(x[i], x[i - 1], dbl));
                     if (i == 0 || brk) {
                         if (i) r++;
                         std::memcpy(vb.exc.data() + r * rsz, c ? (const void*)&c[i] : (const void*)&x[i], rsz);
@@ -855,7 +871,7 @@ int64_t encode_column(uint8_t dtype, uint8_t expr_id, const void* in, uint64_t n
                       uint64_t* metadata_size) {
     Expr e;
     if (!expr_from_id(dtype, expr_id, e) || dtype == DT_STR) return -1;
-    if (dtype == DT_DOUBLE && e.exp_t != EXP_UNCOMPRESSED) return -1;
+    if (!expr_fits_type(e, dtype)) return -1;
     if (n % 1024) return -1;
     Buf     b;
     int64_t r = -1;
@@ -863,7 +879,7 @@ int64_t encode_column(uint8_t dtype, uint8_t expr_id, const void* in, uint64_t n
     case 8: r = encode_t<uint8_t>(e, (const uint8_t*)in, n, b, metadata_size, false); break;
     case 16: r = encode_t<uint16_t>(e, (const uint16_t*)in, n, b, metadata_size, false); break;
     case 32: r = encode_t<uint32_t>(e, (const uint32_t*)in, n, b, metadata_size, false); break;
-    case 64: r = encode_t<uint64_t>(e, (const uint64_t*)in, n, b, metadata_size, dtype == DT_I64); break;
+    case 64: r = encode_t<uint64_t>(e, (const uint64_t*)in, n, b, metadata_size, dtype == DT_I64, dtype == DT_DOUBLE); break;
     default: return -1;
     }
     if (r < 0) return r;

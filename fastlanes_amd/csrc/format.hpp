@@ -69,7 +69,7 @@ bool expr_from_id(uint8_t dtype, uint8_t id, Expr& e);
 bool expr_from_page(uint8_t exp_t, uint8_t var, Expr& e);
 // Whether a column of this type may hold pages of the expression: patches resolve for 32/64-bit values only
 // (src/cor/prm/patch/*/..._compress.cpp ResolveCompressFunc), chains of THzlCmp<uint64_t> hold 64-bit values here,
-// doubles are UNCOMPRESSED only.
+// doubles take UNCOMPRESSED and the RLE forms (their pool's dictionaries are ALP-coded).
 bool expr_fits_type(const Expr& e, uint8_t dtype);
 
 // Variant numbering of DELTA (include/fls/cor/exp/delta/delta_exp.hpp:24-30), RLE

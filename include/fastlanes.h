@@ -65,7 +65,8 @@ enum i64_expr_encoding_t
 };
 
 /* double expression ids; reference include/fastlanes.h:44-47. DBL_ALP is not supported (ALP lives in an external,
- * unpinned dependency of the reference). */
+ * unpinned dependency of the reference). encode_from_memory also takes the double pool's RLE ids 6-10
+ * (src/cor/exp_pool.cpp:105-112): runs split where double != holds, run values kept as 8-byte patterns. */
 enum dbl_expr_encoding_t
 #ifdef __cplusplus
     : uint8_t

@@ -39,6 +39,11 @@
  *                              equality with the library's encoder. PARITY UNPINNED for ids 35-38 (DICT_FRQ with
  *                              DELTA patches): their recorded sizes depend on the reference's std::sort tie order
  *                              over an unordered_map; their layouts are those of ids 23-26.
+ *   double RLE (run_break)     all five RLE rows of dbl_pt_1vec.csv (ids 6-10: 855/2632/2644/867/879 B) at once on
+ *                              one 71-run float64 vector (70 or 72 runs miss each row by 8 B); run splits by
+ *                              double != (Vec::Init) checked against a literal Python float loop on +0.0/-0.0
+ *                              runs, NaN payload runs, infinities and subnormals; byte equality with the library's
+ *                              encoder.
  *   FSST decode                byte-identical to the compiled reference fsst_decompress on tables and strings
  *                              produced by the compiled reference fsst_create/fsst_compress (tests/golden fixtures),
  *                              and round trip with this file's own table builder.
@@ -342,7 +347,8 @@ static int expr_fits(uint8_t dtype, const expr_t* e) {
     unsigned W = dtype_bits(dtype);
     if (dtype == DT_STR) return e->exp_t == EXP_FSST;
     if (!W || e->exp_t == EXP_FSST) return 0;
-    if (dtype == DT_DOUBLE) return e->exp_t == EXP_UNCOMPRESSED;
+    /* doubles: UNCOMPRESSED and RLE (src/cor/exp_pool.cpp:105-112); the double pool's dictionaries are ALP-coded */
+    if (dtype == DT_DOUBLE) return e->exp_t == EXP_UNCOMPRESSED || e->exp_t == EXP_RLE;
     if (!e->dict && expr_patch(e) && W < 32) return 0;
     if (!e->dict && expr_rec(e) && W != 64) return 0;
     return 1;
@@ -718,8 +724,16 @@ static void enc_shaped(unsigned W, const expr_t* e, uint8_t* t, vec_parts* vp, u
     else enc_ffor_parts(W, t, vp, o, bwb, baseb, vp->b2, sgn);
 }
 
-/* sgn: signed ANALYZE of the column values (int64 columns) */
-static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint32_t* codes, vec_parts* vp, int sgn) {
+/* sgn: signed ANALYZE of the column values (int64 columns); dbl: runs split by double `!=` (+0.0 and -0.0 share a
+ * run, a NaN starts one), as Vec::Init compares a double column's values (src/cor/lyt/vec.cpp:50-71) */
+static int run_break(uint64_t a, uint64_t b, int dbl) {
+    if (!dbl) return a != b;
+    double x, y;
+    memcpy(&x, &a, 8), memcpy(&y, &b, 8);
+    return !(x == y);
+}
+static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint32_t* codes, vec_parts* vp, int sgn,
+                       int dbl) {
     static __thread uint8_t t[8 * 1024], bwb[8], baseb[8];
     const unsigned          sz = W / 8;
     switch (e->exp_t) {
@@ -740,7 +754,7 @@ static void enc_vector(unsigned W, const expr_t* e, const void* vals, const uint
         for (unsigned i = 0; i < 1024; ++i) {
             uint64_t x = rw == 32 && codes ? codes[i] : ld(W, vals, i);
             uint64_t p = i ? (rw == 32 && codes ? codes[i - 1] : ld(W, vals, i - 1)) : 0;
-            if (i == 0 || x != p) {
+            if (i == 0 || run_break(x, p, dbl && !codes)) {
                 if (i) r++;
                 st(rw, vp->b0, r, x);
             }
@@ -778,7 +792,7 @@ static int64_t encode_impl(uint8_t dtype, uint8_t expr_id, const void* in, uint6
     expr_t   e;
     unsigned W = dtype_bits(dtype);
     if (!W || expr_lookup(dtype, expr_id, &e)) return -1;
-    if (dtype == DT_DOUBLE && e.exp_t != EXP_UNCOMPRESSED) return -1;
+    if (!expr_fits(dtype, &e)) return -1;
     if (n % 1024) return -1;
     uint64_t  n_vec = n / 1024;
     uint32_t* codes = NULL;
@@ -802,7 +816,7 @@ static int64_t encode_impl(uint8_t dtype, uint8_t expr_id, const void* in, uint6
         if (stop > n_vec) stop = n_vec;
         while (v < stop) {
             enc_vector(W, &e, (const uint8_t*)in + v * 1024 * (W / 8), codes ? codes + v * 1024 : NULL, vp,
-                       dtype == DT_I64);
+                       dtype == DT_I64, dtype == DT_DOUBLE);
             uint64_t add = 4ULL * e.arr_c;
             for (unsigned i = 0; i < e.arr_c; ++i) add += vp->s[i];
             if (pg.vec_c && page_size(&pg) + add >= DATA_PG_SZ) break; /* TryFlushVec limit, compressor.cpp:47 */

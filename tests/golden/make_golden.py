@@ -10,8 +10,8 @@ Fixtures written (data only: inputs and expected outputs):
   unffor_helper.npz       the 124 seeded arrays of the reference's unffor test
                           (fallback_scalar_aav_1024_uf1_unffor_helper.hpp) with the bytes the compiled reference
                           ffor (base 10, as in ..._unffor_test.cpp:44-52) and pack produce for them
-  local_result_sizes.json encoded sizes per expression from test/local_result/col_result/generated/i64_pt1vec.csv
-                          and i64_pt16vec.csv
+  local_result_sizes.json encoded sizes per expression from test/local_result/col_result/generated/i64_pt1vec.csv,
+                          i64_pt16vec.csv and dbl_pt_1vec.csv
   fsst_golden.npz         strings of the synthetic vocabulary, the table and compressed bytes the compiled
                           reference fsst_create/fsst_compress produce, and the reference fsst_decompress output
   config1_golden.json     config "FFOR u32 bw 7, 2^20 values, seed 42": SHA-256 of the reference-encoded packed
@@ -80,7 +80,7 @@ def main():
 
     # local_result sizes
     sizes = {}
-    for fname in ("i64_pt1vec.csv", "i64_pt16vec.csv"):
+    for fname in ("i64_pt1vec.csv", "i64_pt16vec.csv", "dbl_pt_1vec.csv"):
         rows = list(csv.DictReader(open(os.path.join(a.ref, "test/local_result/col_result/generated", fname))))
         sizes[fname] = [{"id": int(r["ID"]), "exp": r["EXP_T/ENCODER_T"], "var": r["VAR_T/COMPRESSOR"],
                          "size": int(r["SIZE"])} for r in rows]

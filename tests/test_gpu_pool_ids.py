@@ -131,3 +131,59 @@ def test_delta_ll_chain_leaving_the_vector_is_reported(torch_cuda, oracle):
     plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
     with pytest.raises(fa.FlsError):
         plan.check()
+
+
+def _dbl_runs_column(n_vec, seed):
+    """float64 runs of 1-40 equal values over normal draws, with NaN payloads, both zeros and infinities mixed in."""
+    g = np.random.default_rng(seed)
+    n = n_vec * 1024
+    vals = np.round(g.standard_normal(n), 1)
+    vals[g.choice(n, n // 50, replace=False)] = -0.0
+    vals[g.choice(n, n // 50, replace=False)] = 0.0
+    vals[g.choice(n, n // 100, replace=False)] = np.inf
+    nan = (np.uint64(0x7FF8000000000000) | g.integers(1, 1 << 30, n // 100).astype(np.uint64)).view(np.float64)
+    vals[g.choice(n, n // 100, replace=False)] = nan
+    return vals[np.repeat(np.arange(n), g.integers(1, 41, n))[:n]]
+
+
+@pytest.mark.parametrize("eid", [6, 7, 8, 9, 10])
+def test_double_rle_pages(torch_cuda, oracle, eid):
+    """The double pool's RLE forms decode on the GPU to the oracle's bit patterns (runs split by IEEE !=: the run's
+    first pattern for +0.0/-0.0 runs, each NaN its own run), with the grid capped too, and predicates on them compare
+    as IEEE values (NaN selects nothing but !=, -0.0 equals 0.0)."""
+    torch = torch_cuda
+    lib = fa.lib()
+    x = _dbl_runs_column(37, seed=eid)
+    print(f"double RLE expr={eid}", flush=True)
+    enc = oracle.encode_column(x, eid)
+    want = oracle.decode_column(enc, np.float64, x.size)
+    plan = fa.Plan(enc, np.float64)
+    d_in = dev(torch, enc)
+    d_out = empty(torch, x.nbytes)
+    plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+    plan.check()
+    np.testing.assert_array_equal(to_np(d_out, np.uint64, x.size), want.view(np.uint64))
+    try:
+        for block, maxb in ((64, 1), (256, 2)):
+            print(f"double RLE expr={eid} block={block} max_blocks={maxb}", flush=True)
+            fa.check(lib.fls_gpu_tune_set(4, 64, 255, block, 1, maxb), "tune_set")
+            d_out = empty(torch, x.nbytes)
+            plan.decode(d_in.data_ptr(), d_out.data_ptr(), x.nbytes)
+            plan.check()
+            np.testing.assert_array_equal(to_np(d_out, np.uint64, x.size), want.view(np.uint64))
+    finally:
+        path = os.path.join(os.path.dirname(os.path.abspath(fa.__file__)), "tuning.txt")
+        lib.fls_gpu_tune_load.argtypes = [C.c_char_p]
+        assert lib.fls_gpu_tune_load(path.encode()) > 0
+    n_vec = x.size // 1024
+    for op, fn in PRED.items():
+        for c in (0.0, -0.0, 0.5, np.inf, np.nan):
+            print(f"double RLE expr={eid} select {op} {c}", flush=True)
+            d_bm = torch.zeros(16 * n_vec, dtype=torch.int64, device="cuda")
+            d_cnt = torch.zeros(n_vec, dtype=torch.int32, device="cuda")
+            plan.select(d_in.data_ptr(), op, c, d_bm.data_ptr(), d_cnt.data_ptr())
+            plan.check()
+            with np.errstate(invalid="ignore"):
+                mask = fn(want, c)
+            np.testing.assert_array_equal(d_bm.cpu().numpy().view(np.uint64), _bitmap_words(mask), err_msg=f"{op} {c}")
+            np.testing.assert_array_equal(d_cnt.cpu().numpy(), mask.reshape(-1, 1024).sum(axis=1))

@@ -215,6 +215,62 @@ def test_frequency_delta_patches_round_trip(oracle, recorded):
     assert sizes[38] == sizes[37] + 12
 
 
+def _dbl_runs_vector(runs=71):
+    """One float64 vector of `runs` equal-length-ish runs of distinct values (run r holds 0.5 r)."""
+    return (np.arange(1024) * runs // 1024).astype(np.float64) * 0.5
+
+
+def test_double_rle_matches_recorded(oracle, recorded):
+    """The double pool's RLE forms (6-10, src/cor/exp_pool.cpp:105-112) on one vector of 71 runs reproduce every RLE
+    row of dbl_pt_1vec.csv at once: the run values are 8-byte patterns (71 x 8 B), positions u16 (7: 2048 B) or
+    DELTA bit-packed at bw 1 with 64 lane bases (6), the chains add 12 B each (8, 9: run values; 10: lane bases too).
+    70 or 72 runs miss every row by 8 B, so the run count is fixed by the sizes themselves."""
+    rows = {r["id"]: r["size"] for r in recorded["dbl_pt_1vec.csv"]}
+    x = _dbl_runs_vector()
+    for eid in (6, 7, 8, 9, 10):
+        enc = oracle.encode_column(x, eid)
+        assert enc.size == rows[eid], (eid, enc.size, rows[eid])
+        np.testing.assert_array_equal(fa.encode_from_memory(x, eid), enc)
+        got = oracle.decode_column(enc, np.float64, x.size)
+        np.testing.assert_array_equal(got.view(np.uint64), x.view(np.uint64))
+    for r in (70, 72):
+        assert oracle.encode_column(_dbl_runs_vector(r), 6).size == rows[6] + 8 * (r - 71)
+
+
+def _ieee_runs(x):
+    """Each value replaced by the first value of its run, runs split where double != holds (Vec::Init,
+    src/cor/lyt/vec.cpp:50-71), by a literal Python loop over float values."""
+    out = x.copy()
+    for i in range(1, x.size):
+        if not (float(x[i]) != float(x[i - 1])):
+            out[i] = out[i - 1]
+    return out
+
+
+@pytest.mark.parametrize("eid", [6, 7, 8, 9, 10])
+def test_double_rle_signed_zero_and_nan(oracle, eid):
+    """Runs of a double column split by IEEE !=: +0.0 and -0.0 share a run and decode as the run's first bit
+    pattern; NaNs never extend a run, so each keeps its own payload; infinities and subnormals are ordinary values.
+    Both encoders agree byte for byte over several vectors."""
+    g = np.random.default_rng(eid)
+    n = 4 * 1024
+    vals = g.choice([0.0, -0.0, 1.5, -2.25, np.inf, -np.inf, 5e-324], n)
+    x = vals[np.repeat(np.arange(n), g.integers(1, 6, n))[:n]]  # runs of 1-5 equal draws
+    nan_payloads = (np.uint64(0x7FF8000000000000) | g.integers(1, 1 << 20, 40).astype(np.uint64)).view(np.float64)
+    x[g.choice(n, 40, replace=False)] = nan_payloads
+    x[100:110] = np.nan                              # ten NaNs in a row: ten runs
+    x[200:204] = [0.0, -0.0, -0.0, 0.0]              # one run, decoded as +0.0
+    x[300:303] = [-0.0, 0.0, -0.0]                   # one run, decoded as -0.0
+    x[[199, 204, 299, 303]] = 7.0                    # fence both runs off their neighbours
+    enc = oracle.encode_column(x, eid)
+    np.testing.assert_array_equal(fa.encode_from_memory(x, eid), enc)
+    got = oracle.decode_column(enc, np.float64, n)
+    want = _ieee_runs(x)
+    np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+    assert np.signbit(got[201]) == False and np.signbit(got[301]) == True  # noqa: E712
+    assert np.isnan(got[100:110]).all()
+
+
 def _page_buffers(enc, arr_c, start=0):
     """Buffer slices of a one-vector page at enc[start:] (the last buffer runs to the end)."""
     off = np.frombuffer(enc[start + 8: start + 8 + 4 * arr_c].tobytes(), np.uint32).astype(int) + start
