@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
     L.fls_gpu_encode_noalloc.argtypes = [_vp, _u64, C.c_uint8, C.c_uint8, _vp, _u64, C.POINTER(_u64), _vp]
     L.fls_gpu_plan_string_offsets.argtypes = [_vp, _vp, _vp, _vp]
     L.fls_gpu_plan_decode_strings.argtypes = [_vp, _vp, _vp, _vp, _u64, _vp]
+    L.fls_gpu_plan_decode_strings_fused.argtypes = [_vp, _vp, _vp, _vp, _u64, _vp]
     L.fls_gpu_unffor_stream.argtypes = [C.c_uint, C.c_uint, _vp, _vp, _vp, _u64, _vp]
     L.fls_gpu_rowgroup_create.argtypes = [_u32, C.POINTER(_vp), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(C.c_uint8),
                                           C.POINTER(_vp)]
@@ -154,6 +155,11 @@ class Plan:
     def decode_strings(self, dev_encoded_ptr, dev_offsets_ptr, dev_bytes_ptr, cap, stream: int = 0) -> None:
         check(lib().fls_gpu_plan_decode_strings(self.h, dev_encoded_ptr, dev_offsets_ptr, dev_bytes_ptr, cap, stream),
               "decode_strings")
+
+    def decode_strings_fused(self, dev_encoded_ptr, dev_offsets_ptr, dev_bytes_ptr, cap, stream: int = 0) -> None:
+        """Offsets and bytes in one pass (one launch per page)."""
+        check(lib().fls_gpu_plan_decode_strings_fused(self.h, dev_encoded_ptr, dev_offsets_ptr, dev_bytes_ptr, cap,
+                                                      stream), "decode_strings_fused")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -262,6 +268,11 @@ class RowGroup:
 
     def check(self, stream: int = 0) -> None:
         check(lib().fls_gpu_rowgroup_check(self.h, stream), "rowgroup_check")
+
+    def decode_strings_fused(self, dev_encoded_ptr, dev_offsets_ptr, dev_bytes_ptr, cap, stream: int = 0) -> None:
+        """Offsets and bytes in one pass (one launch per page)."""
+        check(lib().fls_gpu_plan_decode_strings_fused(self.h, dev_encoded_ptr, dev_offsets_ptr, dev_bytes_ptr, cap,
+                                                      stream), "decode_strings_fused")
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

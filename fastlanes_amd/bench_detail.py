@@ -102,9 +102,8 @@ def cascade(dev, stream, log2n=26, log2_strings=24, verbose=True):
     d_bytes = torch.empty(data.size + 64, dtype=torch.uint8, device=dev)
     sp = int(stream.cuda_stream)
 
-    def fn():
-        plan.string_offsets(d_in.data_ptr(), d_off.data_ptr(), sp)
-        plan.decode_strings(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size, sp)
+    def fn():  # offsets and bytes in one pass
+        plan.decode_strings_fused(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size, sp)
 
     t = _events_time(fn, stream, reps=3)
     ok = bool(np.array_equal(d_bytes[: data.size].cpu().numpy(), data)) and bool(

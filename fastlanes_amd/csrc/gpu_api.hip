@@ -352,6 +352,14 @@ int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, 
                           (hipStream_t)stream);
 }
 
+int fls_gpu_plan_decode_strings_fused(fls_gpu_plan* plan, const uint8_t* dev_encoded, uint64_t* dev_offsets,
+                                      uint8_t* dev_bytes, uint64_t bytes_capacity, void* stream) {
+    if (!plan || !dev_encoded || !dev_offsets || !dev_bytes) return fail(FLS_E_ARG, "null argument");
+    if (plan->col.dtype != STR) return fail(FLS_E_ARG, "not a string column");
+    return strings_decode_all(plan->col, plan->strings, dev_encoded, dev_offsets, dev_bytes, bytes_capacity,
+                              (hipStream_t)stream);
+}
+
 int fls_encode_strings(const uint32_t* offsets, const uint8_t* bytes, uint64_t n, uint8_t* out, uint64_t capacity,
                        uint64_t* out_size) {
     if (!offsets || (!bytes && n) || !out || !out_size) return fail(FLS_E_ARG, "null argument");

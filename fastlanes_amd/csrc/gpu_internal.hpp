@@ -19,10 +19,13 @@ struct FsstTableDev {
 };
 
 struct StringPlan {
-    FsstTableDev* d_tables  = nullptr;  // one per page entry
-    uint64_t*     d_vec_tot = nullptr;  // decoded bytes per half-vector segment, then their exclusive prefix
-    uint32_t*     d_err     = nullptr;  // set when string offsets exceed the output capacity
-    uint64_t      n_vec     = 0;
+    FsstTableDev* d_tables = nullptr;  // one per page entry
+    uint64_t*     d_status = nullptr;  // look-back status word per half-vector segment (zeroed once)
+    uint64_t*     d_ticket = nullptr;  // segment ticket counter of the look-back launches (never reset)
+    uint32_t*     d_err    = nullptr;  // error bits: capacity, look-back
+    uint64_t      n_vec    = 0;
+    uint64_t      tickets  = 0;        // tickets handed out so far (the counter's value between launches)
+    uint32_t      epoch    = 0;        // call number tagging the status words (1..65535)
 };
 
 int  strings_prepare(const Column& col, const uint8_t* host, StringPlan& sp);
@@ -31,5 +34,7 @@ int  strings_offsets(const Column& col, StringPlan& sp, const uint8_t* dev_enc, 
 int  strings_check(StringPlan& sp, hipStream_t s);
 int  strings_decode(const Column& col, StringPlan& sp, const uint8_t* dev_enc, const uint64_t* dev_offsets,
                     uint8_t* dev_bytes, uint64_t cap, hipStream_t s);
+int  strings_decode_all(const Column& col, StringPlan& sp, const uint8_t* dev_enc, uint64_t* dev_offsets,
+                        uint8_t* dev_bytes, uint64_t cap, hipStream_t s);
 
 }  // namespace fls

@@ -24,6 +24,8 @@ def main():
     ap.add_argument("--copy-log2-bytes", type=int, default=30)
     ap.add_argument("--fsst-log2-strings", type=int, default=0,
                     help="instead: decode an FSST string column of 2^k vocabulary strings (offsets + bytes)")
+    ap.add_argument("--fsst-two-step", action="store_true",
+                    help="with --fsst-log2-strings: offsets, then bytes (two launches per page) instead of one pass")
     ap.add_argument("--dict-rle", action="store_true",
                     help="instead: decode a DICT_RLE column (pool id 27) of 2^log2n values of --width bits, runs of "
                          "mean length 16 over 4096 keys")
@@ -105,6 +107,8 @@ def main():
         return
 
     if a.fsst_log2_strings:
+        import numpy as np
+
         from fastlanes_amd import workloads as wl
 
         ns = 1 << a.fsst_log2_strings
@@ -115,9 +119,15 @@ def main():
         d_off = torch.empty(ns + 1, dtype=torch.int64, device="cuda")
         d_bytes = torch.empty(data.size + 64, dtype=torch.uint8, device="cuda")
         for _ in range(a.reps):
-            plan.string_offsets(d_in.data_ptr(), d_off.data_ptr())
-            plan.decode_strings(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size)
+            if a.fsst_two_step:
+                plan.string_offsets(d_in.data_ptr(), d_off.data_ptr())
+                plan.decode_strings(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size)
+            else:
+                plan.decode_strings_fused(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size)
         torch.cuda.synchronize()
+        plan.check()
+        assert np.array_equal(d_bytes[: data.size].cpu().numpy(), data), "decoded bytes differ"
+        assert np.array_equal(d_off.cpu().numpy(), offs.astype(np.int64)), "decoded offsets differ"
         print(json.dumps({"strings": ns, "raw_bytes": int(data.size), "encoded_bytes": int(enc.size),
                           "pages": plan.n_pages, "reps": a.reps}), flush=True)
         return

@@ -59,6 +59,12 @@ int fls_gpu_plan_string_offsets(fls_gpu_plan* plan, const uint8_t* dev_encoded, 
                                 void* stream);
 int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, const uint64_t* dev_offsets,
                                 uint8_t* dev_bytes, uint64_t bytes_capacity, void* stream);
+/* Steps 1 and 2 in one pass over the compressed bytes: writes the n+1 offsets and the bytes (bytes_capacity bounds
+ * the bytes; a column that needs more reports FLS_E_DATA at fls_gpu_plan_check). No reference counterpart: the
+ * reference decodes string by string on the CPU. Calls on one plan must not overlap (they share its segment status
+ * words). */
+int fls_gpu_plan_decode_strings_fused(fls_gpu_plan* plan, const uint8_t* dev_encoded, uint64_t* dev_offsets,
+                                      uint8_t* dev_bytes, uint64_t bytes_capacity, void* stream);
 
 /* ---- predicate pushdown (reference src/primitive/predicate/equal.cpp, include/fls/expression/predicate_operator.hpp,
  * include/fls/expression/selection_ds.hpp) ----------------------------------------------------------------------
