@@ -211,20 +211,30 @@ __device__ VecStats vector_stats(const uint64_t* __restrict__ x) {
     return s;
 }
 
+// Atomics on a block's own region of global memory (the sizer's hash table): only the block's threads touch it,
+// so workgroup scope is enough, and the operations stay in the XCD's L2 instead of the device-coherent path.
+__device__ __forceinline__ uint64_t wg_cas(uint64_t* p, uint64_t expected, uint64_t desired) {
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    return expected;  // the value found (equal to the expected one when the exchange happened)
+}
+__device__ __forceinline__ void wg_min(uint32_t* p, uint32_t v) {
+    __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Insert v (first seen at position o) into an open-addressing table whose empty keys are kEmpty; the value kEmpty
 // itself lives in the extra slot `mask + 1`. Returns the slot (never fails: callers size tables above their input).
-template <typename KeyPtr, typename FirstPtr>
-__device__ __forceinline__ uint32_t ht_insert(KeyPtr keys, FirstPtr first, uint32_t mask, uint64_t v, uint32_t o,
+__device__ __forceinline__ uint32_t ht_insert(uint64_t* keys, uint32_t* first, uint32_t mask, uint64_t v, uint32_t o,
                                               uint32_t start = ~0u) {
     if (v == kEmpty) {
-        atomicMin(&first[mask + 1], o);
+        wg_min(&first[mask + 1], o);
         return mask + 1;
     }
     uint32_t h = start == ~0u ? mix64(v) & mask : start;
     for (uint32_t p = 0; p <= mask; ++p) {
-        const uint64_t old = atomicCAS((unsigned long long*)&keys[h], (unsigned long long)kEmpty, (unsigned long long)v);
+        const uint64_t old = wg_cas(&keys[h], kEmpty, v);
         if (old == kEmpty || old == v) {
-            atomicMin(&first[h], o);
+            wg_min(&first[h], o);
             return h;
         }
         h = (h + 1) & mask;
@@ -389,26 +399,44 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
             uint64_t           v[kBatch];
             uint32_t           h[kBatch];
             unsigned long long old[kBatch];
+            bool               lead[kBatch];
 #pragma unroll
             for (unsigned k = 0; k < kBatch; ++k) v[k] = x[o0 + k * stride];
-#pragma unroll
-            for (unsigned k = 0; k < kBatch; ++k) h[k] = v[k] == kEmpty ? kHt : mix64(v[k]) & (kHt - 1);
-#pragma unroll
-            for (unsigned k = 0; k < kBatch; ++k)
-                old[k] = v[k] == kEmpty ? 0ull
-                                        : atomicCAS((unsigned long long*)&keys[h[k]], (unsigned long long)kEmpty,
-                                                    (unsigned long long)v[k]);
+            // a run of equal values at consecutive positions of the wave: only its first lane inserts (the others'
+            // positions are no first occurrences) and the others take its slot -- many lanes exchanging or
+            // min-ing one slot would serialize on that address
 #pragma unroll
             for (unsigned k = 0; k < kBatch; ++k) {
-                const unsigned o = o0 + k * stride;
-                uint32_t       hs;
-                if (v[k] == kEmpty || old[k] == kEmpty || old[k] == v[k]) {
-                    hs = h[k];
-                    atomicMin(&first[hs], o);
-                } else {  // the first probe met another value: continue probing from the next slot
-                    hs = ht_insert(keys, first, kHt - 1, v[k], o, (h[k] + 1) & (kHt - 1));
+                const uint64_t pv = __shfl_up(v[k], 1, 64);
+                lead[k]           = lane == 0 || pv != v[k];
+            }
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) h[k] = v[k] == kEmpty ? kHt : mix64(v[k]) & (kHt - 1);
+            // a key already in its slot needs no exchange (within a row-group a slot only goes from empty to its
+            // value); agent-scope loads read the L2, where the exchanges happen
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k)
+                old[k] = (!lead[k] || v[k] == kEmpty) ? 0ull
+                                                      : __hip_atomic_load(&keys[h[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k)
+                if (lead[k] && v[k] != kEmpty && old[k] == kEmpty) old[k] = wg_cas(&keys[h[k]], kEmpty, v[k]);
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) {
+                const unsigned o  = o0 + k * stride;
+                uint32_t       hs = 0;
+                if (lead[k]) {
+                    if (v[k] == kEmpty || old[k] == kEmpty || old[k] == v[k]) {
+                        hs               = h[k];
+                        const uint32_t f = __hip_atomic_load(&first[hs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (f > o) wg_min(&first[hs], o);
+                    } else {  // the first probe met another value: continue probing from the next slot
+                        hs = ht_insert(keys, first, kHt - 1, v[k], o, (h[k] + 1) & (kHt - 1));
+                    }
                 }
-                slot[o] = hs;
+                const uint64_t lm  = __ballot(lead[k]);  // lane 0 always leads
+                const unsigned src = 63u - (unsigned)__clzll((long long)(lm & ((2ull << lane) - 1)));
+                slot[o]            = (uint32_t)__shfl((int)hs, (int)src, 64);
             }
         }
         __syncthreads();
@@ -559,6 +587,7 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
             for (unsigned k = 0; k < kBatch; ++k) sl[k] = slot[o0 + k * stride];
 #pragma unroll
             for (unsigned k = 0; k < kBatch; ++k) {
+                if (fpos[o0 + k * stride] != o0 + k * stride) continue;  // each slot once: at its first occurrence
                 keys[sl[k]]  = kEmpty;  // slot kHt (the kEmpty value's own slot) is padding here
                 first[sl[k]] = ~0u;
             }
