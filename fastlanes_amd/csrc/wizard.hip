@@ -70,20 +70,47 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Sorts the 1024 values of buf (the wave's LDS buffer) ascending: a bitonic network, 8 pairs per lane per stage.
+// Sorts the 1024 values of buf (the wave's LDS buffer) ascending with a bitonic network held in registers: lane l
+// keeps elements 16l .. 16l+15, so the stages over distances 1-8 (34 of 55) are compare-exchanges inside a lane and
+// the stages over distances 16-512 exchange with lane l ^ (distance / 16) through shuffles; LDS is read once and
+// written once.
+__device__ __forceinline__ void ce_regs(uint64_t& a, uint64_t& b, bool desc) {
+    const bool     sw = (a > b) != desc;
+    const uint64_t x = sw ? b : a, y = sw ? a : b;
+    a = x, b = y;
+}
 __device__ void wave_sort1024(uint64_t* buf) {
     const unsigned lane = threadIdx.x % 64;
-    for (unsigned k = 2; k <= 1024; k <<= 1)
-        for (unsigned j = k >> 1; j > 0; j >>= 1) {
-            wave_lds_sync();
+    uint64_t       v[16];
+    wave_lds_sync();
 #pragma unroll
-            for (unsigned m = 0; m < 8; ++m) {
-                const unsigned t = lane + 64 * m;           // pair 0..511
-                const unsigned i = 2 * t - (t & (j - 1));   // bit j of i is clear; its partner is i + j
-                const uint64_t a = buf[i], b = buf[i + j];
-                if ((a > b) == ((i & k) == 0)) buf[i] = b, buf[i + j] = a;
+    for (unsigned r = 0; r < 16; ++r) v[r] = buf[64 * r + lane];  // any starting order will do
+#pragma unroll
+    for (unsigned k = 2; k <= 1024; k <<= 1) {
+        // descending blocks of size k: element i = 16 lane + r with (i & k) != 0
+#pragma unroll
+        for (unsigned j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 16) {
+                const unsigned d       = j / 16;
+                const bool     desc    = k < 1024 && (((16 * lane) & k) != 0);
+                const bool     takemin = ((lane & d) == 0) != desc;  // the lower index of an ascending pair
+#pragma unroll
+                for (unsigned r = 0; r < 16; ++r) {
+                    const uint64_t p = __shfl_xor((unsigned long long)v[r], (int)d, 64);
+                    v[r]             = ((v[r] < p) == takemin) ? v[r] : p;
+                }
+            } else {
+#pragma unroll
+                for (unsigned r = 0; r < 16; ++r) {
+                    if (r & j) continue;
+                    const bool desc = k < 16 ? (r & k) != 0 : (k < 1024 && ((16 * lane) & k) != 0);
+                    ce_regs(v[r], v[r | j], desc);
+                }
             }
         }
+    }
+#pragma unroll
+    for (unsigned r = 0; r < 16; ++r) buf[16 * lane + r] = v[r];
     wave_lds_sync();
 }
 
@@ -213,7 +240,7 @@ __device__ VecStats vector_stats(const uint64_t* __restrict__ x) {
 
 // Atomics on a block's own region of global memory (the sizer's hash table): only the block's threads touch it,
 // so workgroup scope is enough, and the operations stay in the XCD's L2 instead of the device-coherent path.
-__device__ __forceinline__ uint64_t wg_cas(uint64_t* p, uint64_t expected, uint64_t desired) {
+__device__ __forceinline__ uint32_t wg_cas(uint32_t* p, uint32_t expected, uint32_t desired) {
     __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_WORKGROUP);
     return expected;  // the value found (equal to the expected one when the exchange happened)
@@ -221,25 +248,30 @@ __device__ __forceinline__ uint64_t wg_cas(uint64_t* p, uint64_t expected, uint6
 __device__ __forceinline__ void wg_min(uint32_t* p, uint32_t v) {
     __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+__device__ __forceinline__ uint32_t l2_load(const uint32_t* p) {  // reads the L2, where the exchanges happen
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
-// Insert v (first seen at position o) into an open-addressing table whose empty keys are kEmpty; the value kEmpty
-// itself lives in the extra slot `mask + 1`. Returns the slot (never fails: callers size tables above their input).
-__device__ __forceinline__ uint32_t ht_insert(uint64_t* keys, uint32_t* first, uint32_t mask, uint64_t v, uint32_t o,
-                                              uint32_t start = ~0u) {
-    if (v == kEmpty) {
-        wg_min(&first[mask + 1], o);
-        return mask + 1;
-    }
-    uint32_t h = start == ~0u ? mix64(v) & mask : start;
+// The sizer's open-addressing table keeps, per slot, the smallest position of its value seen so far (kNoPos: empty);
+// keys compare through the row-group's own values (x[position], resident in the L2 while the row-group is sized),
+// so a slot is 4 bytes and the block's table 512 KiB. A claimed slot only ever holds positions of one value.
+constexpr uint32_t kNoPos = ~0u;
+// Probes from slot h, whose content was read as cur, for value v at position o; returns v's slot.
+__device__ __forceinline__ uint32_t ht_insert(uint32_t* tab, const uint64_t* __restrict__ x, uint32_t mask, uint64_t v,
+                                              uint32_t o, uint32_t h, uint32_t cur) {
     for (uint32_t p = 0; p <= mask; ++p) {
-        const uint64_t old = wg_cas(&keys[h], kEmpty, v);
-        if (old == kEmpty || old == v) {
-            wg_min(&first[h], o);
+        if (cur == kNoPos) {
+            cur = wg_cas(&tab[h], kNoPos, o);
+            if (cur == kNoPos) return h;  // claimed
+        }
+        if (x[cur] == v) {
+            if (o < cur) wg_min(&tab[h], o);
             return h;
         }
-        h = (h + 1) & mask;
+        h   = (h + 1) & mask;
+        cur = l2_load(&tab[h]);
     }
-    return mask + 1;  // unreachable: the table holds more slots than distinct inputs
+    return 0;  // unreachable: the table holds more slots than distinct inputs
 }
 
 __device__ __forceinline__ uint32_t rank_of(const uint32_t* bitmap, const uint32_t* base, uint32_t f) {
@@ -338,15 +370,14 @@ struct SizerLds {
 };
 
 __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restrict__ values, uint64_t n_rg,
-                                                       uint64_t* __restrict__ tables, uint32_t* __restrict__ firsts_all,
-                                                       uint32_t* __restrict__ slots_all, uint64_t* __restrict__ sizes) {
+                                                       uint32_t* __restrict__ firsts_all, uint32_t* __restrict__ slots_all,
+                                                       uint64_t* __restrict__ sizes) {
     // dynamic LDS (the launch passes sizeof(SizerLds), about 150 KB)
     extern __shared__ __attribute__((aligned(16))) uint8_t wiz_lds[];
     SizerLds&      L = *reinterpret_cast<SizerLds*>(wiz_lds);
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
-    uint64_t* keys   = tables + (uint64_t)blockIdx.x * (kHt + 64);  // kHt keys + padding
-    uint32_t* first  = firsts_all + (uint64_t)blockIdx.x * (2 * kHt + 64);
-    uint32_t* fpos   = first + kHt + 32;                               // first occurrence per position (kRG)
+    uint32_t* first  = firsts_all + (uint64_t)blockIdx.x * (2 * kHt + 64);  // the table: first position per slot
+    uint32_t* fpos   = first + kHt + 32;                                   // first occurrence per position (kRG)
     uint32_t* slot   = slots_all + (uint64_t)blockIdx.x * kRG;
     for (uint64_t r = blockIdx.x; r < n_rg; r += gridDim.x) {
         const uint64_t* x = values + r * kRG;
@@ -396,10 +427,9 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
         constexpr unsigned kBatch = 4;
         const unsigned     stride = blockDim.x;
         for (unsigned o0 = t; o0 < kRG; o0 += kBatch * stride) {
-            uint64_t           v[kBatch];
-            uint32_t           h[kBatch];
-            unsigned long long old[kBatch];
-            bool               lead[kBatch];
+            uint64_t v[kBatch];
+            uint32_t h[kBatch], cur[kBatch], hs[kBatch];
+            bool     lead[kBatch], done[kBatch];
 #pragma unroll
             for (unsigned k = 0; k < kBatch; ++k) v[k] = x[o0 + k * stride];
             // a run of equal values at consecutive positions of the wave: only its first lane inserts (the others'
@@ -409,34 +439,40 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
             for (unsigned k = 0; k < kBatch; ++k) {
                 const uint64_t pv = __shfl_up(v[k], 1, 64);
                 lead[k]           = lane == 0 || pv != v[k];
+                h[k]              = mix64(v[k]) & (kHt - 1);
+                hs[k]             = h[k];
             }
+            // the home slots' contents, then the exchanges into the empty ones, then the value comparisons: the
+            // batch's loads and atomics of each step are in flight together
 #pragma unroll
-            for (unsigned k = 0; k < kBatch; ++k) h[k] = v[k] == kEmpty ? kHt : mix64(v[k]) & (kHt - 1);
-            // a key already in its slot needs no exchange (within a row-group a slot only goes from empty to its
-            // value); agent-scope loads read the L2, where the exchanges happen
-#pragma unroll
-            for (unsigned k = 0; k < kBatch; ++k)
-                old[k] = (!lead[k] || v[k] == kEmpty) ? 0ull
-                                                      : __hip_atomic_load(&keys[h[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-            for (unsigned k = 0; k < kBatch; ++k)
-                if (lead[k] && v[k] != kEmpty && old[k] == kEmpty) old[k] = wg_cas(&keys[h[k]], kEmpty, v[k]);
+            for (unsigned k = 0; k < kBatch; ++k) cur[k] = lead[k] ? l2_load(&first[h[k]]) : 0u;
 #pragma unroll
             for (unsigned k = 0; k < kBatch; ++k) {
-                const unsigned o  = o0 + k * stride;
-                uint32_t       hs = 0;
-                if (lead[k]) {
-                    if (v[k] == kEmpty || old[k] == kEmpty || old[k] == v[k]) {
-                        hs               = h[k];
-                        const uint32_t f = __hip_atomic_load(&first[hs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        if (f > o) wg_min(&first[hs], o);
-                    } else {  // the first probe met another value: continue probing from the next slot
-                        hs = ht_insert(keys, first, kHt - 1, v[k], o, (h[k] + 1) & (kHt - 1));
-                    }
+                done[k] = !lead[k];
+                if (!done[k] && cur[k] == kNoPos) {
+                    cur[k]  = wg_cas(&first[h[k]], kNoPos, o0 + k * stride);
+                    done[k] = cur[k] == kNoPos;
                 }
+            }
+            uint64_t xv[kBatch];
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) xv[k] = done[k] ? 0ull : x[cur[k]];
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) {
+                const unsigned o = o0 + k * stride;
+                if (done[k]) continue;
+                if (xv[k] == v[k]) {
+                    if (o < cur[k]) wg_min(&first[h[k]], o);
+                } else {  // the home slot holds another value: probe on
+                    hs[k] = ht_insert(first, x, kHt - 1, v[k], o, (h[k] + 1) & (kHt - 1),
+                                      l2_load(&first[(h[k] + 1) & (kHt - 1)]));
+                }
+            }
+#pragma unroll
+            for (unsigned k = 0; k < kBatch; ++k) {
                 const uint64_t lm  = __ballot(lead[k]);  // lane 0 always leads
                 const unsigned src = 63u - (unsigned)__clzll((long long)(lm & ((2ull << lane) - 1)));
-                slot[o]            = (uint32_t)__shfl((int)hs, (int)src, 64);
+                slot[o0 + k * stride] = (uint32_t)__shfl((int)hs[k], (int)src, 64);
             }
         }
         __syncthreads();
@@ -446,7 +482,7 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
             for (unsigned k = 0; k < kBatch; ++k) sl[k] = slot[o0 + k * stride];
 #pragma unroll
             for (unsigned k = 0; k < kBatch; ++k)
-                f[k] = __hip_atomic_load(&first[sl[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                f[k] = l2_load(&first[sl[k]]);
 #pragma unroll
             for (unsigned k = 0; k < kBatch; ++k) {
                 const unsigned o = o0 + k * stride;
@@ -588,8 +624,7 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
 #pragma unroll
             for (unsigned k = 0; k < kBatch; ++k) {
                 if (fpos[o0 + k * stride] != o0 + k * stride) continue;  // each slot once: at its first occurrence
-                keys[sl[k]]  = kEmpty;  // slot kHt (the kEmpty value's own slot) is padding here
-                first[sl[k]] = ~0u;
+                first[sl[k]] = kNoPos;
             }
         }
         __syncthreads();
@@ -808,7 +843,6 @@ __global__ __launch_bounds__(256) void k_wizard_features(const uint64_t* __restr
 
 // per-device workspace of the exhaustive sizer
 struct SizerWs {
-    uint64_t* tables = nullptr;
     uint32_t* firsts = nullptr;
     uint32_t* slots  = nullptr;
     unsigned  blocks = 0;
@@ -823,17 +857,14 @@ int sizer_ws(SizerWs*& out) {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
     SizerWs& w = g_ws[dev];
-    if (!w.tables) {
+    if (!w.firsts) {
         hipDeviceProp_t prop;
         if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return hip_fail(e, "device properties");
         w.blocks = (unsigned)prop.multiProcessorCount;
-        const size_t tb = (size_t)w.blocks * (kHt + 64) * 8, fb = (size_t)w.blocks * (2 * kHt + 64) * 4,
-                     sb = (size_t)w.blocks * kRG * 4;
-        if ((e = hipMalloc(&w.tables, tb)) != hipSuccess) return hip_fail(e, "hipMalloc");
+        const size_t fb = (size_t)w.blocks * (2 * kHt + 64) * 4, sb = (size_t)w.blocks * kRG * 4;
         if ((e = hipMalloc(&w.firsts, fb)) != hipSuccess) return hip_fail(e, "hipMalloc");
         if ((e = hipMalloc(&w.slots, sb)) != hipSuccess) return hip_fail(e, "hipMalloc");
-        // empty keys and first positions; the kernels restore this state after every row-group
-        if ((e = hipMemset(w.tables, 0xFF, tb)) != hipSuccess) return hip_fail(e, "hipMemset");
+        // empty slots; the kernel restores this state after every row-group
         if ((e = hipMemset(w.firsts, 0xFF, fb)) != hipSuccess) return hip_fail(e, "hipMemset");
         if ((e = hipDeviceSynchronize()) != hipSuccess) return hip_fail(e, "sync");
     }
@@ -860,7 +891,7 @@ int fls_wizard_sizes(const int64_t* dev_values, uint64_t n_rowgroups, uint64_t* 
     if (rc) return rc;
     const unsigned blocks = (unsigned)std::min<uint64_t>(n_rowgroups, w->blocks);
     hipLaunchKernelGGL(k_wizard_sizes, dim3(blocks), dim3(1024), sizeof(SizerLds), (hipStream_t)stream, (const uint64_t*)dev_values,
-                       n_rowgroups, w->tables, w->firsts, w->slots, dev_sizes);
+                       n_rowgroups, w->firsts, w->slots, dev_sizes);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? FLS_OK : hip_fail(e, "wizard sizes launch");
 }

@@ -156,3 +156,23 @@ def test_training_gains_over_rule_without_network(torch_cuda, wiz):
         plan.decode(d_enc.data_ptr(), d_out.data_ptr(), x.nbytes)
         plan.check()
         np.testing.assert_array_equal(d_out.cpu().numpy().view(np.int64), x, err_msg=f"{fam}, expression {eid}")
+
+
+def test_exhaustive_sizes_many_rowgroups_per_block(torch_cuda, wiz):
+    """More row-groups than the launch has blocks (one per compute unit): each block sizes several row-groups in a
+    row on one hash table, restored between them; every row-group's sizes equal the oracle's."""
+    torch = torch_cuda
+    rgs = rowgroups()
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    n = 2 * n_cu + 7
+    order = [(3 * i) % len(rgs) for i in range(n)]  # neighbours differ: a block meets different families in turn
+    host = np.concatenate([rgs[j][1] for j in order])
+    want = {j: wo.candidate_sizes(rgs[j][1]).astype(np.int64) for j in set(order)}
+    d_x = torch.from_numpy(host).cuda()
+    d_s = torch.zeros((n, len(wo.CANDIDATES)), dtype=torch.int64, device="cuda")
+    print(f"wizard sizes over {n} row-groups", flush=True)
+    wiz.exhaustive_sizes(d_x.data_ptr(), n, d_s.data_ptr())
+    torch.cuda.synchronize()
+    sizes = d_s.cpu().numpy()
+    for r, j in enumerate(order):
+        np.testing.assert_array_equal(sizes[r], want[j], err_msg=f"row-group {r} ({rgs[j][0]})")
