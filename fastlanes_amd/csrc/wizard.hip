@@ -335,9 +335,13 @@ __device__ PatchPlan delta_patch(const uint64_t* __restrict__ xv, uint64_t* sb, 
     const unsigned  hop = 1u << (pd.bw < 11 ? pd.bw : 11);
     int             last = -1;
     n = 0, extra = 0;
+    uint64_t dq[16];  // the 16 differences this lane tests, loaded before the ballot chain
+#pragma unroll
+    for (unsigned q = 0; q < 16; ++q) dq[q] = diff(orig_of(64 * q + lane));
+#pragma unroll
     for (unsigned q = 0; q < 16; ++q) {
         const unsigned k  = 64 * q + lane;
-        const bool     ex = is_exception(diff(orig_of(k)), pd);
+        const bool     ex = is_exception(dq[q], pd);
         const uint64_t m  = __ballot(ex);
         if (ex) {
             const uint64_t below = m & ((1ull << lane) - 1);
@@ -401,9 +405,13 @@ __global__ __launch_bounds__(1024) void k_wizard_sizes(const uint64_t* __restric
             const unsigned hop = 1u << (pp.bw < 11 ? pp.bw : 11);
             uint32_t       n = 0, extra = 0;
             int            last = -1;  // last exception index seen by the wave so far
+            uint64_t       xq[16];     // the 16 values this lane tests, loaded before the ballot chain
+#pragma unroll
+            for (unsigned q = 0; q < 16; ++q) xq[q] = x[v * 1024 + orig_of(64 * q + lane)];
+#pragma unroll
             for (unsigned q = 0; q < 16; ++q) {
                 const unsigned k  = 64 * q + lane;
-                const bool     ex = is_exception(x[v * 1024 + orig_of(k)], pp);
+                const bool     ex = is_exception(xq[q], pp);
                 const uint64_t m  = __ballot(ex);
                 if (ex) {
                     const uint64_t below = m & ((1ull << lane) - 1);
