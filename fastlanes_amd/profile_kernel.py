@@ -32,8 +32,8 @@ def main():
     ap.add_argument("--page-ffor", action="store_true",
                     help="instead: decode an FFOR column (pool id 1) of 2^log2n values of --width bits, values in "
                          "[0, 128) (bit width 7)")
-    ap.add_argument("--page-kind", choices=("ffor", "bitpack", "delta", "rle", "dict"),
-                    help="instead: decode the row-group workload's column of that kind (pool id 1, 1, 11, 6, 17) of "
+    ap.add_argument("--page-kind", choices=("ffor", "bitpack", "delta", "rle", "dict", "runs16"),
+                    help="instead: decode the row-group workload's column of that kind (pool id 1, 1, 11, 6, 17; runs16: uint32 geometric runs of mean 16, pool id 6) of "
                          "2^log2n values of --width bits")
     a = ap.parse_args()
     import torch
@@ -43,9 +43,12 @@ def main():
 
         from fastlanes_amd import workloads as wl
 
-        eid = {"ffor": 1, "bitpack": 1, "delta": 11, "rle": 6, "dict": 17}[a.page_kind]
-        x = wl.rowgroup_column(a.page_kind, a.width, 1 << a.log2n, seed=5,
-                               w=7 if a.page_kind in ("ffor", "bitpack") else None)
+        eid = {"ffor": 1, "bitpack": 1, "delta": 11, "rle": 6, "dict": 17, "runs16": 6}[a.page_kind]
+        if a.page_kind == "runs16":  # the cascade's RLE workload: uint32 geometric runs of mean length 16
+            x = wl.geometric_runs(1 << a.log2n, 16.0, seed=4)
+        else:
+            x = wl.rowgroup_column(a.page_kind, a.width, 1 << a.log2n, seed=5,
+                                   w=7 if a.page_kind in ("ffor", "bitpack") else None)
         enc = fa.encode_from_memory(x, eid)
         plan = fa.Plan(enc, x.dtype)
         d_in = torch.from_numpy(enc).cuda()
@@ -61,7 +64,7 @@ def main():
         plan.check()
         sec = t0.elapsed_time(t1) * 1e-3 / a.reps
         ok = bool(np.array_equal(d_out[: x.nbytes].cpu().numpy().view(x.dtype), x))
-        print(json.dumps({"kind": a.page_kind, "values": x.size, "width": a.width, "encoded_bytes": int(enc.size),
+        print(json.dumps({"kind": a.page_kind, "values": x.size, "width": x.dtype.itemsize * 8, "encoded_bytes": int(enc.size),
                           "pages": plan.n_pages, "reps": a.reps, "bit_exact": ok, "ms": round(sec * 1e3, 4),
                           "hbm_frac": round((enc.size + x.nbytes) / sec / 8e12, 3)}), flush=True)
         return
