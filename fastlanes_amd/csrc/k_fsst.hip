@@ -391,7 +391,9 @@ __device__ __forceinline__ void decode_string_global(const uint8_t* cb, uint32_t
 }
 
 // MODE FS_OFFSETS writes the n+1 string end offsets; FS_BYTES writes the bytes at the caller's offsets; FS_ALL
-// writes both. `offsets` is indexed from the column's first string; `first_string` is the page's.
+// writes both. `offsets` is indexed from the column's first string; `first_string` is the page's. The launch bound
+// asks for 6 waves per SIMD (at most 80 VGPRs): three 512-thread blocks per compute unit, which the ~38 KB of LDS
+// per block also allows; without it the compiler takes ~100 VGPRs and two blocks fit.
 template <int MODE>
 __global__ __launch_bounds__(kSeg, 6) void k_fsst(const uint8_t* __restrict__ page, const FsstTableDev* __restrict__ table,
                                                uint64_t first_string, uint64_t* __restrict__ offsets,
