@@ -106,3 +106,22 @@ def test_fused_capacity_error(torch_cuda, oracle):
     with pytest.raises(RuntimeError, match="capacity"):
         plan.check()
     np.testing.assert_array_equal(d_off.cpu().numpy(), offs.astype(np.int64))  # offsets are still whole
+
+
+def test_fused_multi_page_column(torch_cuda):
+    """A string column over the 128 MiB page limit (2^16 vocabulary strings repeated 96 times, ~182 MB encoded, two
+    pages): each page is its own launch and its segments look back into the previous page's; the decode reproduces
+    the input's offsets and bytes."""
+    torch = torch_cuda
+    o0, d0 = wl.vocab_strings(1 << 16, seed=13)
+    reps = 96
+    data = np.tile(d0, reps)
+    offs = np.concatenate([[0], np.cumsum(np.tile(np.diff(o0).astype(np.int64), reps))])
+    enc = fa.encode_strings(offs.astype(np.uint32), data)
+    plan = fa.Plan(enc, "str")
+    assert plan.n_pages >= 2
+    n = plan.n_values
+    print(f"fsst fused multi-page: {n} strings, {plan.n_pages} pages", flush=True)
+    got_off, got = _fused(torch, plan, dev(torch, enc), n, data.size)
+    np.testing.assert_array_equal(got_off, offs)
+    np.testing.assert_array_equal(got[: data.size], data)
