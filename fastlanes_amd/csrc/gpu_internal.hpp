@@ -21,11 +21,9 @@ struct FsstTableDev {
 struct StringPlan {
     FsstTableDev* d_tables = nullptr;  // one per page entry
     uint64_t*     d_status = nullptr;  // look-back status word per half-vector segment (zeroed once)
-    uint64_t*     d_ticket = nullptr;  // segment ticket counter of the look-back launches (never reset)
+    uint64_t*     d_ticket = nullptr;  // look-back tickets: counter (never reset), the call's base, its epoch
     uint32_t*     d_err    = nullptr;  // error bits: capacity, look-back
     uint64_t      n_vec    = 0;
-    uint64_t      tickets  = 0;        // tickets handed out so far (the counter's value between launches)
-    uint32_t      epoch    = 0;        // call number tagging the status words (1..65535)
 };
 
 int  strings_prepare(const Column& col, const uint8_t* host, StringPlan& sp);

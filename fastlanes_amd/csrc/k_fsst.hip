@@ -36,6 +36,8 @@ constexpr uint32_t kMaxComp  = 32 * 1024;  // compressed bytes of a segment on t
 constexpr uint32_t kOutLds   = 24 * 1024;  // decoded bytes of a segment assembled in LDS
 constexpr uint32_t kShare    = 32;         // share granule: one word of the start bitmap
 
+constexpr unsigned kCallBase = 16;  // u64 index of the call state, 128 bytes past the ticket counter
+
 // error bits (StringPlan::d_err)
 constexpr uint32_t kErrCap = 4, kErrSpin = 16, kErrTicket = 32;
 
@@ -44,10 +46,11 @@ constexpr uint64_t kFlagAgg = 1, kFlagPrefix = 2, kValMask = (1ull << 46) - 1;
 
 struct SegCtl {
     uint64_t* status;       // one word per segment of the column
-    uint64_t* ticket;       // dispatch-order ticket counter, never reset
-    uint64_t  ticket_base;  // its value when this launch starts
+    uint64_t* ticket;       // [0] dispatch-order ticket counter (never reset); [kCallBase] its value when the call
+                            // began, [kCallBase + 1] the call's epoch (1..65535), on a cache line of their own: set on
+                            // the device by k_begin_call, so a captured graph of a call stays valid on replay
+    uint64_t  ticket_off;   // tickets taken by the call's earlier page launches
     uint64_t  seg0;         // column-wide index of the page's first segment
-    uint32_t  epoch;        // 1..65535, new for every call
     uint32_t  spin_limit;   // look-back polls before giving up (error bit kErrSpin)
 };
 
@@ -319,7 +322,7 @@ __device__ __forceinline__ void record_starts(uint32_t sbm, const uint32_t (&kp)
 // memory round trip. Status words are read and written with relaxed agent-scope atomics: a block consumes only the
 // words' own values (no data that other blocks wrote), and acquire/release at agent scope would invalidate or write
 // back the whole L2 of the XCD at every poll.
-__device__ __forceinline__ uint64_t look_back(const SegCtl& sc, uint64_t g, uint32_t* err) {
+__device__ __forceinline__ uint64_t look_back(const SegCtl& sc, uint32_t epoch, uint64_t g, uint32_t* err) {
     constexpr unsigned M     = 8;
     const unsigned     lane  = threadIdx.x % 64;
     uint64_t           excl  = 0;
@@ -334,7 +337,7 @@ __device__ __forceinline__ uint64_t look_back(const SegCtl& sc, uint64_t g, uint
             fl[m] = (uint32_t)kFlagPrefix, val[m] = 0;  // before the first segment: a prefix of 0
             if (idx >= 0) {
                 const uint64_t w = __hip_atomic_load(sc.status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                fl[m]            = (uint32_t)(w >> 48) == sc.epoch ? (uint32_t)((w >> 46) & 3u) : 0u;
+                fl[m]            = (uint32_t)(w >> 48) == epoch ? (uint32_t)((w >> 46) & 3u) : 0u;
                 val[m]           = w & kValMask;
             }
         });
@@ -410,12 +413,21 @@ __global__ __launch_bounds__(kSeg, 6) void k_fsst(const uint8_t* __restrict__ pa
     const unsigned      tid = threadIdx.x, lane = tid % 64;
 
     uint64_t local = blockIdx.x;
+    uint32_t epoch = 0;
     if constexpr (LOOKBACK) {
-        if (tid == 0)
-            bcast = __hip_atomic_fetch_add(sc.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - sc.ticket_base;
+        // one thread reads the call's state and takes the ticket; the block shares them through LDS
+        __shared__ uint32_t bepoch;
+        if (tid == 0) {
+            // written by k_begin_call, an earlier kernel of the stream: plain loads see it (and stay in the L2)
+            const uint64_t base = sc.ticket[kCallBase];
+            bepoch = (uint32_t)sc.ticket[kCallBase + 1];
+            bcast  = __hip_atomic_fetch_add(sc.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) -
+                    (base + sc.ticket_off);
+        }
         __syncthreads();
         local = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(bcast >> 32)) << 32) |
                 __builtin_amdgcn_readfirstlane((uint32_t)bcast);  // block-uniform: scalar registers
+        epoch = __builtin_amdgcn_readfirstlane(bepoch);
         __syncthreads();
         if (local >= gridDim.x) {  // the host's ticket count disagrees with the counter: stop, report
             if (tid == 0) atomicOr(err, kErrTicket);
@@ -525,7 +537,7 @@ __global__ __launch_bounds__(kSeg, 6) void k_fsst(const uint8_t* __restrict__ pa
     // publish this segment's aggregate early, so that later segments can sum past it while it decodes
     if constexpr (LOOKBACK) {
         if (tid == 0)
-            __hip_atomic_store(sc.status + g, status_word(sc.epoch, g == 0 ? kFlagPrefix : kFlagAgg, seg_total),
+            __hip_atomic_store(sc.status + g, status_word(epoch, g == 0 ? kFlagPrefix : kFlagAgg, seg_total),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     uint64_t seg_base = 0;
@@ -582,10 +594,10 @@ __global__ __launch_bounds__(kSeg, 6) void k_fsst(const uint8_t* __restrict__ pa
     // have not published yet and polls longer) and publishes the inclusive prefix
     if constexpr (LOOKBACK) {
         if (tid < 64) {
-            const uint64_t excl = g == 0 ? 0ull : look_back(sc, g, err);
+            const uint64_t excl = g == 0 ? 0ull : look_back(sc, epoch, g, err);
             if (tid == 0) {
                 if (g != 0)
-                    __hip_atomic_store(sc.status + g, status_word(sc.epoch, kFlagPrefix, excl + seg_total),
+                    __hip_atomic_store(sc.status + g, status_word(epoch, kFlagPrefix, excl + seg_total),
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 bcast = excl;
             }
@@ -620,27 +632,32 @@ __global__ __launch_bounds__(kSeg, 6) void k_fsst(const uint8_t* __restrict__ pa
     }
 }
 
-uint32_t next_epoch(StringPlan& sp) {
-    sp.epoch = sp.epoch % 0xffffu + 1;
-    return sp.epoch;
+__global__ void k_zero_first(uint64_t* offsets) { offsets[0] = 0; }
+
+// Start of a look-back call: the ticket counter's current value becomes the call's base, and the epoch advances.
+__global__ void k_begin_call(uint64_t* ticket) {
+    ticket[kCallBase]     = __hip_atomic_load(ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ticket[kCallBase + 1] = ticket[kCallBase + 1] % 0xffffu + 1;
 }
 
 template <int MODE>
 int launch_pages(const Column& col, StringPlan& sp, const uint8_t* dev_enc, uint64_t* dev_offsets, uint8_t* dev_bytes,
                  uint64_t cap, hipStream_t s) {
-    const uint32_t epoch = MODE == FS_BYTES ? 0u : next_epoch(sp);
+    if (MODE != FS_BYTES) hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(1), 0, s, sp.d_ticket);
+    uint64_t ticket_off = 0;
     for (size_t i = 0; i < col.pages.size(); ++i) {
         const PageEntry& pe = col.pages[i];
         if (!pe.vec_c) continue;
         const unsigned grid = pe.vec_c * kSegPerV;
-        SegCtl         sc {sp.d_status, sp.d_ticket, sp.tickets, (pe.out_index / 1024) * kSegPerV, epoch, 1u << 20};
+        SegCtl         sc {sp.d_status, sp.d_ticket, ticket_off, (pe.out_index / 1024) * kSegPerV, 1u << 20};
         hipLaunchKernelGGL(k_fsst<MODE>, dim3(grid), dim3(kSeg), 0, s, dev_enc + pe.page_off, sp.d_tables + i,
                            pe.out_index, dev_offsets, dev_bytes, cap, sp.d_err, sc);
-        if (MODE != FS_BYTES) sp.tickets += grid;
+        ticket_off += grid;
     }
     if (MODE != FS_BYTES && col.n_values == 0) {
-        hipError_t e = hipMemsetAsync(dev_offsets, 0, 8, s);
-        if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+        // offsets[0] of an empty column, by a kernel: a memset node recorded into a caller's captured graph is not
+        // reliable on replay
+        hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, s, dev_offsets);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? FLS_OK : hip_fail(e, "kernel launch");
@@ -670,8 +687,8 @@ int strings_prepare(const Column& col, const uint8_t* host, StringPlan& sp) {
         return hip_fail(e, "hipMemcpy");
     if ((e = hipMalloc(&sp.d_status, (nsg + 1) * sizeof(uint64_t))) != hipSuccess) return hip_fail(e, "hipMalloc");
     if ((e = hipMemset(sp.d_status, 0, (nsg + 1) * sizeof(uint64_t))) != hipSuccess) return hip_fail(e, "hipMemset");
-    if ((e = hipMalloc(&sp.d_ticket, sizeof(uint64_t))) != hipSuccess) return hip_fail(e, "hipMalloc");
-    if ((e = hipMemset(sp.d_ticket, 0, sizeof(uint64_t))) != hipSuccess) return hip_fail(e, "hipMemset");
+    if ((e = hipMalloc(&sp.d_ticket, (kCallBase + 2) * sizeof(uint64_t))) != hipSuccess) return hip_fail(e, "hipMalloc");
+    if ((e = hipMemset(sp.d_ticket, 0, (kCallBase + 2) * sizeof(uint64_t))) != hipSuccess) return hip_fail(e, "hipMemset");
     if ((e = hipMalloc(&sp.d_err, 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
     if ((e = hipMemset(sp.d_err, 0, 4)) != hipSuccess) return hip_fail(e, "hipMemset");
     return FLS_OK;

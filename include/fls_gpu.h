@@ -62,7 +62,8 @@ int fls_gpu_plan_decode_strings(fls_gpu_plan* plan, const uint8_t* dev_encoded, 
 /* Steps 1 and 2 in one pass over the compressed bytes: writes the n+1 offsets and the bytes (bytes_capacity bounds
  * the bytes; a column that needs more reports FLS_E_DATA at fls_gpu_plan_check). No reference counterpart: the
  * reference decodes string by string on the CPU. Calls on one plan must not overlap (they share its segment status
- * words). */
+ * words). Kernel launches only: the call may be captured into a HIP graph; each replay takes its ticket base and
+ * epoch on the device. */
 int fls_gpu_plan_decode_strings_fused(fls_gpu_plan* plan, const uint8_t* dev_encoded, uint64_t* dev_offsets,
                                       uint8_t* dev_bytes, uint64_t bytes_capacity, void* stream);
 

@@ -125,3 +125,43 @@ def test_fused_multi_page_column(torch_cuda):
     got_off, got = _fused(torch, plan, dev(torch, enc), n, data.size)
     np.testing.assert_array_equal(got_off, offs)
     np.testing.assert_array_equal(got[: data.size], data)
+
+
+def test_fused_decode_in_a_graph(torch_cuda, oracle):
+    """The fused decode captured in a HIP graph and replayed three times, with eager decodes of the same plan between
+    the replays (they advance its ticket counter and epoch) and the outputs cleared before each replay: every replay
+    decodes the column bit-exact, because each call's ticket base and epoch are taken on the device."""
+    torch = torch_cuda
+    offs, data = wl.vocab_strings(12 * 1024, seed=21)
+    enc = oracle.encode_str_column(offs, data)
+    plan = fa.Plan(enc, "str")
+    n = plan.n_values
+    d_in = dev(torch, enc)
+    d_off = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
+    d_bytes = empty(torch, data.size)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):  # one eager call first, outside the capture
+        plan.decode_strings_fused(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size, s.cuda_stream)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan.decode_strings_fused(d_in.data_ptr(), d_off.data_ptr(), d_bytes.data_ptr(), data.size,
+                                  torch.cuda.current_stream().cuda_stream)
+    for rep in range(3):
+        print(f"fsst graph replay {rep}", flush=True)
+        d_off.zero_()
+        d_bytes.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        plan.check()
+        np.testing.assert_array_equal(d_off.cpu().numpy(), offs.astype(np.int64), err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(d_bytes.cpu().numpy()[: data.size], data, err_msg=f"replay {rep}")
+        # eager work on the same plan between replays
+        e_off = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
+        plan.string_offsets(d_in.data_ptr(), e_off.data_ptr())
+        e_bytes = empty(torch, data.size)
+        plan.decode_strings_fused(d_in.data_ptr(), e_off.data_ptr(), e_bytes.data_ptr(), data.size)
+        plan.check()
+        np.testing.assert_array_equal(e_bytes.cpu().numpy()[: data.size], data)
